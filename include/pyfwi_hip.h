@@ -1,0 +1,183 @@
+/*
+ * pyfwi_hip.h -- C ABI of the MI355X (gfx950) elastic propagator and
+ * adjoint-state gradient.
+ *
+ * The ABI replaces the OpenCL layer under PyFWI's WavePropagator
+ * (src/PyFWI/wave_propagation.py): the kernel build with macro-pasted
+ * constants (:181-192, :409-509), the device buffers (:196-277), the per-step
+ * kernel enqueues of the forward loop (:625-733) and of the adjoint loop
+ * (:735-856) and the gradient read-back (:279-331).  The Python mirror of
+ * WavePropagator (pyfwi_amd/wave_propagation.py) binds it with ctypes.
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success or a PYFWI_E* code; the text of
+ *     the last error on the calling thread is pyfwi_last_error().
+ *   - Arrays are float32, row-major, x fastest, over the grid including the
+ *     absorbing layer (tnz x tnx), like the reference's buffers.
+ *   - `mem` says where caller pointers live: PYFWI_MEM_HOST or
+ *     PYFWI_MEM_DEVICE (a device pointer on the plan's GPU, e.g. a torch
+ *     tensor's data_ptr()).
+ *   - A plan is used by one thread at a time; calls on one plan are
+ *     serialised on the plan's stream.  Plans are independent.
+ */
+#ifndef PYFWI_HIP_H
+#define PYFWI_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PYFWI_OK = 0,
+    PYFWI_EINVAL = 1,   /* invalid argument or geometry             */
+    PYFWI_ENOMEM = 2,   /* device or host allocation failed         */
+    PYFWI_EHIP = 3,     /* HIP runtime error                        */
+    PYFWI_ESTATE = 4,   /* call out of order (e.g. no forward yet)  */
+    PYFWI_ENONFINITE = 5 /* model holds NaN or Inf                   */
+};
+
+enum { PYFWI_MEM_HOST = 0, PYFWI_MEM_DEVICE = 1 };
+
+/* Gradient modes (pyfwi_gradient `mode`). */
+enum {
+    /* The reference's adjoint: reconstruct the forward state by restoring the
+     * snapshots taken at `chp` and reversing single steps in between, inject
+     * residual row k at backward step k, adjoint stresses then adjoint
+     * velocities, correlate (wave_propagation.py:751-848, elastic.cl:372-526). */
+    PYFWI_MODE_PYFWI = 0,
+    /* Exact discrete adjoint of the forward (FD-checkable), forward states
+     * recomputed from checkpoints kept every `ckpt_every` steps. */
+    PYFWI_MODE_EXACT = 1
+};
+
+/* Flags (pyfwi_gradient `flags`). */
+enum {
+    PYFWI_F_PRECOND = 1 /* also accumulate the squared-energy preconditioners */
+};
+
+/* Grid and stencil (the macro block of wave_propagation.py:434-490). */
+typedef struct pyfwi_grid {
+    int32_t tnz, tnx;   /* rows, columns including the absorbing layer (Nz, Nx) */
+    int32_t npml;       /* absorbing-layer thickness in cells (npml)            */
+    int32_t sdo;        /* half stencil width: 2 for order 4, 4 for order 8      */
+    double c[4];        /* stencil coefficients as the kernels see them (c1..c4) */
+    double dt;          /* internal time step as the kernels see it (dt)         */
+    double dx, dz;      /* grid spacing as the kernels see it (dx, dz)           */
+} pyfwi_grid;
+
+/* One shot: a point source and a surface receiver line
+ * (injSrc / Adj_injSrc of elastic_surface.cl:7-72). */
+typedef struct pyfwi_shot {
+    int32_t src_x, src_z;   /* source cell (column, row), absorbing layer incl.   */
+    int32_t rec_z;          /* receiver row (rec_surface_const)                   */
+    int32_t rec_x0;         /* first receiver column (rec_surface_var)            */
+    int32_t rec_dx;         /* receiver spacing in cells (dxr), >= 1              */
+    int32_t n_rec;          /* receivers recorded; all must lie inside the grid   */
+    int32_t rec_stride;     /* floats per record row (>= n_rec; the reference's Nr) */
+    int32_t nt;             /* internal time steps                                 */
+    int32_t dt_scale;       /* internal steps per record sample, >= 1              */
+    int32_t n_samples;      /* record rows (nt_ext)                                */
+    int64_t src_offset;     /* float offset of this shot's source table
+                               [ceil(nt/dt_scale)][5] (vx, vz, txx, tzz, txz)     */
+    int64_t rec_offset;     /* float offset of this shot's record / residual
+                               [5][n_samples][rec_stride]                         */
+} pyfwi_shot;
+
+typedef struct pyfwi_plan pyfwi_plan;
+
+/* Library version (major*100 + minor). */
+int32_t pyfwi_version(void);
+
+/* Text of the last error on this thread ("" if none). */
+const char *pyfwi_last_error(void);
+
+/* Create a plan on the current HIP device for one grid.  Replaces the
+ * program build and buffer set-up of WavePreparation.__init__
+ * (wave_propagation.py:181-246).  `stream` is a hipStream_t or NULL for a
+ * plan-owned stream. */
+int32_t pyfwi_plan_create(const pyfwi_grid *grid, void *stream, pyfwi_plan **out);
+
+/* Release every device buffer of the plan. */
+int32_t pyfwi_plan_destroy(pyfwi_plan *plan);
+
+/* Use another stream for later calls (e.g. torch's current stream). */
+int32_t pyfwi_plan_set_stream(pyfwi_plan *plan, void *stream);
+
+/* Upload the medium: lam, mu, buoyancy (1/rho) [tnz*tnx] and the damping
+ * profiles dx_prof [tnx] (per column) and dz_prof [tnz] (per row), already
+ * scaled by the model's maximum P velocity.  Replaces elastic_buffers and
+ * pml_preparation (wave_propagation.py:570-591, :400-407); the reference's
+ * full-grid damping arrays are constant along one axis. */
+int32_t pyfwi_set_model(pyfwi_plan *plan, const float *lam, const float *mu,
+                        const float *buoy, const float *dx_prof,
+                        const float *dz_prof, int32_t mem);
+
+/* Forward modelling of `nshots` shots in one batch (forward_propagator +
+ * __kernel + make_seismogram, wave_propagation.py:625-733, 334-366).
+ *   src   source tables indexed by shot.src_offset
+ *   seis  records indexed by shot.rec_offset: seis[f][k][r] = field f at
+ *         receiver r after k*dt_scale steps (before that step's injection);
+ *         rows that are never sampled are written as zero
+ *   chp, nchp  steps after which the state is kept on the device for a later
+ *         PYFWI_MODE_PYFWI gradient (the reference's checkpoint list); may be
+ *         NULL/0.  All shots of one call must share nt when nchp > 0.
+ *   ckpt_every  > 0: also keep the state at the start of every ckpt_every-th
+ *         step for a later PYFWI_MODE_EXACT gradient (0: none) */
+int32_t pyfwi_forward(pyfwi_plan *plan, const pyfwi_shot *shots, int32_t nshots,
+                      const float *src, float *seis, const int32_t *chp,
+                      int32_t nchp, int32_t ckpt_every, int32_t mem);
+
+/* Adjoint-state gradient for the shots of the last pyfwi_forward call
+ * (__adjoint_modelling_per_source + __kernel_gradient,
+ * wave_propagation.py:735-856).
+ *   res   residual records, same layout and offsets as `seis`
+ *   grad  output [6][tnz*tnx] summed over shots:
+ *         g_mu, g_mu_precond, g_lam, g_lam_precond, g_rho, g_rho_precond
+ *         (the buffers the reference reads back, wave_propagation.py:286-297)
+ *         for PYFWI_MODE_PYFWI; for PYFWI_MODE_EXACT slots 0, 2, 4 hold
+ *         dJ/dmu, dJ/dlam, dJ/drho (rho through b = 1/rho) and slots 1, 3, 5
+ *         are zero.
+ *   mode  PYFWI_MODE_PYFWI or PYFWI_MODE_EXACT
+ *   ckpt_every  exact mode: checkpoint interval in steps; when the last
+ *         forward kept no checkpoints at this interval the forward is run
+ *         again here (0: use the forward's interval, or 50)
+ *   flags PYFWI_F_PRECOND (pyfwi mode) */
+int32_t pyfwi_gradient(pyfwi_plan *plan, const pyfwi_shot *shots,
+                       int32_t nshots, const float *src, const float *res,
+                       float *grad, int32_t mode, int32_t ckpt_every,
+                       int32_t flags, int32_t mem);
+
+/* Copy kept snapshot `slot` of shot `shot` ([5][tnz*tnx]: vx, vz, txx, tzz,
+ * txz) to `out` (the reference's WavePropagator.W, wave_propagation.py:150-156,
+ * 710-727). */
+int32_t pyfwi_get_snapshot(pyfwi_plan *plan, int32_t shot, int32_t slot,
+                           float *out, int32_t mem);
+
+/* Wait for all work queued on the plan's stream. */
+int32_t pyfwi_synchronize(pyfwi_plan *plan);
+
+/* Per-kernel timing with HIP events on the plan's stream.  While enabled,
+ * every stencil launch is bracketed by an event pair; pyfwi_kernel_timing
+ * returns the summed duration and launch count of one kernel kind since the
+ * last enable.  Kinds: see PYFWI_K_* below. */
+enum {
+    PYFWI_K_FWD_VEL = 0,   /* velocity half step, forward             */
+    PYFWI_K_FWD_STR = 1,   /* stress half step, forward               */
+    PYFWI_K_REV_STR = 2,   /* stress half step, time reversed         */
+    PYFWI_K_REV_VEL = 3,   /* velocity half step, time reversed       */
+    PYFWI_K_ADJ_STR = 4,   /* adjoint stresses                        */
+    PYFWI_K_ADJ_VEL = 5,   /* adjoint velocities + imaging condition  */
+    PYFWI_K_EXA_STR = 6,   /* exact adjoint of the stress half step   */
+    PYFWI_K_EXA_VEL = 7,   /* exact adjoint of the velocity half step */
+    PYFWI_K_COUNT = 8
+};
+int32_t pyfwi_plan_set_timing(pyfwi_plan *plan, int32_t enable);
+int32_t pyfwi_kernel_timing(pyfwi_plan *plan, int32_t kind, double *total_ms,
+                            int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PYFWI_HIP_H */

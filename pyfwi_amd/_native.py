@@ -1,0 +1,164 @@
+"""ctypes binding of ``libpyfwi_hip.so`` (declared in include/pyfwi_hip.h).
+
+There is no fallback: if the library is missing or no GPU is visible, the
+calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpyfwi_hip.so")
+
+MEM_HOST = 0
+MEM_DEVICE = 1
+MODE_PYFWI = 0
+MODE_EXACT = 1
+F_PRECOND = 1
+
+K_FWD_VEL, K_FWD_STR, K_REV_STR, K_REV_VEL, K_ADJ_STR, K_ADJ_VEL, K_EXA_STR, K_EXA_VEL = range(8)
+KERNEL_NAMES = ["fwd_vel", "fwd_str", "rev_str", "rev_vel", "adj_str", "adj_vel_grad",
+                "exact_str", "exact_vel"]
+
+# exported symbols, in header order (checked by the CPU test suite)
+EXPORTS = [
+    "pyfwi_version", "pyfwi_last_error", "pyfwi_plan_create", "pyfwi_plan_destroy",
+    "pyfwi_plan_set_stream", "pyfwi_set_model", "pyfwi_forward", "pyfwi_gradient",
+    "pyfwi_get_snapshot", "pyfwi_synchronize", "pyfwi_plan_set_timing",
+    "pyfwi_kernel_timing",
+]
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("tnz", ctypes.c_int32), ("tnx", ctypes.c_int32),
+                ("npml", ctypes.c_int32), ("sdo", ctypes.c_int32),
+                ("c", ctypes.c_double * 4), ("dt", ctypes.c_double),
+                ("dx", ctypes.c_double), ("dz", ctypes.c_double)]
+
+
+class Shot(ctypes.Structure):
+    _fields_ = [("src_x", ctypes.c_int32), ("src_z", ctypes.c_int32),
+                ("rec_z", ctypes.c_int32), ("rec_x0", ctypes.c_int32),
+                ("rec_dx", ctypes.c_int32), ("n_rec", ctypes.c_int32),
+                ("rec_stride", ctypes.c_int32), ("nt", ctypes.c_int32),
+                ("dt_scale", ctypes.c_int32), ("n_samples", ctypes.c_int32),
+                ("src_offset", ctypes.c_int64), ("rec_offset", ctypes.c_int64)]
+
+
+class NativeError(RuntimeError):
+    """Error reported by libpyfwi_hip (status code + message)."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"libpyfwi_hip error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load(path=None):
+    """Load and prototype the library once; raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+            " (hipcc, gfx950); there is no CPU fallback")
+    lib = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    i32 = ctypes.c_int32
+    fp = ctypes.c_void_p  # float pointers are passed as raw addresses
+    lib.pyfwi_version.restype = i32
+    lib.pyfwi_last_error.restype = ctypes.c_char_p
+    lib.pyfwi_plan_create.argtypes = [ctypes.POINTER(Grid), P, ctypes.POINTER(P)]
+    lib.pyfwi_plan_destroy.argtypes = [P]
+    lib.pyfwi_plan_set_stream.argtypes = [P, P]
+    lib.pyfwi_set_model.argtypes = [P, fp, fp, fp, fp, fp, i32]
+    lib.pyfwi_forward.argtypes = [P, ctypes.POINTER(Shot), i32, fp, fp,
+                                  ctypes.POINTER(ctypes.c_int32), i32, i32, i32]
+    lib.pyfwi_gradient.argtypes = [P, ctypes.POINTER(Shot), i32, fp, fp, fp, i32, i32, i32, i32]
+    lib.pyfwi_get_snapshot.argtypes = [P, i32, i32, fp, i32]
+    lib.pyfwi_synchronize.argtypes = [P]
+    lib.pyfwi_plan_set_timing.argtypes = [P, i32]
+    lib.pyfwi_kernel_timing.argtypes = [P, i32, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_int64)]
+    for name in EXPORTS:
+        if name not in ("pyfwi_version", "pyfwi_last_error"):
+            getattr(lib, name).restype = i32
+    _lib = lib
+    return lib
+
+
+def check(code):
+    if code != 0:
+        msg = _lib.pyfwi_last_error().decode(errors="replace")
+        raise NativeError(code, msg)
+
+
+def ptr(a):
+    """Raw address of a contiguous numpy array or torch tensor."""
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("array must be C-contiguous")
+        return a.ctypes.data
+    return a.data_ptr()
+
+
+class Plan:
+    """Owner of one native plan (one grid on the current GPU)."""
+
+    def __init__(self, grid, stream=None):
+        self.lib = load()
+        self.grid = grid
+        h = ctypes.c_void_p()
+        check(self.lib.pyfwi_plan_create(ctypes.byref(grid), stream, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.pyfwi_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream):
+        check(self.lib.pyfwi_plan_set_stream(self.h, stream))
+
+    def set_model(self, lam, mu, buoy, dxp, dzp, mem):
+        check(self.lib.pyfwi_set_model(self.h, ptr(lam), ptr(mu), ptr(buoy), ptr(dxp),
+                                       ptr(dzp), mem))
+
+    def forward(self, shots, src, seis, chp, ckpt_every, mem):
+        arr = (Shot * len(shots))(*shots)
+        chp = np.ascontiguousarray(chp, np.int32) if chp is not None else np.zeros(0, np.int32)
+        cp = chp.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)) if chp.size else None
+        check(self.lib.pyfwi_forward(self.h, arr, len(shots), ptr(src), ptr(seis), cp,
+                                     int(chp.size), int(ckpt_every), mem))
+
+    def gradient(self, shots, src, res, grad, mode, ckpt_every, flags, mem):
+        arr = (Shot * len(shots))(*shots)
+        check(self.lib.pyfwi_gradient(self.h, arr, len(shots),
+                                      ptr(src) if src is not None else None, ptr(res),
+                                      ptr(grad), mode, int(ckpt_every), flags, mem))
+
+    def snapshot(self, shot, slot, out, mem):
+        check(self.lib.pyfwi_get_snapshot(self.h, shot, slot, ptr(out), mem))
+
+    def synchronize(self):
+        check(self.lib.pyfwi_synchronize(self.h))
+
+    def set_timing(self, enable):
+        check(self.lib.pyfwi_plan_set_timing(self.h, 1 if enable else 0))
+
+    def kernel_timing(self, kind):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        check(self.lib.pyfwi_kernel_timing(self.h, kind, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value

@@ -1,0 +1,61 @@
+// Internal interface between the kernels (pyfwi_kernels.hip) and the plan /
+// C-ABI layer (pyfwi_plan.hip).  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/pyfwi_hip.h"
+
+namespace pyfwi {
+
+constexpr int HALO = 4;      // zero rows above and below each padded field
+constexpr int TILE_X = 64;   // one wavefront along x
+constexpr int TILE_Y = 4;    // 4 wavefronts per block along z
+
+// Constants every kernel takes by value.
+struct DevGrid {
+    int nz, nx;          // tnz, tnx
+    int pitch;           // padded row pitch in floats
+    int sdo;             // half stencil width
+    int npml;
+    int64_t fstride;     // floats per padded field
+    int64_t origin;      // offset of cell (0, 0) inside a padded field
+    float cx[4], cz[4];  // c_k / dx, c_k / dz
+    float dt;
+};
+
+void launch_vel(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
+                float *fields, const float *bu, const float *dxp, const float *dzp,
+                hipStream_t st);
+void launch_str(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
+                float *fields, const float *lam, const float *mu, const float *dxp,
+                const float *dzp, hipStream_t st);
+void launch_inject_record(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
+                          float *fields, const float *src, float *seis, hipStream_t st);
+void launch_adj_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
+                       float *adj, const float *res, hipStream_t st);
+void launch_adj_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
+                    float *adj, const float *bu, const float *dxp, const float *dzp,
+                    hipStream_t st);
+void launch_adj_vel_grad(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
+                         const float *fields, float *adj, const float *lam,
+                         const float *mu, const float *bu, const float *dxp,
+                         const float *dzp, float *gacc, bool precond, hipStream_t st);
+void launch_exact_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
+                      const float *vel, int64_t vel_stride, float *adj, float *av,
+                      const float *lam, const float *mu, const float *dxp,
+                      const float *dzp, float *gacc, hipStream_t st);
+void launch_exact_vel(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
+                      const float *tau, int64_t tau_stride, float *adj, const float *av_in,
+                      float *av_out, const float *bu, const float *dxp, const float *dzp,
+                      float *gacc, hipStream_t st);
+void launch_exact_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
+                         float *adj, float *av, const float *res, hipStream_t st);
+void launch_copy_shots(float *dst, int64_t dst_stride, const float *src, int64_t src_stride,
+                       int64_t count, int nshots, hipStream_t st);
+void launch_grad_sum(const DevGrid &g, int nshots, int nslot, const float *gacc, float *out,
+                     const float *bu, int b_to_rho, hipStream_t st);
+void launch_pad(const DevGrid &g, const float *src, float *dst, hipStream_t st);
+void launch_unpad(const DevGrid &g, const float *src, float *dst, int nf, hipStream_t st);
+
+} // namespace pyfwi

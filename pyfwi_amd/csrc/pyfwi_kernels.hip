@@ -1,0 +1,723 @@
+// Stencil, injection and correlation kernels of the elastic propagator for
+// gfx950 (MI355X).  FP32 throughout, no MFMA: the path is HBM-bandwidth bound.
+//
+// Device layout (see pyfwi_plan.hip): every field and model array is a padded
+// row-major grid with row pitch P (a multiple of 64 floats, P >= tnx + 4) and
+// HALO zero rows above and below; cell (i, j) of the reference's tnz x tnx
+// buffer lives at origin + i*P + j.  Columns tnx..P-1 of one row double as the
+// left halo of the next row, so every stencil tap of an active cell reads
+// either a real cell or a zero, which is what the reference's flat-index taps
+// see inside its never-updated outer ring (src/PyFWI/elastic.cl:163-166).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/pyfwi_hip.h"
+#include "pyfwi_kernels.h"
+
+namespace pyfwi {
+
+// ------------------------------------------------------------------ helpers
+
+// Staggered derivatives at p along stride s with pre-divided coefficients
+// c[k] = c_k / h.  Minus: pairs (0,-1) (1,-2) (2,-3) (3,-4); plus: pairs
+// (1,0) (2,-1) (3,-2) (4,-3) -- the reference's Dx-/Dx+/Dz-/Dz+
+// (src/PyFWI/elastic.cl:168-172, 208-213).
+template <int SDO>
+__device__ __forceinline__ float dminus(const float *__restrict__ p, int s,
+                                        const float c[4])
+{
+    float r = c[0] * (p[0] - p[-s]);
+    r = fmaf(c[1], p[s] - p[-2 * s], r);
+    if constexpr (SDO == 4) {
+        r = fmaf(c[2], p[2 * s] - p[-3 * s], r);
+        r = fmaf(c[3], p[3 * s] - p[-4 * s], r);
+    }
+    return r;
+}
+
+template <int SDO>
+__device__ __forceinline__ float dplus(const float *__restrict__ p, int s,
+                                       const float c[4])
+{
+    float r = c[0] * (p[s] - p[0]);
+    r = fmaf(c[1], p[2 * s] - p[-s], r);
+    if constexpr (SDO == 4) {
+        r = fmaf(c[2], p[3 * s] - p[-2 * s], r);
+        r = fmaf(c[3], p[4 * s] - p[-3 * s], r);
+    }
+    return r;
+}
+
+// Same with each tap the product of two arrays, a[k]*f[k].
+template <int SDO>
+__device__ __forceinline__ float dminus2(const float *__restrict__ f,
+                                         const float *__restrict__ a, int s,
+                                         const float c[4])
+{
+    auto v = [&](int k) { return f[k] * a[k]; };
+    float r = c[0] * (v(0) - v(-s));
+    r = fmaf(c[1], v(s) - v(-2 * s), r);
+    if constexpr (SDO == 4) {
+        r = fmaf(c[2], v(2 * s) - v(-3 * s), r);
+        r = fmaf(c[3], v(3 * s) - v(-4 * s), r);
+    }
+    return r;
+}
+
+template <int SDO>
+__device__ __forceinline__ float dplus2(const float *__restrict__ f,
+                                        const float *__restrict__ a, int s,
+                                        const float c[4])
+{
+    auto v = [&](int k) { return f[k] * a[k]; };
+    float r = c[0] * (v(s) - v(0));
+    r = fmaf(c[1], v(2 * s) - v(-s), r);
+    if constexpr (SDO == 4) {
+        r = fmaf(c[2], v(3 * s) - v(-2 * s), r);
+        r = fmaf(c[3], v(4 * s) - v(-3 * s), r);
+    }
+    return r;
+}
+
+// Taps of (lam + 2 mu) * f.
+template <int SDO>
+__device__ __forceinline__ float dminus_m(const float *__restrict__ f,
+                                          const float *__restrict__ l,
+                                          const float *__restrict__ m, int s,
+                                          const float c[4])
+{
+    auto v = [&](int k) { return (l[k] + 2.0f * m[k]) * f[k]; };
+    float r = c[0] * (v(0) - v(-s));
+    r = fmaf(c[1], v(s) - v(-2 * s), r);
+    if constexpr (SDO == 4) {
+        r = fmaf(c[2], v(2 * s) - v(-3 * s), r);
+        r = fmaf(c[3], v(3 * s) - v(-4 * s), r);
+    }
+    return r;
+}
+
+template <int SDO>
+__device__ __forceinline__ float dplus_m(const float *__restrict__ f,
+                                         const float *__restrict__ l,
+                                         const float *__restrict__ m, int s,
+                                         const float c[4])
+{
+    auto v = [&](int k) { return (l[k] + 2.0f * m[k]) * f[k]; };
+    float r = c[0] * (v(s) - v(0));
+    r = fmaf(c[1], v(2 * s) - v(-s), r);
+    if constexpr (SDO == 4) {
+        r = fmaf(c[2], v(3 * s) - v(-2 * s), r);
+        r = fmaf(c[3], v(4 * s) - v(-3 * s), r);
+    }
+    return r;
+}
+
+// Active shot test and cell coordinates shared by all stencil kernels.
+// Forward steps run for t < nt; backward steps (reconstruction and adjoint)
+// for 1 <= t < nt, the reference's loop `for t in range(nt-1, 0, -1)`.
+template <int SDO>
+__device__ __forceinline__ bool cell(const DevGrid &g, int &i, int &j)
+{
+    j = SDO + (int)(blockIdx.x * TILE_X + threadIdx.x);
+    i = SDO + (int)(blockIdx.y * TILE_Y + threadIdx.y);
+    return i <= g.nz - SDO && j <= g.nx - SDO;
+}
+
+__device__ __forceinline__ bool fwd_active(const pyfwi_shot *shots, int s, int t)
+{
+    return t < shots[s].nt;
+}
+
+__device__ __forceinline__ bool bwd_active(const pyfwi_shot *shots, int s, int t)
+{
+    return t >= 1 && t < shots[s].nt;
+}
+
+// ---------------------------------------------------------- forward sweeps
+
+// Velocity half step of every active shot (src/PyFWI/elastic.cl:155-226).
+//   DIR = +1: v += dt*b*D(tau); v -= dt*(dx+dz)*v   (damping after)
+//   DIR = -1: v -= dt*(dx+dz)*v; v -= dt*b*D(tau)   (time reversed)
+template <int SDO, int DIR>
+__global__ __launch_bounds__(TILE_X *TILE_Y) void k_vel(DevGrid g, const pyfwi_shot *shots,
+                                                        int t, float *fields,
+                                                        const float *__restrict__ bu,
+                                                        const float *__restrict__ dxp,
+                                                        const float *__restrict__ dzp)
+{
+    const int s = blockIdx.z;
+    if (DIR > 0 ? !fwd_active(shots, s, t) : !bwd_active(shots, s, t)) return;
+    int i, j;
+    if (!cell<SDO>(g, i, j)) return;
+    const int64_t fs = g.fstride;
+    float *F = fields + (int64_t)s * 5 * fs + g.origin;
+    const int64_t c = (int64_t)i * g.pitch + j;
+    float *vx = F, *vz = F + fs;
+    const float *txx = F + 2 * fs + c, *tzz = F + 3 * fs + c, *txz = F + 4 * fs + c;
+    const float ax = bu[g.origin + c] * (dminus<SDO>(txx, 1, g.cx) + dminus<SDO>(txz, g.pitch, g.cz));
+    const float az = bu[g.origin + c] * (dplus<SDO>(tzz, g.pitch, g.cz) + dplus<SDO>(txz, 1, g.cx));
+    const float damp = dxp[j] + dzp[i];
+    float ux = vx[c], uz = vz[c];
+    if (DIR > 0) {
+        ux = fmaf(g.dt, ax, ux);
+        ux = fmaf(g.dt, -damp * ux, ux);
+        uz = fmaf(g.dt, az, uz);
+        uz = fmaf(g.dt, -damp * uz, uz);
+    } else {
+        ux = fmaf(-g.dt, damp * ux, ux);
+        ux = fmaf(-g.dt, ax, ux);
+        uz = fmaf(-g.dt, damp * uz, uz);
+        uz = fmaf(-g.dt, az, uz);
+    }
+    vx[c] = ux;
+    vz[c] = uz;
+}
+
+// Stress half step (src/PyFWI/elastic.cl:235-308).
+template <int SDO, int DIR>
+__global__ __launch_bounds__(TILE_X *TILE_Y) void k_str(DevGrid g, const pyfwi_shot *shots,
+                                                        int t, float *fields,
+                                                        const float *__restrict__ lam,
+                                                        const float *__restrict__ mu,
+                                                        const float *__restrict__ dxp,
+                                                        const float *__restrict__ dzp)
+{
+    const int s = blockIdx.z;
+    if (DIR > 0 ? !fwd_active(shots, s, t) : !bwd_active(shots, s, t)) return;
+    int i, j;
+    if (!cell<SDO>(g, i, j)) return;
+    const int64_t fs = g.fstride;
+    float *F = fields + (int64_t)s * 5 * fs + g.origin;
+    const int64_t c = (int64_t)i * g.pitch + j;
+    const float *vx = F + c, *vz = F + fs + c;
+    float *txx = F + 2 * fs, *tzz = F + 3 * fs, *txz = F + 4 * fs;
+    const float exx = dplus<SDO>(vx, 1, g.cx);
+    const float ezz = dminus<SDO>(vz, g.pitch, g.cz);
+    const float gxz = dminus<SDO>(vz, 1, g.cx) + dplus<SDO>(vx, g.pitch, g.cz);
+    const float l = lam[g.origin + c], m = mu[g.origin + c];
+    const float lm = l + 2.0f * m;
+    const float sxx = lm * exx + l * ezz;
+    const float szz = lm * ezz + l * exx;
+    const float sxz = m * gxz;
+    const float damp = dxp[j] + dzp[i];
+    float a = txx[c], b = tzz[c], d = txz[c];
+    if (DIR > 0) {
+        a = fmaf(g.dt, sxx, a);
+        b = fmaf(g.dt, szz, b);
+        d = fmaf(g.dt, sxz, d);
+        a = fmaf(g.dt, -damp * a, a);
+        b = fmaf(g.dt, -damp * b, b);
+        d = fmaf(g.dt, -damp * d, d);
+    } else {
+        a = fmaf(-g.dt, damp * a, a);
+        b = fmaf(-g.dt, damp * b, b);
+        d = fmaf(-g.dt, damp * d, d);
+        a = fmaf(-g.dt, sxx, a);
+        b = fmaf(-g.dt, szz, b);
+        d = fmaf(-g.dt, sxz, d);
+    }
+    txx[c] = a;
+    tzz[c] = b;
+    txz[c] = d;
+}
+
+// Receiver gather then source injection of step t for every shot, one block
+// per shot (src/PyFWI/elastic_surface.cl:7-41).  Record row k = t/dt_scale
+// holds the state after t steps; the source sample k is added to vx, vz, txx,
+// tzz at the source cell afterwards.
+__global__ void k_inject_record(DevGrid g, const pyfwi_shot *shots, int t,
+                                float *fields, const float *__restrict__ src,
+                                float *seis)
+{
+    const int s = blockIdx.x;
+    const pyfwi_shot sh = shots[s];
+    if (t >= sh.nt || (t % sh.dt_scale) != 0) return;
+    const int k = t / sh.dt_scale;
+    const int64_t fs = g.fstride;
+    float *F = fields + (int64_t)s * 5 * fs + g.origin;
+    if (k < sh.n_samples) {
+        float *rec = seis + sh.rec_offset;
+        const int64_t c0 = (int64_t)sh.rec_z * g.pitch + sh.rec_x0;
+        for (int r = threadIdx.x; r < sh.n_rec; r += blockDim.x) {
+            const int64_t c = c0 + (int64_t)r * sh.rec_dx;
+#pragma unroll
+            for (int f = 0; f < 5; ++f)
+                rec[((int64_t)f * sh.n_samples + k) * sh.rec_stride + r] = F[f * fs + c];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float *sv = src + sh.src_offset + (int64_t)k * 5;
+        const int64_t c = (int64_t)sh.src_z * g.pitch + sh.src_x;
+        F[c] += sv[0];
+        F[fs + c] += sv[1];
+        F[2 * fs + c] += sv[2];
+        F[3 * fs + c] += sv[3];
+    }
+}
+
+// ---------------------------------------------------- adjoint, pyfwi mode
+
+// Residual injection at backward step t (src/PyFWI/elastic_surface.cl:48-72,
+// wave_propagation.py:815-824): the reference counts injections down from
+// nt_ext, so step t uses row nt_ext-1-(floor((nt-1)/dt_scale) - t/dt_scale);
+// a negative row wraps like a Python index.
+__global__ void k_adj_inject(DevGrid g, const pyfwi_shot *shots, int t,
+                             float *adj, const float *__restrict__ res)
+{
+    const int s = blockIdx.x;
+    const pyfwi_shot sh = shots[s];
+    if (t < 1 || t >= sh.nt || (t % sh.dt_scale) != 0) return;
+    int row = sh.n_samples - 1 - ((sh.nt - 1) / sh.dt_scale - t / sh.dt_scale);
+    if (row < 0) row += sh.n_samples;
+    if (row < 0) return;
+    const int64_t fs = g.fstride;
+    float *A = adj + (int64_t)s * 5 * fs + g.origin;
+    const float *rr = res + sh.rec_offset;
+    const int64_t c0 = (int64_t)sh.rec_z * g.pitch + sh.rec_x0;
+    for (int r = threadIdx.x; r < sh.n_rec; r += blockDim.x) {
+        const int64_t c = c0 + (int64_t)r * sh.rec_dx;
+#pragma unroll
+        for (int f = 0; f < 5; ++f)
+            A[f * fs + c] += rr[((int64_t)f * sh.n_samples + row) * sh.rec_stride + r];
+    }
+}
+
+// Adjoint stresses from adjoint velocities (src/PyFWI/elastic.cl:372-411).
+template <int SDO>
+__global__ __launch_bounds__(TILE_X *TILE_Y) void k_adj_str(DevGrid g, const pyfwi_shot *shots,
+                                                            int t, float *adj,
+                                                            const float *__restrict__ bu,
+                                                            const float *__restrict__ dxp,
+                                                            const float *__restrict__ dzp)
+{
+    const int s = blockIdx.z;
+    if (!bwd_active(shots, s, t)) return;
+    int i, j;
+    if (!cell<SDO>(g, i, j)) return;
+    const int64_t fs = g.fstride;
+    float *A = adj + (int64_t)s * 5 * fs + g.origin;
+    const int64_t c = (int64_t)i * g.pitch + j;
+    const float *avx = A + c, *avz = A + fs + c;
+    const float *b = bu + g.origin + c;
+    float *atxx = A + 2 * fs, *atzz = A + 3 * fs, *atxz = A + 4 * fs;
+    const float dxpb = dplus2<SDO>(avx, b, 1, g.cx);
+    const float dzmb = dminus2<SDO>(avz, b, g.pitch, g.cz);
+    const float dzpb = dplus2<SDO>(avx, b, g.pitch, g.cz);
+    const float dxmb = dminus2<SDO>(avz, b, 1, g.cx);
+    const float ex = dxp[j], ez = dzp[i];
+    const float a = atxx[c], bb = atzz[c], d = atxz[c];
+    atxx[c] = fmaf(-g.dt, ex * a + dxpb, a);
+    atzz[c] = fmaf(-g.dt, ez * bb + dzmb, bb);
+    atxz[c] = fmaf(-g.dt, (ex + ez) * d + (dzpb + dxmb), d);
+}
+
+// Adjoint velocities (src/PyFWI/elastic.cl:419-468) fused with the imaging
+// condition of the same step (elastic.cl:470-526), which reads the adjoint
+// fields only at the cell itself.  The shear term uses the z-derivative of vz
+// as the reference does (elastic.cl:485-486).  gacc: [nshots][6][fstride]
+// = gmu, gmu_pre, glam, glam_pre, grho, grho_pre.
+template <int SDO, bool PRECOND>
+__global__ __launch_bounds__(TILE_X *TILE_Y) void k_adj_vel_grad(
+    DevGrid g, const pyfwi_shot *shots, int t, const float *fields, float *adj,
+    const float *__restrict__ lam, const float *__restrict__ mu,
+    const float *__restrict__ bu, const float *__restrict__ dxp,
+    const float *__restrict__ dzp, float *gacc)
+{
+    const int s = blockIdx.z;
+    if (!bwd_active(shots, s, t)) return;
+    int i, j;
+    if (!cell<SDO>(g, i, j)) return;
+    const int64_t fs = g.fstride;
+    float *A = adj + (int64_t)s * 5 * fs + g.origin;
+    const int64_t c = (int64_t)i * g.pitch + j;
+    const float *l = lam + g.origin + c, *m = mu + g.origin + c;
+    const float *atxx = A + 2 * fs + c, *atzz = A + 3 * fs + c, *atxz = A + 4 * fs + c;
+    const float a1 = dminus_m<SDO>(atxx, l, m, 1, g.cx);
+    const float a2 = dminus2<SDO>(atzz, l, 1, g.cx);
+    const float a3 = dminus2<SDO>(atxz, m, g.pitch, g.cz);
+    const float b1 = dplus_m<SDO>(atzz, l, m, g.pitch, g.cz);
+    const float b2 = dplus2<SDO>(atxx, l, g.pitch, g.cz);
+    const float b3 = dplus2<SDO>(atxz, m, 1, g.cx);
+    const float ex = dxp[j], ez = dzp[i];
+    float ux = A[c], uz = A[fs + c];
+    ux = fmaf(-g.dt, (2.0f * ex + ez) * ux + (a1 + a2 + a3), ux);
+    uz = fmaf(-g.dt, (ex + 2.0f * ez) * uz + (b2 + b1 + b3), uz);
+    A[c] = ux;
+    A[fs + c] = uz;
+
+    const int lo = g.npml + SDO;
+    if (i < lo || j < lo || i > g.nz - SDO - g.npml || j > g.nx - SDO - g.npml) return;
+    const float *F = fields + (int64_t)s * 5 * fs + g.origin + c;
+    const float *vx = F, *vz = F + fs, *txx = F + 2 * fs, *tzz = F + 3 * fs, *txz = F + 4 * fs;
+    const float dzp_vz = dplus<SDO>(vz, g.pitch, g.cz);
+    const float dxm_vz = dminus<SDO>(vz, 1, g.cx);
+    const float dxp_vx = dplus<SDO>(vx, 1, g.cx);
+    const float dzm_vz = dminus<SDO>(vz, g.pitch, g.cz);
+    const float dzp_tzz = dplus<SDO>(tzz, g.pitch, g.cz);
+    const float dxm_txx = dminus<SDO>(txx, 1, g.cx);
+    const float dxp_txz = dplus<SDO>(txz, 1, g.cx);
+    const float dzm_txz = dminus<SDO>(txz, g.pitch, g.cz);
+    const float aa = atxx[0], ab = atzz[0], ad = atxz[0];
+    const float dt = g.dt;
+    float *G = gacc + (int64_t)s * 6 * fs + g.origin + c;
+    G[0] += 2.0f * dt * (aa * dxp_vx + ab * dzm_vz) + dt * ad * (dzp_vz + dxm_vz);
+    G[2 * fs] += dt * (aa + ab) * (dxp_vx + dzm_vz);
+    const float b = bu[g.origin + c];
+    G[4 * fs] += -(dt * (b * b)) * (ux * (dxm_txx + dzm_txz) + uz * (dzp_tzz + dxp_txz));
+    if constexpr (PRECOND) {
+        const float sxx = txx[0], szz = tzz[0], sxz = txz[0];
+        const float wx = vx[0], wz = vz[0];
+        const float dt2 = dt * dt;
+        G[fs] += 4.0f * dt2 * (sxx * sxx * dxp_vx * dxp_vx + szz * szz * dzm_vz * dzm_vz)
+                 + dt2 * sxz * sxz * (dzp_vz * dzp_vz + dxm_vz * dxm_vz);
+        G[3 * fs] += dt2 * (sxx * sxx + szz * szz) * (dxp_vx * dxp_vx + dzm_vz * dzm_vz);
+        G[5 * fs] += dt2 * (wx * wx * (dxm_txx * dxm_txx + dzm_txz * dzm_txz)
+                            + wz * wz * (dzp_tzz * dzp_tzz + dxp_txz * dxp_txz));
+    }
+}
+
+// ------------------------------------------------- adjoint, exact mode
+//
+// Exact transpose of the forward step, with D+^T = -D- and D-^T = -D+ on
+// fields that vanish outside the active cells, and the damping factor
+// keep = 1 - dt*(dx+dz) transposing to itself.  Per backward step t:
+//   k_exact_str: q = Atau*keep (taps);  Av += -dt*D(C q);  g_lam, g_mu
+//   k_exact_vel: p = Av*keep (taps);  Atau = q - dt*D(b p);  Av' = p;  g_b;
+//                then the receiver sampling of S_t adds the residual row.
+// The velocity adjoint ping-pongs between two buffers because its taps are
+// read while its centre is written.
+
+__device__ __forceinline__ float keepf(const DevGrid &g, const float *dxp,
+                                       const float *dzp, int i, int j)
+{
+    return 1.0f - g.dt * (dxp[j] + dzp[i]);
+}
+
+// Derivative taps of q = w(k) * keep(k) for a field w along x (s=1) or z.
+template <int SDO, int PLUS, bool ALONG_X>
+__device__ __forceinline__ float dq(const DevGrid &g, const float *w, const float *coef_a,
+                                    const float *coef_b, int mode, const float *dxp,
+                                    const float *dzp, int i, int j, const float c[4])
+{
+    // mode 0: w*keep; 1: a*w*keep; 2: (a + 2 b)*w*keep
+    auto val = [&](int k) {
+        const int ii = ALONG_X ? i : i + k;
+        const int jj = ALONG_X ? j + k : j;
+        const int64_t off = ALONG_X ? k : (int64_t)k * g.pitch;
+        float v = w[off] * keepf(g, dxp, dzp, ii, jj);
+        if (mode == 1) v = coef_a[off] * v;
+        if (mode == 2) v = (coef_a[off] + 2.0f * coef_b[off]) * v;
+        return v;
+    };
+    float r;
+    if (PLUS) {
+        r = c[0] * (val(1) - val(0));
+        r = fmaf(c[1], val(2) - val(-1), r);
+        if constexpr (SDO == 4) {
+            r = fmaf(c[2], val(3) - val(-2), r);
+            r = fmaf(c[3], val(4) - val(-3), r);
+        }
+    } else {
+        r = c[0] * (val(0) - val(-1));
+        r = fmaf(c[1], val(1) - val(-2), r);
+        if constexpr (SDO == 4) {
+            r = fmaf(c[2], val(2) - val(-3), r);
+            r = fmaf(c[3], val(3) - val(-4), r);
+        }
+    }
+    return r;
+}
+
+// vel: [nshots][2][fs] velocities after the velocity half step of step t
+// (V_t); gacc: [nshots][6][fs] with slots 0 (mu) and 2 (lam) updated.
+template <int SDO>
+__global__ __launch_bounds__(TILE_X *TILE_Y) void k_exact_str(
+    DevGrid g, const pyfwi_shot *shots, int t, const float *vel, int64_t vel_shot_stride,
+    float *adj, float *av, const float *__restrict__ lam,
+    const float *__restrict__ mu, const float *__restrict__ dxp,
+    const float *__restrict__ dzp, float *gacc)
+{
+    const int s = blockIdx.z;
+    if (t >= shots[s].nt) return;
+    int i, j;
+    if (!cell<SDO>(g, i, j)) return;
+    const int64_t fs = g.fstride;
+    const int64_t c = (int64_t)i * g.pitch + j;
+    float *A = adj + (int64_t)s * 5 * fs + g.origin;
+    float *Av = av + (int64_t)s * 2 * fs + g.origin;
+    const float *qxx = A + 2 * fs + c, *qzz = A + 3 * fs + c, *qxz = A + 4 * fs + c;
+    const float *l = lam + g.origin + c, *m = mu + g.origin + c;
+    const float sx = dq<SDO, 0, true>(g, qxx, l, m, 2, dxp, dzp, i, j, g.cx)
+                   + dq<SDO, 0, true>(g, qzz, l, m, 1, dxp, dzp, i, j, g.cx)
+                   + dq<SDO, 0, false>(g, qxz, m, m, 1, dxp, dzp, i, j, g.cz);
+    const float sz = dq<SDO, 1, false>(g, qxx, l, m, 1, dxp, dzp, i, j, g.cz)
+                   + dq<SDO, 1, false>(g, qzz, l, m, 2, dxp, dzp, i, j, g.cz)
+                   + dq<SDO, 1, true>(g, qxz, m, m, 1, dxp, dzp, i, j, g.cx);
+    // the velocity adjoint lives in the current ping-pong buffer
+    Av[c] = fmaf(-g.dt, sx, Av[c]);
+    Av[fs + c] = fmaf(-g.dt, sz, Av[fs + c]);
+    const float keep = keepf(g, dxp, dzp, i, j);
+    const float q1 = qxx[0] * keep, q2 = qzz[0] * keep, q3 = qxz[0] * keep;
+    const float *V = vel + (int64_t)s * vel_shot_stride + g.origin + c;
+    const float exx = dplus<SDO>(V, 1, g.cx);
+    const float ezz = dminus<SDO>(V + fs, g.pitch, g.cz);
+    const float gxz = dminus<SDO>(V + fs, 1, g.cx) + dplus<SDO>(V, g.pitch, g.cz);
+    float *G = gacc + (int64_t)s * 6 * fs + g.origin + c;
+    G[2 * fs] += g.dt * (q1 + q2) * (exx + ezz);
+    G[0] += g.dt * (2.0f * q1 * exx + 2.0f * q2 * ezz + q3 * gxz);
+}
+
+// tau: [nshots][3][fs] stresses after the injection of step t (input of the
+// update); gacc slot 4 accumulates dJ/db.  Writes the new stress adjoint in
+// place and p = Av*keep to av_out.  Then adds residual row t/dt_scale at the
+// receivers when step t is sampled.
+template <int SDO>
+__global__ __launch_bounds__(TILE_X *TILE_Y) void k_exact_vel(
+    DevGrid g, const pyfwi_shot *shots, int t, const float *tau, int64_t tau_shot_stride,
+    float *adj, const float *av_in, float *av_out, const float *__restrict__ bu,
+    const float *__restrict__ dxp, const float *__restrict__ dzp, float *gacc)
+{
+    const int s = blockIdx.z;
+    if (t >= shots[s].nt) return;
+    int i, j;
+    if (!cell<SDO>(g, i, j)) return;
+    const int64_t fs = g.fstride;
+    const int64_t c = (int64_t)i * g.pitch + j;
+    float *A = adj + (int64_t)s * 5 * fs + g.origin;
+    const float *Ain = av_in + (int64_t)s * 2 * fs + g.origin;
+    float *Aout = av_out + (int64_t)s * 2 * fs + g.origin;
+    const float *pvx = Ain + c, *pvz = Ain + fs + c;
+    const float *b = bu + g.origin + c;
+    const float rxx = dq<SDO, 1, true>(g, pvx, b, b, 1, dxp, dzp, i, j, g.cx);
+    const float rzz = dq<SDO, 0, false>(g, pvz, b, b, 1, dxp, dzp, i, j, g.cz);
+    const float rxz = dq<SDO, 1, false>(g, pvx, b, b, 1, dxp, dzp, i, j, g.cz)
+                    + dq<SDO, 0, true>(g, pvz, b, b, 1, dxp, dzp, i, j, g.cx);
+    const float keep = keepf(g, dxp, dzp, i, j);
+    const float px = pvx[0] * keep, pz = pvz[0] * keep;
+    float *atxx = A + 2 * fs, *atzz = A + 3 * fs, *atxz = A + 4 * fs;
+    atxx[c] = fmaf(-g.dt, rxx, atxx[c] * keep);
+    atzz[c] = fmaf(-g.dt, rzz, atzz[c] * keep);
+    atxz[c] = fmaf(-g.dt, rxz, atxz[c] * keep);
+    Aout[c] = px;
+    Aout[fs + c] = pz;
+    const float *T = tau + (int64_t)s * tau_shot_stride + g.origin + c;
+    const float dtau_x = dminus<SDO>(T, 1, g.cx) + dminus<SDO>(T + 2 * fs, g.pitch, g.cz);
+    const float dtau_z = dplus<SDO>(T + fs, g.pitch, g.cz) + dplus<SDO>(T + 2 * fs, 1, g.cx);
+    float *G = gacc + (int64_t)s * 6 * fs + g.origin + c;
+    G[4 * fs] += g.dt * (px * dtau_x + pz * dtau_z);
+}
+
+// Residual of sample t/dt_scale added at the receivers after backward step t
+// (adjoint of the gather of S_t).  Velocity components go to the ping-pong
+// buffer `av`, stresses to adj[2..4].
+__global__ void k_exact_inject(DevGrid g, const pyfwi_shot *shots, int t, float *adj,
+                               float *av, const float *__restrict__ res)
+{
+    const int s = blockIdx.x;
+    const pyfwi_shot sh = shots[s];
+    if (t >= sh.nt || (t % sh.dt_scale) != 0) return;
+    const int row = t / sh.dt_scale;
+    if (row >= sh.n_samples) return;
+    const int64_t fs = g.fstride;
+    float *A = adj + (int64_t)s * 5 * fs + g.origin;
+    float *V = av + (int64_t)s * 2 * fs + g.origin;
+    const float *rr = res + sh.rec_offset;
+    const int64_t c0 = (int64_t)sh.rec_z * g.pitch + sh.rec_x0;
+    for (int r = threadIdx.x; r < sh.n_rec; r += blockDim.x) {
+        const int64_t c = c0 + (int64_t)r * sh.rec_dx;
+        V[c] += rr[((int64_t)0 * sh.n_samples + row) * sh.rec_stride + r];
+        V[fs + c] += rr[((int64_t)1 * sh.n_samples + row) * sh.rec_stride + r];
+        for (int f = 2; f < 5; ++f)
+            A[f * fs + c] += rr[((int64_t)f * sh.n_samples + row) * sh.rec_stride + r];
+    }
+}
+
+// ------------------------------------------------------------- utilities
+
+// Copy `count` floats per shot (count % 4 == 0, 16-byte aligned bases).
+__global__ void k_copy_shots(float *dst, int64_t dst_stride, const float *src,
+                             int64_t src_stride, int64_t count)
+{
+    const int s = blockIdx.y;
+    const float4 *a = reinterpret_cast<const float4 *>(src + (int64_t)s * src_stride);
+    float4 *b = reinterpret_cast<float4 *>(dst + (int64_t)s * dst_stride);
+    const int64_t n4 = count / 4;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n4;
+         k += (int64_t)gridDim.x * blockDim.x)
+        b[k] = a[k];
+}
+
+// Sum per-shot accumulators into the unpadded output [nslot][tnz*tnx];
+// exact mode converts dJ/db (slot 4) to dJ/drho = -b^2 dJ/db.
+__global__ void k_grad_sum(DevGrid g, int nshots, int nslot, const float *gacc,
+                           float *out, const float *__restrict__ bu, int b_to_rho)
+{
+    const int64_t n = (int64_t)g.nz * g.nx;
+    const int64_t fs = g.fstride;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n * nslot;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int slot = (int)(k / n);
+        const int64_t cell = k - (int64_t)slot * n;
+        const int i = (int)(cell / g.nx), j = (int)(cell - (int64_t)i * g.nx);
+        const int64_t p = g.origin + (int64_t)i * g.pitch + j;
+        float acc = 0.0f;
+        for (int s = 0; s < nshots; ++s) acc += gacc[((int64_t)s * 6 + slot) * fs + p];
+        if (b_to_rho && slot == 4) acc = -(bu[p] * bu[p]) * acc;
+        out[k] = acc;
+    }
+}
+
+// Scatter a tnz x tnx array into the padded layout (padding untouched).
+__global__ void k_pad(DevGrid g, const float *src, float *dst)
+{
+    const int64_t n = (int64_t)g.nz * g.nx;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(k / g.nx), j = (int)(k - (int64_t)i * g.nx);
+        dst[g.origin + (int64_t)i * g.pitch + j] = src[k];
+    }
+}
+
+// Gather the padded layout back to tnz x tnx, `nf` consecutive fields.
+__global__ void k_unpad(DevGrid g, const float *src, float *dst, int nf)
+{
+    const int64_t n = (int64_t)g.nz * g.nx;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n * nf;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int f = (int)(k / n);
+        const int64_t cell = k - (int64_t)f * n;
+        const int i = (int)(cell / g.nx), j = (int)(cell - (int64_t)i * g.nx);
+        dst[k] = src[(int64_t)f * g.fstride + g.origin + (int64_t)i * g.pitch + j];
+    }
+}
+
+// ------------------------------------------------------- launch wrappers
+
+static dim3 stencil_grid(const DevGrid &g, int sdo, int nshots)
+{
+    const int nx_act = g.nx - 2 * sdo + 1, nz_act = g.nz - 2 * sdo + 1;
+    return dim3((nx_act + TILE_X - 1) / TILE_X, (nz_act + TILE_Y - 1) / TILE_Y, nshots);
+}
+
+#define DISPATCH_SDO(sdo, KERNEL, ...)                                    \
+    do {                                                                  \
+        if ((sdo) == 2) KERNEL<2 __VA_ARGS__><<<grid, block, 0, st>>>(ARGS); \
+        else KERNEL<4 __VA_ARGS__><<<grid, block, 0, st>>>(ARGS);          \
+    } while (0)
+
+void launch_vel(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
+                float *fields, const float *bu, const float *dxp, const float *dzp,
+                hipStream_t st)
+{
+    dim3 grid = stencil_grid(g, g.sdo, nshots), block(TILE_X, TILE_Y);
+#define ARGS g, shots, t, fields, bu, dxp, dzp
+    if (dir > 0) DISPATCH_SDO(g.sdo, k_vel, , 1);
+    else DISPATCH_SDO(g.sdo, k_vel, , -1);
+#undef ARGS
+}
+
+void launch_str(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
+                float *fields, const float *lam, const float *mu, const float *dxp,
+                const float *dzp, hipStream_t st)
+{
+    dim3 grid = stencil_grid(g, g.sdo, nshots), block(TILE_X, TILE_Y);
+#define ARGS g, shots, t, fields, lam, mu, dxp, dzp
+    if (dir > 0) DISPATCH_SDO(g.sdo, k_str, , 1);
+    else DISPATCH_SDO(g.sdo, k_str, , -1);
+#undef ARGS
+}
+
+void launch_inject_record(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
+                          float *fields, const float *src, float *seis, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_inject_record, dim3(nshots), dim3(256), 0, st, g, shots, t,
+                       fields, src, seis);
+}
+
+void launch_adj_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
+                       float *adj, const float *res, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_adj_inject, dim3(nshots), dim3(256), 0, st, g, shots, t, adj, res);
+}
+
+void launch_adj_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
+                    float *adj, const float *bu, const float *dxp, const float *dzp,
+                    hipStream_t st)
+{
+    dim3 grid = stencil_grid(g, g.sdo, nshots), block(TILE_X, TILE_Y);
+#define ARGS g, shots, t, adj, bu, dxp, dzp
+    DISPATCH_SDO(g.sdo, k_adj_str, );
+#undef ARGS
+}
+
+void launch_adj_vel_grad(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
+                         const float *fields, float *adj, const float *lam,
+                         const float *mu, const float *bu, const float *dxp,
+                         const float *dzp, float *gacc, bool precond, hipStream_t st)
+{
+    dim3 grid = stencil_grid(g, g.sdo, nshots), block(TILE_X, TILE_Y);
+#define ARGS g, shots, t, fields, adj, lam, mu, bu, dxp, dzp, gacc
+    if (precond) DISPATCH_SDO(g.sdo, k_adj_vel_grad, , true);
+    else DISPATCH_SDO(g.sdo, k_adj_vel_grad, , false);
+#undef ARGS
+}
+
+void launch_exact_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
+                      const float *vel, int64_t vel_stride, float *adj, float *av,
+                      const float *lam, const float *mu, const float *dxp,
+                      const float *dzp, float *gacc, hipStream_t st)
+{
+    dim3 grid = stencil_grid(g, g.sdo, nshots), block(TILE_X, TILE_Y);
+#define ARGS g, shots, t, vel, vel_stride, adj, av, lam, mu, dxp, dzp, gacc
+    DISPATCH_SDO(g.sdo, k_exact_str, );
+#undef ARGS
+}
+
+void launch_exact_vel(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
+                      const float *tau, int64_t tau_stride, float *adj, const float *av_in,
+                      float *av_out, const float *bu, const float *dxp, const float *dzp,
+                      float *gacc, hipStream_t st)
+{
+    dim3 grid = stencil_grid(g, g.sdo, nshots), block(TILE_X, TILE_Y);
+#define ARGS g, shots, t, tau, tau_stride, adj, av_in, av_out, bu, dxp, dzp, gacc
+    DISPATCH_SDO(g.sdo, k_exact_vel, );
+#undef ARGS
+}
+
+void launch_exact_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
+                         float *adj, float *av, const float *res, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_exact_inject, dim3(nshots), dim3(256), 0, st, g, shots, t, adj,
+                       av, res);
+}
+
+void launch_copy_shots(float *dst, int64_t dst_stride, const float *src, int64_t src_stride,
+                       int64_t count, int nshots, hipStream_t st)
+{
+    int64_t n4 = count / 4;
+    int blocks = (int)((n4 + 255) / 256);
+    if (blocks > 2048) blocks = 2048;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_copy_shots, dim3(blocks, nshots), dim3(256), 0, st, dst, dst_stride,
+                       src, src_stride, count);
+}
+
+void launch_grad_sum(const DevGrid &g, int nshots, int nslot, const float *gacc, float *out,
+                     const float *bu, int b_to_rho, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_grad_sum, dim3(1024), dim3(256), 0, st, g, nshots, nslot, gacc, out,
+                       bu, b_to_rho);
+}
+
+void launch_pad(const DevGrid &g, const float *src, float *dst, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_pad, dim3(512), dim3(256), 0, st, g, src, dst);
+}
+
+void launch_unpad(const DevGrid &g, const float *src, float *dst, int nf, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_unpad, dim3(512), dim3(256), 0, st, g, src, dst, nf);
+}
+
+} // namespace pyfwi

@@ -1,0 +1,683 @@
+// Plan objects and the extern "C" entry points declared in include/pyfwi_hip.h.
+//
+// A plan owns, on one GPU:
+//   model   lam, mu, buoyancy in the padded layout; damping profiles (1-D)
+//   fields  [nshots][5][fstride]  forward state of every shot of the batch
+//   adj     [nshots][5][fstride]  adjoint state
+//   av      2 x [nshots][2][fstride]  exact mode velocity adjoint ping-pong
+//   gacc    [nshots][6][fstride]  per-shot gradient accumulators
+//   snaps   [nshots][nchp][5][fstride]  reference checkpoints (pyfwi mode)
+//   ckpts   [nshots][ncp][5][fstride]   exact-mode checkpoints every K steps
+//   seg     [nshots][K][5][fstride]     exact-mode recomputed segment
+// Batched shots are the z dimension of every stencil launch, so one launch
+// updates all shots of a step (the reference runs shots one after another,
+// src/PyFWI/wave_propagation.py:639-645, 740-749).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pyfwi_hip.h"
+#include "pyfwi_kernels.h"
+
+using namespace pyfwi;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                 \
+    do {                                                                             \
+        hipError_t e_ = (expr);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            return fail(PYFWI_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct DevBuf {
+    float *p = nullptr;
+    size_t n = 0;  // floats
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+// Grow-only device buffer; zeroed on (re)allocation.
+int ensure(DevBuf &b, size_t n, hipStream_t st, const char *what)
+{
+    if (b.n >= n && b.p) return 0;
+    b.release();
+    if (n == 0) return 0;
+    hipError_t e = hipMalloc(&b.p, n * sizeof(float));
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(PYFWI_ENOMEM, std::string("hipMalloc failed for ") + what + " (" +
+                                      std::to_string(n * sizeof(float)) + " bytes)");
+    }
+    b.n = n;
+    HIPCHK(hipMemsetAsync(b.p, 0, n * sizeof(float), st));
+    return 0;
+}
+
+} // namespace
+
+struct pyfwi_plan {
+    pyfwi_grid grid{};
+    DevGrid dg{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    bool model_set = false;
+    DevBuf lam, mu, bu, dxp_buf, dzp_buf, scratch_in;
+    // damping profiles with HALO zero entries on both sides: dxp[-HALO..nx+HALO)
+    float *dxp = nullptr, *dzp = nullptr;
+    DevBuf fields, adj, av0, av1, gacc, snaps, ckpts, seg;
+    DevBuf src_stage, seis_stage, res_stage, grad_stage, shot_tab_f;
+    pyfwi_shot *d_shots = nullptr;
+    int shots_cap = 0;
+    // state of the last forward
+    std::vector<pyfwi_shot> last_shots;
+    std::vector<int32_t> chp;
+    int ckpt_every = 0;
+    int ncp = 0;
+    bool have_forward = false;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<int> ev_kind;
+    size_t ev_used = 0;
+    double kt_ms[PYFWI_K_COUNT] = {0};
+    int64_t kt_n[PYFWI_K_COUNT] = {0};
+};
+
+namespace {
+
+int64_t fields_per_shot(const pyfwi_plan *p) { return 5 * p->dg.fstride; }
+
+int max_nt(const std::vector<pyfwi_shot> &s)
+{
+    int m = 0;
+    for (auto &x : s) m = std::max(m, x.nt);
+    return m;
+}
+
+// Bracket a launch with an event pair when timing is on.
+struct Timed {
+    pyfwi_plan *p;
+    int kind;
+    size_t slot = 0;
+    bool on;
+    Timed(pyfwi_plan *p_, int k) : p(p_), kind(k), on(p_->timing)
+    {
+        if (!on) return;
+        if (p->ev_used + 2 > p->ev_pool.size()) {
+            for (int n = 0; n < 256; ++n) {
+                hipEvent_t e;
+                if (hipEventCreate(&e) != hipSuccess) { on = false; return; }
+                p->ev_pool.push_back(e);
+                p->ev_kind.push_back(-1);
+            }
+        }
+        slot = p->ev_used;
+        p->ev_used += 2;
+        p->ev_kind[slot] = kind;
+        (void)hipEventRecord(p->ev_pool[slot], p->stream);
+    }
+    ~Timed()
+    {
+        if (on) (void)hipEventRecord(p->ev_pool[slot + 1], p->stream);
+    }
+};
+
+// Fold the recorded event pairs into the per-kind totals.
+int harvest_timing(pyfwi_plan *p)
+{
+    if (!p->timing || p->ev_used == 0) return 0;
+    HIPCHK(hipStreamSynchronize(p->stream));
+    for (size_t k = 0; k + 1 < p->ev_used; k += 2) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, p->ev_pool[k], p->ev_pool[k + 1]));
+        int kind = p->ev_kind[k];
+        if (kind >= 0 && kind < PYFWI_K_COUNT) {
+            p->kt_ms[kind] += ms;
+            p->kt_n[kind] += 1;
+        }
+    }
+    p->ev_used = 0;
+    return 0;
+}
+
+// Flush the event pool when it grows large (long time loops).
+int maybe_harvest(pyfwi_plan *p)
+{
+    if (p->timing && p->ev_used >= 8192) return harvest_timing(p);
+    return 0;
+}
+
+int check_shots(const pyfwi_plan *p, const pyfwi_shot *shots, int nshots)
+{
+    const pyfwi_grid &g = p->grid;
+    if (!shots || nshots <= 0) return fail(PYFWI_EINVAL, "no shots");
+    for (int s = 0; s < nshots; ++s) {
+        const pyfwi_shot &h = shots[s];
+        std::string w = "shot " + std::to_string(s) + ": ";
+        if (h.nt < 1) return fail(PYFWI_EINVAL, w + "nt must be >= 1");
+        if (h.dt_scale < 1) return fail(PYFWI_EINVAL, w + "dt_scale must be >= 1");
+        if (h.n_samples < 1) return fail(PYFWI_EINVAL, w + "n_samples must be >= 1");
+        if (h.src_x < 0 || h.src_x >= g.tnx || h.src_z < 0 || h.src_z >= g.tnz)
+            return fail(PYFWI_EINVAL, w + "source cell outside the grid");
+        if (h.n_rec < 0 || h.rec_stride < h.n_rec)
+            return fail(PYFWI_EINVAL, w + "need 0 <= n_rec <= rec_stride");
+        if (h.n_rec > 0) {
+            if (h.rec_dx < 1) return fail(PYFWI_EINVAL, w + "receiver spacing must be >= 1 cell");
+            int64_t last = (int64_t)h.rec_x0 + (int64_t)(h.n_rec - 1) * h.rec_dx;
+            if (h.rec_z < 0 || h.rec_z >= g.tnz || h.rec_x0 < 0 || last >= g.tnx)
+                return fail(PYFWI_EINVAL, w + "receiver line leaves the grid");
+        }
+        if (h.src_offset < 0 || h.rec_offset < 0) return fail(PYFWI_EINVAL, w + "negative offset");
+    }
+    return 0;
+}
+
+size_t src_extent(const pyfwi_shot *shots, int nshots)
+{
+    size_t m = 0;
+    for (int s = 0; s < nshots; ++s) {
+        size_t ninj = (size_t)((shots[s].nt + shots[s].dt_scale - 1) / shots[s].dt_scale);
+        m = std::max(m, (size_t)shots[s].src_offset + ninj * 5);
+    }
+    return m;
+}
+
+size_t rec_extent(const pyfwi_shot *shots, int nshots)
+{
+    size_t m = 0;
+    for (int s = 0; s < nshots; ++s)
+        m = std::max(m, (size_t)shots[s].rec_offset +
+                            (size_t)5 * shots[s].n_samples * shots[s].rec_stride);
+    return m;
+}
+
+int upload_shots(pyfwi_plan *p, const pyfwi_shot *shots, int nshots)
+{
+    if (nshots > p->shots_cap) {
+        if (p->d_shots) (void)hipFree(p->d_shots);
+        p->d_shots = nullptr;
+        HIPCHK(hipMalloc(&p->d_shots, sizeof(pyfwi_shot) * nshots));
+        p->shots_cap = nshots;
+    }
+    HIPCHK(hipMemcpyAsync(p->d_shots, shots, sizeof(pyfwi_shot) * nshots,
+                          hipMemcpyHostToDevice, p->stream));
+    // the caller's table may be freed as soon as the entry point returns
+    HIPCHK(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+// Device pointer for a caller array: the pointer itself for device memory,
+// otherwise a staged copy (direction given by `in`).
+int stage_in(pyfwi_plan *p, DevBuf &stage, const float *ptr, size_t n, int mem,
+             const float **out, const char *what)
+{
+    if (mem == PYFWI_MEM_DEVICE) {
+        *out = ptr;
+        return 0;
+    }
+    if (int e = ensure(stage, n, p->stream, what)) return e;
+    HIPCHK(hipMemcpyAsync(stage.p, ptr, n * sizeof(float), hipMemcpyHostToDevice, p->stream));
+    *out = stage.p;
+    return 0;
+}
+
+} // namespace
+
+// =================================================================== C ABI
+
+extern "C" {
+
+int32_t pyfwi_version(void) { return 100; }
+
+const char *pyfwi_last_error(void) { return g_err.c_str(); }
+
+int32_t pyfwi_plan_create(const pyfwi_grid *grid, void *stream, pyfwi_plan **out)
+{
+    if (!grid || !out) return fail(PYFWI_EINVAL, "null argument");
+    *out = nullptr;
+    const pyfwi_grid &g = *grid;
+    if (g.sdo != 2 && g.sdo != 4) return fail(PYFWI_EINVAL, "sdo must be 2 (order 4) or 4 (order 8)");
+    if (g.tnz < 2 * g.sdo + 1 || g.tnx < 2 * g.sdo + 1)
+        return fail(PYFWI_EINVAL, "grid smaller than the stencil");
+    if (g.npml < 0) return fail(PYFWI_EINVAL, "npml must be >= 0");
+    if (!(g.dt > 0) || !(g.dx > 0) || !(g.dz > 0) || !std::isfinite(g.dt))
+        return fail(PYFWI_EINVAL, "dt, dx, dz must be positive");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    pyfwi_plan *p = new pyfwi_plan();
+    p->grid = g;
+    p->device = dev;
+    DevGrid &d = p->dg;
+    d.nz = g.tnz;
+    d.nx = g.tnx;
+    d.pitch = ((g.tnx + HALO + 63) / 64) * 64;
+    d.sdo = g.sdo;
+    d.npml = g.npml;
+    d.origin = (int64_t)HALO * d.pitch;
+    d.fstride = (int64_t)(g.tnz + 2 * HALO) * d.pitch + 64;  // tail for the last left halo
+    d.fstride = ((d.fstride + 63) / 64) * 64;
+    for (int k = 0; k < 4; ++k) {
+        d.cx[k] = (float)(g.c[k] / g.dx);
+        d.cz[k] = (float)(g.c[k] / g.dz);
+    }
+    d.dt = (float)g.dt;
+    if (stream) {
+        p->stream = (hipStream_t)stream;
+    } else {
+        hipError_t e = hipStreamCreate(&p->own_stream);
+        if (e != hipSuccess) {
+            delete p;
+            return fail(PYFWI_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        }
+        p->stream = p->own_stream;
+    }
+    *out = p;
+    return 0;
+}
+
+int32_t pyfwi_plan_destroy(pyfwi_plan *p)
+{
+    if (!p) return 0;
+    (void)hipStreamSynchronize(p->stream);
+    for (DevBuf *b : {&p->lam, &p->mu, &p->bu, &p->dxp_buf, &p->dzp_buf, &p->scratch_in, &p->fields,
+                      &p->adj, &p->av0, &p->av1, &p->gacc, &p->snaps, &p->ckpts, &p->seg,
+                      &p->src_stage, &p->seis_stage, &p->res_stage, &p->grad_stage,
+                      &p->shot_tab_f})
+        b->release();
+    if (p->d_shots) (void)hipFree(p->d_shots);
+    for (auto e : p->ev_pool) (void)hipEventDestroy(e);
+    if (p->own_stream) (void)hipStreamDestroy(p->own_stream);
+    delete p;
+    return 0;
+}
+
+int32_t pyfwi_plan_set_stream(pyfwi_plan *p, void *stream)
+{
+    if (!p) return fail(PYFWI_EINVAL, "null plan");
+    HIPCHK(hipStreamSynchronize(p->stream));
+    p->stream = stream ? (hipStream_t)stream : p->own_stream;
+    if (!p->stream) {
+        HIPCHK(hipStreamCreate(&p->own_stream));
+        p->stream = p->own_stream;
+    }
+    return 0;
+}
+
+int32_t pyfwi_set_model(pyfwi_plan *p, const float *lam, const float *mu, const float *buoy,
+                        const float *dx_prof, const float *dz_prof, int32_t mem)
+{
+    if (!p || !lam || !mu || !buoy || !dx_prof || !dz_prof) return fail(PYFWI_EINVAL, "null argument");
+    const DevGrid &g = p->dg;
+    const size_t n = (size_t)g.nz * g.nx;
+    if (mem == PYFWI_MEM_HOST) {
+        for (const float *a : {lam, mu, buoy})
+            for (size_t k = 0; k < n; ++k)
+                if (!std::isfinite(a[k])) return fail(PYFWI_ENONFINITE, "model holds NaN or Inf");
+        for (int k = 0; k < g.nx; ++k)
+            if (!std::isfinite(dx_prof[k])) return fail(PYFWI_ENONFINITE, "damping holds NaN or Inf");
+        for (int k = 0; k < g.nz; ++k)
+            if (!std::isfinite(dz_prof[k])) return fail(PYFWI_ENONFINITE, "damping holds NaN or Inf");
+    }
+    for (DevBuf *b : {&p->lam, &p->mu, &p->bu})
+        if (int e = ensure(*b, g.fstride, p->stream, "model")) return e;
+    if (int e = ensure(p->dxp_buf, g.nx + 2 * HALO, p->stream, "damping")) return e;
+    if (int e = ensure(p->dzp_buf, g.nz + 2 * HALO, p->stream, "damping")) return e;
+    p->dxp = p->dxp_buf.p + HALO;
+    p->dzp = p->dzp_buf.p + HALO;
+    const hipMemcpyKind kind = mem == PYFWI_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (mem == PYFWI_MEM_DEVICE) {
+        launch_pad(g, lam, p->lam.p, p->stream);
+        launch_pad(g, mu, p->mu.p, p->stream);
+        launch_pad(g, buoy, p->bu.p, p->stream);
+    } else {
+        if (int e = ensure(p->scratch_in, n, p->stream, "staging")) return e;
+        const float *srcs[3] = {lam, mu, buoy};
+        DevBuf *dsts[3] = {&p->lam, &p->mu, &p->bu};
+        for (int k = 0; k < 3; ++k) {
+            HIPCHK(hipMemcpyAsync(p->scratch_in.p, srcs[k], n * sizeof(float), kind, p->stream));
+            launch_pad(g, p->scratch_in.p, dsts[k]->p, p->stream);
+        }
+    }
+    HIPCHK(hipMemcpyAsync(p->dxp, dx_prof, g.nx * sizeof(float), kind, p->stream));
+    HIPCHK(hipMemcpyAsync(p->dzp, dz_prof, g.nz * sizeof(float), kind, p->stream));
+    HIPCHK(hipGetLastError());
+    if (mem == PYFWI_MEM_HOST) HIPCHK(hipStreamSynchronize(p->stream));  // host arrays may go away
+    p->model_set = true;
+    p->have_forward = false;
+    return 0;
+}
+
+namespace {
+
+// One forward step of every shot of the batch (gather + inject, velocity,
+// stress), the reference's loop body at wave_propagation.py:704-708.
+int forward_step(pyfwi_plan *p, int nshots, int t, const float *src, float *seis)
+{
+    const DevGrid &g = p->dg;
+    launch_inject_record(g, nshots, p->d_shots, t, p->fields.p, src, seis, p->stream);
+    {
+        Timed tm(p, PYFWI_K_FWD_VEL);
+        launch_vel(g, +1, nshots, p->d_shots, t, p->fields.p, p->bu.p, p->dxp, p->dzp, p->stream);
+    }
+    {
+        Timed tm(p, PYFWI_K_FWD_STR);
+        launch_str(g, +1, nshots, p->d_shots, t, p->fields.p, p->lam.p, p->mu.p, p->dxp,
+                   p->dzp, p->stream);
+    }
+    return 0;
+}
+
+} // namespace
+
+int32_t pyfwi_forward(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, const float *src,
+                      float *seis, const int32_t *chp, int32_t nchp, int32_t ckpt_every,
+                      int32_t mem)
+{
+    if (!p || !src || !seis) return fail(PYFWI_EINVAL, "null argument");
+    if (!p->model_set) return fail(PYFWI_ESTATE, "pyfwi_set_model must come first");
+    if (int e = check_shots(p, shots, nshots)) return e;
+    if (nchp < 0 || (nchp > 0 && !chp)) return fail(PYFWI_EINVAL, "bad checkpoint list");
+    if (ckpt_every < 0) return fail(PYFWI_EINVAL, "ckpt_every must be >= 0");
+    std::vector<pyfwi_shot> sh(shots, shots + nshots);
+    const int nt = max_nt(sh);
+    if (nchp > 0)
+        for (auto &x : sh)
+            if (x.nt != nt) return fail(PYFWI_EINVAL, "checkpoint lists need equal nt for all shots");
+    const DevGrid &g = p->dg;
+    const int64_t fps = fields_per_shot(p);
+    p->have_forward = false;
+    if (int e = ensure(p->fields, (size_t)nshots * fps, p->stream, "fields")) return e;
+    HIPCHK(hipMemsetAsync(p->fields.p, 0, (size_t)nshots * fps * sizeof(float), p->stream));
+    if (nchp > 0)
+        if (int e = ensure(p->snaps, (size_t)nshots * nchp * fps, p->stream, "checkpoints")) return e;
+    int ncp = 0;
+    if (ckpt_every > 0) {
+        ncp = (nt + ckpt_every - 1) / ckpt_every;
+        if (int e = ensure(p->ckpts, (size_t)nshots * ncp * fps, p->stream, "checkpoints")) return e;
+    }
+    if (int e = upload_shots(p, shots, nshots)) return e;
+    const size_t nsrc = src_extent(shots, nshots), nrec = rec_extent(shots, nshots);
+    const float *d_src = nullptr;
+    if (int e = stage_in(p, p->src_stage, src, nsrc, mem, &d_src, "source table")) return e;
+    float *d_seis = seis;
+    if (mem != PYFWI_MEM_DEVICE) {
+        if (int e = ensure(p->seis_stage, nrec, p->stream, "records")) return e;
+        d_seis = p->seis_stage.p;
+    }
+    HIPCHK(hipMemsetAsync(d_seis, 0, nrec * sizeof(float), p->stream));
+    std::vector<int32_t> cl(chp, chp + std::max(nchp, 0));
+    int chpc = 0;
+    for (int t = 0; t < nt; ++t) {
+        if (ncp > 0 && t % ckpt_every == 0)
+            launch_copy_shots(p->ckpts.p + (int64_t)(t / ckpt_every) * fps, (int64_t)ncp * fps,
+                              p->fields.p, fps, fps, nshots, p->stream);
+        forward_step(p, nshots, t, d_src, d_seis);
+        if (nchp > 0 && std::find(cl.begin(), cl.end(), t) != cl.end()) {
+            if (chpc < nchp)
+                launch_copy_shots(p->snaps.p + (int64_t)chpc * fps, (int64_t)nchp * fps,
+                                  p->fields.p, fps, fps, nshots, p->stream);
+            chpc++;
+        }
+        if (int e = maybe_harvest(p)) return e;
+    }
+    HIPCHK(hipGetLastError());
+    if (mem != PYFWI_MEM_DEVICE) {
+        HIPCHK(hipMemcpyAsync(seis, d_seis, nrec * sizeof(float), hipMemcpyDeviceToHost, p->stream));
+        HIPCHK(hipStreamSynchronize(p->stream));
+    }
+    if (int e = harvest_timing(p)) return e;
+    p->last_shots = sh;
+    p->chp = cl;
+    p->ckpt_every = ckpt_every;
+    p->ncp = ncp;
+    p->have_forward = true;
+    return 0;
+}
+
+namespace {
+
+int gradient_pyfwi(pyfwi_plan *p, int nshots, const float *d_res, bool precond)
+{
+    const DevGrid &g = p->dg;
+    (void)g;
+    const int64_t fps = fields_per_shot(p);
+    const int nt = max_nt(p->last_shots);
+    const int nchp = (int)p->chp.size();
+    if (nchp == 0) return fail(PYFWI_ESTATE, "pyfwi-mode gradient needs a forward with a checkpoint list");
+    if (int e = ensure(p->adj, (size_t)nshots * fps, p->stream, "adjoint fields")) return e;
+    HIPCHK(hipMemsetAsync(p->adj.p, 0, (size_t)nshots * fps * sizeof(float), p->stream));
+    HIPCHK(hipMemsetAsync(p->fields.p, 0, (size_t)nshots * fps * sizeof(float), p->stream));
+    int chpc = nchp - 1;
+    for (int t = nt - 1; t > 0; --t) {
+        if (chpc >= 0 && t == p->chp[chpc]) {
+            launch_copy_shots(p->fields.p, fps, p->snaps.p + (int64_t)chpc * fps,
+                              (int64_t)nchp * fps, fps, nshots, p->stream);
+            chpc--;
+        } else {
+            {
+                Timed tm(p, PYFWI_K_REV_STR);
+                launch_str(g, -1, nshots, p->d_shots, t, p->fields.p, p->lam.p, p->mu.p,
+                           p->dxp, p->dzp, p->stream);
+            }
+            {
+                Timed tm(p, PYFWI_K_REV_VEL);
+                launch_vel(g, -1, nshots, p->d_shots, t, p->fields.p, p->bu.p, p->dxp,
+                           p->dzp, p->stream);
+            }
+        }
+        launch_adj_inject(g, nshots, p->d_shots, t, p->adj.p, d_res, p->stream);
+        {
+            Timed tm(p, PYFWI_K_ADJ_STR);
+            launch_adj_str(g, nshots, p->d_shots, t, p->adj.p, p->bu.p, p->dxp, p->dzp,
+                           p->stream);
+        }
+        {
+            Timed tm(p, PYFWI_K_ADJ_VEL);
+            launch_adj_vel_grad(g, nshots, p->d_shots, t, p->fields.p, p->adj.p, p->lam.p,
+                                p->mu.p, p->bu.p, p->dxp, p->dzp, p->gacc.p, precond,
+                                p->stream);
+        }
+        if (int e = maybe_harvest(p)) return e;
+    }
+    return 0;
+}
+
+// Exact adjoint with forward recomputation from checkpoints every K steps.
+int gradient_exact(pyfwi_plan *p, int nshots, const float *d_src, const float *d_res, int K)
+{
+    const DevGrid &g = p->dg;
+    const int64_t fps = fields_per_shot(p);
+    const int nt = max_nt(p->last_shots);
+    const int ncp = p->ncp;
+    if (int e = ensure(p->adj, (size_t)nshots * fps, p->stream, "adjoint fields")) return e;
+    if (int e = ensure(p->av0, (size_t)nshots * 2 * g.fstride, p->stream, "adjoint velocity")) return e;
+    if (int e = ensure(p->av1, (size_t)nshots * 2 * g.fstride, p->stream, "adjoint velocity")) return e;
+    if (int e = ensure(p->seg, (size_t)nshots * K * fps, p->stream, "segment")) return e;
+    HIPCHK(hipMemsetAsync(p->adj.p, 0, (size_t)nshots * fps * sizeof(float), p->stream));
+    HIPCHK(hipMemsetAsync(p->av0.p, 0, (size_t)nshots * 2 * g.fstride * sizeof(float), p->stream));
+    HIPCHK(hipMemsetAsync(p->av1.p, 0, (size_t)nshots * 2 * g.fstride * sizeof(float), p->stream));
+    float *av_cur = p->av0.p, *av_nxt = p->av1.p;
+    const int64_t seg_stride = (int64_t)K * fps;  // per shot
+    for (int c = ncp - 1; c >= 0; --c) {
+        const int t0 = c * K, t1 = std::min(nt, t0 + K);
+        // recompute the segment: after the velocity update of step t the field
+        // buffers hold V_t (velocities) and the stresses that step t updates
+        launch_copy_shots(p->fields.p, fps, p->ckpts.p + (int64_t)c * fps, (int64_t)ncp * fps,
+                          fps, nshots, p->stream);
+        for (int t = t0; t < t1; ++t) {
+            launch_inject_record(g, nshots, p->d_shots, t, p->fields.p, d_src, p->seis_stage.p,
+                                 p->stream);
+            {
+                Timed tm(p, PYFWI_K_FWD_VEL);
+                launch_vel(g, +1, nshots, p->d_shots, t, p->fields.p, p->bu.p, p->dxp,
+                           p->dzp, p->stream);
+            }
+            launch_copy_shots(p->seg.p + (int64_t)(t - t0) * fps, seg_stride, p->fields.p, fps,
+                              fps, nshots, p->stream);
+            if (t + 1 < t1) {
+                Timed tm(p, PYFWI_K_FWD_STR);
+                launch_str(g, +1, nshots, p->d_shots, t, p->fields.p, p->lam.p, p->mu.p,
+                           p->dxp, p->dzp, p->stream);
+            }
+        }
+        for (int t = t1 - 1; t >= t0; --t) {
+            const float *slot = p->seg.p + (int64_t)(t - t0) * fps;
+            {
+                Timed tm(p, PYFWI_K_EXA_STR);
+                launch_exact_str(g, nshots, p->d_shots, t, slot, seg_stride, p->adj.p, av_cur,
+                                 p->lam.p, p->mu.p, p->dxp, p->dzp, p->gacc.p, p->stream);
+            }
+            {
+                Timed tm(p, PYFWI_K_EXA_VEL);
+                launch_exact_vel(g, nshots, p->d_shots, t, slot + 2 * g.fstride, seg_stride,
+                                 p->adj.p, av_cur, av_nxt, p->bu.p, p->dxp, p->dzp, p->gacc.p,
+                                 p->stream);
+            }
+            std::swap(av_cur, av_nxt);
+            launch_exact_inject(g, nshots, p->d_shots, t, p->adj.p, av_cur, d_res, p->stream);
+            if (int e = maybe_harvest(p)) return e;
+        }
+    }
+    return 0;
+}
+
+} // namespace
+
+int32_t pyfwi_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, const float *src,
+                       const float *res, float *grad, int32_t mode, int32_t ckpt_every,
+                       int32_t flags, int32_t mem)
+{
+    if (!p || !res || !grad) return fail(PYFWI_EINVAL, "null argument");
+    if (!p->model_set) return fail(PYFWI_ESTATE, "pyfwi_set_model must come first");
+    if (mode != PYFWI_MODE_PYFWI && mode != PYFWI_MODE_EXACT) return fail(PYFWI_EINVAL, "unknown mode");
+    if (int e = check_shots(p, shots, nshots)) return e;
+    if (ckpt_every < 0) return fail(PYFWI_EINVAL, "ckpt_every must be >= 0");
+    const DevGrid &g = p->dg;
+    const int64_t fps = fields_per_shot(p);
+    const size_t nrec = rec_extent(shots, nshots);
+    bool same = p->have_forward && (int)p->last_shots.size() == nshots &&
+                std::memcmp(p->last_shots.data(), shots, sizeof(pyfwi_shot) * nshots) == 0;
+    if (mode == PYFWI_MODE_PYFWI && !same)
+        return fail(PYFWI_ESTATE, "pyfwi-mode gradient must follow a forward of the same shots");
+    const float *d_res = nullptr;
+    if (int e = stage_in(p, p->res_stage, res, nrec, mem, &d_res, "residuals")) return e;
+    if (int e = ensure(p->gacc, (size_t)nshots * 6 * g.fstride, p->stream, "gradient")) return e;
+    HIPCHK(hipMemsetAsync(p->gacc.p, 0, (size_t)nshots * 6 * g.fstride * sizeof(float), p->stream));
+    if (mode == PYFWI_MODE_PYFWI) {
+        if (int e = upload_shots(p, shots, nshots)) return e;
+        if (int e = gradient_pyfwi(p, nshots, d_res, (flags & PYFWI_F_PRECOND) != 0)) return e;
+    } else {
+        if (!src) return fail(PYFWI_EINVAL, "exact mode needs the source table");
+        int K = ckpt_every > 0 ? ckpt_every : (p->ckpt_every > 0 ? p->ckpt_every : 50);
+        if (!same || p->ckpt_every != K) {
+            // run the forward again, keeping checkpoints every K steps
+            const float *d_src = nullptr;
+            if (int e = stage_in(p, p->src_stage, src, src_extent(shots, nshots), mem, &d_src,
+                                 "source table"))
+                return e;
+            if (int e = ensure(p->seis_stage, nrec, p->stream, "records")) return e;
+            std::vector<float> none;
+            (void)none;
+            // pyfwi_forward stages host arrays itself; pass device pointers here
+            int32_t e = pyfwi_forward(p, shots, nshots, d_src, p->seis_stage.p, nullptr, 0, K,
+                                      PYFWI_MEM_DEVICE);
+            if (e) return e;
+        }
+        const float *d_src = nullptr;
+        if (int e = stage_in(p, p->src_stage, src, src_extent(shots, nshots), mem, &d_src,
+                             "source table"))
+            return e;
+        if (int e = ensure(p->seis_stage, nrec, p->stream, "records")) return e;
+        if (int e = upload_shots(p, shots, nshots)) return e;
+        if (int e = gradient_exact(p, nshots, d_src, d_res, K)) return e;
+    }
+    float *d_grad = grad;
+    const size_t n = (size_t)g.nz * g.nx;
+    if (mem != PYFWI_MEM_DEVICE) {
+        if (int e = ensure(p->grad_stage, 6 * n, p->stream, "gradient output")) return e;
+        d_grad = p->grad_stage.p;
+    }
+    launch_grad_sum(g, nshots, 6, p->gacc.p, d_grad, p->bu.p, mode == PYFWI_MODE_EXACT, p->stream);
+    HIPCHK(hipGetLastError());
+    if (mem != PYFWI_MEM_DEVICE) {
+        HIPCHK(hipMemcpyAsync(grad, d_grad, 6 * n * sizeof(float), hipMemcpyDeviceToHost, p->stream));
+        HIPCHK(hipStreamSynchronize(p->stream));
+    }
+    if (int e = harvest_timing(p)) return e;
+    return 0;
+}
+
+int32_t pyfwi_get_snapshot(pyfwi_plan *p, int32_t shot, int32_t slot, float *out, int32_t mem)
+{
+    if (!p || !out) return fail(PYFWI_EINVAL, "null argument");
+    if (!p->have_forward) return fail(PYFWI_ESTATE, "no forward run yet");
+    const int nchp = (int)p->chp.size();
+    if (shot < 0 || shot >= (int)p->last_shots.size() || slot < 0 || slot >= nchp)
+        return fail(PYFWI_EINVAL, "snapshot index out of range");
+    const DevGrid &g = p->dg;
+    const int64_t fps = fields_per_shot(p);
+    const size_t n = (size_t)g.nz * g.nx;
+    float *d_out = out;
+    if (mem != PYFWI_MEM_DEVICE) {
+        if (int e = ensure(p->grad_stage, 6 * n, p->stream, "snapshot staging")) return e;
+        d_out = p->grad_stage.p;
+    }
+    launch_unpad(g, p->snaps.p + ((int64_t)shot * nchp + slot) * fps, d_out, 5, p->stream);
+    HIPCHK(hipGetLastError());
+    if (mem != PYFWI_MEM_DEVICE) {
+        HIPCHK(hipMemcpyAsync(out, d_out, 5 * n * sizeof(float), hipMemcpyDeviceToHost, p->stream));
+        HIPCHK(hipStreamSynchronize(p->stream));
+    }
+    return 0;
+}
+
+int32_t pyfwi_synchronize(pyfwi_plan *p)
+{
+    if (!p) return fail(PYFWI_EINVAL, "null plan");
+    HIPCHK(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+int32_t pyfwi_plan_set_timing(pyfwi_plan *p, int32_t enable)
+{
+    if (!p) return fail(PYFWI_EINVAL, "null plan");
+    if (int e = harvest_timing(p)) return e;
+    p->timing = enable != 0;
+    for (int k = 0; k < PYFWI_K_COUNT; ++k) {
+        p->kt_ms[k] = 0;
+        p->kt_n[k] = 0;
+    }
+    return 0;
+}
+
+int32_t pyfwi_kernel_timing(pyfwi_plan *p, int32_t kind, double *total_ms, int64_t *launches)
+{
+    if (!p || !total_ms || !launches) return fail(PYFWI_EINVAL, "null argument");
+    if (kind < 0 || kind >= PYFWI_K_COUNT) return fail(PYFWI_EINVAL, "unknown kernel kind");
+    if (!p->timing) return fail(PYFWI_ESTATE, "timing is not enabled");
+    if (int e = harvest_timing(p)) return e;
+    *total_ms = p->kt_ms[kind];
+    *launches = p->kt_n[kind];
+    return 0;
+}
+
+} // extern "C"

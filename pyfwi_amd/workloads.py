@@ -1,0 +1,173 @@
+"""Seeded synthetic workloads for tests and benchmarks.
+
+Each builder returns a ``Workload`` holding the ``inpa`` dictionary, a
+``Source``, receiver positions and models in the layout PyFWI's
+``WavePropagator`` takes (src/PyFWI/wave_propagation.py:617-623).  Models are
+random-layered with Gardner density ``rho = 310 vp^0.25``; all randomness comes
+from ``numpy.random.default_rng(seed)``.  The absorbing layer is 20 cells with
+``pmlR = 1e-5`` on all four sides (``pml_dir = 2``) and the stencil is of
+order 4, as in PyFWI's examples (example/fwi_example_surface.py:24-32).
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .acquisition import Source
+
+__all__ = ["Workload", "layered3", "random_layered", "gaussian_anomaly",
+           "plain_small", "plain_medium", "differentiation", "large_step",
+           "t_for_nt", "gardner"]
+
+
+def gardner(vp):
+    return (310.0 * np.asarray(vp, np.float64) ** 0.25).astype(np.float32)
+
+
+def t_for_nt(nt, dt):
+    """Record length giving exactly ``nt`` samples under
+    ``nt = int(1 + t // dt)`` (src/PyFWI/wave_propagation.py:34-35)."""
+    return (nt - 1) * dt + dt / 2
+
+
+@dataclass
+class Workload:
+    name: str
+    inpa: dict
+    src: Source
+    rec_loc: np.ndarray
+    model: dict
+    model_obs: dict = None
+    components: int = 0
+    chpr: int = 0
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def model_shape(self):
+        return self.model["vp"].shape
+
+    @property
+    def cells(self):
+        npml = self.inpa.get("npml", 0)
+        nz, nx = self.model_shape
+        return (nz + 2 * npml) * (nx + 2 * npml)
+
+
+def _base_inpa(dh, dt, nt, npml=20):
+    return {"dh": dh, "dt": dt, "t": t_for_nt(nt, dt), "npml": npml, "pmlR": 1e-5,
+            "pml_dir": 2, "sdo": 4, "acq_type": 1}
+
+
+def layered3(nz=100, nx=100):
+    """Three equal horizontal layers, vp 1500/2000/2500, vs = vp/1.73."""
+    vp = np.empty((nz, nx), np.float32)
+    b1, b2 = nz // 3 + 1, 2 * nz // 3 + 1
+    vp[:b1], vp[b1:b2], vp[b2:] = 1500, 2000, 2500
+    return {"vp": vp, "vs": (vp / 1.73).astype(np.float32), "rho": gardner(vp)}
+
+
+def random_layered(nz, nx, nlayers, seed, vp_range=(1800.0, 4000.0), ratio=(1.6, 2.0)):
+    """Horizontal layers with sorted uniform interface depths, vp uniform in
+    ``vp_range``, vs = vp / U(ratio), Gardner density."""
+    rng = np.random.default_rng(seed)
+    tops = np.sort(rng.integers(1, nz, nlayers - 1))
+    vp_l = rng.uniform(*vp_range, nlayers)
+    r_l = rng.uniform(*ratio, nlayers)
+    layer = np.searchsorted(tops, np.arange(nz), side="right")
+    vp = np.repeat(vp_l[layer][:, None], nx, axis=1).astype(np.float32)
+    vs = np.repeat((vp_l / r_l)[layer][:, None], nx, axis=1).astype(np.float32)
+    return {"vp": vp, "vs": vs, "rho": gardner(vp)}
+
+
+def gaussian_anomaly(model, z0, x0, sigma, amp, keep_vs=True):
+    """vp * (1 + amp * exp(-r^2 / (2 sigma^2))) around cell (z0, x0)."""
+    nz, nx = model["vp"].shape
+    z, x = np.mgrid[0:nz, 0:nx]
+    g = np.exp(-((z - z0) ** 2 + (x - x0) ** 2) / (2.0 * sigma ** 2))
+    out = {k: v.copy() for k, v in model.items()}
+    out["vp"] = (model["vp"] * (1 + amp * g)).astype(np.float32)
+    return out
+
+
+def _surface_geometry(ns, nx, dh, nrec, rec_dis_cells, src_z_cells, rec_z_cells, x0_cells):
+    src_loc = np.zeros((ns, 2), np.float32)
+    src_loc[:, 0] = np.linspace(4 * dh, nx * dh - 9 * dh, ns)
+    src_loc[:, 1] = src_z_cells * dh
+    rec_loc = np.zeros((nrec, 2), np.float64)
+    rec_loc[:, 0] = (x0_cells + rec_dis_cells * np.arange(nrec)) * dh
+    rec_loc[:, 1] = rec_z_cells * dh
+    return src_loc, rec_loc
+
+
+def plain_small():
+    """nz=nx=100, dh=1 m, 3 layers, 30 Hz Ricker, dt=0.1 ms, nt=1000, 5 surface
+    explosive shots (z=4 cells, x=linspace(4, 91)), 50 receivers (z=2 cells,
+    x=1+2k), vx/vz records; observed data from a +5% Gaussian vp anomaly
+    (sigma 8 cells) at the centre; gradient w.r.t. lambda/mu/rho."""
+    dh, dt, nt = 1.0, 1e-4, 1000
+    inpa = _base_inpa(dh, dt, nt)
+    inpa["rec_dis"] = 2 * dh
+    model = layered3(100, 100)
+    src_loc, rec_loc = _surface_geometry(5, 100, dh, 50, 2, 4, 2, 1)
+    src = Source(src_loc, dh, dt)
+    src.Ricker(30)
+    obs = gaussian_anomaly(model, 50, 50, 8.0, 0.05)
+    return Workload("plain_small_100x100", inpa, src, rec_loc, model, obs, components=2,
+                    chpr=100, meta={"nt": nt, "shots": 5, "receivers": 50})
+
+
+def _cfl_dt(model, dh, cfl=0.45):
+    return cfl * dh / float(model["vp"].max())
+
+
+def plain_medium(seed=2, ns=20, nt=2000):
+    """nz=500, nx=1000, dh=5 m, 20-layer random model, 20 Hz Ricker, dt from
+    CFL 0.45, nt=2000, 20 evenly spaced shots, 400 receivers (10 m spacing)."""
+    dh = 5.0
+    model = random_layered(500, 1000, 20, seed)
+    dt = _cfl_dt(model, dh)
+    inpa = _base_inpa(dh, dt, nt)
+    inpa["rec_dis"] = 2 * dh
+    src_loc, rec_loc = _surface_geometry(ns, 1000, dh, 400, 2, 4, 2, 100)
+    src = Source(src_loc, dh, dt)
+    src.Ricker(20)
+    return Workload("plain_medium_500x1000", inpa, src, rec_loc, model, None, components=0,
+                    chpr=0, meta={"nt": nt, "shots": ns, "receivers": 400, "seed": seed})
+
+
+def differentiation(seed=3, ns=10, nt=2500):
+    """nz=300, nx=600, dh=5 m, random layers plus 3 Gaussian anomalies of
+    +-10% vp, 10 shots, nt=2500."""
+    dh = 5.0
+    rng = np.random.default_rng(seed + 1000)
+    model = random_layered(300, 600, 20, seed)
+    obs = model
+    for _ in range(3):
+        obs = gaussian_anomaly(obs, rng.uniform(60, 240), rng.uniform(100, 500),
+                               rng.uniform(10, 25), rng.choice([-0.1, 0.1]))
+    dt = _cfl_dt(obs, dh)
+    inpa = _base_inpa(dh, dt, nt)
+    inpa["rec_dis"] = 2 * dh
+    src_loc, rec_loc = _surface_geometry(ns, 600, dh, 280, 2, 4, 2, 20)
+    src = Source(src_loc, dh, dt)
+    src.Ricker(20)
+    return Workload("differentiation_300x600", inpa, src, rec_loc, model, obs, components=0,
+                    chpr=0, meta={"nt": nt, "shots": ns, "seed": seed})
+
+
+def large_step(seed=5, ns=32, nt=6000):
+    """nz=2000, nx=6000, dh=2.5 m, random layers with 5 anomalies, 32 shots,
+    nt=6000, 15 Hz, 3000 receivers."""
+    dh = 2.5
+    rng = np.random.default_rng(seed + 1000)
+    model = random_layered(2000, 6000, 20, seed)
+    for _ in range(5):
+        model = gaussian_anomaly(model, rng.uniform(300, 1700), rng.uniform(500, 5500),
+                                 rng.uniform(30, 80), rng.choice([-0.1, 0.1]))
+    dt = _cfl_dt(model, dh)
+    inpa = _base_inpa(dh, dt, nt)
+    inpa["rec_dis"] = 2 * dh
+    src_loc, rec_loc = _surface_geometry(ns, 6000, dh, 3000, 2, 4, 2, 0)
+    src = Source(src_loc, dh, dt)
+    src.Ricker(15)
+    return Workload("large_step_2000x6000", inpa, src, rec_loc, model, None, components=0,
+                    chpr=0, meta={"nt": nt, "shots": ns, "seed": seed})
