@@ -1,0 +1,251 @@
+"""Benchmark: fwd+adj gradient Gcell-updates/s on one MI355X, batched vs per-shot.
+
+Workload (default): plain_medium_500x1000 -- nz=500, nx=1000, dh=5 m,
+seeded 20-layer random model, 20 Hz Ricker, dt from CFL 0.45, nt=2000,
+20 surface shots, 400 receivers, npml=20 (540 x 1040 cells with the
+absorbing layer), order-4 stencil, FP32.  One step = one full gradient of the
+L2 misfit: batched forward modelling of all shots (records + reference
+checkpoints), residual against observed data on the device, and the
+reference-semantics adjoint (time-reversed reconstruction, adjoint sweeps,
+imaging condition) of all shots.  Inputs are resident in HBM before the
+timed region.
+
+A gradient cell-update is one (cell incl. absorbing layer, time step, shot)
+taken through forward plus adjoint; value = shots * cells * nt / seconds.
+
+Prints ONE JSON line.  --gpus above 1 is rejected: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per cell of one launch (FP32 words read + written)
+KERNEL_BYTES = {
+    "fwd_vel": 32,       # R txx tzz txz vx vz b, W vx vz
+    "fwd_str": 40,       # R vx vz txx tzz txz lam mu, W txx tzz txz
+    "rev_str": 40,
+    "rev_vel": 32,
+    "adj_str": 36,       # R avx avz b atxx atzz atxz, W atxx atzz atxz
+    "adj_vel_grad": 84,  # R 3 atau, lam, mu, 2 av, 5 fwd, b, 3 acc; W 2 av, 3 acc
+    "exact_str": 48,
+    "exact_vel": 52,
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_problem(workload):
+    from pyfwi_amd import workloads
+    from pyfwi_amd.wave_propagation import WavePropagator
+    w = getattr(workloads, workload)()
+    W = WavePropagator(w.inpa, w.src, w.rec_loc, w.model_shape, components=w.components,
+                       chpr=w.chpr)
+    return w, W
+
+
+def device_setup(w, W, torch):
+    """Upload model and tables once; return device buffers and shot tables."""
+    from pyfwi_amd import _native
+    from pyfwi_amd.fwi_tools import expand_model
+    dev = torch.device("cuda:0")
+    plan = W._plan
+    plan.set_stream(torch.cuda.current_stream().cuda_stream)
+    m = {k: expand_model(v, int(W.tnz), int(W.tnx), W.npml) for k, v in w.model.items()}
+    W.pml_preparation(m["vp"].max())
+    W.elastic_buffers(m)
+    ids = list(range(int(W.ns)))
+    shots = W._shots(ids)
+    src = torch.from_numpy(W._source_table()).to(dev)
+    nrec = len(ids) * 5 * int(W.nt_ext) * int(W.nr)
+    seis = torch.zeros(nrec, dtype=torch.float32, device=dev)
+    obs = torch.zeros(nrec, dtype=torch.float32, device=dev)
+    res = torch.zeros(nrec, dtype=torch.float32, device=dev)
+    grad = torch.zeros(6 * int(W.tnz) * int(W.tnx), dtype=torch.float32, device=dev)
+    # observed data: the same acquisition over a +5% vp anomaly
+    from pyfwi_amd.workloads import gaussian_anomaly
+    mo = gaussian_anomaly(w.model, w.model_shape[0] / 2, w.model_shape[1] / 2, 40.0, 0.05)
+    mo = {k: expand_model(v, int(W.tnz), int(W.tnx), W.npml) for k, v in mo.items()}
+    W.elastic_buffers(mo)
+    plan.forward(shots, src, obs, W.chp, 0, _native.MEM_DEVICE)
+    W.elastic_buffers(m)
+    torch.cuda.synchronize()
+    return dict(plan=plan, shots=shots, src=src, seis=seis, obs=obs, res=res, grad=grad)
+
+
+def gradient_step(W, b, shots=None, seis=None, obs=None, res=None):
+    """One fwd+adj gradient of the given shots (all by default)."""
+    from pyfwi_amd import _native
+    plan = b["plan"]
+    shots = b["shots"] if shots is None else shots
+    seis = b["seis"] if seis is None else seis
+    obs = b["obs"] if obs is None else obs
+    res = b["res"] if res is None else res
+    plan.forward(shots, b["src"], seis, W.chp, 0, _native.MEM_DEVICE)
+    res.copy_(seis)
+    res.sub_(obs)
+    plan.gradient(shots, b["src"], res, b["grad"], _native.MODE_PYFWI, 0, 0, _native.MEM_DEVICE)
+
+
+def per_shot_rate(W, b, torch, nsub):
+    """The plain per-shot path: shots one after another through the same
+    kernels (the reference's schedule), on the first `nsub` shots."""
+    import copy
+    cells = int(W.tnz) * int(W.tnx)
+    nrec1 = 5 * int(W.nt_ext) * int(W.nr)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(nsub):
+        sh = copy.copy(b["shots"][s])
+        sh.rec_offset = 0
+        gradient_step(W, b, [sh], b["seis"][:nrec1], b["obs"][s * nrec1:(s + 1) * nrec1],
+                      b["res"][:nrec1])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return nsub * cells * W.nt / dt / 1e9, dt
+
+
+def kernel_profile(W, b, torch):
+    """Average launch duration of each stencil kernel over one gradient, from
+    HIP events on the plan's stream."""
+    from pyfwi_amd import _native
+    plan = b["plan"]
+    plan.set_timing(True)
+    gradient_step(W, b)
+    torch.cuda.synchronize()
+    out = {}
+    for k, name in enumerate(_native.KERNEL_NAMES):
+        ms, n = plan.kernel_timing(k)
+        if n:
+            out[name] = (ms, n)
+    plan.set_timing(False)
+    return out
+
+
+def cpu_baseline(w, steps_cap):
+    """The CPU oracle (C, OpenMP over rows) on one shot of the same workload,
+    forward + reference adjoint, for `steps_cap` time steps."""
+    from oracle import oracle as O
+    import math
+    inpa = dict(w.inpa)
+    nt = steps_cap
+    inpa["t"] = (nt - 1) * inpa["dt"] + inpa["dt"] / 2
+    src_loc = np.c_[w.src.i, w.src.j].astype(np.float64) * inpa["dh"]
+    st = O.Setup(inpa, src_loc[:1], w.rec_loc, w.model_shape, w.src.w, w.src.component,
+                 chpr=0, components=w.components)
+    O.lib()
+    t0 = time.perf_counter()
+    seis, states = O.forward(st, w.model, np.float32)
+    res = O.section(seis, 4)
+    O.raw_gradient(st, w.model, res, states, np.float32)
+    dt = time.perf_counter() - t0
+    cells = st.tnz * st.tnx
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(cells * nt / dt / 1e9, 5), "unit": "Gcell-updates/s",
+            "cores": threads, "kind": "port",
+            "sample": f"1 shot x {nt} steps of the same grid ({st.tnz}x{st.tnx}), forward + "
+                      f"reference adjoint, C oracle, {threads} OpenMP threads, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="plain_medium")
+    ap.add_argument("--per-shot-sample", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU visible"}))
+        sys.exit(2)
+    torch.cuda.set_device(0)
+
+    log("bench: building problem")
+    w, W = build_problem(args.workload)
+    b = device_setup(w, W, torch)
+    cells = int(W.tnz) * int(W.tnx)
+    units = int(W.ns) * cells * W.nt
+    log(f"bench: {w.name}: {int(W.ns)} shots x {W.nt} steps x {cells} cells")
+
+    for i in range(args.warmup):
+        gradient_step(W, b)
+        torch.cuda.synchronize()
+        log(f"bench: warmup {i + 1}/{args.warmup} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        gradient_step(W, b)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed / args.steps * 1e3
+    value = units / (elapsed / args.steps) / 1e9
+    log(f"bench: timed {args.steps} steps, {ms_per_step:.1f} ms/step, {value:.2f} Gcell/s")
+
+    prof = kernel_profile(W, b, torch)
+    dom = max(prof, key=lambda k: prof[k][0])
+    ms, n = prof[dom]
+    avg_s = ms / n / 1e3
+    per_launch_bytes = KERNEL_BYTES[dom] * cells * int(W.ns)
+    achieved = per_launch_bytes / avg_s / 1e9
+    kernels = {k: {"avg_us": round(v[0] / v[1] * 1e3, 2), "launches": v[1],
+                   "GBps": round(KERNEL_BYTES[k] * cells * int(W.ns) / (v[0] / v[1] / 1e3) / 1e9, 1)}
+               for k, v in prof.items()}
+    log(f"bench: kernels {kernels}")
+
+    ps_rate, ps_time = per_shot_rate(W, b, torch, args.per_shot_sample)
+    log(f"bench: per-shot path {ps_rate:.2f} Gcell/s")
+
+    cpu = None
+    if not args.no_cpu:
+        log("bench: cpu baseline")
+        cpu = cpu_baseline(w, args.cpu_steps)
+
+    grad_bytes = 148.0
+    line = {
+        "metric": "fwd+adj gradient Gcell-updates/s, batched vs per-shot, % HBM roofline",
+        "value": round(value, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded random-layered model, Ricker sources)",
+        "config": {"workload": w.name, "nz": int(W.nz), "nx": int(W.nx), "npml": int(W.npml),
+                   "cells": cells, "shots": int(W.ns), "nt": int(W.nt), "receivers": int(W.nr),
+                   "order": int(W.sdo) * 2, "gradient": "reference adjoint, chpr=0",
+                   "parallelism": "single GPU, shots batched in one launch"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_cell": KERNEL_BYTES[dom]},
+        "gradient_roofline_frac": round(value * grad_bytes / HBM_PEAK_GBS, 4),
+        "per_shot": {"value": round(ps_rate, 3), "shots_sampled": args.per_shot_sample,
+                     "batched_over_per_shot": round(value / ps_rate, 2)},
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -204,10 +204,11 @@ def _suffix(dtype):
     return "f" if dtype == np.float32 else "d"
 
 
-def forward(setup, model, dtype=np.float32, shots=None, keep_states=True):
-    """Records [ns, 5, nt_ext, nr] and kept states [ns, nchp, 5, tnz, tnx]."""
+def forward(setup, model, dtype=np.float32, shots=None, keep_states=True, medium=None):
+    """Records [ns, 5, nt_ext, nr] and kept states [ns, nchp, 5, tnz, tnx].
+    ``medium`` overrides the lam/mu/b/damping arrays derived from ``model``."""
     L = lib()
-    md = setup.medium(model, dtype)
+    md = setup.medium(model, dtype) if medium is None else medium
     acq = setup.acq(shots)
     ns = acq.ns
     seis = np.zeros((5, ns, setup.nt_ext, setup.nr), dtype)
@@ -316,10 +317,10 @@ def gradient(setup, model, res2d, W, parameterization="dv", dtype=np.float32):
     return {"lam": glam, "mu": gmu, "rho": grho}
 
 
-def exact_gradient(setup, model, res_records, dtype=np.float64, shots=None):
+def exact_gradient(setup, model, res_records, dtype=np.float64, shots=None, medium=None):
     """dJ/d(lam, mu, b) [3, tnz, tnx] for J = sum(res * records), all shots."""
     L = lib()
-    md = setup.medium(model, dtype)
+    md = setup.medium(model, dtype) if medium is None else medium
     acq = setup.acq(shots)
     ns = acq.ns
     src = np.ascontiguousarray(np.broadcast_to(setup.src_table.astype(dtype),

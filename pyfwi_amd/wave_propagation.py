@@ -122,7 +122,7 @@ class WavePreparation:
         grid.dt = kernel_constant(self.dt)
         grid.dx = grid.dz = kernel_constant(self.dh)
         self._grid = grid
-        self._plan = _native.Plan(grid)
+        self._plan_obj = None
         self._seis = None
         self._forward_done = False
         self.per_shot = bool(inpa.get("per_shot", False))
@@ -131,6 +131,13 @@ class WavePreparation:
                            for f in COMPONENTS}
 
     # ------------------------------------------------------------ helpers
+    @property
+    def _plan(self):
+        # created on first use so that the geometry can be inspected without a GPU
+        if self._plan_obj is None:
+            self._plan_obj = _native.Plan(self._grid)
+        return self._plan_obj
+
     @property
     def n_injections(self):
         return int(math.ceil(self.nt / float(self.dt_scale)))

@@ -1,0 +1,48 @@
+"""The C-ABI library loads and exports every entry point include/*.h declares
+(no compute calls: this suite runs without a GPU)."""
+import ctypes
+import glob
+import os
+import re
+
+from pyfwi_amd import _native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(pyfwi_\w+)\s*\(", text, re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_header_declares_the_binding_exports():
+    assert declared_functions() == set(_native.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_native.LIB_PATH)
+    for name in sorted(declared_functions()):
+        assert hasattr(lib, name), name
+
+
+def test_version_and_error_string():
+    lib = _native.load()
+    assert lib.pyfwi_version() == 100
+    assert isinstance(lib.pyfwi_last_error(), bytes)
+
+
+def test_invalid_grid_is_rejected_before_any_device_call():
+    lib = _native.load()
+    g = _native.Grid()
+    g.tnz, g.tnx, g.npml, g.sdo = 50, 50, 10, 3   # sdo must be 2 or 4
+    g.dt, g.dx, g.dz = 1e-3, 5.0, 5.0
+    h = ctypes.c_void_p()
+    rc = lib.pyfwi_plan_create(ctypes.byref(g), None, ctypes.byref(h))
+    assert rc == 1 and b"sdo" in lib.pyfwi_last_error()
+    g.sdo, g.dt = 2, -1.0
+    assert lib.pyfwi_plan_create(ctypes.byref(g), None, ctypes.byref(h)) == 1

@@ -1,0 +1,223 @@
+"""Host-side logic of the drop-in modules against closed forms and the
+reference's own known-answer test (no GPU)."""
+import numpy as np
+import pytest
+
+from pyfwi_amd import acquisition as acq
+from pyfwi_amd import fwi_tools as tools
+from pyfwi_amd import grad_switcher as gs
+from pyfwi_amd.processing import prepare_residual
+from pyfwi_amd.wave_propagation import WavePropagator, checkpoint_steps
+from pyfwi_amd import workloads
+
+
+def test_ricker_known_answer():
+    """The reference's only unit test (test/test_ricker.py:16-22), verbatim in
+    meaning: exact float32 equality with the closed form."""
+    fdom, dt = 20.0, 0.004
+    src = acq.Source(src_loc=np.array([[0, 0]]), dh=1.0, dt=dt)
+    src.Ricker(fdom=fdom)
+    t = np.arange(-1.0 / fdom, 1.0 / fdom + dt / 3, dt)
+    assert np.all(src.w == np.float32((1.0 - 2.0 * (np.pi * fdom * t) ** 2)
+                                      * np.exp(-(np.pi * fdom * t) ** 2)))
+
+
+def test_source_samples_and_components():
+    src = acq.Source(np.array([[10.0, 4.0], [30.0, 4.0]]), 1.0, 1e-3)
+    src.Ricker(25)
+    tab = src.samples(src.w.size + 5)
+    assert tab.dtype == np.float32 and tab.shape == (src.w.size + 5, 5)
+    np.testing.assert_array_equal(tab[:, 2], tab[:, 3])
+    assert not tab[:, [0, 1, 4]].any() and not tab[src.w.size:].any()
+    np.testing.assert_array_equal(tab[3], src(3))
+    np.testing.assert_array_equal(src(src.w.size + 1), np.zeros(5, np.float32))
+    assert list(src.i) == [10, 30] and list(src.j) == [4, 4]
+    for kind, comp in ((1, 0), (2, 1)):
+        s = acq.Source(np.array([[1.0, 1.0]]), 1.0, 1e-3, src_type=kind)
+        assert s.component[comp] == 1 and s.component.sum() == 1
+
+
+def test_surface_geometry():
+    src, rec = acq.surface_seismic(5, 2.0, 100.0, 1.0, 4)
+    np.testing.assert_allclose(src[:, 0], np.linspace(4, 91, 5))
+    assert np.all(src[:, 1] == 4) and np.all(rec[:, 1] == 4)
+    np.testing.assert_allclose(rec[:, 0], np.arange(4, 96, 2))
+    s1, _ = acq.surface_seismic(1, 2.0, 100.0, 1.0, 4)
+    assert s1[0, 0] == 50
+    with pytest.raises(Exception):
+        acq.acq_parameters(2, 0.5, 100, 100, 1.0, 4, 1)
+
+
+def test_kernel_constants_are_printed_values():
+    """The reference pastes constants into kernel source with %f
+    (wave_propagation.py:434-490)."""
+    assert tools.kernel_constant(-1 / 24) == -0.041667
+    assert tools.kernel_constant(0.45 * 5 / 4000) == 0.000562
+    assert [tools.kernel_constant(c) for c in tools.fd_coefficients(8)] == \
+        [1.195955, -0.079498, 0.009763, -0.000697]
+
+
+def test_damping_profile_closed_form():
+    """d(r) = -(3/(2L)) (r/L)^2 log10(R), r = 0..npml-1 cells, mirrored on the
+    left/top (fwi_tools.py:943-1063)."""
+    dh, npml, R = 5.0, 20, 1e-5
+    a = tools.pml_delta_calculation(np.float32(dh), npml, R)
+    L = npml * dh
+    for k in (0, 1, 7, 19):
+        want = -(3 / (2 * L)) * ((k * dh / L) ** 2) * np.log10(R)
+        assert a[k] == np.float32(want)
+    dxg, dzg = tools.pml_counstruction(60, 70, np.float32(dh), npml, R, 2)
+    assert dxg.shape == (60, 70) and dxg.dtype == np.float32
+    np.testing.assert_array_equal(dxg[:, 70 - npml:], np.broadcast_to(a, (60, npml)))
+    np.testing.assert_array_equal(dxg[:, :npml], np.broadcast_to(a[::-1], (60, npml)))
+    assert not dxg[:, npml:70 - npml].any()
+    np.testing.assert_array_equal(dzg[60 - npml:, 5], a)
+    _, dz3 = tools.pml_counstruction(60, 70, np.float32(dh), npml, R, 3)
+    assert not dz3[:npml].any() and dz3[-1, 0] == a[-1]
+    dx0, _ = tools.pml_counstruction(60, 70, np.float32(dh), npml, R, 0)
+    _, dz1 = tools.pml_counstruction(60, 70, np.float32(dh), npml, R, 1)
+    assert not dx0.any() and not dz1.any()
+
+
+def test_expand_model_replicates_edges():
+    m = np.arange(12, dtype=np.float32).reshape(3, 4)
+    e = tools.expand_model(m, 7, 8, 2)
+    np.testing.assert_array_equal(e[2:5, 2:6], m)
+    np.testing.assert_array_equal(e[0, 2:6], m[0])
+    np.testing.assert_array_equal(e[6, 2:6], m[2])
+    np.testing.assert_array_equal(e[3, :2], [m[1, 0]] * 2)
+    assert e[0, 0] == m[0, 0] and e[6, 7] == m[2, 3] and e[0, 7] == m[0, 3]
+
+
+def test_checkpoint_list():
+    """wave_propagation.py:95-100."""
+    np.testing.assert_array_equal(checkpoint_steps(1000, 0), [1, 999])
+    np.testing.assert_array_equal(checkpoint_steps(1000, 100), np.arange(1000))
+    c = checkpoint_steps(1000, 20)
+    assert len(c) == 200 and c[0] == 0 and c[-1] == 999 and np.all(np.diff(c) > 0)
+    np.testing.assert_array_equal(checkpoint_steps(50, 1), [1, 49])
+
+
+def test_seismic_section_layout():
+    rng = np.random.default_rng(0)
+    seismo = {f: rng.standard_normal((7, 3, 2)).astype(np.float32)
+              for f in ("vx", "vz", "taux", "tauz", "tauxz")}
+    d = acq.seismic_section(seismo, 0, "2d")
+    assert set(d) == {"taux", "tauz"} and d["taux"].shape == (7, 6)
+    p = (seismo["taux"] + seismo["tauz"]) / 2
+    for s in range(2):
+        for r in range(3):
+            np.testing.assert_array_equal(d["tauz"][:, s * 3 + r], p[:, r, s])
+    d2 = acq.seismic_section(seismo, 2, "3d")
+    assert set(d2) == {"vx", "vz"} and d2["vx"].shape == (7, 3, 2)
+    assert set(acq.seismic_section(seismo, 3)) == {"taux", "tauz", "tauxz"}
+    assert set(acq.seismic_section(seismo, 4)) == set(seismo)
+
+
+def test_prepare_residual_and_l2():
+    r = {"taux": np.ones((4, 6), np.float32)}
+    d = prepare_residual(r, 2.0)
+    assert set(d) == {"vx", "vz", "taux", "tauz", "tauxz"}
+    assert np.all(d["taux"] == 2) and not d["vx"].any() and d["vx"].dtype == np.float32
+    cf = tools.CostFunction("l2")
+    a = {"vx": np.full((3, 2), 2.0, np.float32)}
+    b = {"vx": np.full((3, 2), 0.5, np.float32)}
+    rms, res = cf(a, b)
+    assert rms == pytest.approx(0.5 * 6 * 1.5 ** 2) and np.all(res["vx"] == 1.5)
+
+
+def _fd_jacobian(f, x, eps=1e-6):
+    """Central-difference Jacobian of an elementwise map R^3 -> R^3 at each
+    point: J[i][j] = d f_i / d x_j."""
+    J = np.zeros((3, 3) + x[0].shape)
+    for j in range(3):
+        xp = [v.copy() for v in x]
+        xm = [v.copy() for v in x]
+        xp[j] = xp[j] + eps * np.abs(x[j])
+        xm[j] = xm[j] - eps * np.abs(x[j])
+        fp, fm = f(*xp), f(*xm)
+        for i in range(3):
+            J[i, j] = (fp[i] - fm[i]) / (2 * eps * np.abs(x[j]))
+    return J
+
+
+def test_chain_rules_against_finite_differences():
+    """g_new = J^T g_old, with J the Jacobian of the parameter map, checked by
+    central differences in float64 (grad_switcher.py:246-266, 331-448)."""
+    rng = np.random.default_rng(1)
+    vp = rng.uniform(1800, 4000, 50)
+    vs = vp / rng.uniform(1.6, 2.0, 50)
+    rho = rng.uniform(1800, 2600, 50)
+    g = [rng.standard_normal(50) for _ in range(3)]
+    # (vp, vs, rho) -> (lam, mu, rho)
+    to_lmd = lambda a, b, r: (r * a ** 2 - 2 * r * b ** 2, r * b ** 2, r)
+    J = _fd_jacobian(to_lmd, [vp, vs, rho])
+    want = [sum(J[i, j] * g[i] for i in range(3)) for j in range(3)]
+    lam, mu = rho * vp ** 2 - 2 * rho * vs ** 2, rho * vs ** 2
+    got = gs.grad_lmd_to_vd(g[0], g[1], g[2], lam, mu, rho)
+    for a, b in zip(got, want):
+        np.testing.assert_allclose(a, b, rtol=2e-5)
+    # (lam, mu, rho) -> (vp, vs, rho)
+    to_vd = lambda l, m, r: (np.sqrt((l + 2 * m) / r), np.sqrt(m / r), r)
+    J = _fd_jacobian(to_vd, [lam, mu, rho])
+    want = [sum(J[i, j] * g[i] for i in range(3)) for j in range(3)]
+    got = gs.grad_vd_to_lmd(g[0], g[1], g[2], vp, vs, rho)
+    for a, b in zip(got, want):
+        np.testing.assert_allclose(a, b, rtol=2e-5)
+    # (K, mu, rho) -> (vp, vs, rho)
+    k = lam + 2 * mu / 3
+    to_vd_k = lambda kk, m, r: (np.sqrt((kk + 4 * m / 3) / r), np.sqrt(m / r), r)
+    J = _fd_jacobian(to_vd_k, [k, mu, rho])
+    want = [sum(J[i, j] * g[i] for i in range(3)) for j in range(3)]
+    got = gs.grad_dv2kmd({"vp": g[0], "vs": g[1], "rho": g[2]}, vp, vs, rho)
+    for a, b in zip(got, want):
+        np.testing.assert_allclose(a, b, rtol=2e-5)
+
+
+def test_chain_rule_on_torch_tensors():
+    torch = pytest.importorskip("torch")
+    a = [torch.rand(5, dtype=torch.float64) + 1 for _ in range(6)]
+    out = gs.grad_lmd_to_vd(*a)
+    ref = gs.grad_lmd_to_vd(*[x.numpy() for x in a])
+    for t, r in zip(out, ref):
+        assert t.dtype == torch.float32
+        np.testing.assert_allclose(t.numpy(), r, rtol=1e-6)
+
+
+def test_fdm_self_checks():
+    for order in (4, 8):
+        D = tools.Fdm(order)
+        ex, ez = D.dot_product_test_derivatives(seed=3)
+        assert abs(ex) < 1e-9 and abs(ez) < 1e-9
+    D = tools.Fdm(4)
+    c_sum = 9 / 8 + 1 / 24
+    assert D.dt_computation(4000.0, 5.0) == pytest.approx(2 / 4000 / np.sqrt(2 * c_sum / 25 * 5))
+    with pytest.raises(AssertionError):
+        tools.Fdm(6)
+
+
+def test_propagator_geometry_without_gpu():
+    """The drop-in's timing and geometry (wave_propagation.py:28-144) are
+    computed on the host; the device plan is created lazily."""
+    w = workloads.plain_small()
+    W = WavePropagator(w.inpa, w.src, w.rec_loc, w.model_shape, components=2, chpr=100)
+    assert (W.nt, W.nt_ext, W.dt_scale) == (1000, 1000, 1)
+    assert (int(W.tnz), int(W.tnx), int(W.dxr)) == (140, 140, 2)
+    assert (int(W.rec_surface_const), int(W.rec_surface_var)) == (22, 21)
+    assert list(W.srcx) == [24, 45, 67, 89, 111] and set(W.srcz) == {24}
+    assert W.nchp == 1000 and W._n_live == 50
+    assert W._grid.c[1] == -0.041667 and W._grid.dt == 1e-4
+    m = workloads.plain_medium()
+    Wm = WavePropagator(m.inpa, m.src, m.rec_loc, m.model_shape, components=0)
+    assert Wm.nt == 2000 and list(Wm.chp) == [1, 1999]
+    with pytest.raises(NotImplementedError):
+        WavePropagator(dict(w.inpa, acq_type=0), w.src, w.rec_loc, w.model_shape)
+
+
+def test_workloads_are_seeded():
+    a, b = workloads.plain_medium(), workloads.plain_medium()
+    np.testing.assert_array_equal(a.model["vp"], b.model["vp"])
+    assert a.model["vp"].shape == (500, 1000) and a.src.i.size == 20 and a.rec_loc.shape[0] == 400
+    vp = a.model["vp"]
+    assert vp.min() >= 1800 and vp.max() <= 4000
+    assert a.inpa["dt"] == pytest.approx(0.45 * 5 / vp.max())
