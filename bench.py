@@ -28,27 +28,34 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per cell of one launch (FP32 words read + written)
+# Algorithmic bytes of one launch (FP32 words read + written once per cell):
+# (bytes per cell of the whole grid, extra bytes per cell of the interior where
+# the imaging condition runs).
 KERNEL_BYTES = {
-    "fwd_vel": 32,       # R txx tzz txz vx vz b, W vx vz
-    "fwd_str": 40,       # R vx vz txx tzz txz lam mu, W txx tzz txz
-    "rev_str": 40,
-    "rev_vel": 32,
-    "adj_str": 36,       # R avx avz b atxx atzz atxz, W atxx atzz atxz
-    "adj_vel_grad": 84,  # R 3 atau, lam, mu, 2 av, 5 fwd, b, 3 acc; W 2 av, 3 acc
-    "exact_str": 48,
-    "exact_vel": 52,
+    "fwd_vel": (32, 0),       # R txx tzz txz vx vz b, W vx vz
+    "fwd_str": (40, 0),       # R vx vz txx tzz txz lam mu, W txx tzz txz
+    "rev_str": (40, 0),
+    "rev_vel": (32, 0),
+    "adj_str": (36, 0),       # R avx avz b atxx atzz atxz, W atxx atzz atxz
+    "adj_vel_grad": (36, 48),  # all: R 3 atau lam mu 2 av, W 2 av; interior: R 5 fwd b 3 acc, W 3 acc
+    "exact_str": (48, 0),
+    "exact_vel": (52, 0),
 }
+
+
+def launch_bytes(kernel, cells, interior, shots):
+    a, b = KERNEL_BYTES[kernel]
+    return (a * cells + b * interior) * shots
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def build_problem(workload):
+def build_problem(workload, nt=None):
     from pyfwi_amd import workloads
     from pyfwi_amd.wave_propagation import WavePropagator
-    w = getattr(workloads, workload)()
+    w = getattr(workloads, workload)(**({"nt": nt} if nt else {}))
     W = WavePropagator(w.inpa, w.src, w.rec_loc, w.model_shape, components=w.components,
                        chpr=w.chpr)
     return w, W
@@ -132,11 +139,19 @@ def kernel_profile(W, b, torch):
     return out
 
 
-def cpu_baseline(w, steps_cap):
+def _set_omp_threads(n):
+    import ctypes
+    try:
+        ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+        return True
+    except OSError:
+        return False
+
+
+def cpu_baseline(w, steps_cap, threads):
     """The CPU oracle (C, OpenMP over rows) on one shot of the same workload,
     forward + reference adjoint, for `steps_cap` time steps."""
     from oracle import oracle as O
-    import math
     inpa = dict(w.inpa)
     nt = steps_cap
     inpa["t"] = (nt - 1) * inpa["dt"] + inpa["dt"] / 2
@@ -144,17 +159,29 @@ def cpu_baseline(w, steps_cap):
     st = O.Setup(inpa, src_loc[:1], w.rec_loc, w.model_shape, w.src.w, w.src.component,
                  chpr=0, components=w.components)
     O.lib()
+    _set_omp_threads(threads)
     t0 = time.perf_counter()
     seis, states = O.forward(st, w.model, np.float32)
     res = O.section(seis, 4)
     O.raw_gradient(st, w.model, res, states, np.float32)
     dt = time.perf_counter() - t0
     cells = st.tnz * st.tnx
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": round(cells * nt / dt / 1e9, 5), "unit": "Gcell-updates/s",
             "cores": threads, "kind": "port",
             "sample": f"1 shot x {nt} steps of the same grid ({st.tnz}x{st.tnx}), forward + "
-                      f"reference adjoint, C oracle, {threads} OpenMP threads, {dt:.1f} s"}
+                      f"reference adjoint, C oracle, {threads} OpenMP thread(s), {dt:.1f} s"}
+
+
+def copy_calibration(torch, nbytes=1 << 30):
+    """A plain device copy of known size, for scaling HBM byte counters."""
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"bench: copy calibration {nbytes} bytes read + {nbytes} bytes written, 3 launches")
 
 
 def main():
@@ -163,9 +190,12 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="plain_medium")
-    ap.add_argument("--per-shot-sample", type=int, default=2)
-    ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--nt", type=int, default=0, help="override time steps (profiling runs)")
+    ap.add_argument("--per-shot-sample", type=int, default=2, help="0 skips the per-shot pass")
+    ap.add_argument("--cpu-steps", type=int, default=2000)
+    ap.add_argument("--cpu-steps-1t", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -177,9 +207,13 @@ def main():
     torch.cuda.set_device(0)
 
     log("bench: building problem")
-    w, W = build_problem(args.workload)
+    if args.copy_calibration:
+        copy_calibration(torch)
+    w, W = build_problem(args.workload, args.nt or None)
     b = device_setup(w, W, torch)
     cells = int(W.tnz) * int(W.tnx)
+    sdo, npml = int(W.sdo), int(W.npml)
+    interior = (int(W.tnz) - 2 * (npml + sdo) + 1) * (int(W.tnx) - 2 * (npml + sdo) + 1)
     units = int(W.ns) * cells * W.nt
     log(f"bench: {w.name}: {int(W.ns)} shots x {W.nt} steps x {cells} cells")
 
@@ -201,20 +235,28 @@ def main():
     dom = max(prof, key=lambda k: prof[k][0])
     ms, n = prof[dom]
     avg_s = ms / n / 1e3
-    per_launch_bytes = KERNEL_BYTES[dom] * cells * int(W.ns)
+    per_launch_bytes = launch_bytes(dom, cells, interior, int(W.ns))
     achieved = per_launch_bytes / avg_s / 1e9
     kernels = {k: {"avg_us": round(v[0] / v[1] * 1e3, 2), "launches": v[1],
-                   "GBps": round(KERNEL_BYTES[k] * cells * int(W.ns) / (v[0] / v[1] / 1e3) / 1e9, 1)}
+                   "GBps": round(launch_bytes(k, cells, interior, int(W.ns))
+                                 / (v[0] / v[1] / 1e3) / 1e9, 1)}
                for k, v in prof.items()}
     log(f"bench: kernels {kernels}")
 
-    ps_rate, ps_time = per_shot_rate(W, b, torch, args.per_shot_sample)
-    log(f"bench: per-shot path {ps_rate:.2f} Gcell/s")
+    ps = None
+    if args.per_shot_sample > 0:
+        ps_rate, ps_time = per_shot_rate(W, b, torch, args.per_shot_sample)
+        log(f"bench: per-shot path {ps_rate:.2f} Gcell/s")
+        ps = {"value": round(ps_rate, 3), "shots_sampled": args.per_shot_sample,
+              "batched_over_per_shot": round(value / ps_rate, 2)}
 
-    cpu = None
+    cpu = cpu1 = None
     if not args.no_cpu:
-        log("bench: cpu baseline")
-        cpu = cpu_baseline(w, args.cpu_steps)
+        threads = int(os.environ.get("OMP_NUM_THREADS", 0) or os.cpu_count() or 1)
+        log(f"bench: cpu baseline, {threads} threads")
+        cpu = cpu_baseline(w, args.cpu_steps, threads)
+        log(f"bench: cpu baseline, 1 thread")
+        cpu1 = cpu_baseline(w, args.cpu_steps_1t, 1)
 
     grad_bytes = 148.0
     line = {
@@ -237,12 +279,12 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_cell": KERNEL_BYTES[dom]},
+                     "bytes_per_launch": per_launch_bytes, "avg_launch_us": round(avg_s * 1e6, 2)},
         "gradient_roofline_frac": round(value * grad_bytes / HBM_PEAK_GBS, 4),
-        "per_shot": {"value": round(ps_rate, 3), "shots_sampled": args.per_shot_sample,
-                     "batched_over_per_shot": round(value / ps_rate, 2)},
+        "per_shot": ps,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "cpu_baseline_1thread": cpu1,
     }
     print(json.dumps(line), flush=True)
 
