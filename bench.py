@@ -52,6 +52,26 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def kernel_source_sha1():
+    import glob
+    import hashlib
+    files = sorted(glob.glob(os.path.join(ROOT, "pyfwi_amd", "csrc", "*.hip"))
+                   + glob.glob(os.path.join(ROOT, "pyfwi_amd", "csrc", "*.h")))
+    return hashlib.sha1(b"".join(open(f, "rb").read() for f in files)).hexdigest()
+
+
+def measured_traffic(workload_name, kernel):
+    """HBM bytes per dispatch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary, when it was taken with the current kernel sources."""
+    try:
+        d = json.load(open(os.path.join(ROOT, "profiles", "hbm_traffic.json")))
+    except (OSError, ValueError):
+        return None
+    if d.get("workload") != workload_name or d.get("kernel_source_sha1") != kernel_source_sha1():
+        return None
+    return d.get("bytes_per_dispatch", {}).get(kernel)
+
+
 def build_problem(workload, nt=None):
     from pyfwi_amd import workloads
     from pyfwi_amd.wave_propagation import WavePropagator
@@ -278,7 +298,8 @@ def main():
                    "parallelism": "single GPU, shots batched in one launch"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(w.name, dom),
                      "bytes_per_launch": per_launch_bytes, "avg_launch_us": round(avg_s * 1e6, 2)},
         "gradient_roofline_frac": round(value * grad_bytes / HBM_PEAK_GBS, 4),
         "per_shot": ps,
