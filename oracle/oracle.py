@@ -108,9 +108,11 @@ def damping_grids(tnz, tnx, dh, npml, R, pml_dir):
 
 
 def pad_edge(a, npml):
-    a = np.asarray(a, np.float32)
-    out = np.pad(a, npml, mode="edge")
-    return out.astype(np.float32)
+    """expand_model (fwi_tools.py:776-811): edge replication; float32 unless
+    given float64 (FP64 finite-difference checks)."""
+    a = np.asarray(a)
+    dt = np.float64 if a.dtype == np.float64 else np.float32
+    return np.pad(a.astype(dt), npml, mode="edge")
 
 
 class Setup:

@@ -309,13 +309,13 @@ class WavePropagator(WavePreparation):
         (no mask), returned as {'lam','mu','rho'} or {'vp','vs','rho'}."""
         if not self._forward_done:
             raise RuntimeError("forward_modeling must run before gradient")
+        if mode == "exact":
+            res = _section_vjp(res, self.components)
         G = self.raw_gradient(res, mode=mode)
         if mode == "pyfwi":
             glam, gmu, grho0 = self.gradient_reading(G)
         else:
-            crop = (slice(self.npml, int(self.tnz) - self.npml),
-                    slice(self.npml, int(self.tnx) - self.npml))
-            glam, gmu, grho0 = G[2][crop], G[0][crop], G[4][crop]
+            glam, gmu, grho0 = (_fold_padding(G[k], int(self.npml)) for k in (2, 0, 4))
         crop = (slice(self.npml, int(self.tnz) - self.npml),
                 slice(self.npml, int(self.tnx) - self.npml))
         if parameterization == "dv":
@@ -328,3 +328,172 @@ class WavePropagator(WavePreparation):
             from scipy.ndimage import gaussian_filter
             out = {k: gaussian_filter(v, self.g_smooth) for k, v in out.items()}
         return out
+
+    # ------------------------------------------------ device-resident path
+    # Used by pyfwi_amd.torchfwi: model, records and gradients stay in torch
+    # tensors on the GPU; the plan reads and writes them in place.
+
+    def _tensor_setup(self, vp, vs, rho):
+        import torch
+        from torch.nn.functional import pad
+        dev = vp.device
+        npml = int(self.npml)
+
+        def expand(t):
+            t = t.detach().to(device=dev, dtype=torch.float32).contiguous()
+            if npml == 0:
+                return t
+            return pad(t[None, None], (npml, npml, npml, npml), mode="replicate")[0, 0]
+
+        m = {"vp": expand(vp), "vs": expand(vs), "rho": expand(rho)}
+        vmax = np.float32(m["vp"].max().item())
+        self.pml_vmax = vmax
+        self.pml_preparation(vmax)
+        self.mu = m["rho"] * (m["vs"] ** 2)
+        self.lam = m["rho"] * (m["vp"] ** 2) - 2 * self.mu
+        self.rho, self.vp, self.vs = m["rho"], m["vp"], m["vs"]
+        self.rho_b = (1 / m["rho"]).contiguous()
+        plan = self._plan
+        plan.set_stream(torch.cuda.current_stream(dev).cuda_stream)
+        self._dev = dev
+        self._vdx_t = torch.from_numpy(self.vdx_pml).to(dev)
+        self._vdz_t = torch.from_numpy(self.vdz_pml).to(dev)
+        plan.set_model(self.lam.contiguous(), self.mu.contiguous(), self.rho_b,
+                       self._vdx_t, self._vdz_t, _native.MEM_DEVICE)
+        self._src_t = torch.from_numpy(self._source_table()).to(dev)
+        return dev
+
+    def forward_modeling_tensor(self, vp, vs, rho, ckpt_every=0):
+        """Device version of ``forward_modeling`` for torch tensors on the GPU;
+        returns the seismic section as device tensors.  ``ckpt_every`` > 0 also
+        keeps checkpoints for an exact-mode gradient."""
+        import torch
+        dev = self._tensor_setup(vp, vs, rho)
+        ns, nt, nr = int(self.ns), int(self.nt_ext), int(self.nr)
+        seis = torch.empty(ns * 5 * nt * nr, dtype=torch.float32, device=dev)
+        self._plan.forward(self._shots(list(range(ns))), self._src_t, seis, self.chp,
+                           ckpt_every, _native.MEM_DEVICE)
+        self._forward_done = True
+        self._seis_t = seis.view(ns, 5, nt, nr)
+        seismo = {f: self._seis_t[:, k].permute(1, 2, 0) for k, f in enumerate(COMPONENTS)}
+        return _section_tensor(seismo, self.components, self.seismo_shape)
+
+    def _records_tensor(self, res):
+        """{f: [nt_ext, nr*ns] or [nt_ext, nr, ns]} tensors -> [ns, 5, nt, nr]."""
+        import torch
+        ns, nt, nr = int(self.ns), int(self.nt_ext), int(self.nr)
+        out = torch.zeros(ns, 5, nt, nr, dtype=torch.float32, device=self._dev)
+        for k, f in enumerate(COMPONENTS):
+            if f not in res:
+                continue
+            r = res[f].to(device=self._dev, dtype=torch.float32)
+            if r.dim() == 2:
+                r = r.reshape(nt, ns, nr).permute(1, 0, 2)
+            else:
+                r = r.permute(2, 0, 1)
+            out[:, k] = r
+        return out.contiguous()
+
+    def gradient_tensor(self, res, parameterization="dv", mode="pyfwi", ckpt_every=0):
+        """Device version of ``gradient``: residual tensors in, gradient
+        tensors out.  ``mode='exact'`` returns the exact derivative w.r.t. the
+        unpadded model (absorbing-layer cells folded onto the edge cells; the
+        damping's dependence on max(vp) is held fixed)."""
+        import torch
+        if not self._forward_done:
+            raise RuntimeError("forward_modeling must run before gradient")
+        tnz, tnx, npml = int(self.tnz), int(self.tnx), int(self.npml)
+        if mode == "exact":
+            res = _section_vjp(res, self.components)
+        rec = self._records_tensor(res)
+        G = torch.zeros(6, tnz, tnx, dtype=torch.float32, device=self._dev)
+        mode_id = _native.MODE_PYFWI if mode == "pyfwi" else _native.MODE_EXACT
+        flags = _native.F_PRECOND if self.energy_balancing else 0
+        self._plan.gradient(self._shots(list(range(int(self.ns)))), self._src_t, rec, G,
+                            mode_id, ckpt_every, flags, _native.MEM_DEVICE)
+        crop = (slice(npml, tnz - npml), slice(npml, tnx - npml))
+        if mode == "pyfwi":
+            glam, gmu, grho0 = _gradient_reading_tensor(self, G)
+        else:
+            glam, gmu, grho0 = (_fold_padding(G[k], npml) for k in (2, 0, 4))
+        if parameterization == "dv":
+            gvp, gvs, grho = grad_lmd_to_vd(glam, gmu, grho0, self.lam[crop], self.mu[crop],
+                                            self.rho[crop])
+            out = {"vp": gvp, "vs": gvs, "rho": grho}
+        else:
+            out = {"lam": glam, "mu": gmu, "rho": grho0}
+        if self.g_smooth:
+            from scipy.ndimage import gaussian_filter
+            out = {k: torch.from_numpy(gaussian_filter(v.cpu().numpy(), self.g_smooth)).to(self._dev)
+                   for k, v in out.items()}
+        return out
+
+
+def _section_tensor(seismo, components, shape):
+    """seismic_section for tensors {f: [nt, nr, ns]} (acquisition.py:313-333)."""
+    plan = {1: ["taux"], 2: ["vx", "vz"], 3: ["taux", "tauz", "tauxz"],
+            4: ["vx", "vz", "taux", "tauz", "tauxz"]}
+    if components == 0:
+        p = (seismo["taux"] + seismo["tauz"]) / 2
+        data = {"taux": p, "tauz": p.clone()}
+    else:
+        data = {k: seismo[k] for k in plan[int(components)]}
+    out = {}
+    for k, v in data.items():
+        if shape == "2d":
+            nt, nr, ns = v.shape
+            v = v.permute(0, 2, 1).reshape(nt, ns * nr)
+        out[k] = v.contiguous()
+    return out
+
+
+def _section_vjp(res, components):
+    """Exact adjoint of the component selection: for the pressure output
+    (components 0) both returned records are (txx + tzz)/2, so each stress
+    receives half of the sum of both adjoints."""
+    if components != 0:
+        return res
+    s = 0
+    for k in ("taux", "tauz"):
+        if k in res:
+            s = s + res[k]
+    return {"taux": s / 2, "tauz": s / 2}
+
+
+def _fold_padding(g, npml):
+    """Transpose of expand_model: sum the absorbing-layer cells onto the edge
+    cells they replicate (NumPy array or torch tensor)."""
+    if npml == 0:
+        return g
+    g = g.copy() if isinstance(g, np.ndarray) else g.clone()
+    g[:, npml] += g[:, :npml].sum(1)
+    g[:, -npml - 1] += g[:, -npml:].sum(1)
+    g[npml, :] += g[:npml, :].sum(0)
+    g[-npml - 1, :] += g[-npml:, :].sum(0)
+    out = g[npml:-npml, npml:-npml]
+    return np.ascontiguousarray(out) if isinstance(out, np.ndarray) else out.contiguous()
+
+
+def _gradient_reading_tensor(W, G):
+    """gradient_reading for device tensors (wave_propagation.py:279-331)."""
+    import torch
+    npml, sdo = int(W.npml), int(W.sdo)
+    tnz, tnx, nz, nx = int(W.tnz), int(W.tnx), int(W.nz), int(W.nx)
+    r0 = sdo + int(W.src_cts) + 2
+
+    def factor_of(pre):
+        f = torch.zeros(nz, nx, dtype=torch.float32, device=G.device)
+        if pre is None:
+            f[r0:nz - sdo, sdo:nx - sdo] = 1.0
+            return f
+        d = torch.sqrt(pre)[npml + r0:tnz - npml - sdo, npml + sdo:tnx - npml - sdo]
+        f[r0:nz - sdo, sdo:nx - sdo] = 1 / d
+        return f / f.abs().max()
+
+    eb = W.energy_balancing
+    crop = (slice(npml, tnz - npml), slice(npml, tnx - npml))
+    grho = G[4][crop] * factor_of(G[5] if eb else None)
+    gmu = G[0][crop] * factor_of(G[1] if eb else None)
+    glam = G[2][crop] * factor_of(G[3] if eb else None)
+    return glam, gmu, grho
+

@@ -1,0 +1,155 @@
+"""The Fwi autograd surface on GPU tensors (src/PyFWI/torchfwi.py:7-91).
+
+- 'pyfwi' mode: gradients equal the oracle's reference-semantics gradient
+  (<= 1e-4 relative L2) and the CPU-tensor path.
+- 'exact' mode (checkpoints every K steps, forward recomputed): the
+  directional derivative along a random smooth perturbation matches FP64
+  central differences of the same loss to 1e-4 of sum|g*dm| (FP32
+  gradients; the directional sum cancels), at eps = 1e-5; eps = 1e-3 is
+  reported with its truncation.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from pyfwi_amd import workloads
+from pyfwi_amd.torchfwi import Fwi
+from pyfwi_amd.wave_propagation import WavePropagator
+
+from _helpers import oracle_setup, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _small(nt=500, components=0, **over):
+    w = workloads.plain_small()
+    w.inpa["t"] = workloads.t_for_nt(nt, w.inpa["dt"])
+    w.inpa.update(over)
+    w.components = components
+    return w
+
+
+def _loss(taux, tauz, obs_x, obs_z):
+    return 0.5 * ((taux - obs_x) ** 2).sum() + 0.5 * ((tauz - obs_z) ** 2).sum()
+
+
+def _observed(w, dev):
+    Wo = WavePropagator(w.inpa, w.src, w.rec_loc, w.model_shape, components=w.components)
+    t = {k: torch.tensor(v, device=dev) for k, v in w.model_obs.items()}
+    with torch.no_grad():
+        ox, oz = Fwi.apply(Wo, t["vp"], t["vs"], t["rho"])
+    return ox.clone(), oz.clone()
+
+
+@pytest.mark.parametrize("shape", ["2d", "3d"])
+def test_fwi_pyfwi_mode_matches_oracle(gpu, shape):
+    w = _small(chpr=100, seimogram_shape=shape)
+    obs_x, obs_z = _observed(w, gpu)
+    W = WavePropagator(w.inpa, w.src, w.rec_loc, w.model_shape, components=0, chpr=100)
+    m = {k: torch.tensor(v, device=gpu, requires_grad=True) for k, v in w.model.items()}
+    tx, tz = Fwi.apply(W, m["vp"], m["vs"], m["rho"])
+    assert tx.is_cuda and tx.shape == obs_x.shape
+    _loss(tx, tz, obs_x, obs_z).backward()
+    # oracle: same residuals, reference gradient with the 'dv' chain rule
+    res = {"taux": (tx - obs_x).detach().cpu().numpy(), "tauz": (tz - obs_z).detach().cpu().numpy()}
+    if shape == "3d":
+        res = {k: np.reshape(v, (v.shape[0], -1), "F") for k, v in res.items()}
+    st = oracle_setup(w, chpr=100, components=0)
+    seis, states = O.forward(st, w.model, np.float32)
+    ref_sec = O.section(seis, 0, "2d")
+    got = tx.detach().cpu().numpy()
+    if shape == "3d":
+        got = np.reshape(got, (got.shape[0], -1), "F")
+    assert rel(got, ref_sec["taux"]) < 1e-5
+    gr = O.gradient(st, w.model, res, states, parameterization="dv")
+    for k in ("vp", "vs", "rho"):
+        e = rel(m[k].grad.cpu().numpy(), gr[k])
+        print(f"Fwi pyfwi-mode grad {k} ({shape}): {e:.2e}")
+        assert e < 1e-4, (k, e)
+
+
+def test_fwi_gpu_equals_cpu_path(gpu):
+    w = _small(nt=300, chpr=100)
+    outs = []
+    for dev in (gpu, torch.device("cpu")):
+        obs_x, obs_z = _observed(w, dev) if dev.type == "cuda" else (None, None)
+        if dev.type == "cpu":
+            obs_x, obs_z = outs[0][1]
+            obs_x, obs_z = obs_x.cpu(), obs_z.cpu()
+        W = WavePropagator(w.inpa, w.src, w.rec_loc, w.model_shape, components=0, chpr=100)
+        m = {k: torch.tensor(v, device=dev, requires_grad=True) for k, v in w.model.items()}
+        tx, tz = Fwi.apply(W, m["vp"], m["vs"], m["rho"])
+        _loss(tx, tz, obs_x, obs_z).backward()
+        outs.append(({k: m[k].grad.cpu().numpy() for k in m}, (obs_x, obs_z)))
+    for k in ("vp", "vs", "rho"):
+        assert rel(outs[0][0][k], outs[1][0][k]) < 1e-5
+
+
+def _smooth_direction(shape, seed, mask):
+    rng = np.random.default_rng(seed)
+    z, x = np.mgrid[0:shape[0], 0:shape[1]].astype(np.float64)
+    d = np.zeros(shape)
+    for _ in range(4):
+        kz, kx, ph = rng.uniform(0.02, 0.12), rng.uniform(0.02, 0.12), rng.uniform(0, 6.3)
+        d += rng.standard_normal() * np.cos(kz * z + kx * x + ph)
+    return d * mask
+
+
+def _fd_check(w, dev, ckpt_every, tol=1e-4):
+    """Exact-mode autograd gradient vs FP64 central differences of the same
+    loss (oracle forward) along a smooth direction that vanishes where vp is
+    maximal (the damping scales with max(vp)).
+
+    eps = 1e-3 (relative) is reported: its O(eps^2) truncation reaches
+    1e-2..1e-1 of the derivative on these long propagations (measured; the
+    error falls 100x per decade of eps).  The assertion uses eps = 1e-5, where
+    FP64 differences are accurate to ~1e-6, and checks that the eps = 1e-3
+    error is the truncation (at least 100x the eps = 1e-5 error or already
+    below the tolerance).  The FP32 gradient is compared against
+    sum |g dm| since the directional sum cancels."""
+    obs_x, obs_z = _observed(w, dev)
+    W = WavePropagator(w.inpa, w.src, w.rec_loc, w.model_shape, components=w.components)
+    W.grad_mode, W.ckpt_every = "exact", ckpt_every
+    m = {k: torch.tensor(v, device=dev, requires_grad=True) for k, v in w.model.items()}
+    tx, tz = Fwi.apply(W, m["vp"], m["vs"], m["rho"])
+    _loss(tx, tz, obs_x, obs_z).backward()
+    st = oracle_setup(w, chpr=0, components=w.components)
+    ox, oz = obs_x.cpu().numpy().astype(np.float64), obs_z.cpu().numpy().astype(np.float64)
+    mask = (w.model["vp"] < w.model["vp"].max() * 0.999).astype(np.float64)
+    for k, seed in (("vp", 1), ("vs", 2), ("rho", 3)):
+        d = _smooth_direction(w.model_shape, seed, mask)
+        gk = m[k].grad.cpu().numpy().astype(np.float64)
+        errs = {}
+        for eps in (1e-3, 1e-5):
+            dm = eps * d * w.model[k].astype(np.float64)
+
+            def L(sign):
+                m2 = {kk: w.model[kk].astype(np.float64) for kk in w.model}
+                m2[k] = m2[k] + sign * dm
+                seis, _ = O.forward(st, m2, np.float64, keep_states=False)
+                sec = O.section(seis, w.components, w.inpa.get("seimogram_shape", "2d"))
+                return (0.5 * np.sum((sec["taux"] - ox) ** 2)
+                        + 0.5 * np.sum((sec["tauz"] - oz) ** 2))
+
+            fd = (L(1) - L(-1)) / 2
+            an = float(np.sum(gk * dm))
+            scale = float(np.sum(np.abs(gk * dm)))
+            errs[eps] = abs(fd - an) / scale
+            print(f"exact-mode FD {k} eps={eps:g}: fd={fd:.6e} autograd={an:.6e} "
+                  f"err/sum|g dm|={errs[eps]:.2e} err/|fd|={abs(fd - an) / abs(fd):.2e}")
+        assert errs[1e-5] < tol, (k, errs)
+        assert errs[1e-3] < tol or errs[1e-3] > 100 * errs[1e-5], (k, errs)
+
+
+def test_fwi_exact_mode_central_differences_small(gpu):
+    w = _small(nt=400, components=0)
+    _fd_check(w, gpu, ckpt_every=37)
+
+
+def test_fwi_exact_mode_central_differences_differentiation_size(gpu):
+    """nz=300, nx=600, dh=5 m, random layers + 3 Gaussian anomalies, nt=2500,
+    checkpoints every 50 steps; 2 of the 10 shots so that the FP64 oracle
+    finishes in seconds."""
+    w = workloads.differentiation(ns=2)
+    _fd_check(w, gpu, ckpt_every=50)

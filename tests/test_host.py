@@ -221,3 +221,49 @@ def test_workloads_are_seeded():
     vp = a.model["vp"]
     assert vp.min() >= 1800 and vp.max() <= 4000
     assert a.inpa["dt"] == pytest.approx(0.45 * 5 / vp.max())
+
+
+def test_fold_chain_and_section_vjp_against_fp64_differences():
+    """Exact gradient w.r.t. the unpadded (vp, vs, rho): the oracle's FP64
+    exact gradient over the padded grid, folded onto the model cells
+    (transpose of expand_model) and chained (grad_lmd_to_vd), matches FP64
+    central differences of the pressure-record loss; exercises the product's
+    _fold_padding, _section_vjp and grad_lmd_to_vd."""
+    from oracle import oracle as O
+    from pyfwi_amd.wave_propagation import _fold_padding, _section_vjp
+    from _helpers import oracle_setup
+    w = workloads.plain_small()
+    w.inpa["t"] = workloads.t_for_nt(150, w.inpa["dt"])
+    w.inpa["npml"] = 8
+    st = oracle_setup(w, chpr=0, components=0)
+    m64 = {k: v.astype(np.float64) for k, v in w.model.items()}
+    rng = np.random.default_rng(9)
+    seis, _ = O.forward(st, m64, np.float64, keep_states=False)
+    sec = O.section(seis, 0)
+    obs = {k: v * 0.9 + 0.01 * rng.standard_normal(v.shape) * np.abs(v).max() for k, v in sec.items()}
+    res = _section_vjp({k: sec[k] - obs[k] for k in sec}, 0)
+    g3 = O.exact_gradient(st, m64, O.residual_records(st, res, np.float64), np.float64)
+    md = st.medium(m64, np.float64)
+    glam, gmu = _fold_padding(g3[0], 8), _fold_padding(g3[1], 8)
+    grho = _fold_padding(-(md["b"] ** 2) * g3[2], 8)
+    c = (slice(8, -8), slice(8, -8))
+    gvp, gvs, gr = (np.asarray(a, np.float64) for a in
+                    gs.grad_lmd_to_vd(glam, gmu, grho, md["lam"][c], md["mu"][c], md["rho"][c]))
+    mask = (w.model["vp"] < w.model["vp"].max()).astype(np.float64)
+    z, x = np.mgrid[0:100, 0:100]
+    for key, g in (("vp", gvp), ("vs", gvs), ("rho", gr)):
+        dm = 1e-3 * np.cos(0.07 * z + 0.11 * x) * mask * m64[key]
+
+        def L(sign):
+            m2 = dict(m64)
+            m2[key] = m64[key] + sign * dm
+            s2, _ = O.forward(st, m2, np.float64, keep_states=False)
+            q = O.section(s2, 0)
+            return sum(0.5 * np.sum((q[k] - obs[k]) ** 2) for k in q)
+
+        fd = (L(1) - L(-1)) / 2
+        an = np.sum(g * dm)
+        # grad_lmd_to_vd returns float32 (as the reference); the directional
+        # sum cancels heavily, so the error is measured against sum |g dm|
+        scale = np.sum(np.abs(g * dm))
+        assert abs(fd - an) < 1e-6 * scale, (key, fd, an, scale)
