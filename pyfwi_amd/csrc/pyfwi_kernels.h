@@ -11,6 +11,8 @@ namespace pyfwi {
 constexpr int HALO = 4;      // zero rows above and below each padded field
 constexpr int TILE_X = 64;   // one wavefront along x
 constexpr int TILE_Y = 4;    // 4 wavefronts per block along z
+constexpr int MARCH_X = 64;      // column-marching kernels: one wavefront of columns
+constexpr int MARCH_ROWS = 32;   // rows walked by each thread
 
 // Constants every kernel takes by value.
 struct DevGrid {

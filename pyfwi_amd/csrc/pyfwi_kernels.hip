@@ -133,92 +133,172 @@ __device__ __forceinline__ bool bwd_active(const pyfwi_shot *shots, int s, int t
     return t >= 1 && t < shots[s].nt;
 }
 
+// ------------------------------------------------------ column marching
+//
+// The stencil kernels below give each thread one column j and a strip of
+// MARCH_ROWS rows.  The thread walks down the strip keeping, for every field
+// differentiated along z, a register window of the 2*SDO+1 rows around the
+// current row, so each row of such a field is fetched from memory once.
+// Derivatives along x read the neighbouring columns of the same row; those
+// loads are coalesced across the wavefront and served by L1 / L2.
+
+template <int SDO>
+struct ZWin {
+    float v[2 * SDO + 1];  // v[SDO + k] = row r + k
+    __device__ __forceinline__ void push(float x)
+    {
+#pragma unroll
+        for (int k = 0; k < 2 * SDO; ++k) v[k] = v[k + 1];
+        v[2 * SDO] = x;
+    }
+    __device__ __forceinline__ float centre() const { return v[SDO]; }
+    // backward-staggered: pairs (0,-1) (1,-2) (2,-3) (3,-4)
+    __device__ __forceinline__ float dminus(const float c[4]) const
+    {
+        float r = c[0] * (v[SDO] - v[SDO - 1]);
+        r = fmaf(c[1], v[SDO + 1] - v[SDO - 2], r);
+        if constexpr (SDO == 4) {
+            r = fmaf(c[2], v[SDO + 2] - v[SDO - 3], r);
+            r = fmaf(c[3], v[SDO + 3] - v[SDO - 4], r);
+        }
+        return r;
+    }
+    // forward-staggered: pairs (1,0) (2,-1) (3,-2) (4,-3)
+    __device__ __forceinline__ float dplus(const float c[4]) const
+    {
+        float r = c[0] * (v[SDO + 1] - v[SDO]);
+        r = fmaf(c[1], v[SDO + 2] - v[SDO - 1], r);
+        if constexpr (SDO == 4) {
+            r = fmaf(c[2], v[SDO + 3] - v[SDO - 2], r);
+            r = fmaf(c[3], v[SDO + 4] - v[SDO - 3], r);
+        }
+        return r;
+    }
+};
+
+// Column and row strip of this thread; false when the column is inactive.
+template <int SDO>
+__device__ __forceinline__ bool strip(const DevGrid &g, int &j, int &i0, int &i1)
+{
+    j = SDO + (int)(blockIdx.x * MARCH_X + threadIdx.x);
+    i0 = SDO + (int)blockIdx.y * MARCH_ROWS;
+    i1 = min(i0 + MARCH_ROWS - 1, g.nz - SDO);
+    return j <= g.nx - SDO;
+}
+
+// Prime a window for the first row i0 of a strip: rows i0-SDO .. i0+SDO-1.
+// `val(k)` returns the windowed quantity at padded offset k.
+template <int SDO, class F>
+__device__ __forceinline__ void prime(ZWin<SDO> &w, int64_t c0, int pitch, F val)
+{
+#pragma unroll
+    for (int k = 0; k < 2 * SDO; ++k) w.v[k + 1] = val(c0 + (int64_t)(k - SDO) * pitch);
+}
+
 // ---------------------------------------------------------- forward sweeps
 
 // Velocity half step of every active shot (src/PyFWI/elastic.cl:155-226).
 //   DIR = +1: v += dt*b*D(tau); v -= dt*(dx+dz)*v   (damping after)
 //   DIR = -1: v -= dt*(dx+dz)*v; v -= dt*b*D(tau)   (time reversed)
 template <int SDO, int DIR>
-__global__ __launch_bounds__(TILE_X *TILE_Y) void k_vel(DevGrid g, const pyfwi_shot *shots,
-                                                        int t, float *fields,
-                                                        const float *__restrict__ bu,
-                                                        const float *__restrict__ dxp,
-                                                        const float *__restrict__ dzp)
+__global__ __launch_bounds__(MARCH_X) void k_vel(DevGrid g, const pyfwi_shot *shots, int t,
+                                                 float *fields, const float *__restrict__ bu,
+                                                 const float *__restrict__ dxp,
+                                                 const float *__restrict__ dzp)
 {
     const int s = blockIdx.z;
     if (DIR > 0 ? !fwd_active(shots, s, t) : !bwd_active(shots, s, t)) return;
-    int i, j;
-    if (!cell<SDO>(g, i, j)) return;
+    int j, i0, i1;
+    if (!strip<SDO>(g, j, i0, i1)) return;
     const int64_t fs = g.fstride;
+    const int P = g.pitch;
     float *F = fields + (int64_t)s * 5 * fs + g.origin;
-    const int64_t c = (int64_t)i * g.pitch + j;
     float *vx = F, *vz = F + fs;
-    const float *txx = F + 2 * fs + c, *tzz = F + 3 * fs + c, *txz = F + 4 * fs + c;
-    const float ax = bu[g.origin + c] * (dminus<SDO>(txx, 1, g.cx) + dminus<SDO>(txz, g.pitch, g.cz));
-    const float az = bu[g.origin + c] * (dplus<SDO>(tzz, g.pitch, g.cz) + dplus<SDO>(txz, 1, g.cx));
-    const float damp = dxp[j] + dzp[i];
-    float ux = vx[c], uz = vz[c];
-    if (DIR > 0) {
-        ux = fmaf(g.dt, ax, ux);
-        ux = fmaf(g.dt, -damp * ux, ux);
-        uz = fmaf(g.dt, az, uz);
-        uz = fmaf(g.dt, -damp * uz, uz);
-    } else {
-        ux = fmaf(-g.dt, damp * ux, ux);
-        ux = fmaf(-g.dt, ax, ux);
-        uz = fmaf(-g.dt, damp * uz, uz);
-        uz = fmaf(-g.dt, az, uz);
+    const float *txx = F + 2 * fs, *tzz = F + 3 * fs, *txz = F + 4 * fs;
+    const float *b = bu + g.origin;
+    ZWin<SDO> wxz, wzz;  // txz for d/dz-, tzz for d/dz+
+    int64_t c = (int64_t)i0 * P + j;
+    prime(wxz, c, P, [&](int64_t k) { return txz[k]; });
+    prime(wzz, c, P, [&](int64_t k) { return tzz[k]; });
+    const float ddx = dxp[j];
+    for (int i = i0; i <= i1; ++i, c += P) {
+        wxz.push(txz[c + (int64_t)SDO * P]);
+        wzz.push(tzz[c + (int64_t)SDO * P]);
+        const float ax = b[c] * (dminus<SDO>(txx + c, 1, g.cx) + wxz.dminus(g.cz));
+        const float az = b[c] * (wzz.dplus(g.cz) + dplus<SDO>(txz + c, 1, g.cx));
+        const float damp = ddx + dzp[i];
+        float ux = vx[c], uz = vz[c];
+        if (DIR > 0) {
+            ux = fmaf(g.dt, ax, ux);
+            ux = fmaf(g.dt, -damp * ux, ux);
+            uz = fmaf(g.dt, az, uz);
+            uz = fmaf(g.dt, -damp * uz, uz);
+        } else {
+            ux = fmaf(-g.dt, damp * ux, ux);
+            ux = fmaf(-g.dt, ax, ux);
+            uz = fmaf(-g.dt, damp * uz, uz);
+            uz = fmaf(-g.dt, az, uz);
+        }
+        vx[c] = ux;
+        vz[c] = uz;
     }
-    vx[c] = ux;
-    vz[c] = uz;
 }
 
 // Stress half step (src/PyFWI/elastic.cl:235-308).
 template <int SDO, int DIR>
-__global__ __launch_bounds__(TILE_X *TILE_Y) void k_str(DevGrid g, const pyfwi_shot *shots,
-                                                        int t, float *fields,
-                                                        const float *__restrict__ lam,
-                                                        const float *__restrict__ mu,
-                                                        const float *__restrict__ dxp,
-                                                        const float *__restrict__ dzp)
+__global__ __launch_bounds__(MARCH_X) void k_str(DevGrid g, const pyfwi_shot *shots, int t,
+                                                 float *fields, const float *__restrict__ lam,
+                                                 const float *__restrict__ mu,
+                                                 const float *__restrict__ dxp,
+                                                 const float *__restrict__ dzp)
 {
     const int s = blockIdx.z;
     if (DIR > 0 ? !fwd_active(shots, s, t) : !bwd_active(shots, s, t)) return;
-    int i, j;
-    if (!cell<SDO>(g, i, j)) return;
+    int j, i0, i1;
+    if (!strip<SDO>(g, j, i0, i1)) return;
     const int64_t fs = g.fstride;
+    const int P = g.pitch;
     float *F = fields + (int64_t)s * 5 * fs + g.origin;
-    const int64_t c = (int64_t)i * g.pitch + j;
-    const float *vx = F + c, *vz = F + fs + c;
+    const float *vx = F, *vz = F + fs;
     float *txx = F + 2 * fs, *tzz = F + 3 * fs, *txz = F + 4 * fs;
-    const float exx = dplus<SDO>(vx, 1, g.cx);
-    const float ezz = dminus<SDO>(vz, g.pitch, g.cz);
-    const float gxz = dminus<SDO>(vz, 1, g.cx) + dplus<SDO>(vx, g.pitch, g.cz);
-    const float l = lam[g.origin + c], m = mu[g.origin + c];
-    const float lm = l + 2.0f * m;
-    const float sxx = lm * exx + l * ezz;
-    const float szz = lm * ezz + l * exx;
-    const float sxz = m * gxz;
-    const float damp = dxp[j] + dzp[i];
-    float a = txx[c], b = tzz[c], d = txz[c];
-    if (DIR > 0) {
-        a = fmaf(g.dt, sxx, a);
-        b = fmaf(g.dt, szz, b);
-        d = fmaf(g.dt, sxz, d);
-        a = fmaf(g.dt, -damp * a, a);
-        b = fmaf(g.dt, -damp * b, b);
-        d = fmaf(g.dt, -damp * d, d);
-    } else {
-        a = fmaf(-g.dt, damp * a, a);
-        b = fmaf(-g.dt, damp * b, b);
-        d = fmaf(-g.dt, damp * d, d);
-        a = fmaf(-g.dt, sxx, a);
-        b = fmaf(-g.dt, szz, b);
-        d = fmaf(-g.dt, sxz, d);
+    const float *L = lam + g.origin, *M = mu + g.origin;
+    ZWin<SDO> wvz, wvx;  // vz for d/dz-, vx for d/dz+
+    int64_t c = (int64_t)i0 * P + j;
+    prime(wvz, c, P, [&](int64_t k) { return vz[k]; });
+    prime(wvx, c, P, [&](int64_t k) { return vx[k]; });
+    const float ddx = dxp[j];
+    for (int i = i0; i <= i1; ++i, c += P) {
+        wvz.push(vz[c + (int64_t)SDO * P]);
+        wvx.push(vx[c + (int64_t)SDO * P]);
+        const float exx = dplus<SDO>(vx + c, 1, g.cx);
+        const float ezz = wvz.dminus(g.cz);
+        const float gxz = dminus<SDO>(vz + c, 1, g.cx) + wvx.dplus(g.cz);
+        const float l = L[c], m = M[c];
+        const float lm = l + 2.0f * m;
+        const float sxx = lm * exx + l * ezz;
+        const float szz = lm * ezz + l * exx;
+        const float sxz = m * gxz;
+        const float damp = ddx + dzp[i];
+        float a = txx[c], bb = tzz[c], d = txz[c];
+        if (DIR > 0) {
+            a = fmaf(g.dt, sxx, a);
+            bb = fmaf(g.dt, szz, bb);
+            d = fmaf(g.dt, sxz, d);
+            a = fmaf(g.dt, -damp * a, a);
+            bb = fmaf(g.dt, -damp * bb, bb);
+            d = fmaf(g.dt, -damp * d, d);
+        } else {
+            a = fmaf(-g.dt, damp * a, a);
+            bb = fmaf(-g.dt, damp * bb, bb);
+            d = fmaf(-g.dt, damp * d, d);
+            a = fmaf(-g.dt, sxx, a);
+            bb = fmaf(-g.dt, szz, bb);
+            d = fmaf(-g.dt, sxz, d);
+        }
+        txx[c] = a;
+        tzz[c] = bb;
+        txz[c] = d;
     }
-    txx[c] = a;
-    tzz[c] = b;
-    txz[c] = d;
 }
 
 // Receiver gather then source injection of step t for every shot, one block
@@ -285,31 +365,40 @@ __global__ void k_adj_inject(DevGrid g, const pyfwi_shot *shots, int t,
 
 // Adjoint stresses from adjoint velocities (src/PyFWI/elastic.cl:372-411).
 template <int SDO>
-__global__ __launch_bounds__(TILE_X *TILE_Y) void k_adj_str(DevGrid g, const pyfwi_shot *shots,
-                                                            int t, float *adj,
-                                                            const float *__restrict__ bu,
-                                                            const float *__restrict__ dxp,
-                                                            const float *__restrict__ dzp)
+__global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot *shots, int t,
+                                                     float *adj, const float *__restrict__ bu,
+                                                     const float *__restrict__ dxp,
+                                                     const float *__restrict__ dzp)
 {
     const int s = blockIdx.z;
     if (!bwd_active(shots, s, t)) return;
-    int i, j;
-    if (!cell<SDO>(g, i, j)) return;
+    int j, i0, i1;
+    if (!strip<SDO>(g, j, i0, i1)) return;
     const int64_t fs = g.fstride;
+    const int P = g.pitch;
     float *A = adj + (int64_t)s * 5 * fs + g.origin;
-    const int64_t c = (int64_t)i * g.pitch + j;
-    const float *avx = A + c, *avz = A + fs + c;
-    const float *b = bu + g.origin + c;
+    const float *avx = A, *avz = A + fs;
+    const float *b = bu + g.origin;
     float *atxx = A + 2 * fs, *atzz = A + 3 * fs, *atxz = A + 4 * fs;
-    const float dxpb = dplus2<SDO>(avx, b, 1, g.cx);
-    const float dzmb = dminus2<SDO>(avz, b, g.pitch, g.cz);
-    const float dzpb = dplus2<SDO>(avx, b, g.pitch, g.cz);
-    const float dxmb = dminus2<SDO>(avz, b, 1, g.cx);
-    const float ex = dxp[j], ez = dzp[i];
-    const float a = atxx[c], bb = atzz[c], d = atxz[c];
-    atxx[c] = fmaf(-g.dt, ex * a + dxpb, a);
-    atzz[c] = fmaf(-g.dt, ez * bb + dzmb, bb);
-    atxz[c] = fmaf(-g.dt, (ex + ez) * d + (dzpb + dxmb), d);
+    ZWin<SDO> wbz, wbx;  // b*avz for d/dz-, b*avx for d/dz+
+    int64_t c = (int64_t)i0 * P + j;
+    prime(wbz, c, P, [&](int64_t k) { return avz[k] * b[k]; });
+    prime(wbx, c, P, [&](int64_t k) { return avx[k] * b[k]; });
+    const float ex = dxp[j];
+    for (int i = i0; i <= i1; ++i, c += P) {
+        const int64_t n = c + (int64_t)SDO * P;
+        wbz.push(avz[n] * b[n]);
+        wbx.push(avx[n] * b[n]);
+        const float dxpb = dplus2<SDO>(avx + c, b + c, 1, g.cx);
+        const float dzmb = wbz.dminus(g.cz);
+        const float dzpb = wbx.dplus(g.cz);
+        const float dxmb = dminus2<SDO>(avz + c, b + c, 1, g.cx);
+        const float ez = dzp[i];
+        const float a = atxx[c], bb = atzz[c], d = atxz[c];
+        atxx[c] = fmaf(-g.dt, ex * a + dxpb, a);
+        atzz[c] = fmaf(-g.dt, ez * bb + dzmb, bb);
+        atxz[c] = fmaf(-g.dt, (ex + ez) * d + (dzpb + dxmb), d);
+    }
 }
 
 // Adjoint velocities (src/PyFWI/elastic.cl:419-468) fused with the imaging
@@ -318,7 +407,7 @@ __global__ __launch_bounds__(TILE_X *TILE_Y) void k_adj_str(DevGrid g, const pyf
 // as the reference does (elastic.cl:485-486).  gacc: [nshots][6][fstride]
 // = gmu, gmu_pre, glam, glam_pre, grho, grho_pre.
 template <int SDO, bool PRECOND>
-__global__ __launch_bounds__(TILE_X *TILE_Y) void k_adj_vel_grad(
+__global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
     DevGrid g, const pyfwi_shot *shots, int t, const float *fields, float *adj,
     const float *__restrict__ lam, const float *__restrict__ mu,
     const float *__restrict__ bu, const float *__restrict__ dxp,
@@ -326,54 +415,81 @@ __global__ __launch_bounds__(TILE_X *TILE_Y) void k_adj_vel_grad(
 {
     const int s = blockIdx.z;
     if (!bwd_active(shots, s, t)) return;
-    int i, j;
-    if (!cell<SDO>(g, i, j)) return;
+    int j, i0, i1;
+    if (!strip<SDO>(g, j, i0, i1)) return;
     const int64_t fs = g.fstride;
+    const int P = g.pitch;
     float *A = adj + (int64_t)s * 5 * fs + g.origin;
-    const int64_t c = (int64_t)i * g.pitch + j;
-    const float *l = lam + g.origin + c, *m = mu + g.origin + c;
-    const float *atxx = A + 2 * fs + c, *atzz = A + 3 * fs + c, *atxz = A + 4 * fs + c;
-    const float a1 = dminus_m<SDO>(atxx, l, m, 1, g.cx);
-    const float a2 = dminus2<SDO>(atzz, l, 1, g.cx);
-    const float a3 = dminus2<SDO>(atxz, m, g.pitch, g.cz);
-    const float b1 = dplus_m<SDO>(atzz, l, m, g.pitch, g.cz);
-    const float b2 = dplus2<SDO>(atxx, l, g.pitch, g.cz);
-    const float b3 = dplus2<SDO>(atxz, m, 1, g.cx);
-    const float ex = dxp[j], ez = dzp[i];
-    float ux = A[c], uz = A[fs + c];
-    ux = fmaf(-g.dt, (2.0f * ex + ez) * ux + (a1 + a2 + a3), ux);
-    uz = fmaf(-g.dt, (ex + 2.0f * ez) * uz + (b2 + b1 + b3), uz);
-    A[c] = ux;
-    A[fs + c] = uz;
-
-    const int lo = g.npml + SDO;
-    if (i < lo || j < lo || i > g.nz - SDO - g.npml || j > g.nx - SDO - g.npml) return;
-    const float *F = fields + (int64_t)s * 5 * fs + g.origin + c;
+    const float *atxx = A + 2 * fs, *atzz = A + 3 * fs, *atxz = A + 4 * fs;
+    const float *L = lam + g.origin, *M = mu + g.origin;
+    // imaging condition: interior rows lo..hi and columns lo..hix only
+    const int lo = g.npml + SDO, hi = g.nz - SDO - g.npml, hix = g.nx - SDO - g.npml;
+    const bool gcol = j >= lo && j <= hix;
+    const bool grow_strip = i1 >= lo && i0 <= hi;
+    const bool img = gcol && grow_strip;
+    const float *F = fields + (int64_t)s * 5 * fs + g.origin;
     const float *vx = F, *vz = F + fs, *txx = F + 2 * fs, *tzz = F + 3 * fs, *txz = F + 4 * fs;
-    const float dzp_vz = dplus<SDO>(vz, g.pitch, g.cz);
-    const float dxm_vz = dminus<SDO>(vz, 1, g.cx);
-    const float dxp_vx = dplus<SDO>(vx, 1, g.cx);
-    const float dzm_vz = dminus<SDO>(vz, g.pitch, g.cz);
-    const float dzp_tzz = dplus<SDO>(tzz, g.pitch, g.cz);
-    const float dxm_txx = dminus<SDO>(txx, 1, g.cx);
-    const float dxp_txz = dplus<SDO>(txz, 1, g.cx);
-    const float dzm_txz = dminus<SDO>(txz, g.pitch, g.cz);
-    const float aa = atxx[0], ab = atzz[0], ad = atxz[0];
+    float *G = gacc + (int64_t)s * 6 * fs + g.origin;
+    const float *b = bu + g.origin;
+
+    ZWin<SDO> wa3, wb1, wb2;  // mu*atxz (d/dz-), (lam+2mu)*atzz, lam*atxx (d/dz+)
+    ZWin<SDO> fvz, ftzz, ftxz;  // forward vz, tzz, txz for the imaging condition
+    int64_t c = (int64_t)i0 * P + j;
+    prime(wa3, c, P, [&](int64_t k) { return M[k] * atxz[k]; });
+    prime(wb1, c, P, [&](int64_t k) { return (L[k] + 2.0f * M[k]) * atzz[k]; });
+    prime(wb2, c, P, [&](int64_t k) { return L[k] * atxx[k]; });
+    if (img) {
+        prime(fvz, c, P, [&](int64_t k) { return vz[k]; });
+        prime(ftzz, c, P, [&](int64_t k) { return tzz[k]; });
+        prime(ftxz, c, P, [&](int64_t k) { return txz[k]; });
+    }
+    const float ex = dxp[j];
     const float dt = g.dt;
-    float *G = gacc + (int64_t)s * 6 * fs + g.origin + c;
-    G[0] += 2.0f * dt * (aa * dxp_vx + ab * dzm_vz) + dt * ad * (dzp_vz + dxm_vz);
-    G[2 * fs] += dt * (aa + ab) * (dxp_vx + dzm_vz);
-    const float b = bu[g.origin + c];
-    G[4 * fs] += -(dt * (b * b)) * (ux * (dxm_txx + dzm_txz) + uz * (dzp_tzz + dxp_txz));
-    if constexpr (PRECOND) {
-        const float sxx = txx[0], szz = tzz[0], sxz = txz[0];
-        const float wx = vx[0], wz = vz[0];
-        const float dt2 = dt * dt;
-        G[fs] += 4.0f * dt2 * (sxx * sxx * dxp_vx * dxp_vx + szz * szz * dzm_vz * dzm_vz)
-                 + dt2 * sxz * sxz * (dzp_vz * dzp_vz + dxm_vz * dxm_vz);
-        G[3 * fs] += dt2 * (sxx * sxx + szz * szz) * (dxp_vx * dxp_vx + dzm_vz * dzm_vz);
-        G[5 * fs] += dt2 * (wx * wx * (dxm_txx * dxm_txx + dzm_txz * dzm_txz)
-                            + wz * wz * (dzp_tzz * dzp_tzz + dxp_txz * dxp_txz));
+    for (int i = i0; i <= i1; ++i, c += P) {
+        const int64_t n = c + (int64_t)SDO * P;
+        wa3.push(M[n] * atxz[n]);
+        wb1.push((L[n] + 2.0f * M[n]) * atzz[n]);
+        wb2.push(L[n] * atxx[n]);
+        const float a1 = dminus_m<SDO>(atxx + c, L + c, M + c, 1, g.cx);
+        const float a2 = dminus2<SDO>(atzz + c, L + c, 1, g.cx);
+        const float a3 = wa3.dminus(g.cz);
+        const float b1 = wb1.dplus(g.cz);
+        const float b2 = wb2.dplus(g.cz);
+        const float b3 = dplus2<SDO>(atxz + c, M + c, 1, g.cx);
+        const float ez = dzp[i];
+        float ux = A[c], uz = A[fs + c];
+        ux = fmaf(-dt, (2.0f * ex + ez) * ux + (a1 + a2 + a3), ux);
+        uz = fmaf(-dt, (ex + 2.0f * ez) * uz + (b2 + b1 + b3), uz);
+        A[c] = ux;
+        A[fs + c] = uz;
+        if (!img) continue;
+        fvz.push(vz[n]);
+        ftzz.push(tzz[n]);
+        ftxz.push(txz[n]);
+        if (i < lo || i > hi) continue;
+        const float dzp_vz = fvz.dplus(g.cz);
+        const float dxm_vz = dminus<SDO>(vz + c, 1, g.cx);
+        const float dxp_vx = dplus<SDO>(vx + c, 1, g.cx);
+        const float dzm_vz = fvz.dminus(g.cz);
+        const float dzp_tzz = ftzz.dplus(g.cz);
+        const float dxm_txx = dminus<SDO>(txx + c, 1, g.cx);
+        const float dxp_txz = dplus<SDO>(txz + c, 1, g.cx);
+        const float dzm_txz = ftxz.dminus(g.cz);
+        const float aa = atxx[c], ab = atzz[c], ad = atxz[c];
+        G[c] += 2.0f * dt * (aa * dxp_vx + ab * dzm_vz) + dt * ad * (dzp_vz + dxm_vz);
+        G[2 * fs + c] += dt * (aa + ab) * (dxp_vx + dzm_vz);
+        const float bb = b[c];
+        G[4 * fs + c] += -(dt * (bb * bb)) * (ux * (dxm_txx + dzm_txz) + uz * (dzp_tzz + dxp_txz));
+        if constexpr (PRECOND) {
+            const float sxx = txx[c], szz = ftzz.centre(), sxz = ftxz.centre();
+            const float wx = vx[c], wz = fvz.centre();
+            const float dt2 = dt * dt;
+            G[fs + c] += 4.0f * dt2 * (sxx * sxx * dxp_vx * dxp_vx + szz * szz * dzm_vz * dzm_vz)
+                         + dt2 * sxz * sxz * (dzp_vz * dzp_vz + dxm_vz * dxm_vz);
+            G[3 * fs + c] += dt2 * (sxx * sxx + szz * szz) * (dxp_vx * dxp_vx + dzm_vz * dzm_vz);
+            G[5 * fs + c] += dt2 * (wx * wx * (dxm_txx * dxm_txx + dzm_txz * dzm_txz)
+                                    + wz * wz * (dzp_tzz * dzp_tzz + dxp_txz * dxp_txz));
+        }
     }
 }
 
@@ -600,6 +716,13 @@ static dim3 stencil_grid(const DevGrid &g, int sdo, int nshots)
     return dim3((nx_act + TILE_X - 1) / TILE_X, (nz_act + TILE_Y - 1) / TILE_Y, nshots);
 }
 
+// One thread per active column, one block row per strip of MARCH_ROWS rows.
+static dim3 march_grid(const DevGrid &g, int sdo, int nshots)
+{
+    const int nx_act = g.nx - 2 * sdo + 1, nz_act = g.nz - 2 * sdo + 1;
+    return dim3((nx_act + MARCH_X - 1) / MARCH_X, (nz_act + MARCH_ROWS - 1) / MARCH_ROWS, nshots);
+}
+
 #define DISPATCH_SDO(sdo, KERNEL, ...)                                    \
     do {                                                                  \
         if ((sdo) == 2) KERNEL<2 __VA_ARGS__><<<grid, block, 0, st>>>(ARGS); \
@@ -610,7 +733,7 @@ void launch_vel(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, 
                 float *fields, const float *bu, const float *dxp, const float *dzp,
                 hipStream_t st)
 {
-    dim3 grid = stencil_grid(g, g.sdo, nshots), block(TILE_X, TILE_Y);
+    dim3 grid = march_grid(g, g.sdo, nshots), block(MARCH_X);
 #define ARGS g, shots, t, fields, bu, dxp, dzp
     if (dir > 0) DISPATCH_SDO(g.sdo, k_vel, , 1);
     else DISPATCH_SDO(g.sdo, k_vel, , -1);
@@ -621,7 +744,7 @@ void launch_str(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, 
                 float *fields, const float *lam, const float *mu, const float *dxp,
                 const float *dzp, hipStream_t st)
 {
-    dim3 grid = stencil_grid(g, g.sdo, nshots), block(TILE_X, TILE_Y);
+    dim3 grid = march_grid(g, g.sdo, nshots), block(MARCH_X);
 #define ARGS g, shots, t, fields, lam, mu, dxp, dzp
     if (dir > 0) DISPATCH_SDO(g.sdo, k_str, , 1);
     else DISPATCH_SDO(g.sdo, k_str, , -1);
@@ -645,7 +768,7 @@ void launch_adj_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t
                     float *adj, const float *bu, const float *dxp, const float *dzp,
                     hipStream_t st)
 {
-    dim3 grid = stencil_grid(g, g.sdo, nshots), block(TILE_X, TILE_Y);
+    dim3 grid = march_grid(g, g.sdo, nshots), block(MARCH_X);
 #define ARGS g, shots, t, adj, bu, dxp, dzp
     DISPATCH_SDO(g.sdo, k_adj_str, );
 #undef ARGS
@@ -656,7 +779,7 @@ void launch_adj_vel_grad(const DevGrid &g, int nshots, const pyfwi_shot *shots, 
                          const float *mu, const float *bu, const float *dxp,
                          const float *dzp, float *gacc, bool precond, hipStream_t st)
 {
-    dim3 grid = stencil_grid(g, g.sdo, nshots), block(TILE_X, TILE_Y);
+    dim3 grid = march_grid(g, g.sdo, nshots), block(MARCH_X);
 #define ARGS g, shots, t, fields, adj, lam, mu, bu, dxp, dzp, gacc
     if (precond) DISPATCH_SDO(g.sdo, k_adj_vel_grad, , true);
     else DISPATCH_SDO(g.sdo, k_adj_vel_grad, , false);
