@@ -11,8 +11,11 @@ namespace pyfwi {
 constexpr int HALO = 4;      // zero rows above and below each padded field
 constexpr int TILE_X = 64;   // one wavefront along x
 constexpr int TILE_Y = 4;    // 4 wavefronts per block along z
-constexpr int MARCH_X = 64;      // column-marching kernels: one wavefront of columns
-constexpr int MARCH_ROWS = 32;   // rows walked by each thread
+constexpr int MARCH_X = 256;        // column-marching kernels: columns per block (4 waves)
+constexpr int MARCH_ROWS_MIN = 8;   // strip height bounds (rows walked by a thread)
+constexpr int MARCH_ROWS_MAX = 64;
+constexpr int TARGET_WAVES = 8192;  // wavefronts per launch to aim for (32 per CU)
+constexpr int NUM_XCD = 8;          // MI355X: 8 XCDs, each with its own L2
 
 // Constants every kernel takes by value.
 struct DevGrid {
@@ -24,6 +27,8 @@ struct DevGrid {
     int64_t origin;      // offset of cell (0, 0) inside a padded field
     float cx[4], cz[4];  // c_k / dx, c_k / dz
     float dt;
+    // set per launch by the column-marching launchers
+    int mrows = 0, ncb = 0, nstrips = 0, nbatch = 0;
 };
 
 void launch_vel(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,

@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "../../include/pyfwi_hip.h"
 #include "pyfwi_kernels.h"
 
@@ -136,7 +138,7 @@ __device__ __forceinline__ bool bwd_active(const pyfwi_shot *shots, int s, int t
 // ------------------------------------------------------ column marching
 //
 // The stencil kernels below give each thread one column j and a strip of
-// MARCH_ROWS rows.  The thread walks down the strip keeping, for every field
+// rows (height chosen per launch, see march_grid).  The thread walks down the strip keeping, for every field
 // differentiated along z, a register window of the 2*SDO+1 rows around the
 // current row, so each row of such a field is fetched from memory once.
 // Derivatives along x read the neighbouring columns of the same row; those
@@ -176,13 +178,28 @@ struct ZWin {
     }
 };
 
-// Column and row strip of this thread; false when the column is inactive.
+// Tile of this block and column / row strip of this thread.
+//
+// Launches are 1-D.  The tiles (column block, row strip, shot) are numbered
+// column block fastest, then strip, then shot; workgroup w runs tile
+// (w % 8) * ceil(T/8) + w / 8.  Workgroups are handed to the 8 XCDs in turn,
+// so each XCD receives one contiguous run of tiles and neighbouring tiles,
+// which share halo rows and cache lines, meet in the same L2.
 template <int SDO>
-__device__ __forceinline__ bool strip(const DevGrid &g, int &j, int &i0, int &i1)
+__device__ __forceinline__ bool tile(const DevGrid &g, int &s, int &j, int &i0, int &i1)
 {
-    j = SDO + (int)(blockIdx.x * MARCH_X + threadIdx.x);
-    i0 = SDO + (int)blockIdx.y * MARCH_ROWS;
-    i1 = min(i0 + MARCH_ROWS - 1, g.nz - SDO);
+    const int T = g.ncb * g.nstrips * g.nbatch;
+    const int per = (T + NUM_XCD - 1) / NUM_XCD;
+    const int w = (int)blockIdx.x;
+    const int logical = (w % NUM_XCD) * per + w / NUM_XCD;
+    if (logical >= T) return false;
+    const int cb = logical % g.ncb;
+    const int rest = logical / g.ncb;
+    const int st = rest % g.nstrips;
+    s = rest / g.nstrips;
+    j = SDO + cb * MARCH_X + (int)threadIdx.x;
+    i0 = SDO + st * g.mrows;
+    i1 = min(i0 + g.mrows - 1, g.nz - SDO);
     return j <= g.nx - SDO;
 }
 
@@ -206,10 +223,9 @@ __global__ __launch_bounds__(MARCH_X) void k_vel(DevGrid g, const pyfwi_shot *sh
                                                  const float *__restrict__ dxp,
                                                  const float *__restrict__ dzp)
 {
-    const int s = blockIdx.z;
+    int s, j, i0, i1;
+    if (!tile<SDO>(g, s, j, i0, i1)) return;
     if (DIR > 0 ? !fwd_active(shots, s, t) : !bwd_active(shots, s, t)) return;
-    int j, i0, i1;
-    if (!strip<SDO>(g, j, i0, i1)) return;
     const int64_t fs = g.fstride;
     const int P = g.pitch;
     float *F = fields + (int64_t)s * 5 * fs + g.origin;
@@ -252,10 +268,9 @@ __global__ __launch_bounds__(MARCH_X) void k_str(DevGrid g, const pyfwi_shot *sh
                                                  const float *__restrict__ dxp,
                                                  const float *__restrict__ dzp)
 {
-    const int s = blockIdx.z;
+    int s, j, i0, i1;
+    if (!tile<SDO>(g, s, j, i0, i1)) return;
     if (DIR > 0 ? !fwd_active(shots, s, t) : !bwd_active(shots, s, t)) return;
-    int j, i0, i1;
-    if (!strip<SDO>(g, j, i0, i1)) return;
     const int64_t fs = g.fstride;
     const int P = g.pitch;
     float *F = fields + (int64_t)s * 5 * fs + g.origin;
@@ -370,10 +385,9 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot
                                                      const float *__restrict__ dxp,
                                                      const float *__restrict__ dzp)
 {
-    const int s = blockIdx.z;
+    int s, j, i0, i1;
+    if (!tile<SDO>(g, s, j, i0, i1)) return;
     if (!bwd_active(shots, s, t)) return;
-    int j, i0, i1;
-    if (!strip<SDO>(g, j, i0, i1)) return;
     const int64_t fs = g.fstride;
     const int P = g.pitch;
     float *A = adj + (int64_t)s * 5 * fs + g.origin;
@@ -413,10 +427,9 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
     const float *__restrict__ bu, const float *__restrict__ dxp,
     const float *__restrict__ dzp, float *gacc)
 {
-    const int s = blockIdx.z;
+    int s, j, i0, i1;
+    if (!tile<SDO>(g, s, j, i0, i1)) return;
     if (!bwd_active(shots, s, t)) return;
-    int j, i0, i1;
-    if (!strip<SDO>(g, j, i0, i1)) return;
     const int64_t fs = g.fstride;
     const int P = g.pitch;
     float *A = adj + (int64_t)s * 5 * fs + g.origin;
@@ -716,11 +729,23 @@ static dim3 stencil_grid(const DevGrid &g, int sdo, int nshots)
     return dim3((nx_act + TILE_X - 1) / TILE_X, (nz_act + TILE_Y - 1) / TILE_Y, nshots);
 }
 
-// One thread per active column, one block row per strip of MARCH_ROWS rows.
-static dim3 march_grid(const DevGrid &g, int sdo, int nshots)
+// 1-D grid of (column block, row strip, shot) tiles, XCD-interleaved (see
+// tile()).  The strip height is chosen per launch so that a launch carries
+// about TARGET_WAVES wavefronts: tall strips (fewer halo rows) when many shots
+// are batched, short strips when a single shot must still fill the GPU.
+static dim3 march_grid(DevGrid &g, int sdo, int nshots)
 {
     const int nx_act = g.nx - 2 * sdo + 1, nz_act = g.nz - 2 * sdo + 1;
-    return dim3((nx_act + MARCH_X - 1) / MARCH_X, (nz_act + MARCH_ROWS - 1) / MARCH_ROWS, nshots);
+    g.ncb = (nx_act + MARCH_X - 1) / MARCH_X;
+    const int64_t waves_per_row = (int64_t)((nx_act + 63) / 64) * nshots;
+    int rows = (int)((waves_per_row * nz_act + TARGET_WAVES - 1) / TARGET_WAVES);
+    rows = std::max(MARCH_ROWS_MIN, std::min(MARCH_ROWS_MAX, rows));
+    g.mrows = rows;
+    g.nstrips = (nz_act + rows - 1) / rows;
+    g.nbatch = nshots;
+    const int T = g.ncb * g.nstrips * nshots;
+    const int per = (T + NUM_XCD - 1) / NUM_XCD;
+    return dim3(per * NUM_XCD);
 }
 
 #define DISPATCH_SDO(sdo, KERNEL, ...)                                    \
@@ -733,8 +758,9 @@ void launch_vel(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, 
                 float *fields, const float *bu, const float *dxp, const float *dzp,
                 hipStream_t st)
 {
-    dim3 grid = march_grid(g, g.sdo, nshots), block(MARCH_X);
-#define ARGS g, shots, t, fields, bu, dxp, dzp
+    DevGrid gm = g;
+    dim3 grid = march_grid(gm, g.sdo, nshots), block(MARCH_X);
+#define ARGS gm, shots, t, fields, bu, dxp, dzp
     if (dir > 0) DISPATCH_SDO(g.sdo, k_vel, , 1);
     else DISPATCH_SDO(g.sdo, k_vel, , -1);
 #undef ARGS
@@ -744,8 +770,9 @@ void launch_str(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, 
                 float *fields, const float *lam, const float *mu, const float *dxp,
                 const float *dzp, hipStream_t st)
 {
-    dim3 grid = march_grid(g, g.sdo, nshots), block(MARCH_X);
-#define ARGS g, shots, t, fields, lam, mu, dxp, dzp
+    DevGrid gm = g;
+    dim3 grid = march_grid(gm, g.sdo, nshots), block(MARCH_X);
+#define ARGS gm, shots, t, fields, lam, mu, dxp, dzp
     if (dir > 0) DISPATCH_SDO(g.sdo, k_str, , 1);
     else DISPATCH_SDO(g.sdo, k_str, , -1);
 #undef ARGS
@@ -768,8 +795,9 @@ void launch_adj_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t
                     float *adj, const float *bu, const float *dxp, const float *dzp,
                     hipStream_t st)
 {
-    dim3 grid = march_grid(g, g.sdo, nshots), block(MARCH_X);
-#define ARGS g, shots, t, adj, bu, dxp, dzp
+    DevGrid gm = g;
+    dim3 grid = march_grid(gm, g.sdo, nshots), block(MARCH_X);
+#define ARGS gm, shots, t, adj, bu, dxp, dzp
     DISPATCH_SDO(g.sdo, k_adj_str, );
 #undef ARGS
 }
@@ -779,8 +807,9 @@ void launch_adj_vel_grad(const DevGrid &g, int nshots, const pyfwi_shot *shots, 
                          const float *mu, const float *bu, const float *dxp,
                          const float *dzp, float *gacc, bool precond, hipStream_t st)
 {
-    dim3 grid = march_grid(g, g.sdo, nshots), block(MARCH_X);
-#define ARGS g, shots, t, fields, adj, lam, mu, bu, dxp, dzp, gacc
+    DevGrid gm = g;
+    dim3 grid = march_grid(gm, g.sdo, nshots), block(MARCH_X);
+#define ARGS gm, shots, t, fields, adj, lam, mu, bu, dxp, dzp, gacc
     if (precond) DISPATCH_SDO(g.sdo, k_adj_vel_grad, , true);
     else DISPATCH_SDO(g.sdo, k_adj_vel_grad, , false);
 #undef ARGS
