@@ -267,3 +267,44 @@ def test_fold_chain_and_section_vjp_against_fp64_differences():
         # sum cancels heavily, so the error is measured against sum |g dm|
         scale = np.sum(np.abs(g * dm))
         assert abs(fd - an) < 1e-6 * scale, (key, fd, an, scale)
+
+
+def test_pcs_parameterisations_match_reference_vectors():
+    """PCS chain rules and the rock-physics relations they use, against
+    input/output vectors written from the reference (tests/golden/
+    make_pcs_vectors.py -> pcs_vectors.npz)."""
+    import os
+    from pyfwi_amd import rock_physics as rp
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "pcs_vectors.npz"))
+    pcs = {k: g["in_" + k] for k in ("phi", "cc", "sw")}
+    gdv = {k: g["in_g" + k] for k in ("vp", "vs", "rho")}
+    rock = {"k_q": 37.0, "k_c": 21.0, "k_w": 2.25, "k_h": 0.8, "mu_q": 44.0, "mu_c": 7.0,
+            "rho_q": 2.65, "rho_c": 2.55, "rho_w": 1.0, "rho_h": 0.7, "cs": 20.0}
+    close = lambda a, b: np.testing.assert_allclose(np.asarray(a, np.float64), b, rtol=1e-6, atol=0)
+    for method in ("VRH", "Voigt", "Reuss"):
+        dv = rp.pcs2dv_gassmann(pcs, rock, method)
+        for k in dv:
+            close(dv[k], g[f"pcs2dv_gassmann_{method}_{k}"])
+        out = gs.grad_dv2pcs_gassmann(gdv, rock, pcs, method)
+        for k in out:
+            close(out[k], g[f"grad_dv2pcs_gassmann_{method}_{k}"])
+        close(gs.grad_vrh(rock["k_c"], rock["k_q"], pcs["cc"], method), g[f"grad_vrh_{method}"])
+        close(rp.vrh(rock["k_c"], rock["k_q"], pcs["cc"], method), g[f"vrh_{method}"])
+    for k, v in rp.pcs2dv_vrh(pcs, rock).items():
+        close(v, g["pcs2dv_vrh_" + k])
+    for k, v in rp.pcs2dv_han(pcs, rock).items():
+        close(v, g["pcs2dv_han_" + k])
+    for k, v in gs.grad_dv2pcs_han(gdv, rock, pcs).items():
+        close(v, g["grad_dv2pcs_han_" + k])
+    kd, gd = rp.drained_moduli(pcs["phi"], 30.0, 20.0, rock["cs"])
+    close(kd, g["drained_kd"]); close(gd, g["drained_gd"])
+    for a, b in zip(rp.biot_gassmann(pcs["phi"], 2.0, 30.0, kd), ("ku", "C", "M")):
+        close(a, g["biot_" + b])
+    assert int(g["grad_dv2pcs_vrh_raises"]) == 1
+    with pytest.raises(TypeError):
+        gs.grad_dv2pcs_vrh(gdv, rock, pcs)
+    out = gs.grad_dv2pcs_vrh(gdv, rock, pcs, method="VRH")
+    assert set(out) == {"phi", "cc", "sw"}
+    par = gs.PcsParameterization(grad_args={"method": "VRH"}, model_args={"method": "VRH"})
+    for k, v in par.grad_dv2pcs(gdv, rock, pcs).items():
+        close(v, g[f"grad_dv2pcs_gassmann_VRH_{k}"])
