@@ -40,6 +40,8 @@ KERNEL_BYTES = {
     "adj_vel_grad": (36, 48),  # all: R 3 atau lam mu 2 av, W 2 av; interior: R 5 fwd b 3 acc, W 3 acc
     "exact_str": (48, 0),
     "exact_vel": (52, 0),
+    "fwd_step": (52, 0),       # R 5 fields lam mu b, W 5 fields
+    "rev_step": (52, 0),
 }
 
 
@@ -124,19 +126,15 @@ def gradient_step(W, b, shots=None, seis=None, obs=None, res=None):
     plan.gradient(shots, b["src"], res, b["grad"], _native.MODE_PYFWI, 0, 0, _native.MEM_DEVICE)
 
 
-def per_shot_rate(W, b, torch, nsub):
-    """The plain per-shot path: shots one after another through the same
-    kernels (the reference's schedule), on the first `nsub` shots."""
-    import copy
+def per_shot_rate(W, b, torch, nsub, fused):
+    """The plain per-shot path: the same kernels on one shot at a time (one
+    shot per pass, the reference's schedule), on the first `nsub` shots."""
     cells = int(W.tnz) * int(W.tnx)
-    nrec1 = 5 * int(W.nt_ext) * int(W.nr)
+    plan = b["plan"]
+    plan.set_tuning(1, fused, 0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for s in range(nsub):
-        sh = copy.copy(b["shots"][s])
-        sh.rec_offset = 0
-        gradient_step(W, b, [sh], b["seis"][:nrec1], b["obs"][s * nrec1:(s + 1) * nrec1],
-                      b["res"][:nrec1])
+    gradient_step(W, b, b["shots"][:nsub])
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     return nsub * cells * W.nt / dt / 1e9, dt
@@ -216,6 +214,8 @@ def main():
     ap.add_argument("--cpu-steps-1t", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the execution-shape autotuner")
+    ap.add_argument("--tune", default="", help="fixed tuning 'shots_per_pass,fused,rows'")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -236,6 +236,19 @@ def main():
     interior = (int(W.tnz) - 2 * (npml + sdo) + 1) * (int(W.tnx) - 2 * (npml + sdo) + 1)
     units = int(W.ns) * cells * W.nt
     log(f"bench: {w.name}: {int(W.ns)} shots x {W.nt} steps x {cells} cells")
+
+    tuning = (0, True, 0)
+    if args.tune:
+        g, f, r = (int(x) for x in args.tune.split(","))
+        tuning = (g, bool(f), r)
+        b["plan"].set_tuning(*tuning)
+    elif not args.no_tune:
+        from pyfwi_amd.tuning import autotune_gradient
+        tuning, _ = autotune_gradient(b["plan"], b["shots"], b["src"], b["seis"], b["res"],
+                                      b["grad"], torch.cuda.synchronize, log=log)
+    else:
+        b["plan"].set_tuning(*tuning)
+    log(f"bench: tuning shots/pass={tuning[0]} fused={int(tuning[1])} rows={tuning[2]}")
 
     for i in range(args.warmup):
         gradient_step(W, b)
@@ -265,7 +278,8 @@ def main():
 
     ps = None
     if args.per_shot_sample > 0:
-        ps_rate, ps_time = per_shot_rate(W, b, torch, args.per_shot_sample)
+        ps_rate, ps_time = per_shot_rate(W, b, torch, args.per_shot_sample, tuning[1])
+        b["plan"].set_tuning(*tuning)
         log(f"bench: per-shot path {ps_rate:.2f} Gcell/s")
         ps = {"value": round(ps_rate, 3), "shots_sampled": args.per_shot_sample,
               "batched_over_per_shot": round(value / ps_rate, 2)}
@@ -295,7 +309,9 @@ def main():
         "config": {"workload": w.name, "nz": int(W.nz), "nx": int(W.nx), "npml": int(W.npml),
                    "cells": cells, "shots": int(W.ns), "nt": int(W.nt), "receivers": int(W.nr),
                    "order": int(W.sdo) * 2, "gradient": "reference adjoint, chpr=0",
-                   "parallelism": "single GPU, shots batched in one launch"},
+                   "parallelism": "single GPU, shots batched per launch",
+                   "tuning": {"shots_per_pass": tuning[0], "fused_steps": bool(tuning[1]),
+                              "strip_rows": tuning[2]}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),

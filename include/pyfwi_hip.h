@@ -149,6 +149,16 @@ int32_t pyfwi_gradient(pyfwi_plan *plan, const pyfwi_shot *shots,
                        float *grad, int32_t mode, int32_t ckpt_every,
                        int32_t flags, int32_t mem);
 
+/* Execution shape of later calls (no counterpart in the reference, which runs
+ * one shot at a time): shots_per_pass > 0 runs a call's shots in passes of
+ * that many shots (0: all in one pass; smaller passes need less memory and
+ * can keep the working set in the 256 MB on-package cache); fused != 0 does
+ * each time step as one fused launch (ping-pong buffers) instead of two
+ * in-place half-step launches; strip_rows > 0 fixes the rows each thread
+ * walks (0: chosen per launch).  Results do not depend on these choices. */
+int32_t pyfwi_plan_set_tuning(pyfwi_plan *plan, int32_t shots_per_pass, int32_t fused,
+                              int32_t strip_rows);
+
 /* Copy kept snapshot `slot` of shot `shot` ([5][tnz*tnx]: vx, vz, txx, tzz,
  * txz) to `out` (the reference's WavePropagator.W, wave_propagation.py:150-156,
  * 710-727). */
@@ -171,7 +181,9 @@ enum {
     PYFWI_K_ADJ_VEL = 5,   /* adjoint velocities + imaging condition  */
     PYFWI_K_EXA_STR = 6,   /* exact adjoint of the stress half step   */
     PYFWI_K_EXA_VEL = 7,   /* exact adjoint of the velocity half step */
-    PYFWI_K_COUNT = 8
+    PYFWI_K_FWD_STEP = 8,  /* fused forward step (velocities + stresses) */
+    PYFWI_K_REV_STEP = 9,  /* fused time-reversed step                   */
+    PYFWI_K_COUNT = 10
 };
 int32_t pyfwi_plan_set_timing(pyfwi_plan *plan, int32_t enable);
 int32_t pyfwi_kernel_timing(pyfwi_plan *plan, int32_t kind, double *total_ms,

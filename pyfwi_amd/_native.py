@@ -17,15 +17,16 @@ MODE_PYFWI = 0
 MODE_EXACT = 1
 F_PRECOND = 1
 
-K_FWD_VEL, K_FWD_STR, K_REV_STR, K_REV_VEL, K_ADJ_STR, K_ADJ_VEL, K_EXA_STR, K_EXA_VEL = range(8)
+(K_FWD_VEL, K_FWD_STR, K_REV_STR, K_REV_VEL, K_ADJ_STR, K_ADJ_VEL, K_EXA_STR, K_EXA_VEL,
+ K_FWD_STEP, K_REV_STEP) = range(10)
 KERNEL_NAMES = ["fwd_vel", "fwd_str", "rev_str", "rev_vel", "adj_str", "adj_vel_grad",
-                "exact_str", "exact_vel"]
+                "exact_str", "exact_vel", "fwd_step", "rev_step"]
 
 # exported symbols, in header order (checked by the CPU test suite)
 EXPORTS = [
     "pyfwi_version", "pyfwi_last_error", "pyfwi_plan_create", "pyfwi_plan_destroy",
     "pyfwi_plan_set_stream", "pyfwi_set_model", "pyfwi_forward", "pyfwi_gradient",
-    "pyfwi_get_snapshot", "pyfwi_synchronize", "pyfwi_plan_set_timing",
+    "pyfwi_plan_set_tuning", "pyfwi_get_snapshot", "pyfwi_synchronize", "pyfwi_plan_set_timing",
     "pyfwi_kernel_timing",
 ]
 
@@ -80,6 +81,7 @@ def load(path=None):
     lib.pyfwi_forward.argtypes = [P, ctypes.POINTER(Shot), i32, fp, fp,
                                   ctypes.POINTER(ctypes.c_int32), i32, i32, i32]
     lib.pyfwi_gradient.argtypes = [P, ctypes.POINTER(Shot), i32, fp, fp, fp, i32, i32, i32, i32]
+    lib.pyfwi_plan_set_tuning.argtypes = [P, i32, i32, i32]
     lib.pyfwi_get_snapshot.argtypes = [P, i32, i32, fp, i32]
     lib.pyfwi_synchronize.argtypes = [P]
     lib.pyfwi_plan_set_timing.argtypes = [P, i32]
@@ -147,6 +149,10 @@ class Plan:
         check(self.lib.pyfwi_gradient(self.h, arr, len(shots),
                                       ptr(src) if src is not None else None, ptr(res),
                                       ptr(grad), mode, int(ckpt_every), flags, mem))
+
+    def set_tuning(self, shots_per_pass=0, fused=True, strip_rows=0):
+        check(self.lib.pyfwi_plan_set_tuning(self.h, int(shots_per_pass), 1 if fused else 0,
+                                             int(strip_rows)))
 
     def snapshot(self, shot, slot, out, mem):
         check(self.lib.pyfwi_get_snapshot(self.h, shot, slot, ptr(out), mem))

@@ -8,7 +8,7 @@
 
 namespace pyfwi {
 
-constexpr int HALO = 4;      // zero rows above and below each padded field
+constexpr int HALO = 8;      // zero rows above and below each padded field (>= 2*SDO)
 constexpr int TILE_X = 64;   // one wavefront along x
 constexpr int TILE_Y = 4;    // 4 wavefronts per block along z
 constexpr int MARCH_X = 256;        // column-marching kernels: columns per block (4 waves)
@@ -16,6 +16,10 @@ constexpr int MARCH_ROWS_MIN = 8;   // strip height bounds (rows walked by a thr
 constexpr int MARCH_ROWS_MAX = 64;
 constexpr int TARGET_WAVES = 8192;  // wavefronts per launch to aim for (32 per CU)
 constexpr int NUM_XCD = 8;          // MI355X: 8 XCDs, each with its own L2
+constexpr int FUSED_WAVES = 4;      // fused two-stage kernels: wavefronts per block
+#ifndef MARCH_UNROLL
+#define MARCH_UNROLL 1              // row-loop unrolling (register pressure outweighs it on gfx950)
+#endif
 
 // Constants every kernel takes by value.
 struct DevGrid {
@@ -29,11 +33,17 @@ struct DevGrid {
     float dt;
     // set per launch by the column-marching launchers
     int mrows = 0, ncb = 0, nstrips = 0, nbatch = 0;
+    int rows_override = 0;  // strip height chosen by the caller (0: automatic)
 };
 
 void launch_vel(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
                 float *fields, const float *bu, const float *dxp, const float *dzp,
                 hipStream_t st);
+// Fused velocity + stress step (forward, DIR=+1) or stress + velocity step
+// (time reversed, DIR=-1): reads `in`, writes `out` (both [nshots][5][fs]).
+void launch_fused_step(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
+                       const float *in, float *out, const float *bu, const float *lam,
+                       const float *mu, const float *dxp, const float *dzp, hipStream_t st);
 void launch_str(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
                 float *fields, const float *lam, const float *mu, const float *dxp,
                 const float *dzp, hipStream_t st);
@@ -61,7 +71,7 @@ void launch_exact_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, 
 void launch_copy_shots(float *dst, int64_t dst_stride, const float *src, int64_t src_stride,
                        int64_t count, int nshots, hipStream_t st);
 void launch_grad_sum(const DevGrid &g, int nshots, int nslot, const float *gacc, float *out,
-                     const float *bu, int b_to_rho, hipStream_t st);
+                     const float *bu, int b_to_rho, int accumulate, hipStream_t st);
 void launch_pad(const DevGrid &g, const float *src, float *dst, hipStream_t st);
 void launch_unpad(const DevGrid &g, const float *src, float *dst, int nf, hipStream_t st);
 

@@ -237,6 +237,7 @@ __global__ __launch_bounds__(MARCH_X) void k_vel(DevGrid g, const pyfwi_shot *sh
     prime(wxz, c, P, [&](int64_t k) { return txz[k]; });
     prime(wzz, c, P, [&](int64_t k) { return tzz[k]; });
     const float ddx = dxp[j];
+#pragma unroll MARCH_UNROLL
     for (int i = i0; i <= i1; ++i, c += P) {
         wxz.push(txz[c + (int64_t)SDO * P]);
         wzz.push(tzz[c + (int64_t)SDO * P]);
@@ -282,6 +283,7 @@ __global__ __launch_bounds__(MARCH_X) void k_str(DevGrid g, const pyfwi_shot *sh
     prime(wvz, c, P, [&](int64_t k) { return vz[k]; });
     prime(wvx, c, P, [&](int64_t k) { return vx[k]; });
     const float ddx = dxp[j];
+#pragma unroll MARCH_UNROLL
     for (int i = i0; i <= i1; ++i, c += P) {
         wvz.push(vz[c + (int64_t)SDO * P]);
         wvx.push(vx[c + (int64_t)SDO * P]);
@@ -313,6 +315,254 @@ __global__ __launch_bounds__(MARCH_X) void k_str(DevGrid g, const pyfwi_shot *sh
         txx[c] = a;
         tzz[c] = bb;
         txz[c] = d;
+    }
+}
+
+// ------------------------------------------------ fused two-stage steps
+//
+// One launch performs both half steps of a time step, reading the state from
+// `in` and writing the new state to `out`, so every field is read once and
+// written once (52 B per cell instead of 72 B for two launches).
+//
+// Stage 1 updates field group X at row r; stage 2 updates group Y at row
+// q = r - SDO from the NEW X values, whose z-neighbours sit in a register
+// window and whose x-neighbours come from the adjacent lanes by shuffle.
+// A wavefront covers 64 columns; its SDO lanes on each side only recompute
+// stage-1 values for their neighbours, the 64 - 2*SDO lanes in between own
+// their column and write.  Each strip also recomputes stage 1 on the SDO rows
+// above and below it.  Cells outside the active range stay zero.
+
+template <int SDO>
+struct FusedTile {
+    int s, col, i0, i1;
+    bool own, colact;
+};
+
+template <int SDO>
+__device__ __forceinline__ bool fused_tile(const DevGrid &g, FusedTile<SDO> &ft)
+{
+    const int T = g.ncb * g.nstrips * g.nbatch;
+    const int per = (T + NUM_XCD - 1) / NUM_XCD;
+    const int w = (int)blockIdx.x;
+    const int logical = (w % NUM_XCD) * per + w / NUM_XCD;
+    if (logical >= T) return false;
+    const int cb = logical % g.ncb;
+    const int rest = logical / g.ncb;
+    const int st = rest % g.nstrips;
+    ft.s = rest / g.nstrips;
+    const int wave = (int)threadIdx.x / 64, lane = (int)threadIdx.x % 64;
+    const int span = 64 - 2 * SDO;
+    const int x0 = SDO + (cb * FUSED_WAVES + wave) * span;  // first owned column
+    if (x0 > g.nx - SDO) return false;                      // whole wavefront idle
+    ft.col = x0 - SDO + lane;
+    ft.colact = ft.col >= SDO && ft.col <= g.nx - SDO;
+    ft.own = lane >= SDO && lane < 64 - SDO && ft.colact;
+    ft.i0 = SDO + st * g.mrows;
+    ft.i1 = min(ft.i0 + g.mrows - 1, g.nz - SDO);
+    return true;
+}
+
+// x-derivatives from the neighbouring lanes' values (same row).
+template <int SDO>
+__device__ __forceinline__ float lane_dminus(float v, int lane, const float c[4])
+{
+    auto at = [&](int k) { return __shfl(v, lane + k, 64); };
+    float r = c[0] * (v - at(-1));
+    r = fmaf(c[1], at(1) - at(-2), r);
+    if constexpr (SDO == 4) {
+        r = fmaf(c[2], at(2) - at(-3), r);
+        r = fmaf(c[3], at(3) - at(-4), r);
+    }
+    return r;
+}
+
+template <int SDO>
+__device__ __forceinline__ float lane_dplus(float v, int lane, const float c[4])
+{
+    auto at = [&](int k) { return __shfl(v, lane + k, 64); };
+    float r = c[0] * (at(1) - v);
+    r = fmaf(c[1], at(2) - at(-1), r);
+    if constexpr (SDO == 4) {
+        r = fmaf(c[2], at(3) - at(-2), r);
+        r = fmaf(c[3], at(4) - at(-3), r);
+    }
+    return r;
+}
+
+// Forward step: stage 1 velocities (elastic.cl:155-226), stage 2 stresses
+// (elastic.cl:235-308), damping after each update.
+template <int SDO>
+__global__ __launch_bounds__(64 * FUSED_WAVES) void k_fused_fwd(
+    DevGrid g, const pyfwi_shot *shots, int t, const float *in, float *out,
+    const float *__restrict__ bu, const float *__restrict__ lam, const float *__restrict__ mu,
+    const float *__restrict__ dxp, const float *__restrict__ dzp)
+{
+    FusedTile<SDO> ft;
+    if (!fused_tile<SDO>(g, ft)) return;
+    if (!fwd_active(shots, ft.s, t)) return;
+    const int64_t fs = g.fstride;
+    const int P = g.pitch, lane = (int)threadIdx.x % 64, col = ft.col;
+    const float *I = in + (int64_t)ft.s * 5 * fs + g.origin;
+    float *O = out + (int64_t)ft.s * 5 * fs + g.origin;
+    const float *vx = I, *vz = I + fs, *txx = I + 2 * fs, *tzz = I + 3 * fs, *txz = I + 4 * fs;
+    const float *b = bu + g.origin, *L = lam + g.origin, *M = mu + g.origin;
+    const bool ld = ft.colact;  // only active columns read memory
+    ZWin<SDO> wxz{}, wzz{};     // old txz (d/dz-), old tzz (d/dz+): rows r-SDO..r+SDO
+    ZWin<SDO> nvx{}, nvz{};         // new vx (d/dz+), new vz (d/dz-): rows q-SDO..q+SDO
+    const int rs = ft.i0 - SDO;
+    const int64_t c0 = (int64_t)rs * P + col;
+    int64_t c = c0;
+#pragma unroll
+    for (int k = 0; k < 2 * SDO; ++k) {
+        const int64_t o = c + (int64_t)(k - SDO) * P;
+        wxz.v[k + 1] = ld ? txz[o] : 0.0f;
+        wzz.v[k + 1] = ld ? tzz[o] : 0.0f;
+    }
+    const float ddx = ld ? dxp[col] : 0.0f;
+    const float dt = g.dt;
+    // rows in groups of MARCH_UNROLL, the inner loop fully unrolled: the cross-lane
+    // shuffles stop the compiler from unrolling a loop with a run-time trip count.
+    // Rows past the strip end (at most MARCH_UNROLL - 1) stay inside the zero halo
+    // and write nothing.
+    const int nrow = ft.i1 + SDO - rs + 1;
+    for (int rr = 0; rr < nrow; rr += MARCH_UNROLL) {
+#pragma unroll
+    for (int u = 0; u < MARCH_UNROLL; ++u) {
+        const int r = rs + rr + u;
+        c = c0 + (int64_t)(rr + u) * P;
+        const int64_t n = c + (int64_t)SDO * P;
+        wxz.push(ld ? txz[n] : 0.0f);
+        wzz.push(ld ? tzz[n] : 0.0f);
+        // stage 1: velocities at row r
+        float ux = 0.0f, uz = 0.0f;
+        const bool act1 = ld && r >= SDO && r <= g.nz - SDO;
+        if (act1) {
+            const float ax = b[c] * (dminus<SDO>(txx + c, 1, g.cx) + wxz.dminus(g.cz));
+            const float az = b[c] * (wzz.dplus(g.cz) + dplus<SDO>(txz + c, 1, g.cx));
+            const float damp = ddx + dzp[r];
+            ux = fmaf(dt, ax, vx[c]);
+            ux = fmaf(dt, -damp * ux, ux);
+            uz = fmaf(dt, az, vz[c]);
+            uz = fmaf(dt, -damp * uz, uz);
+            if (ft.own && r >= ft.i0 && r <= ft.i1) {
+                O[c] = ux;
+                O[fs + c] = uz;
+            }
+        }
+        nvx.push(ux);
+        nvz.push(uz);
+        // stage 2: stresses at row q = r - SDO (every lane shuffles)
+        const int q = r - SDO;
+        const float qvx = nvx.centre(), qvz = nvz.centre();
+        const float exx = lane_dplus<SDO>(qvx, lane, g.cx);
+        const float gzx = lane_dminus<SDO>(qvz, lane, g.cx);
+        if (q < ft.i0 || q > ft.i1 || !ft.own) continue;
+        const int64_t cq = c - (int64_t)SDO * P;
+        const float ezz = nvz.dminus(g.cz);
+        const float gxz = gzx + nvx.dplus(g.cz);
+        const float l = L[cq], m = M[cq];
+        const float lm = l + 2.0f * m;
+        const float sxx = lm * exx + l * ezz;
+        const float szz = lm * ezz + l * exx;
+        const float sxz = m * gxz;
+        const float damp = ddx + dzp[q];
+        float a = fmaf(dt, sxx, txx[cq]);
+        float bb = fmaf(dt, szz, wzz.v[0]);
+        float d = fmaf(dt, sxz, wxz.v[0]);
+        O[2 * fs + cq] = fmaf(dt, -damp * a, a);
+        O[3 * fs + cq] = fmaf(dt, -damp * bb, bb);
+        O[4 * fs + cq] = fmaf(dt, -damp * d, d);
+    }
+    }
+}
+
+// Time-reversed step (coeff = -1, wave_propagation.py:780-812): stage 1
+// stresses, stage 2 velocities, damping before each update.
+template <int SDO>
+__global__ __launch_bounds__(64 * FUSED_WAVES) void k_fused_rev(
+    DevGrid g, const pyfwi_shot *shots, int t, const float *in, float *out,
+    const float *__restrict__ bu, const float *__restrict__ lam, const float *__restrict__ mu,
+    const float *__restrict__ dxp, const float *__restrict__ dzp)
+{
+    FusedTile<SDO> ft;
+    if (!fused_tile<SDO>(g, ft)) return;
+    if (!bwd_active(shots, ft.s, t)) return;
+    const int64_t fs = g.fstride;
+    const int P = g.pitch, lane = (int)threadIdx.x % 64, col = ft.col;
+    const float *I = in + (int64_t)ft.s * 5 * fs + g.origin;
+    float *O = out + (int64_t)ft.s * 5 * fs + g.origin;
+    const float *vx = I, *vz = I + fs, *txx = I + 2 * fs, *tzz = I + 3 * fs, *txz = I + 4 * fs;
+    const float *b = bu + g.origin, *L = lam + g.origin, *M = mu + g.origin;
+    const bool ld = ft.colact;
+    ZWin<SDO> ovz{}, ovx{};      // old vz (d/dz-), old vx (d/dz+): rows r-SDO..r+SDO
+    ZWin<SDO> ntxx{}, ntzz{}, ntxz{};  // new stresses: rows q-SDO..q+SDO
+    const int rs = ft.i0 - SDO;
+    const int64_t c0 = (int64_t)rs * P + col;
+    int64_t c = c0;
+#pragma unroll
+    for (int k = 0; k < 2 * SDO; ++k) {
+        const int64_t o = c + (int64_t)(k - SDO) * P;
+        ovz.v[k + 1] = ld ? vz[o] : 0.0f;
+        ovx.v[k + 1] = ld ? vx[o] : 0.0f;
+    }
+    const float ddx = ld ? dxp[col] : 0.0f;
+    const float dt = g.dt;
+    // rows in groups of MARCH_UNROLL, the inner loop fully unrolled: the cross-lane
+    // shuffles stop the compiler from unrolling a loop with a run-time trip count.
+    // Rows past the strip end (at most MARCH_UNROLL - 1) stay inside the zero halo
+    // and write nothing.
+    const int nrow = ft.i1 + SDO - rs + 1;
+    for (int rr = 0; rr < nrow; rr += MARCH_UNROLL) {
+#pragma unroll
+    for (int u = 0; u < MARCH_UNROLL; ++u) {
+        const int r = rs + rr + u;
+        c = c0 + (int64_t)(rr + u) * P;
+        const int64_t n = c + (int64_t)SDO * P;
+        ovz.push(ld ? vz[n] : 0.0f);
+        ovx.push(ld ? vx[n] : 0.0f);
+        float a = 0.0f, bb = 0.0f, d = 0.0f;
+        const bool act1 = ld && r >= SDO && r <= g.nz - SDO;
+        if (act1) {
+            const float exx = dplus<SDO>(vx + c, 1, g.cx);
+            const float ezz = ovz.dminus(g.cz);
+            const float gxz = dminus<SDO>(vz + c, 1, g.cx) + ovx.dplus(g.cz);
+            const float l = L[c], m = M[c];
+            const float lm = l + 2.0f * m;
+            const float damp = ddx + dzp[r];
+            a = txx[c];
+            bb = tzz[c];
+            d = txz[c];
+            a = fmaf(-dt, damp * a, a);
+            bb = fmaf(-dt, damp * bb, bb);
+            d = fmaf(-dt, damp * d, d);
+            a = fmaf(-dt, lm * exx + l * ezz, a);
+            bb = fmaf(-dt, lm * ezz + l * exx, bb);
+            d = fmaf(-dt, m * gxz, d);
+            if (ft.own && r >= ft.i0 && r <= ft.i1) {
+                O[2 * fs + c] = a;
+                O[3 * fs + c] = bb;
+                O[4 * fs + c] = d;
+            }
+        }
+        ntxx.push(a);
+        ntzz.push(bb);
+        ntxz.push(d);
+        const int q = r - SDO;
+        const float dxm_txx = lane_dminus<SDO>(ntxx.centre(), lane, g.cx);
+        const float dxp_txz = lane_dplus<SDO>(ntxz.centre(), lane, g.cx);
+        if (q < ft.i0 || q > ft.i1 || !ft.own) continue;
+        const int64_t cq = c - (int64_t)SDO * P;
+        const float ax = b[cq] * (dxm_txx + ntxz.dminus(g.cz));
+        const float az = b[cq] * (ntzz.dplus(g.cz) + dxp_txz);
+        const float damp = ddx + dzp[q];
+        float ux = ovx.v[0], uz = ovz.v[0];
+        ux = fmaf(-dt, damp * ux, ux);
+        ux = fmaf(-dt, ax, ux);
+        uz = fmaf(-dt, damp * uz, uz);
+        uz = fmaf(-dt, az, uz);
+        O[cq] = ux;
+        O[fs + cq] = uz;
+    }
     }
 }
 
@@ -399,6 +649,7 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot
     prime(wbz, c, P, [&](int64_t k) { return avz[k] * b[k]; });
     prime(wbx, c, P, [&](int64_t k) { return avx[k] * b[k]; });
     const float ex = dxp[j];
+#pragma unroll MARCH_UNROLL
     for (int i = i0; i <= i1; ++i, c += P) {
         const int64_t n = c + (int64_t)SDO * P;
         wbz.push(avz[n] * b[n]);
@@ -458,6 +709,7 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
     }
     const float ex = dxp[j];
     const float dt = g.dt;
+#pragma unroll MARCH_UNROLL
     for (int i = i0; i <= i1; ++i, c += P) {
         const int64_t n = c + (int64_t)SDO * P;
         wa3.push(M[n] * atxz[n]);
@@ -680,7 +932,7 @@ __global__ void k_copy_shots(float *dst, int64_t dst_stride, const float *src,
 // Sum per-shot accumulators into the unpadded output [nslot][tnz*tnx];
 // exact mode converts dJ/db (slot 4) to dJ/drho = -b^2 dJ/db.
 __global__ void k_grad_sum(DevGrid g, int nshots, int nslot, const float *gacc,
-                           float *out, const float *__restrict__ bu, int b_to_rho)
+                           float *out, const float *__restrict__ bu, int b_to_rho, int accumulate)
 {
     const int64_t n = (int64_t)g.nz * g.nx;
     const int64_t fs = g.fstride;
@@ -693,7 +945,7 @@ __global__ void k_grad_sum(DevGrid g, int nshots, int nslot, const float *gacc,
         float acc = 0.0f;
         for (int s = 0; s < nshots; ++s) acc += gacc[((int64_t)s * 6 + slot) * fs + p];
         if (b_to_rho && slot == 4) acc = -(bu[p] * bu[p]) * acc;
-        out[k] = acc;
+        out[k] = accumulate ? out[k] + acc : acc;
     }
 }
 
@@ -740,6 +992,7 @@ static dim3 march_grid(DevGrid &g, int sdo, int nshots)
     const int64_t waves_per_row = (int64_t)((nx_act + 63) / 64) * nshots;
     int rows = (int)((waves_per_row * nz_act + TARGET_WAVES - 1) / TARGET_WAVES);
     rows = std::max(MARCH_ROWS_MIN, std::min(MARCH_ROWS_MAX, rows));
+    if (g.rows_override > 0) rows = g.rows_override;
     g.mrows = rows;
     g.nstrips = (nz_act + rows - 1) / rows;
     g.nbatch = nshots;
@@ -775,6 +1028,37 @@ void launch_str(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, 
 #define ARGS gm, shots, t, fields, lam, mu, dxp, dzp
     if (dir > 0) DISPATCH_SDO(g.sdo, k_str, , 1);
     else DISPATCH_SDO(g.sdo, k_str, , -1);
+#undef ARGS
+}
+
+// Grid of the fused kernels: column blocks of FUSED_WAVES wavefronts, each
+// owning 64 - 2*SDO columns; strips as in march_grid.
+static dim3 fused_grid(DevGrid &g, int sdo, int nshots)
+{
+    const int nx_act = g.nx - 2 * sdo + 1, nz_act = g.nz - 2 * sdo + 1;
+    const int span = FUSED_WAVES * (64 - 2 * sdo);
+    g.ncb = (nx_act + span - 1) / span;
+    const int64_t waves_per_row = (int64_t)g.ncb * FUSED_WAVES * nshots;
+    int rows = (int)((waves_per_row * nz_act + TARGET_WAVES - 1) / TARGET_WAVES);
+    rows = std::max(MARCH_ROWS_MIN, std::min(MARCH_ROWS_MAX, rows));
+    if (g.rows_override > 0) rows = g.rows_override;
+    g.mrows = rows;
+    g.nstrips = (nz_act + rows - 1) / rows;
+    g.nbatch = nshots;
+    const int T = g.ncb * g.nstrips * nshots;
+    const int per = (T + NUM_XCD - 1) / NUM_XCD;
+    return dim3(per * NUM_XCD);
+}
+
+void launch_fused_step(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
+                       const float *in, float *out, const float *bu, const float *lam,
+                       const float *mu, const float *dxp, const float *dzp, hipStream_t st)
+{
+    DevGrid gm = g;
+    dim3 grid = fused_grid(gm, g.sdo, nshots), block(64 * FUSED_WAVES);
+#define ARGS gm, shots, t, in, out, bu, lam, mu, dxp, dzp
+    if (dir > 0) DISPATCH_SDO(g.sdo, k_fused_fwd, );
+    else DISPATCH_SDO(g.sdo, k_fused_rev, );
 #undef ARGS
 }
 
@@ -856,10 +1140,10 @@ void launch_copy_shots(float *dst, int64_t dst_stride, const float *src, int64_t
 }
 
 void launch_grad_sum(const DevGrid &g, int nshots, int nslot, const float *gacc, float *out,
-                     const float *bu, int b_to_rho, hipStream_t st)
+                     const float *bu, int b_to_rho, int accumulate, hipStream_t st)
 {
     hipLaunchKernelGGL(k_grad_sum, dim3(1024), dim3(256), 0, st, g, nshots, nslot, gacc, out,
-                       bu, b_to_rho);
+                       bu, b_to_rho, accumulate);
 }
 
 void launch_pad(const DevGrid &g, const float *src, float *dst, hipStream_t st)

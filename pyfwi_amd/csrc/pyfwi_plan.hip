@@ -83,12 +83,17 @@ struct pyfwi_plan {
     DevBuf lam, mu, bu, dxp_buf, dzp_buf, scratch_in;
     // damping profiles with HALO zero entries on both sides: dxp[-HALO..nx+HALO)
     float *dxp = nullptr, *dzp = nullptr;
-    DevBuf fields, adj, av0, av1, gacc, snaps, ckpts, seg;
+    DevBuf fields, fields2, adj, av0, av1, gacc, snaps, ckpts, seg;
+    float *cur = nullptr;  // buffer (fields or fields2) holding the current forward state
+    // tuning: shots per pass (0: all), fused steps, strip height (0: automatic)
+    int tune_group = 0, tune_fused = 1, tune_rows = 0;
+    bool ckpts_all = false;  // ckpts holds every shot of the last forward
     DevBuf src_stage, seis_stage, res_stage, grad_stage, shot_tab_f;
     pyfwi_shot *d_shots = nullptr;
     int shots_cap = 0;
     // state of the last forward
     std::vector<pyfwi_shot> last_shots;
+    std::vector<pyfwi_shot> h_shots;  // host copy of the table on the device
     std::vector<int32_t> chp;
     int ckpt_every = 0;
     int ncp = 0;
@@ -212,6 +217,7 @@ size_t rec_extent(const pyfwi_shot *shots, int nshots)
 
 int upload_shots(pyfwi_plan *p, const pyfwi_shot *shots, int nshots)
 {
+    p->h_shots.assign(shots, shots + nshots);
     if (nshots > p->shots_cap) {
         if (p->d_shots) (void)hipFree(p->d_shots);
         p->d_shots = nullptr;
@@ -269,7 +275,7 @@ int32_t pyfwi_plan_create(const pyfwi_grid *grid, void *stream, pyfwi_plan **out
     DevGrid &d = p->dg;
     d.nz = g.tnz;
     d.nx = g.tnx;
-    d.pitch = ((g.tnx + HALO + 63) / 64) * 64;
+    d.pitch = ((g.tnx + HALO + 63) / 64) * 64;  // >= tnx + 8 zero columns
     d.sdo = g.sdo;
     d.npml = g.npml;
     d.origin = (int64_t)HALO * d.pitch;
@@ -298,7 +304,7 @@ int32_t pyfwi_plan_destroy(pyfwi_plan *p)
 {
     if (!p) return 0;
     (void)hipStreamSynchronize(p->stream);
-    for (DevBuf *b : {&p->lam, &p->mu, &p->bu, &p->dxp_buf, &p->dzp_buf, &p->scratch_in, &p->fields,
+    for (DevBuf *b : {&p->lam, &p->mu, &p->bu, &p->dxp_buf, &p->dzp_buf, &p->scratch_in, &p->fields, &p->fields2,
                       &p->adj, &p->av0, &p->av1, &p->gacc, &p->snaps, &p->ckpts, &p->seg,
                       &p->src_stage, &p->seis_stage, &p->res_stage, &p->grad_stage,
                       &p->shot_tab_f})
@@ -368,20 +374,96 @@ int32_t pyfwi_set_model(pyfwi_plan *p, const float *lam, const float *mu, const 
 
 namespace {
 
-// One forward step of every shot of the batch (gather + inject, velocity,
-// stress), the reference's loop body at wave_propagation.py:704-708.
-int forward_step(pyfwi_plan *p, int nshots, int t, const float *src, float *seis)
+// One forward step of the shots of a pass, the reference's loop body at
+// wave_propagation.py:704-708: gather + inject on the current state, then the
+// velocity and stress half steps, either as one fused launch into the other
+// buffer or as two in-place launches.
+int forward_step(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t, const float *src,
+                 float *seis)
 {
     const DevGrid &g = p->dg;
-    launch_inject_record(g, nshots, p->d_shots, t, p->fields.p, src, seis, p->stream);
-    {
-        Timed tm(p, PYFWI_K_FWD_VEL);
-        launch_vel(g, +1, nshots, p->d_shots, t, p->fields.p, p->bu.p, p->dxp, p->dzp, p->stream);
-    }
-    {
+    launch_inject_record(g, ng, dsh, t, p->cur, src, seis, p->stream);
+    if (p->tune_fused) {
+        float *nxt = p->cur == p->fields.p ? p->fields2.p : p->fields.p;
+        Timed tm(p, PYFWI_K_FWD_STEP);
+        launch_fused_step(g, +1, ng, dsh, t, p->cur, nxt, p->bu.p, p->lam.p, p->mu.p, p->dxp,
+                          p->dzp, p->stream);
+        p->cur = nxt;
+    } else {
+        {
+            Timed tm(p, PYFWI_K_FWD_VEL);
+            launch_vel(g, +1, ng, dsh, t, p->cur, p->bu.p, p->dxp, p->dzp, p->stream);
+        }
         Timed tm(p, PYFWI_K_FWD_STR);
-        launch_str(g, +1, nshots, p->d_shots, t, p->fields.p, p->lam.p, p->mu.p, p->dxp,
-                   p->dzp, p->stream);
+        launch_str(g, +1, ng, dsh, t, p->cur, p->lam.p, p->mu.p, p->dxp, p->dzp, p->stream);
+    }
+    return 0;
+}
+
+// Time-reversed step (wave_propagation.py:780-812).
+int reverse_step(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t)
+{
+    const DevGrid &g = p->dg;
+    if (p->tune_fused) {
+        float *nxt = p->cur == p->fields.p ? p->fields2.p : p->fields.p;
+        Timed tm(p, PYFWI_K_REV_STEP);
+        launch_fused_step(g, -1, ng, dsh, t, p->cur, nxt, p->bu.p, p->lam.p, p->mu.p, p->dxp,
+                          p->dzp, p->stream);
+        p->cur = nxt;
+    } else {
+        {
+            Timed tm(p, PYFWI_K_REV_STR);
+            launch_str(g, -1, ng, dsh, t, p->cur, p->lam.p, p->mu.p, p->dxp, p->dzp, p->stream);
+        }
+        Timed tm(p, PYFWI_K_REV_VEL);
+        launch_vel(g, -1, ng, dsh, t, p->cur, p->bu.p, p->dxp, p->dzp, p->stream);
+    }
+    return 0;
+}
+
+int pass_size(const pyfwi_plan *p, int nshots)
+{
+    return p->tune_group > 0 ? std::min(p->tune_group, nshots) : nshots;
+}
+
+// Zeroed state buffers for a pass of ng shots.
+int prepare_fields(pyfwi_plan *p, int ng)
+{
+    const int64_t fps = fields_per_shot(p);
+    if (int e = ensure(p->fields, (size_t)ng * fps, p->stream, "fields")) return e;
+    if (p->tune_fused)
+        if (int e = ensure(p->fields2, (size_t)ng * fps, p->stream, "fields")) return e;
+    HIPCHK(hipMemsetAsync(p->fields.p, 0, (size_t)ng * fps * sizeof(float), p->stream));
+    if (p->tune_fused)
+        HIPCHK(hipMemsetAsync(p->fields2.p, 0, (size_t)ng * fps * sizeof(float), p->stream));
+    p->cur = p->fields.p;
+    return 0;
+}
+
+// Forward time loop of one pass of ng shots starting at shot g0.  `snaps`
+// and `ckpts` (or null) have per-shot strides nchp*fps and ncp*fps.
+int forward_pass(pyfwi_plan *p, int g0, int ng, const float *d_src, float *d_seis,
+                 const std::vector<int32_t> &cl, float *snaps, float *ckpts, int ncp, int K)
+{
+    const int64_t fps = fields_per_shot(p);
+    const pyfwi_shot *dsh = p->d_shots + g0;
+    std::vector<pyfwi_shot> sh(p->h_shots.begin() + g0, p->h_shots.begin() + g0 + ng);
+    const int nt = max_nt(sh);
+    const int nchp = (int)cl.size();
+    if (int e = prepare_fields(p, ng)) return e;
+    int chpc = 0;
+    for (int t = 0; t < nt; ++t) {
+        if (ckpts && t % K == 0)
+            launch_copy_shots(ckpts + (int64_t)(t / K) * fps, (int64_t)ncp * fps, p->cur, fps, fps,
+                              ng, p->stream);
+        forward_step(p, dsh, ng, t, d_src, d_seis);
+        if (snaps && std::find(cl.begin(), cl.end(), t) != cl.end()) {
+            if (chpc < nchp)
+                launch_copy_shots(snaps + (int64_t)chpc * fps, (int64_t)nchp * fps, p->cur, fps, fps,
+                                  ng, p->stream);
+            chpc++;
+        }
+        if (int e = maybe_harvest(p)) return e;
     }
     return 0;
 }
@@ -402,11 +484,9 @@ int32_t pyfwi_forward(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, co
     if (nchp > 0)
         for (auto &x : sh)
             if (x.nt != nt) return fail(PYFWI_EINVAL, "checkpoint lists need equal nt for all shots");
-    const DevGrid &g = p->dg;
     const int64_t fps = fields_per_shot(p);
     p->have_forward = false;
-    if (int e = ensure(p->fields, (size_t)nshots * fps, p->stream, "fields")) return e;
-    HIPCHK(hipMemsetAsync(p->fields.p, 0, (size_t)nshots * fps * sizeof(float), p->stream));
+    p->ckpts_all = false;
     if (nchp > 0)
         if (int e = ensure(p->snaps, (size_t)nshots * nchp * fps, p->stream, "checkpoints")) return e;
     int ncp = 0;
@@ -425,19 +505,12 @@ int32_t pyfwi_forward(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, co
     }
     HIPCHK(hipMemsetAsync(d_seis, 0, nrec * sizeof(float), p->stream));
     std::vector<int32_t> cl(chp, chp + std::max(nchp, 0));
-    int chpc = 0;
-    for (int t = 0; t < nt; ++t) {
-        if (ncp > 0 && t % ckpt_every == 0)
-            launch_copy_shots(p->ckpts.p + (int64_t)(t / ckpt_every) * fps, (int64_t)ncp * fps,
-                              p->fields.p, fps, fps, nshots, p->stream);
-        forward_step(p, nshots, t, d_src, d_seis);
-        if (nchp > 0 && std::find(cl.begin(), cl.end(), t) != cl.end()) {
-            if (chpc < nchp)
-                launch_copy_shots(p->snaps.p + (int64_t)chpc * fps, (int64_t)nchp * fps,
-                                  p->fields.p, fps, fps, nshots, p->stream);
-            chpc++;
-        }
-        if (int e = maybe_harvest(p)) return e;
+    const int G = pass_size(p, nshots);
+    for (int g0 = 0; g0 < nshots; g0 += G) {
+        const int ng = std::min(G, nshots - g0);
+        float *sn = nchp > 0 ? p->snaps.p + (int64_t)g0 * nchp * fps : nullptr;
+        float *ck = ncp > 0 ? p->ckpts.p + (int64_t)g0 * ncp * fps : nullptr;
+        if (int e = forward_pass(p, g0, ng, d_src, d_seis, cl, sn, ck, ncp, ckpt_every)) return e;
     }
     HIPCHK(hipGetLastError());
     if (mem != PYFWI_MEM_DEVICE) {
@@ -449,111 +522,103 @@ int32_t pyfwi_forward(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, co
     p->chp = cl;
     p->ckpt_every = ckpt_every;
     p->ncp = ncp;
+    p->ckpts_all = ncp > 0;
     p->have_forward = true;
     return 0;
 }
 
 namespace {
 
-int gradient_pyfwi(pyfwi_plan *p, int nshots, const float *d_res, bool precond)
+// Reference adjoint of one pass (shots g0 .. g0+ng-1) into gacc[ng].
+int gradient_pyfwi(pyfwi_plan *p, int g0, int ng, const float *d_res, bool precond)
 {
     const DevGrid &g = p->dg;
-    (void)g;
     const int64_t fps = fields_per_shot(p);
+    const pyfwi_shot *dsh = p->d_shots + g0;
     const int nt = max_nt(p->last_shots);
     const int nchp = (int)p->chp.size();
-    if (nchp == 0) return fail(PYFWI_ESTATE, "pyfwi-mode gradient needs a forward with a checkpoint list");
-    if (int e = ensure(p->adj, (size_t)nshots * fps, p->stream, "adjoint fields")) return e;
-    HIPCHK(hipMemsetAsync(p->adj.p, 0, (size_t)nshots * fps * sizeof(float), p->stream));
-    HIPCHK(hipMemsetAsync(p->fields.p, 0, (size_t)nshots * fps * sizeof(float), p->stream));
+    const float *snaps = p->snaps.p + (int64_t)g0 * nchp * fps;
+    if (int e = ensure(p->adj, (size_t)ng * fps, p->stream, "adjoint fields")) return e;
+    HIPCHK(hipMemsetAsync(p->adj.p, 0, (size_t)ng * fps * sizeof(float), p->stream));
+    if (int e = prepare_fields(p, ng)) return e;
     int chpc = nchp - 1;
     for (int t = nt - 1; t > 0; --t) {
         if (chpc >= 0 && t == p->chp[chpc]) {
-            launch_copy_shots(p->fields.p, fps, p->snaps.p + (int64_t)chpc * fps,
-                              (int64_t)nchp * fps, fps, nshots, p->stream);
+            launch_copy_shots(p->cur, fps, snaps + (int64_t)chpc * fps, (int64_t)nchp * fps, fps,
+                              ng, p->stream);
             chpc--;
         } else {
-            {
-                Timed tm(p, PYFWI_K_REV_STR);
-                launch_str(g, -1, nshots, p->d_shots, t, p->fields.p, p->lam.p, p->mu.p,
-                           p->dxp, p->dzp, p->stream);
-            }
-            {
-                Timed tm(p, PYFWI_K_REV_VEL);
-                launch_vel(g, -1, nshots, p->d_shots, t, p->fields.p, p->bu.p, p->dxp,
-                           p->dzp, p->stream);
-            }
+            reverse_step(p, dsh, ng, t);
         }
-        launch_adj_inject(g, nshots, p->d_shots, t, p->adj.p, d_res, p->stream);
+        launch_adj_inject(g, ng, dsh, t, p->adj.p, d_res, p->stream);
         {
             Timed tm(p, PYFWI_K_ADJ_STR);
-            launch_adj_str(g, nshots, p->d_shots, t, p->adj.p, p->bu.p, p->dxp, p->dzp,
-                           p->stream);
+            launch_adj_str(g, ng, dsh, t, p->adj.p, p->bu.p, p->dxp, p->dzp, p->stream);
         }
         {
             Timed tm(p, PYFWI_K_ADJ_VEL);
-            launch_adj_vel_grad(g, nshots, p->d_shots, t, p->fields.p, p->adj.p, p->lam.p,
-                                p->mu.p, p->bu.p, p->dxp, p->dzp, p->gacc.p, precond,
-                                p->stream);
+            launch_adj_vel_grad(g, ng, dsh, t, p->cur, p->adj.p, p->lam.p, p->mu.p, p->bu.p,
+                                p->dxp, p->dzp, p->gacc.p, precond, p->stream);
         }
         if (int e = maybe_harvest(p)) return e;
     }
     return 0;
 }
 
-// Exact adjoint with forward recomputation from checkpoints every K steps.
-int gradient_exact(pyfwi_plan *p, int nshots, const float *d_src, const float *d_res, int K)
+// Exact adjoint of one pass, forward states recomputed from checkpoints
+// every K steps (ckpts: per-shot stride ncp*fps, first shot of the pass).
+int gradient_exact(pyfwi_plan *p, int g0, int ng, const float *d_src, const float *d_res,
+                   const float *ckpts, int ncp, int K)
 {
     const DevGrid &g = p->dg;
     const int64_t fps = fields_per_shot(p);
-    const int nt = max_nt(p->last_shots);
-    const int ncp = p->ncp;
-    if (int e = ensure(p->adj, (size_t)nshots * fps, p->stream, "adjoint fields")) return e;
-    if (int e = ensure(p->av0, (size_t)nshots * 2 * g.fstride, p->stream, "adjoint velocity")) return e;
-    if (int e = ensure(p->av1, (size_t)nshots * 2 * g.fstride, p->stream, "adjoint velocity")) return e;
-    if (int e = ensure(p->seg, (size_t)nshots * K * fps, p->stream, "segment")) return e;
-    HIPCHK(hipMemsetAsync(p->adj.p, 0, (size_t)nshots * fps * sizeof(float), p->stream));
-    HIPCHK(hipMemsetAsync(p->av0.p, 0, (size_t)nshots * 2 * g.fstride * sizeof(float), p->stream));
-    HIPCHK(hipMemsetAsync(p->av1.p, 0, (size_t)nshots * 2 * g.fstride * sizeof(float), p->stream));
+    const pyfwi_shot *dsh = p->d_shots + g0;
+    std::vector<pyfwi_shot> sh(p->h_shots.begin() + g0, p->h_shots.begin() + g0 + ng);
+    const int nt = max_nt(sh);
+    if (int e = ensure(p->adj, (size_t)ng * fps, p->stream, "adjoint fields")) return e;
+    if (int e = ensure(p->av0, (size_t)ng * 2 * g.fstride, p->stream, "adjoint velocity")) return e;
+    if (int e = ensure(p->av1, (size_t)ng * 2 * g.fstride, p->stream, "adjoint velocity")) return e;
+    if (int e = ensure(p->seg, (size_t)ng * K * fps, p->stream, "segment")) return e;
+    if (int e = ensure(p->fields, (size_t)ng * fps, p->stream, "fields")) return e;
+    HIPCHK(hipMemsetAsync(p->adj.p, 0, (size_t)ng * fps * sizeof(float), p->stream));
+    HIPCHK(hipMemsetAsync(p->av0.p, 0, (size_t)ng * 2 * g.fstride * sizeof(float), p->stream));
+    HIPCHK(hipMemsetAsync(p->av1.p, 0, (size_t)ng * 2 * g.fstride * sizeof(float), p->stream));
     float *av_cur = p->av0.p, *av_nxt = p->av1.p;
     const int64_t seg_stride = (int64_t)K * fps;  // per shot
     for (int c = ncp - 1; c >= 0; --c) {
         const int t0 = c * K, t1 = std::min(nt, t0 + K);
         // recompute the segment: after the velocity update of step t the field
-        // buffers hold V_t (velocities) and the stresses that step t updates
-        launch_copy_shots(p->fields.p, fps, p->ckpts.p + (int64_t)c * fps, (int64_t)ncp * fps,
-                          fps, nshots, p->stream);
+        // buffer holds V_t (velocities) and the stresses that step t updates
+        launch_copy_shots(p->fields.p, fps, ckpts + (int64_t)c * fps, (int64_t)ncp * fps, fps, ng,
+                          p->stream);
         for (int t = t0; t < t1; ++t) {
-            launch_inject_record(g, nshots, p->d_shots, t, p->fields.p, d_src, p->seis_stage.p,
-                                 p->stream);
+            launch_inject_record(g, ng, dsh, t, p->fields.p, d_src, p->seis_stage.p, p->stream);
             {
                 Timed tm(p, PYFWI_K_FWD_VEL);
-                launch_vel(g, +1, nshots, p->d_shots, t, p->fields.p, p->bu.p, p->dxp,
-                           p->dzp, p->stream);
+                launch_vel(g, +1, ng, dsh, t, p->fields.p, p->bu.p, p->dxp, p->dzp, p->stream);
             }
             launch_copy_shots(p->seg.p + (int64_t)(t - t0) * fps, seg_stride, p->fields.p, fps,
-                              fps, nshots, p->stream);
+                              fps, ng, p->stream);
             if (t + 1 < t1) {
                 Timed tm(p, PYFWI_K_FWD_STR);
-                launch_str(g, +1, nshots, p->d_shots, t, p->fields.p, p->lam.p, p->mu.p,
-                           p->dxp, p->dzp, p->stream);
+                launch_str(g, +1, ng, dsh, t, p->fields.p, p->lam.p, p->mu.p, p->dxp, p->dzp,
+                           p->stream);
             }
         }
         for (int t = t1 - 1; t >= t0; --t) {
             const float *slot = p->seg.p + (int64_t)(t - t0) * fps;
             {
                 Timed tm(p, PYFWI_K_EXA_STR);
-                launch_exact_str(g, nshots, p->d_shots, t, slot, seg_stride, p->adj.p, av_cur,
-                                 p->lam.p, p->mu.p, p->dxp, p->dzp, p->gacc.p, p->stream);
+                launch_exact_str(g, ng, dsh, t, slot, seg_stride, p->adj.p, av_cur, p->lam.p,
+                                 p->mu.p, p->dxp, p->dzp, p->gacc.p, p->stream);
             }
             {
                 Timed tm(p, PYFWI_K_EXA_VEL);
-                launch_exact_vel(g, nshots, p->d_shots, t, slot + 2 * g.fstride, seg_stride,
-                                 p->adj.p, av_cur, av_nxt, p->bu.p, p->dxp, p->dzp, p->gacc.p,
-                                 p->stream);
+                launch_exact_vel(g, ng, dsh, t, slot + 2 * g.fstride, seg_stride, p->adj.p, av_cur,
+                                 av_nxt, p->bu.p, p->dxp, p->dzp, p->gacc.p, p->stream);
             }
             std::swap(av_cur, av_nxt);
-            launch_exact_inject(g, nshots, p->d_shots, t, p->adj.p, av_cur, d_res, p->stream);
+            launch_exact_inject(g, ng, dsh, t, p->adj.p, av_cur, d_res, p->stream);
             if (int e = maybe_harvest(p)) return e;
         }
     }
@@ -576,53 +641,78 @@ int32_t pyfwi_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, c
     const size_t nrec = rec_extent(shots, nshots);
     bool same = p->have_forward && (int)p->last_shots.size() == nshots &&
                 std::memcmp(p->last_shots.data(), shots, sizeof(pyfwi_shot) * nshots) == 0;
-    if (mode == PYFWI_MODE_PYFWI && !same)
-        return fail(PYFWI_ESTATE, "pyfwi-mode gradient must follow a forward of the same shots");
+    if (mode == PYFWI_MODE_PYFWI && (!same || p->chp.empty()))
+        return fail(PYFWI_ESTATE, "pyfwi-mode gradient must follow a forward of the same shots "
+                                  "with a checkpoint list");
     const float *d_res = nullptr;
     if (int e = stage_in(p, p->res_stage, res, nrec, mem, &d_res, "residuals")) return e;
-    if (int e = ensure(p->gacc, (size_t)nshots * 6 * g.fstride, p->stream, "gradient")) return e;
-    HIPCHK(hipMemsetAsync(p->gacc.p, 0, (size_t)nshots * 6 * g.fstride * sizeof(float), p->stream));
-    if (mode == PYFWI_MODE_PYFWI) {
-        if (int e = upload_shots(p, shots, nshots)) return e;
-        if (int e = gradient_pyfwi(p, nshots, d_res, (flags & PYFWI_F_PRECOND) != 0)) return e;
-    } else {
+    const float *d_src = nullptr;
+    if (mode == PYFWI_MODE_EXACT) {
         if (!src) return fail(PYFWI_EINVAL, "exact mode needs the source table");
-        int K = ckpt_every > 0 ? ckpt_every : (p->ckpt_every > 0 ? p->ckpt_every : 50);
-        if (!same || p->ckpt_every != K) {
-            // run the forward again, keeping checkpoints every K steps
-            const float *d_src = nullptr;
-            if (int e = stage_in(p, p->src_stage, src, src_extent(shots, nshots), mem, &d_src,
-                                 "source table"))
-                return e;
-            if (int e = ensure(p->seis_stage, nrec, p->stream, "records")) return e;
-            std::vector<float> none;
-            (void)none;
-            // pyfwi_forward stages host arrays itself; pass device pointers here
-            int32_t e = pyfwi_forward(p, shots, nshots, d_src, p->seis_stage.p, nullptr, 0, K,
-                                      PYFWI_MEM_DEVICE);
-            if (e) return e;
-        }
-        const float *d_src = nullptr;
         if (int e = stage_in(p, p->src_stage, src, src_extent(shots, nshots), mem, &d_src,
                              "source table"))
             return e;
         if (int e = ensure(p->seis_stage, nrec, p->stream, "records")) return e;
-        if (int e = upload_shots(p, shots, nshots)) return e;
-        if (int e = gradient_exact(p, nshots, d_src, d_res, K)) return e;
     }
+    if (int e = upload_shots(p, shots, nshots)) return e;
     float *d_grad = grad;
     const size_t n = (size_t)g.nz * g.nx;
     if (mem != PYFWI_MEM_DEVICE) {
         if (int e = ensure(p->grad_stage, 6 * n, p->stream, "gradient output")) return e;
         d_grad = p->grad_stage.p;
     }
-    launch_grad_sum(g, nshots, 6, p->gacc.p, d_grad, p->bu.p, mode == PYFWI_MODE_EXACT, p->stream);
+    const int G = pass_size(p, nshots);
+    const int K = ckpt_every > 0 ? ckpt_every : (p->ckpt_every > 0 ? p->ckpt_every : 50);
+    const bool reuse = mode == PYFWI_MODE_EXACT && same && p->ckpts_all && p->ckpt_every == K;
+    for (int g0 = 0; g0 < nshots; g0 += G) {
+        const int ng = std::min(G, nshots - g0);
+        if (int e = ensure(p->gacc, (size_t)ng * 6 * g.fstride, p->stream, "gradient")) return e;
+        HIPCHK(hipMemsetAsync(p->gacc.p, 0, (size_t)ng * 6 * g.fstride * sizeof(float), p->stream));
+        if (mode == PYFWI_MODE_PYFWI) {
+            if (int e = gradient_pyfwi(p, g0, ng, d_res, (flags & PYFWI_F_PRECOND) != 0)) return e;
+        } else {
+            std::vector<pyfwi_shot> shp(shots + g0, shots + g0 + ng);
+            const int ncp = (max_nt(shp) + K - 1) / K;
+            const float *ck;
+            if (reuse) {
+                ck = p->ckpts.p + (int64_t)g0 * p->ncp * fps;
+                if (int e = gradient_exact(p, g0, ng, d_src, d_res, ck, p->ncp, K)) return e;
+            } else {
+                // forward of this pass again, keeping its checkpoints
+                if (int e = ensure(p->ckpts, (size_t)ng * ncp * fps, p->stream, "checkpoints"))
+                    return e;
+                p->ckpts_all = false;
+                std::vector<int32_t> none;
+                if (int e = forward_pass(p, g0, ng, d_src, p->seis_stage.p, none, nullptr,
+                                         p->ckpts.p, ncp, K))
+                    return e;
+                if (int e = gradient_exact(p, g0, ng, d_src, d_res, p->ckpts.p, ncp, K)) return e;
+            }
+        }
+        launch_grad_sum(g, ng, 6, p->gacc.p, d_grad, p->bu.p, mode == PYFWI_MODE_EXACT,
+                        g0 > 0, p->stream);
+    }
     HIPCHK(hipGetLastError());
     if (mem != PYFWI_MEM_DEVICE) {
         HIPCHK(hipMemcpyAsync(grad, d_grad, 6 * n * sizeof(float), hipMemcpyDeviceToHost, p->stream));
         HIPCHK(hipStreamSynchronize(p->stream));
     }
     if (int e = harvest_timing(p)) return e;
+    return 0;
+}
+
+int32_t pyfwi_plan_set_tuning(pyfwi_plan *p, int32_t shots_per_pass, int32_t fused,
+                              int32_t strip_rows)
+{
+    if (!p) return fail(PYFWI_EINVAL, "null plan");
+    if (shots_per_pass < 0 || strip_rows < 0 || (strip_rows > 0 && strip_rows < 2 * p->grid.sdo))
+        return fail(PYFWI_EINVAL, "shots_per_pass and strip_rows must be >= 0 "
+                                  "(strip_rows 0 or >= 2*sdo)");
+    p->tune_group = shots_per_pass;
+    p->tune_fused = fused ? 1 : 0;
+    p->tune_rows = strip_rows;
+    p->dg.rows_override = strip_rows;
+    p->have_forward = false;
     return 0;
 }
 

@@ -156,3 +156,31 @@ def test_exact_gradient_vs_fp64(gpu):
         e = rel(a, b)
         print(f"exact gradient {name} vs FP64: {e:.2e}")
         assert e < GRAD_TOL, (name, e)
+
+
+@pytest.mark.parametrize("tuning", [(2, False, 0), (3, True, 0), (1, True, 0), (0, False, 16),
+                                    (2, True, 12)])
+def test_execution_shapes_agree(gpu, tuning):
+    """Shots per pass, fused or two-launch steps and the strip height change
+    only the schedule: records and reference gradients match the oracle, and
+    the exact gradient (recomputed per pass when the forward's checkpoints
+    are not reused) matches the FP64 oracle."""
+    w = _small(nt=300)
+    W = WavePropagator(w.inpa, w.src, w.rec_loc, w.model_shape, components=4, chpr=0)
+    W._plan.set_tuning(*tuning)
+    d = W.forward_modeling(w.model)
+    st = oracle_setup(w, chpr=0, components=4)
+    seis, states = O.forward(st, w.model, np.float32)
+    ref = O.section(seis, 4)
+    for k in ref:
+        assert rel(d[k], ref[k]) < SEIS_TOL, (k, tuning)
+    res = {k: np.float32(0.3 * v) for k, v in d.items()}
+    g = W.gradient(res, parameterization="lmd")
+    gr = O.gradient(st, w.model, res, states, parameterization="lmd")
+    for k in gr:
+        assert rel(g[k], gr[k]) < GRAD_TOL, (k, tuning)
+    G = W.raw_gradient(res, mode="exact", ckpt_every=41)
+    g3 = O.exact_gradient(st, w.model, O.residual_records(st, res, np.float64), np.float64)
+    md = st.medium(w.model, np.float64)
+    for a, b in ((G[2], g3[0]), (G[0], g3[1]), (G[4], -(md["b"] ** 2) * g3[2])):
+        assert rel(a, b) < GRAD_TOL, tuning
