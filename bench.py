@@ -268,10 +268,13 @@ def main():
     dom = max(prof, key=lambda k: prof[k][0])
     ms, n = prof[dom]
     avg_s = ms / n / 1e3
-    per_launch_bytes = launch_bytes(dom, cells, interior, int(W.ns))
+    # shots per launch: passes of `tuning[0]` shots (the last one may be smaller)
+    npass = -(-int(W.ns) // tuning[0]) if tuning[0] else 1
+    shots_per_launch = int(W.ns) / npass
+    per_launch_bytes = launch_bytes(dom, cells, interior, shots_per_launch)
     achieved = per_launch_bytes / avg_s / 1e9
     kernels = {k: {"avg_us": round(v[0] / v[1] * 1e3, 2), "launches": v[1],
-                   "GBps": round(launch_bytes(k, cells, interior, int(W.ns))
+                   "GBps": round(launch_bytes(k, cells, interior, shots_per_launch)
                                  / (v[0] / v[1] / 1e3) / 1e9, 1)}
                for k, v in prof.items()}
     log(f"bench: kernels {kernels}")
@@ -316,7 +319,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(w.name, dom),
-                     "bytes_per_launch": per_launch_bytes, "avg_launch_us": round(avg_s * 1e6, 2)},
+                     "bytes_per_launch": per_launch_bytes, "avg_launch_us": round(avg_s * 1e6, 2),
+                     "shots_per_launch": round(shots_per_launch, 3)},
         "gradient_roofline_frac": round(value * grad_bytes / HBM_PEAK_GBS, 4),
         "per_shot": ps,
         "kernels": kernels,

@@ -149,6 +149,25 @@ int32_t pyfwi_gradient(pyfwi_plan *plan, const pyfwi_shot *shots,
                        float *grad, int32_t mode, int32_t ckpt_every,
                        int32_t flags, int32_t mem);
 
+/* L2 misfit and its reference-semantics gradient in one call, the sequence
+ * of FWI.fprime (src/PyFWI/fwi.py:229-252): per pass of shots, forward
+ * modelling, residual res = seis - obs with misfit 0.5*sum(res^2)
+ * (CostFunction.l2, src/PyFWI/fwi_tools.py:1402-1422), then the adjoint of
+ * __kernel_gradient (wave_propagation.py:751-856) with the checkpoint list
+ * `chp` (all shots must share nt).  When chp is every step 0..nt-1
+ * (chpr = 100) the forward states are kept in HBM for the pass and read
+ * directly by the adjoint, so no time reversal is needed.
+ *   obs        observed records, layout of `seis` (rec_offset per shot)
+ *   seis       optional output of the modelled records (may be NULL)
+ *   grad       output [6][tnz*tnx] as pyfwi_gradient
+ *   misfit     output scalar
+ *   comp_mask  bit f selects component f (vx, vz, txx, tzz, txz); 0 compares
+ *              the pressure (txx + tzz)/2 with obs[txx] (components = 0) */
+int32_t pyfwi_misfit_gradient(pyfwi_plan *plan, const pyfwi_shot *shots, int32_t nshots,
+                              const float *src, const float *obs, float *seis, float *grad,
+                              double *misfit, const int32_t *chp, int32_t nchp,
+                              int32_t comp_mask, int32_t flags, int32_t mem);
+
 /* Execution shape of later calls (no counterpart in the reference, which runs
  * one shot at a time): shots_per_pass > 0 runs a call's shots in passes of
  * that many shots (0: all in one pass; smaller passes need less memory and

@@ -26,7 +26,7 @@ KERNEL_NAMES = ["fwd_vel", "fwd_str", "rev_str", "rev_vel", "adj_str", "adj_vel_
 EXPORTS = [
     "pyfwi_version", "pyfwi_last_error", "pyfwi_plan_create", "pyfwi_plan_destroy",
     "pyfwi_plan_set_stream", "pyfwi_set_model", "pyfwi_forward", "pyfwi_gradient",
-    "pyfwi_plan_set_tuning", "pyfwi_get_snapshot", "pyfwi_synchronize", "pyfwi_plan_set_timing",
+    "pyfwi_misfit_gradient", "pyfwi_plan_set_tuning", "pyfwi_get_snapshot", "pyfwi_synchronize", "pyfwi_plan_set_timing",
     "pyfwi_kernel_timing",
 ]
 
@@ -82,6 +82,9 @@ def load(path=None):
                                   ctypes.POINTER(ctypes.c_int32), i32, i32, i32]
     lib.pyfwi_gradient.argtypes = [P, ctypes.POINTER(Shot), i32, fp, fp, fp, i32, i32, i32, i32]
     lib.pyfwi_plan_set_tuning.argtypes = [P, i32, i32, i32]
+    lib.pyfwi_misfit_gradient.argtypes = [P, ctypes.POINTER(Shot), i32, fp, fp, fp, fp,
+                                          ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_int32), i32, i32, i32, i32]
     lib.pyfwi_get_snapshot.argtypes = [P, i32, i32, fp, i32]
     lib.pyfwi_synchronize.argtypes = [P]
     lib.pyfwi_plan_set_timing.argtypes = [P, i32]
@@ -149,6 +152,17 @@ class Plan:
         check(self.lib.pyfwi_gradient(self.h, arr, len(shots),
                                       ptr(src) if src is not None else None, ptr(res),
                                       ptr(grad), mode, int(ckpt_every), flags, mem))
+
+    def misfit_gradient(self, shots, src, obs, seis, grad, chp, comp_mask, flags, mem):
+        """Returns the L2 misfit; see pyfwi_misfit_gradient."""
+        arr = (Shot * len(shots))(*shots)
+        chp = np.ascontiguousarray(chp, np.int32)
+        out = ctypes.c_double()
+        check(self.lib.pyfwi_misfit_gradient(
+            self.h, arr, len(shots), ptr(src), ptr(obs), ptr(seis) if seis is not None else None,
+            ptr(grad), ctypes.byref(out), chp.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+            int(chp.size), int(comp_mask), int(flags), mem))
+        return out.value
 
     def set_tuning(self, shots_per_pass=0, fused=True, strip_rows=0):
         check(self.lib.pyfwi_plan_set_tuning(self.h, int(shots_per_pass), 1 if fused else 0,

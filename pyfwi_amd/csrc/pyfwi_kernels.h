@@ -36,9 +36,11 @@ struct DevGrid {
     int rows_override = 0;  // strip height chosen by the caller (0: automatic)
 };
 
+// snap (optional, forward only): also write the updated fields to
+// snap + shot*snap_stride (the kept state of this step)
 void launch_vel(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
                 float *fields, const float *bu, const float *dxp, const float *dzp,
-                hipStream_t st);
+                hipStream_t st, float *snap = nullptr, int64_t snap_stride = 0);
 // Fused velocity + stress step (forward, DIR=+1) or stress + velocity step
 // (time reversed, DIR=-1): reads `in`, writes `out` (both [nshots][5][fs]).
 void launch_fused_step(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
@@ -46,7 +48,8 @@ void launch_fused_step(const DevGrid &g, int dir, int nshots, const pyfwi_shot *
                        const float *mu, const float *dxp, const float *dzp, hipStream_t st);
 void launch_str(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
                 float *fields, const float *lam, const float *mu, const float *dxp,
-                const float *dzp, hipStream_t st);
+                const float *dzp, hipStream_t st, float *snap = nullptr,
+                int64_t snap_stride = 0);
 void launch_inject_record(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
                           float *fields, const float *src, float *seis, hipStream_t st);
 void launch_adj_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
@@ -55,9 +58,12 @@ void launch_adj_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t
                     float *adj, const float *bu, const float *dxp, const float *dzp,
                     hipStream_t st);
 void launch_adj_vel_grad(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
-                         const float *fields, float *adj, const float *lam,
-                         const float *mu, const float *bu, const float *dxp,
-                         const float *dzp, float *gacc, bool precond, hipStream_t st);
+                         const float *fields, int64_t fields_stride, float *adj,
+                         const float *lam, const float *mu, const float *bu,
+                         const float *dxp, const float *dzp, float *gacc, bool precond,
+                         hipStream_t st);
+void launch_l2_residual(int nshots, const pyfwi_shot *shots, const float *seis, const float *obs,
+                        float *res, int comp_mask, double *misfit, hipStream_t st);
 void launch_exact_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
                       const float *vel, int64_t vel_stride, float *adj, float *av,
                       const float *lam, const float *mu, const float *dxp,

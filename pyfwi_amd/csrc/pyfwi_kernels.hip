@@ -221,7 +221,8 @@ template <int SDO, int DIR>
 __global__ __launch_bounds__(MARCH_X) void k_vel(DevGrid g, const pyfwi_shot *shots, int t,
                                                  float *fields, const float *__restrict__ bu,
                                                  const float *__restrict__ dxp,
-                                                 const float *__restrict__ dzp)
+                                                 const float *__restrict__ dzp, float *snap,
+                                                 int64_t snap_stride)
 {
     int s, j, i0, i1;
     if (!tile<SDO>(g, s, j, i0, i1)) return;
@@ -258,6 +259,11 @@ __global__ __launch_bounds__(MARCH_X) void k_vel(DevGrid g, const pyfwi_shot *sh
         }
         vx[c] = ux;
         vz[c] = uz;
+        if (DIR > 0 && snap) {
+            float *S = snap + (int64_t)s * snap_stride + g.origin;
+            S[c] = ux;
+            S[fs + c] = uz;
+        }
     }
 }
 
@@ -267,7 +273,8 @@ __global__ __launch_bounds__(MARCH_X) void k_str(DevGrid g, const pyfwi_shot *sh
                                                  float *fields, const float *__restrict__ lam,
                                                  const float *__restrict__ mu,
                                                  const float *__restrict__ dxp,
-                                                 const float *__restrict__ dzp)
+                                                 const float *__restrict__ dzp, float *snap,
+                                                 int64_t snap_stride)
 {
     int s, j, i0, i1;
     if (!tile<SDO>(g, s, j, i0, i1)) return;
@@ -315,6 +322,12 @@ __global__ __launch_bounds__(MARCH_X) void k_str(DevGrid g, const pyfwi_shot *sh
         txx[c] = a;
         tzz[c] = bb;
         txz[c] = d;
+        if (DIR > 0 && snap) {
+            float *S = snap + (int64_t)s * snap_stride + g.origin;
+            S[2 * fs + c] = a;
+            S[3 * fs + c] = bb;
+            S[4 * fs + c] = d;
+        }
     }
 }
 
@@ -673,8 +686,8 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot
 // = gmu, gmu_pre, glam, glam_pre, grho, grho_pre.
 template <int SDO, bool PRECOND>
 __global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
-    DevGrid g, const pyfwi_shot *shots, int t, const float *fields, float *adj,
-    const float *__restrict__ lam, const float *__restrict__ mu,
+    DevGrid g, const pyfwi_shot *shots, int t, const float *fields, int64_t fields_stride,
+    float *adj, const float *__restrict__ lam, const float *__restrict__ mu,
     const float *__restrict__ bu, const float *__restrict__ dxp,
     const float *__restrict__ dzp, float *gacc)
 {
@@ -691,7 +704,7 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
     const bool gcol = j >= lo && j <= hix;
     const bool grow_strip = i1 >= lo && i0 <= hi;
     const bool img = gcol && grow_strip;
-    const float *F = fields + (int64_t)s * 5 * fs + g.origin;
+    const float *F = fields + (int64_t)s * fields_stride + g.origin;
     const float *vx = F, *vz = F + fs, *txx = F + 2 * fs, *tzz = F + 3 * fs, *txz = F + 4 * fs;
     float *G = gacc + (int64_t)s * 6 * fs + g.origin;
     const float *b = bu + g.origin;
@@ -914,6 +927,69 @@ __global__ void k_exact_inject(DevGrid g, const pyfwi_shot *shots, int t, float 
     }
 }
 
+// ------------------------------------------------------------ L2 misfit
+//
+// Residual of the records of the shots of a pass against observed records
+// (CostFunction.l2, src/PyFWI/fwi_tools.py:1402-1422): res = seis - obs in
+// FP32 for the selected components, misfit = 0.5 * sum(res^2) accumulated in
+// FP64.  comp_mask bit f selects component f (vx, vz, txx, tzz, txz); mask 0
+// means the pressure output (txx + tzz)/2 compared with obs[txx], whose
+// residual goes to both normal stresses and counts twice, as the reference's
+// 'taux'/'tauz' pair does.
+__global__ void k_l2_residual(const pyfwi_shot *shots, const float *__restrict__ seis,
+                              const float *__restrict__ obs, float *res, int comp_mask,
+                              double *misfit)
+{
+    const int s = blockIdx.y;
+    const pyfwi_shot sh = shots[s];
+    const int64_t plane = (int64_t)sh.n_samples * sh.rec_stride;
+    const float *S = seis + sh.rec_offset, *D = obs + sh.rec_offset;
+    float *R = res + sh.rec_offset;
+    double acc = 0.0;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < plane;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        if ((int)(k % sh.rec_stride) >= sh.n_rec) {
+#pragma unroll
+            for (int f = 0; f < 5; ++f) R[f * plane + k] = 0.0f;
+            continue;
+        }
+        if (comp_mask == 0) {
+            const float p = (S[2 * plane + k] + S[3 * plane + k]) / 2;
+            const float r = p - D[2 * plane + k];
+            R[k] = R[plane + k] = R[4 * plane + k] = 0.0f;
+            R[2 * plane + k] = r;
+            R[3 * plane + k] = r;
+            acc += (double)r * r;
+        } else {
+#pragma unroll
+            for (int f = 0; f < 5; ++f) {
+                float r = 0.0f;
+                if (comp_mask & (1 << f)) {
+                    r = S[f * plane + k] - D[f * plane + k];
+                    acc += 0.5 * (double)r * r;
+                }
+                R[f * plane + k] = r;
+            }
+        }
+    }
+    // block reduction, one atomic per block
+    __shared__ double part[256];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(misfit, part[0]);
+}
+
+void launch_l2_residual(int nshots, const pyfwi_shot *shots, const float *seis, const float *obs,
+                        float *res, int comp_mask, double *misfit, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_l2_residual, dim3(64, nshots), dim3(256), 0, st, shots, seis, obs, res,
+                       comp_mask, misfit);
+}
+
 // ------------------------------------------------------------- utilities
 
 // Copy `count` floats per shot (count % 4 == 0, 16-byte aligned bases).
@@ -1009,11 +1085,11 @@ static dim3 march_grid(DevGrid &g, int sdo, int nshots)
 
 void launch_vel(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
                 float *fields, const float *bu, const float *dxp, const float *dzp,
-                hipStream_t st)
+                hipStream_t st, float *snap, int64_t snap_stride)
 {
     DevGrid gm = g;
     dim3 grid = march_grid(gm, g.sdo, nshots), block(MARCH_X);
-#define ARGS gm, shots, t, fields, bu, dxp, dzp
+#define ARGS gm, shots, t, fields, bu, dxp, dzp, snap, snap_stride
     if (dir > 0) DISPATCH_SDO(g.sdo, k_vel, , 1);
     else DISPATCH_SDO(g.sdo, k_vel, , -1);
 #undef ARGS
@@ -1021,11 +1097,11 @@ void launch_vel(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, 
 
 void launch_str(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
                 float *fields, const float *lam, const float *mu, const float *dxp,
-                const float *dzp, hipStream_t st)
+                const float *dzp, hipStream_t st, float *snap, int64_t snap_stride)
 {
     DevGrid gm = g;
     dim3 grid = march_grid(gm, g.sdo, nshots), block(MARCH_X);
-#define ARGS gm, shots, t, fields, lam, mu, dxp, dzp
+#define ARGS gm, shots, t, fields, lam, mu, dxp, dzp, snap, snap_stride
     if (dir > 0) DISPATCH_SDO(g.sdo, k_str, , 1);
     else DISPATCH_SDO(g.sdo, k_str, , -1);
 #undef ARGS
@@ -1087,13 +1163,14 @@ void launch_adj_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t
 }
 
 void launch_adj_vel_grad(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
-                         const float *fields, float *adj, const float *lam,
-                         const float *mu, const float *bu, const float *dxp,
-                         const float *dzp, float *gacc, bool precond, hipStream_t st)
+                         const float *fields, int64_t fields_stride, float *adj,
+                         const float *lam, const float *mu, const float *bu,
+                         const float *dxp, const float *dzp, float *gacc, bool precond,
+                         hipStream_t st)
 {
     DevGrid gm = g;
     dim3 grid = march_grid(gm, g.sdo, nshots), block(MARCH_X);
-#define ARGS gm, shots, t, fields, adj, lam, mu, bu, dxp, dzp, gacc
+#define ARGS gm, shots, t, fields, fields_stride, adj, lam, mu, bu, dxp, dzp, gacc
     if (precond) DISPATCH_SDO(g.sdo, k_adj_vel_grad, , true);
     else DISPATCH_SDO(g.sdo, k_adj_vel_grad, , false);
 #undef ARGS

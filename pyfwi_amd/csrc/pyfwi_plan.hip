@@ -88,7 +88,8 @@ struct pyfwi_plan {
     // tuning: shots per pass (0: all), fused steps, strip height (0: automatic)
     int tune_group = 0, tune_fused = 1, tune_rows = 0;
     bool ckpts_all = false;  // ckpts holds every shot of the last forward
-    DevBuf src_stage, seis_stage, res_stage, grad_stage, shot_tab_f;
+    DevBuf src_stage, seis_stage, res_stage, grad_stage, shot_tab_f, resid;
+    double *d_misfit = nullptr;
     pyfwi_shot *d_shots = nullptr;
     int shots_cap = 0;
     // state of the last forward
@@ -307,8 +308,9 @@ int32_t pyfwi_plan_destroy(pyfwi_plan *p)
     for (DevBuf *b : {&p->lam, &p->mu, &p->bu, &p->dxp_buf, &p->dzp_buf, &p->scratch_in, &p->fields, &p->fields2,
                       &p->adj, &p->av0, &p->av1, &p->gacc, &p->snaps, &p->ckpts, &p->seg,
                       &p->src_stage, &p->seis_stage, &p->res_stage, &p->grad_stage,
-                      &p->shot_tab_f})
+                      &p->shot_tab_f, &p->resid})
         b->release();
+    if (p->d_misfit) (void)hipFree(p->d_misfit);
     if (p->d_shots) (void)hipFree(p->d_shots);
     for (auto e : p->ev_pool) (void)hipEventDestroy(e);
     if (p->own_stream) (void)hipStreamDestroy(p->own_stream);
@@ -379,7 +381,7 @@ namespace {
 // velocity and stress half steps, either as one fused launch into the other
 // buffer or as two in-place launches.
 int forward_step(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t, const float *src,
-                 float *seis)
+                 float *seis, float *snap = nullptr, int64_t snap_stride = 0)
 {
     const DevGrid &g = p->dg;
     launch_inject_record(g, ng, dsh, t, p->cur, src, seis, p->stream);
@@ -392,10 +394,12 @@ int forward_step(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t, const floa
     } else {
         {
             Timed tm(p, PYFWI_K_FWD_VEL);
-            launch_vel(g, +1, ng, dsh, t, p->cur, p->bu.p, p->dxp, p->dzp, p->stream);
+            launch_vel(g, +1, ng, dsh, t, p->cur, p->bu.p, p->dxp, p->dzp, p->stream, snap,
+                       snap_stride);
         }
         Timed tm(p, PYFWI_K_FWD_STR);
-        launch_str(g, +1, ng, dsh, t, p->cur, p->lam.p, p->mu.p, p->dxp, p->dzp, p->stream);
+        launch_str(g, +1, ng, dsh, t, p->cur, p->lam.p, p->mu.p, p->dxp, p->dzp, p->stream, snap,
+                   snap_stride);
     }
     return 0;
 }
@@ -419,6 +423,16 @@ int reverse_step(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t)
         launch_vel(g, -1, ng, dsh, t, p->cur, p->bu.p, p->dxp, p->dzp, p->stream);
     }
     return 0;
+}
+
+// True when the checkpoint list is every step 0..nt-1 (slot t = state after
+// step t).
+bool keeps_every_step(const std::vector<int32_t> &cl, int nt)
+{
+    if ((int)cl.size() != nt) return false;
+    for (int k = 0; k < nt; ++k)
+        if (cl[k] != k) return false;
+    return true;
 }
 
 int pass_size(const pyfwi_plan *p, int nshots)
@@ -451,11 +465,20 @@ int forward_pass(pyfwi_plan *p, int g0, int ng, const float *d_src, float *d_sei
     const int nt = max_nt(sh);
     const int nchp = (int)cl.size();
     if (int e = prepare_fields(p, ng)) return e;
+    // every step kept (chpr=100): the two-launch kernels write the state
+    // straight into its slot
+    const bool direct = snaps && keeps_every_step(cl, nt) && !p->tune_fused;
     int chpc = 0;
     for (int t = 0; t < nt; ++t) {
         if (ckpts && t % K == 0)
             launch_copy_shots(ckpts + (int64_t)(t / K) * fps, (int64_t)ncp * fps, p->cur, fps, fps,
                               ng, p->stream);
+        if (direct) {
+            forward_step(p, dsh, ng, t, d_src, d_seis, snaps + (int64_t)t * fps,
+                         (int64_t)nchp * fps);
+            if (int e = maybe_harvest(p)) return e;
+            continue;
+        }
         forward_step(p, dsh, ng, t, d_src, d_seis);
         if (snaps && std::find(cl.begin(), cl.end(), t) != cl.end()) {
             if (chpc < nchp)
@@ -530,25 +553,34 @@ int32_t pyfwi_forward(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, co
 namespace {
 
 // Reference adjoint of one pass (shots g0 .. g0+ng-1) into gacc[ng].
-int gradient_pyfwi(pyfwi_plan *p, int g0, int ng, const float *d_res, bool precond)
+int gradient_pyfwi(pyfwi_plan *p, int g0, int ng, const float *d_res, bool precond,
+                   const float *snaps)
 {
     const DevGrid &g = p->dg;
     const int64_t fps = fields_per_shot(p);
     const pyfwi_shot *dsh = p->d_shots + g0;
     const int nt = max_nt(p->last_shots);
     const int nchp = (int)p->chp.size();
-    const float *snaps = p->snaps.p + (int64_t)g0 * nchp * fps;
     if (int e = ensure(p->adj, (size_t)ng * fps, p->stream, "adjoint fields")) return e;
     HIPCHK(hipMemsetAsync(p->adj.p, 0, (size_t)ng * fps * sizeof(float), p->stream));
     if (int e = prepare_fields(p, ng)) return e;
+    // every step kept: read the forward state of step t from its slot
+    const bool direct = keeps_every_step(p->chp, nt);
     int chpc = nchp - 1;
     for (int t = nt - 1; t > 0; --t) {
-        if (chpc >= 0 && t == p->chp[chpc]) {
+        const float *fwd = p->cur;
+        int64_t fwd_stride = fps;
+        if (direct) {
+            fwd = snaps + (int64_t)t * fps;
+            fwd_stride = (int64_t)nchp * fps;
+        } else if (chpc >= 0 && t == p->chp[chpc]) {
             launch_copy_shots(p->cur, fps, snaps + (int64_t)chpc * fps, (int64_t)nchp * fps, fps,
                               ng, p->stream);
             chpc--;
+            fwd = p->cur;
         } else {
             reverse_step(p, dsh, ng, t);
+            fwd = p->cur;
         }
         launch_adj_inject(g, ng, dsh, t, p->adj.p, d_res, p->stream);
         {
@@ -557,8 +589,8 @@ int gradient_pyfwi(pyfwi_plan *p, int g0, int ng, const float *d_res, bool preco
         }
         {
             Timed tm(p, PYFWI_K_ADJ_VEL);
-            launch_adj_vel_grad(g, ng, dsh, t, p->cur, p->adj.p, p->lam.p, p->mu.p, p->bu.p,
-                                p->dxp, p->dzp, p->gacc.p, precond, p->stream);
+            launch_adj_vel_grad(g, ng, dsh, t, fwd, fwd_stride, p->adj.p, p->lam.p, p->mu.p,
+                                p->bu.p, p->dxp, p->dzp, p->gacc.p, precond, p->stream);
         }
         if (int e = maybe_harvest(p)) return e;
     }
@@ -669,7 +701,9 @@ int32_t pyfwi_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, c
         if (int e = ensure(p->gacc, (size_t)ng * 6 * g.fstride, p->stream, "gradient")) return e;
         HIPCHK(hipMemsetAsync(p->gacc.p, 0, (size_t)ng * 6 * g.fstride * sizeof(float), p->stream));
         if (mode == PYFWI_MODE_PYFWI) {
-            if (int e = gradient_pyfwi(p, g0, ng, d_res, (flags & PYFWI_F_PRECOND) != 0)) return e;
+            const float *sn = p->snaps.p + (int64_t)g0 * p->chp.size() * fps;
+            if (int e = gradient_pyfwi(p, g0, ng, d_res, (flags & PYFWI_F_PRECOND) != 0, sn))
+                return e;
         } else {
             std::vector<pyfwi_shot> shp(shots + g0, shots + g0 + ng);
             const int ncp = (max_nt(shp) + K - 1) / K;
@@ -697,6 +731,73 @@ int32_t pyfwi_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, c
         HIPCHK(hipMemcpyAsync(grad, d_grad, 6 * n * sizeof(float), hipMemcpyDeviceToHost, p->stream));
         HIPCHK(hipStreamSynchronize(p->stream));
     }
+    if (int e = harvest_timing(p)) return e;
+    return 0;
+}
+
+int32_t pyfwi_misfit_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots,
+                              const float *src, const float *obs, float *seis, float *grad,
+                              double *misfit, const int32_t *chp, int32_t nchp,
+                              int32_t comp_mask, int32_t flags, int32_t mem)
+{
+    if (!p || !src || !obs || !grad || !misfit || !chp || nchp <= 0)
+        return fail(PYFWI_EINVAL, "null argument or empty checkpoint list");
+    if (!p->model_set) return fail(PYFWI_ESTATE, "pyfwi_set_model must come first");
+    if (int e = check_shots(p, shots, nshots)) return e;
+    if (comp_mask < 0 || comp_mask > 31) return fail(PYFWI_EINVAL, "comp_mask must be 0..31");
+    std::vector<pyfwi_shot> sh(shots, shots + nshots);
+    const int nt = max_nt(sh);
+    for (auto &x : sh)
+        if (x.nt != nt) return fail(PYFWI_EINVAL, "checkpoint lists need equal nt for all shots");
+    const DevGrid &g = p->dg;
+    const int64_t fps = fields_per_shot(p);
+    const size_t nrec = rec_extent(shots, nshots), nsrc = src_extent(shots, nshots);
+    const size_t n = (size_t)g.nz * g.nx;
+    if (int e = upload_shots(p, shots, nshots)) return e;
+    const float *d_src = nullptr, *d_obs = nullptr;
+    if (int e = stage_in(p, p->src_stage, src, nsrc, mem, &d_src, "source table")) return e;
+    if (int e = stage_in(p, p->res_stage, obs, nrec, mem, &d_obs, "observed records")) return e;
+    float *d_seis = seis;
+    if (mem != PYFWI_MEM_DEVICE || !seis) {
+        if (int e = ensure(p->seis_stage, nrec, p->stream, "records")) return e;
+        d_seis = p->seis_stage.p;
+    }
+    float *d_grad = grad;
+    if (mem != PYFWI_MEM_DEVICE) {
+        if (int e = ensure(p->grad_stage, 6 * n, p->stream, "gradient output")) return e;
+        d_grad = p->grad_stage.p;
+    }
+    if (int e = ensure(p->resid, nrec, p->stream, "residuals")) return e;
+    if (!p->d_misfit) HIPCHK(hipMalloc(&p->d_misfit, sizeof(double)));
+    HIPCHK(hipMemsetAsync(p->d_misfit, 0, sizeof(double), p->stream));
+    HIPCHK(hipMemsetAsync(d_seis, 0, nrec * sizeof(float), p->stream));
+    std::vector<int32_t> cl(chp, chp + nchp);
+    p->chp = cl;
+    p->last_shots = sh;
+    p->have_forward = false;  // kept states belong to the last pass only
+    const int G = pass_size(p, nshots);
+    if (int e = ensure(p->snaps, (size_t)G * nchp * fps, p->stream, "checkpoints")) return e;
+    for (int g0 = 0; g0 < nshots; g0 += G) {
+        const int ng = std::min(G, nshots - g0);
+        if (int e = forward_pass(p, g0, ng, d_src, d_seis, cl, p->snaps.p, nullptr, 0, 1)) return e;
+        launch_l2_residual(ng, p->d_shots + g0, d_seis, d_obs, p->resid.p, comp_mask, p->d_misfit,
+                           p->stream);
+        if (int e = ensure(p->gacc, (size_t)ng * 6 * g.fstride, p->stream, "gradient")) return e;
+        HIPCHK(hipMemsetAsync(p->gacc.p, 0, (size_t)ng * 6 * g.fstride * sizeof(float), p->stream));
+        if (int e = gradient_pyfwi(p, g0, ng, p->resid.p, (flags & PYFWI_F_PRECOND) != 0,
+                                   p->snaps.p))
+            return e;
+        launch_grad_sum(g, ng, 6, p->gacc.p, d_grad, p->bu.p, 0, g0 > 0, p->stream);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(misfit, p->d_misfit, sizeof(double), hipMemcpyDeviceToHost, p->stream));
+    if (mem != PYFWI_MEM_DEVICE) {
+        HIPCHK(hipMemcpyAsync(grad, d_grad, 6 * n * sizeof(float), hipMemcpyDeviceToHost, p->stream));
+        if (seis)
+            HIPCHK(hipMemcpyAsync(seis, d_seis, nrec * sizeof(float), hipMemcpyDeviceToHost,
+                                  p->stream));
+    }
+    HIPCHK(hipStreamSynchronize(p->stream));
     if (int e = harvest_timing(p)) return e;
     return 0;
 }

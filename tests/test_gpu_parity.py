@@ -184,3 +184,33 @@ def test_execution_shapes_agree(gpu, tuning):
     md = st.medium(w.model, np.float64)
     for a, b in ((G[2], g3[0]), (G[0], g3[1]), (G[4], -(md["b"] ** 2) * g3[2])):
         assert rel(a, b) < GRAD_TOL, tuning
+
+
+@pytest.mark.parametrize("chpr,comp,tuning", [(100, 0, (2, False, 0)), (0, 0, (0, True, 0)),
+                                              (100, 4, (3, False, 0)), (100, 2, (0, True, 0))])
+def test_misfit_gradient_matches_separate_calls(gpu, chpr, comp, tuning):
+    """pyfwi_misfit_gradient (forward, L2 residual and reference adjoint per
+    pass) equals forward_modeling -> CostFunction.l2 -> gradient."""
+    from pyfwi_amd import _native
+    from pyfwi_amd.fwi_tools import CostFunction
+    w = _small(nt=300)
+    W = WavePropagator(w.inpa, w.src, w.rec_loc, w.model_shape, components=comp, chpr=chpr)
+    Wo = WavePropagator(w.inpa, w.src, w.rec_loc, w.model_shape, components=comp, chpr=chpr)
+    d_obs = Wo.forward_modeling(w.model_obs)
+    d_est = W.forward_modeling(w.model)
+    rms, res = CostFunction("l2")(d_est, d_obs)
+    G_ref = W.raw_gradient(res)
+    # observed records in the plan's layout [ns, 5, nt, nr]
+    obs = np.zeros((int(W.ns), 5, int(W.nt_ext), int(W.nr)), np.float32)
+    for f, name in enumerate(("vx", "vz", "taux", "tauz", "tauxz")):
+        if name in d_obs:
+            obs[:, f] = np.reshape(d_obs[name], (int(W.nt_ext), int(W.ns), int(W.nr))).transpose(1, 0, 2)
+    mask = 0 if comp == 0 else {2: 0b00011, 4: 0b11111}[comp]
+    W._plan.set_tuning(*tuning)
+    G = np.zeros_like(G_ref)
+    seis = np.zeros_like(obs)
+    misfit = W._plan.misfit_gradient(W._shots(list(range(int(W.ns)))), W._source_table(), obs,
+                                     seis, G, W.chp, mask, 0, _native.MEM_HOST)
+    assert abs(misfit - rms) < 1e-5 * abs(rms), (misfit, rms)
+    for k in (0, 2, 4):
+        assert rel(G[k], G_ref[k]) < 1e-5, (k, rel(G[k], G_ref[k]))
