@@ -27,6 +27,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from pyfwi_amd import _native  # noqa: E402
+from pyfwi_amd.wave_propagation import checkpoint_steps  # noqa: E402
+
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes of one launch (FP32 words read + written once per cell):
 # (bytes per cell of the whole grid, extra bytes per cell of the interior where
@@ -109,7 +112,18 @@ def device_setup(w, W, torch):
     plan.forward(shots, src, obs, W.chp, 0, _native.MEM_DEVICE)
     W.elastic_buffers(m)
     torch.cuda.synchronize()
-    return dict(plan=plan, shots=shots, src=src, seis=seis, obs=obs, res=res, grad=grad)
+    from pyfwi_amd.wave_propagation import checkpoint_steps
+    return dict(plan=plan, shots=shots, src=src, seis=seis, obs=obs, res=res, grad=grad,
+                chp_all=checkpoint_steps(W.nt, 100))
+
+
+def kept_step(W, b, shots=None):
+    """One misfit + gradient with every forward state kept in HBM per pass
+    (the reference's chpr = 100), residual on the GPU."""
+    from pyfwi_amd import _native
+    shots = b["shots"] if shots is None else shots
+    return b["plan"].misfit_gradient(shots, b["src"], b["obs"], b["seis"], b["grad"],
+                                     b["chp_all"], 0, 0, _native.MEM_DEVICE)
 
 
 def gradient_step(W, b, shots=None, seis=None, obs=None, res=None):
@@ -126,27 +140,27 @@ def gradient_step(W, b, shots=None, seis=None, obs=None, res=None):
     plan.gradient(shots, b["src"], res, b["grad"], _native.MODE_PYFWI, 0, 0, _native.MEM_DEVICE)
 
 
-def per_shot_rate(W, b, torch, nsub, fused):
+def per_shot_rate(W, b, torch, nsub, step):
     """The plain per-shot path: the same kernels on one shot at a time (one
     shot per pass, the reference's schedule), on the first `nsub` shots."""
     cells = int(W.tnz) * int(W.tnx)
     plan = b["plan"]
-    plan.set_tuning(1, fused, 0)
+    plan.set_tuning(1, False, 0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    gradient_step(W, b, b["shots"][:nsub])
+    step(W, b, b["shots"][:nsub])
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     return nsub * cells * W.nt / dt / 1e9, dt
 
 
-def kernel_profile(W, b, torch):
-    """Average launch duration of each stencil kernel over one gradient, from
-    HIP events on the plan's stream."""
+def kernel_profile(W, b, torch, step):
+    """Average launch duration of each stencil kernel over one step, from HIP
+    events on the plan's stream."""
     from pyfwi_amd import _native
     plan = b["plan"]
     plan.set_timing(True)
-    gradient_step(W, b)
+    step(W, b)
     torch.cuda.synchronize()
     out = {}
     for k, name in enumerate(_native.KERNEL_NAMES):
@@ -216,6 +230,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the execution-shape autotuner")
     ap.add_argument("--tune", default="", help="fixed tuning 'shots_per_pass,fused,rows'")
+    ap.add_argument("--mode", default="kept", choices=["kept", "reverse"],
+                    help="kept: every forward state kept in HBM per pass (chpr=100), misfit and "
+                         "gradient in one call; reverse: chpr=0, states rebuilt by time reversal")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -237,34 +254,53 @@ def main():
     units = int(W.ns) * cells * W.nt
     log(f"bench: {w.name}: {int(W.ns)} shots x {W.nt} steps x {cells} cells")
 
-    tuning = (0, True, 0)
+    step = kept_step if args.mode == "kept" else gradient_step
+    state_bytes = 5 * 4 * (int(W.tnz) + 16) * (((int(W.tnx) + 8 + 63) // 64) * 64)
+    free, _total = torch.cuda.mem_get_info()
+    max_pass = int(0.8 * free // (state_bytes * W.nt)) if args.mode == "kept" else None
+    if args.mode == "kept" and max_pass < 1:
+        print(json.dumps({"error": "kept states of one shot do not fit in memory"}))
+        sys.exit(2)
+    tuning = (max_pass or 0, False, 0)
     if args.tune:
         g, f, r = (int(x) for x in args.tune.split(","))
         tuning = (g, bool(f), r)
         b["plan"].set_tuning(*tuning)
     elif not args.no_tune:
-        from pyfwi_amd.tuning import autotune_gradient
-        tuning, _ = autotune_gradient(b["plan"], b["shots"], b["src"], b["seis"], b["res"],
-                                      b["grad"], torch.cuda.synchronize, log=log)
+        from pyfwi_amd import tuning as tn
+        if args.mode == "kept":
+            trial = tn.truncate(b["shots"], 120)
+            chp_t = checkpoint_steps(trial[0].nt, 100)
+
+            def run():
+                b["plan"].misfit_gradient(trial, b["src"], b["obs"], b["seis"], b["grad"],
+                                          chp_t, 0, 0, _native.MEM_DEVICE)
+            cands = tn.default_candidates(int(W.ns), max_pass, fused=(False,))
+            tuning, _ = tn.autotune(b["plan"], run, torch.cuda.synchronize, cands, log=log)
+        else:
+            tuning, _ = tn.autotune_gradient(b["plan"], b["shots"], b["src"], b["seis"], b["res"],
+                                             b["grad"], torch.cuda.synchronize, log=log)
     else:
         b["plan"].set_tuning(*tuning)
-    log(f"bench: tuning shots/pass={tuning[0]} fused={int(tuning[1])} rows={tuning[2]}")
+    log(f"bench: mode={args.mode} tuning shots/pass={tuning[0]} fused={int(tuning[1])} "
+        f"rows={tuning[2]}")
 
     for i in range(args.warmup):
-        gradient_step(W, b)
+        step(W, b)
         torch.cuda.synchronize()
         log(f"bench: warmup {i + 1}/{args.warmup} done")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    misfit = None
     for i in range(args.steps):
-        gradient_step(W, b)
+        misfit = step(W, b)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed / args.steps * 1e3
     value = units / (elapsed / args.steps) / 1e9
     log(f"bench: timed {args.steps} steps, {ms_per_step:.1f} ms/step, {value:.2f} Gcell/s")
 
-    prof = kernel_profile(W, b, torch)
+    prof = kernel_profile(W, b, torch, step)
     dom = max(prof, key=lambda k: prof[k][0])
     ms, n = prof[dom]
     avg_s = ms / n / 1e3
@@ -281,7 +317,7 @@ def main():
 
     ps = None
     if args.per_shot_sample > 0:
-        ps_rate, ps_time = per_shot_rate(W, b, torch, args.per_shot_sample, tuning[1])
+        ps_rate, ps_time = per_shot_rate(W, b, torch, args.per_shot_sample, step)
         b["plan"].set_tuning(*tuning)
         log(f"bench: per-shot path {ps_rate:.2f} Gcell/s")
         ps = {"value": round(ps_rate, 3), "shots_sampled": args.per_shot_sample,
@@ -311,7 +347,10 @@ def main():
         "data": "synthetic (seeded random-layered model, Ricker sources)",
         "config": {"workload": w.name, "nz": int(W.nz), "nx": int(W.nx), "npml": int(W.npml),
                    "cells": cells, "shots": int(W.ns), "nt": int(W.nt), "receivers": int(W.nr),
-                   "order": int(W.sdo) * 2, "gradient": "reference adjoint, chpr=0",
+                   "order": int(W.sdo) * 2,
+                   "gradient": ("reference adjoint, chpr=100: every forward state kept in HBM "
+                                "per pass; L2 misfit on the GPU" if args.mode == "kept" else
+                                "reference adjoint, chpr=0: states rebuilt by time reversal"),
                    "parallelism": "single GPU, shots batched per launch",
                    "tuning": {"shots_per_pass": tuning[0], "fused_steps": bool(tuning[1]),
                               "strip_rows": tuning[2]}},

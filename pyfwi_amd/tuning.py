@@ -15,21 +15,23 @@ import time
 from . import _native
 from .wave_propagation import checkpoint_steps
 
-__all__ = ["default_candidates", "autotune_gradient"]
+__all__ = ["default_candidates", "truncate", "autotune", "autotune_gradient"]
 
 
-def default_candidates(nshots):
+def default_candidates(nshots, max_pass=None, fused=(False, True)):
     """(shots per pass, fused, strip rows) triples; 0 rows = automatic."""
-    passes = sorted({g for g in (nshots, 10, 8, 5, 4, 2) if 1 <= g <= nshots}, reverse=True)
+    top = nshots if max_pass is None else max(1, min(nshots, max_pass))
+    passes = sorted({g for g in (top, 10, 8, 5, 4, 2) if 1 <= g <= top}, reverse=True)
     out = []
     for g in passes:
-        for fused in (False, True):
-            out.append((g, fused, 0))
+        for f in fused:
+            out.append((g, f, 0))
     out += [(passes[0], False, 16), (passes[0], False, 48)]
     return out
 
 
-def _truncate(shots, nt):
+def truncate(shots, nt):
+    """Copies of the shot table entries limited to `nt` time steps."""
     out = []
     for sh in shots:
         s = copy.copy(sh)
@@ -39,32 +41,39 @@ def _truncate(shots, nt):
     return out
 
 
-def autotune_gradient(plan, shots, src, seis, res, grad, synchronize, candidates=None,
-                      nt_trial=200, log=None):
-    """Time each candidate on a truncated forward + reference gradient and
-    configure ``plan`` with the fastest.  Device buffers as for
-    pyfwi_forward / pyfwi_gradient (MEM_DEVICE); ``synchronize()`` waits for
-    the device.  Returns (best, {candidate: seconds})."""
-    trial = _truncate(shots, nt_trial)
-    nt = max(s.nt for s in trial)
-    chp = checkpoint_steps(nt, 0)
-    candidates = candidates or default_candidates(len(shots))
+def autotune(plan, run, synchronize, candidates, log=None):
+    """Time ``run()`` (a truncated workload on the caller's buffers) for each
+    candidate tuning and configure ``plan`` with the fastest.  Returns
+    (best, {candidate: seconds})."""
     times = {}
     for cand in candidates:
-        g, fused, rows = cand
-        plan.set_tuning(g, fused, rows)
+        plan.set_tuning(*cand)
         best = float("inf")
-        for rep in range(2):  # first run warms up (allocation, caches)
+        for rep in range(2):  # the first run warms up (allocation, caches)
             synchronize()
             t0 = time.perf_counter()
-            plan.forward(trial, src, seis, chp, 0, _native.MEM_DEVICE)
-            plan.gradient(trial, src, res, grad, _native.MODE_PYFWI, 0, 0, _native.MEM_DEVICE)
+            run()
             synchronize()
             if rep:
                 best = min(best, time.perf_counter() - t0)
         times[cand] = best
         if log:
-            log(f"tune: shots/pass={g} fused={int(fused)} rows={rows}: {best * 1e3:.1f} ms")
+            log(f"tune: shots/pass={cand[0]} fused={int(cand[1])} rows={cand[2]}: "
+                f"{best * 1e3:.1f} ms")
     best = min(times, key=times.get)
     plan.set_tuning(*best)
     return best, times
+
+
+def autotune_gradient(plan, shots, src, seis, res, grad, synchronize, candidates=None,
+                      nt_trial=200, log=None):
+    """Autotune the forward + reference gradient pair (time reversal between
+    the checkpoints of chpr = 0)."""
+    trial = truncate(shots, nt_trial)
+    chp = checkpoint_steps(max(s.nt for s in trial), 0)
+
+    def run():
+        plan.forward(trial, src, seis, chp, 0, _native.MEM_DEVICE)
+        plan.gradient(trial, src, res, grad, _native.MODE_PYFWI, 0, 0, _native.MEM_DEVICE)
+
+    return autotune(plan, run, synchronize, candidates or default_candidates(len(shots)), log)
