@@ -36,22 +36,34 @@ struct DevGrid {
     int rows_override = 0;  // strip height chosen by the caller (0: automatic)
 };
 
-// snap (optional, forward only): also write the updated fields to
-// snap + shot*snap_stride (the kept state of this step)
+// Velocity half step: state read from `in`, velocities written to `out`
+// (per-shot strides in floats; in == out with stride 5*fstride updates in
+// place, the only form the time-reversed direction takes).
 void launch_vel(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
-                float *fields, const float *bu, const float *dxp, const float *dzp,
-                hipStream_t st, float *snap = nullptr, int64_t snap_stride = 0);
+                const float *in, int64_t in_stride, float *out, int64_t out_stride,
+                const float *bu, const float *dxp, const float *dzp, hipStream_t st);
 // Fused velocity + stress step (forward, DIR=+1) or stress + velocity step
 // (time reversed, DIR=-1): reads `in`, writes `out` (both [nshots][5][fs]).
 void launch_fused_step(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
                        const float *in, float *out, const float *bu, const float *lam,
                        const float *mu, const float *dxp, const float *dzp, hipStream_t st);
+// Stress half step: forward reads this step's velocities from `out` and the
+// previous stresses from `in`, writes the stresses to `out`.
 void launch_str(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
-                float *fields, const float *lam, const float *mu, const float *dxp,
-                const float *dzp, hipStream_t st, float *snap = nullptr,
-                int64_t snap_stride = 0);
+                const float *in, int64_t in_stride, float *out, int64_t out_stride,
+                const float *lam, const float *mu, const float *dxp, const float *dzp,
+                hipStream_t st);
 void launch_inject_record(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
                           float *fields, const float *src, float *seis, hipStream_t st);
+// Kept-state form: record from and inject into `fields` (the slot of step
+// t-1, per-shot stride fields_stride) after restoring the source cell of
+// `prev` (the slot injected at step t-1) from saved[nshots][4], whose values
+// are then replaced by the clean source-cell values of `fields`.  fields ==
+// nullptr only restores `prev`.
+void launch_inject_record_kept(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
+                               float *fields, int64_t fields_stride, float *prev,
+                               int64_t prev_stride, float *saved, const float *src, float *seis,
+                               hipStream_t st);
 void launch_adj_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
                        float *adj, const float *res, hipStream_t st);
 void launch_adj_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,

@@ -217,20 +217,27 @@ __device__ __forceinline__ void prime(ZWin<SDO> &w, int64_t c0, int pitch, F val
 // Velocity half step of every active shot (src/PyFWI/elastic.cl:155-226).
 //   DIR = +1: v += dt*b*D(tau); v -= dt*(dx+dz)*v   (damping after)
 //   DIR = -1: v -= dt*(dx+dz)*v; v -= dt*b*D(tau)   (time reversed)
+//
+// The state is read from `in` and the updated velocities written to `out`
+// (per-shot strides in_stride / out_stride, in floats).  in == out updates in
+// place; the forward of a kept-state run reads the state of step t-1 from its
+// slot and writes step t into the next slot.
 template <int SDO, int DIR>
 __global__ __launch_bounds__(MARCH_X) void k_vel(DevGrid g, const pyfwi_shot *shots, int t,
-                                                 float *fields, const float *__restrict__ bu,
+                                                 const float *in, int64_t in_stride, float *out,
+                                                 int64_t out_stride, const float *__restrict__ bu,
                                                  const float *__restrict__ dxp,
-                                                 const float *__restrict__ dzp, float *snap,
-                                                 int64_t snap_stride)
+                                                 const float *__restrict__ dzp)
 {
     int s, j, i0, i1;
     if (!tile<SDO>(g, s, j, i0, i1)) return;
     if (DIR > 0 ? !fwd_active(shots, s, t) : !bwd_active(shots, s, t)) return;
     const int64_t fs = g.fstride;
     const int P = g.pitch;
-    float *F = fields + (int64_t)s * 5 * fs + g.origin;
-    float *vx = F, *vz = F + fs;
+    const float *F = in + (int64_t)s * in_stride + g.origin;
+    float *O = out + (int64_t)s * out_stride + g.origin;
+    const float *vx = F, *vz = F + fs;
+    float *ox = O, *oz = O + fs;
     const float *txx = F + 2 * fs, *tzz = F + 3 * fs, *txz = F + 4 * fs;
     const float *b = bu + g.origin;
     ZWin<SDO> wxz, wzz;  // txz for d/dz-, tzz for d/dz+
@@ -257,33 +264,33 @@ __global__ __launch_bounds__(MARCH_X) void k_vel(DevGrid g, const pyfwi_shot *sh
             uz = fmaf(-g.dt, damp * uz, uz);
             uz = fmaf(-g.dt, az, uz);
         }
-        vx[c] = ux;
-        vz[c] = uz;
-        if (DIR > 0 && snap) {
-            float *S = snap + (int64_t)s * snap_stride + g.origin;
-            S[c] = ux;
-            S[fs + c] = uz;
-        }
+        ox[c] = ux;
+        oz[c] = uz;
     }
 }
 
-// Stress half step (src/PyFWI/elastic.cl:235-308).
+// Stress half step (src/PyFWI/elastic.cl:235-308).  Forward: the velocities
+// of this step are read from `out` (written there by k_vel), the stresses of
+// the previous step from `in`, and the new stresses written to `out`.  Time
+// reversed: in == out.
 template <int SDO, int DIR>
 __global__ __launch_bounds__(MARCH_X) void k_str(DevGrid g, const pyfwi_shot *shots, int t,
-                                                 float *fields, const float *__restrict__ lam,
+                                                 const float *in, int64_t in_stride, float *out,
+                                                 int64_t out_stride, const float *__restrict__ lam,
                                                  const float *__restrict__ mu,
                                                  const float *__restrict__ dxp,
-                                                 const float *__restrict__ dzp, float *snap,
-                                                 int64_t snap_stride)
+                                                 const float *__restrict__ dzp)
 {
     int s, j, i0, i1;
     if (!tile<SDO>(g, s, j, i0, i1)) return;
     if (DIR > 0 ? !fwd_active(shots, s, t) : !bwd_active(shots, s, t)) return;
     const int64_t fs = g.fstride;
     const int P = g.pitch;
-    float *F = fields + (int64_t)s * 5 * fs + g.origin;
-    const float *vx = F, *vz = F + fs;
-    float *txx = F + 2 * fs, *tzz = F + 3 * fs, *txz = F + 4 * fs;
+    const float *F = in + (int64_t)s * in_stride + g.origin;
+    float *O = out + (int64_t)s * out_stride + g.origin;
+    const float *vx = O, *vz = O + fs;
+    const float *txx = F + 2 * fs, *tzz = F + 3 * fs, *txz = F + 4 * fs;
+    float *oxx = O + 2 * fs, *ozz = O + 3 * fs, *oxz = O + 4 * fs;
     const float *L = lam + g.origin, *M = mu + g.origin;
     ZWin<SDO> wvz, wvx;  // vz for d/dz-, vx for d/dz+
     int64_t c = (int64_t)i0 * P + j;
@@ -319,15 +326,9 @@ __global__ __launch_bounds__(MARCH_X) void k_str(DevGrid g, const pyfwi_shot *sh
             bb = fmaf(-g.dt, szz, bb);
             d = fmaf(-g.dt, sxz, d);
         }
-        txx[c] = a;
-        tzz[c] = bb;
-        txz[c] = d;
-        if (DIR > 0 && snap) {
-            float *S = snap + (int64_t)s * snap_stride + g.origin;
-            S[2 * fs + c] = a;
-            S[3 * fs + c] = bb;
-            S[4 * fs + c] = d;
-        }
+        oxx[c] = a;
+        ozz[c] = bb;
+        oxz[c] = d;
     }
 }
 
@@ -584,16 +585,27 @@ __global__ __launch_bounds__(64 * FUSED_WAVES) void k_fused_rev(
 // holds the state after t steps; the source sample k is added to vx, vz, txx,
 // tzz at the source cell afterwards.
 __global__ void k_inject_record(DevGrid g, const pyfwi_shot *shots, int t,
-                                float *fields, const float *__restrict__ src,
-                                float *seis)
+                                float *fields, int64_t fields_stride, float *prev,
+                                int64_t prev_stride, float *saved,
+                                const float *__restrict__ src, float *seis)
 {
     const int s = blockIdx.x;
     const pyfwi_shot sh = shots[s];
-    if (t >= sh.nt || (t % sh.dt_scale) != 0) return;
-    const int k = t / sh.dt_scale;
     const int64_t fs = g.fstride;
-    float *F = fields + (int64_t)s * 5 * fs + g.origin;
-    if (k < sh.n_samples) {
+    const int64_t cs = (int64_t)sh.src_z * g.pitch + sh.src_x;
+    // kept-state runs inject into the previous step's slot: put back the
+    // source cell of the slot injected one step earlier, and remember the
+    // clean values of this one
+    if (saved && prev && threadIdx.x == 0) {
+        float *Q = prev + (int64_t)s * prev_stride + g.origin;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) Q[f * fs + cs] = saved[s * 4 + f];
+    }
+    if (!fields || t >= sh.nt) return;
+    float *F = fields + (int64_t)s * fields_stride + g.origin;
+    const bool on = (t % sh.dt_scale) == 0;
+    const int k = t / sh.dt_scale;
+    if (on && k < sh.n_samples) {
         float *rec = seis + sh.rec_offset;
         const int64_t c0 = (int64_t)sh.rec_z * g.pitch + sh.rec_x0;
         for (int r = threadIdx.x; r < sh.n_rec; r += blockDim.x) {
@@ -605,12 +617,17 @@ __global__ void k_inject_record(DevGrid g, const pyfwi_shot *shots, int t,
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const float *sv = src + sh.src_offset + (int64_t)k * 5;
-        const int64_t c = (int64_t)sh.src_z * g.pitch + sh.src_x;
-        F[c] += sv[0];
-        F[fs + c] += sv[1];
-        F[2 * fs + c] += sv[2];
-        F[3 * fs + c] += sv[3];
+        if (saved) {
+#pragma unroll
+            for (int f = 0; f < 4; ++f) saved[s * 4 + f] = F[f * fs + cs];
+        }
+        if (on) {
+            const float *sv = src + sh.src_offset + (int64_t)k * 5;
+            F[cs] += sv[0];
+            F[fs + cs] += sv[1];
+            F[2 * fs + cs] += sv[2];
+            F[3 * fs + cs] += sv[3];
+        }
     }
 }
 
@@ -1084,24 +1101,25 @@ static dim3 march_grid(DevGrid &g, int sdo, int nshots)
     } while (0)
 
 void launch_vel(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
-                float *fields, const float *bu, const float *dxp, const float *dzp,
-                hipStream_t st, float *snap, int64_t snap_stride)
+                const float *in, int64_t in_stride, float *out, int64_t out_stride,
+                const float *bu, const float *dxp, const float *dzp, hipStream_t st)
 {
     DevGrid gm = g;
     dim3 grid = march_grid(gm, g.sdo, nshots), block(MARCH_X);
-#define ARGS gm, shots, t, fields, bu, dxp, dzp, snap, snap_stride
+#define ARGS gm, shots, t, in, in_stride, out, out_stride, bu, dxp, dzp
     if (dir > 0) DISPATCH_SDO(g.sdo, k_vel, , 1);
     else DISPATCH_SDO(g.sdo, k_vel, , -1);
 #undef ARGS
 }
 
 void launch_str(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
-                float *fields, const float *lam, const float *mu, const float *dxp,
-                const float *dzp, hipStream_t st, float *snap, int64_t snap_stride)
+                const float *in, int64_t in_stride, float *out, int64_t out_stride,
+                const float *lam, const float *mu, const float *dxp, const float *dzp,
+                hipStream_t st)
 {
     DevGrid gm = g;
     dim3 grid = march_grid(gm, g.sdo, nshots), block(MARCH_X);
-#define ARGS gm, shots, t, fields, lam, mu, dxp, dzp, snap, snap_stride
+#define ARGS gm, shots, t, in, in_stride, out, out_stride, lam, mu, dxp, dzp
     if (dir > 0) DISPATCH_SDO(g.sdo, k_str, , 1);
     else DISPATCH_SDO(g.sdo, k_str, , -1);
 #undef ARGS
@@ -1141,8 +1159,17 @@ void launch_fused_step(const DevGrid &g, int dir, int nshots, const pyfwi_shot *
 void launch_inject_record(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
                           float *fields, const float *src, float *seis, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_inject_record, dim3(nshots), dim3(256), 0, st, g, shots, t,
-                       fields, src, seis);
+    hipLaunchKernelGGL(k_inject_record, dim3(nshots), dim3(256), 0, st, g, shots, t, fields,
+                       (int64_t)5 * g.fstride, nullptr, (int64_t)0, nullptr, src, seis);
+}
+
+void launch_inject_record_kept(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
+                               float *fields, int64_t fields_stride, float *prev,
+                               int64_t prev_stride, float *saved, const float *src, float *seis,
+                               hipStream_t st)
+{
+    hipLaunchKernelGGL(k_inject_record, dim3(nshots), dim3(256), 0, st, g, shots, t, fields,
+                       fields_stride, prev, prev_stride, saved, src, seis);
 }
 
 void launch_adj_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,

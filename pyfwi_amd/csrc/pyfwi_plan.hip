@@ -88,7 +88,7 @@ struct pyfwi_plan {
     // tuning: shots per pass (0: all), fused steps, strip height (0: automatic)
     int tune_group = 0, tune_fused = 1, tune_rows = 0;
     bool ckpts_all = false;  // ckpts holds every shot of the last forward
-    DevBuf src_stage, seis_stage, res_stage, grad_stage, shot_tab_f, resid;
+    DevBuf src_stage, seis_stage, res_stage, grad_stage, shot_tab_f, resid, src_saved;
     double *d_misfit = nullptr;
     pyfwi_shot *d_shots = nullptr;
     int shots_cap = 0;
@@ -308,7 +308,7 @@ int32_t pyfwi_plan_destroy(pyfwi_plan *p)
     for (DevBuf *b : {&p->lam, &p->mu, &p->bu, &p->dxp_buf, &p->dzp_buf, &p->scratch_in, &p->fields, &p->fields2,
                       &p->adj, &p->av0, &p->av1, &p->gacc, &p->snaps, &p->ckpts, &p->seg,
                       &p->src_stage, &p->seis_stage, &p->res_stage, &p->grad_stage,
-                      &p->shot_tab_f, &p->resid})
+                      &p->shot_tab_f, &p->resid, &p->src_saved})
         b->release();
     if (p->d_misfit) (void)hipFree(p->d_misfit);
     if (p->d_shots) (void)hipFree(p->d_shots);
@@ -381,9 +381,10 @@ namespace {
 // velocity and stress half steps, either as one fused launch into the other
 // buffer or as two in-place launches.
 int forward_step(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t, const float *src,
-                 float *seis, float *snap = nullptr, int64_t snap_stride = 0)
+                 float *seis)
 {
     const DevGrid &g = p->dg;
+    const int64_t fps = fields_per_shot(p);
     launch_inject_record(g, ng, dsh, t, p->cur, src, seis, p->stream);
     if (p->tune_fused) {
         float *nxt = p->cur == p->fields.p ? p->fields2.p : p->fields.p;
@@ -394,13 +395,35 @@ int forward_step(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t, const floa
     } else {
         {
             Timed tm(p, PYFWI_K_FWD_VEL);
-            launch_vel(g, +1, ng, dsh, t, p->cur, p->bu.p, p->dxp, p->dzp, p->stream, snap,
-                       snap_stride);
+            launch_vel(g, +1, ng, dsh, t, p->cur, fps, p->cur, fps, p->bu.p, p->dxp, p->dzp,
+                       p->stream);
         }
         Timed tm(p, PYFWI_K_FWD_STR);
-        launch_str(g, +1, ng, dsh, t, p->cur, p->lam.p, p->mu.p, p->dxp, p->dzp, p->stream, snap,
-                   snap_stride);
+        launch_str(g, +1, ng, dsh, t, p->cur, fps, p->cur, fps, p->lam.p, p->mu.p, p->dxp, p->dzp,
+                   p->stream);
     }
+    return 0;
+}
+
+// Forward step of a kept-state run: the state of step t-1 is read from `in`
+// (per-shot stride in_stride) and step t written to `out`.  The source is
+// injected into `in` and taken out again by the next step's injection launch
+// (or finish_kept), so every slot ends up holding the clean state.
+int forward_step_kept(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t, const float *src,
+                      float *seis, float *in, int64_t in_stride, float *prev, int64_t prev_stride,
+                      float *out, int64_t out_stride)
+{
+    const DevGrid &g = p->dg;
+    launch_inject_record_kept(g, ng, dsh, t, in, in_stride, prev, prev_stride, p->src_saved.p,
+                              src, seis, p->stream);
+    {
+        Timed tm(p, PYFWI_K_FWD_VEL);
+        launch_vel(g, +1, ng, dsh, t, in, in_stride, out, out_stride, p->bu.p, p->dxp, p->dzp,
+                   p->stream);
+    }
+    Timed tm(p, PYFWI_K_FWD_STR);
+    launch_str(g, +1, ng, dsh, t, in, in_stride, out, out_stride, p->lam.p, p->mu.p, p->dxp,
+               p->dzp, p->stream);
     return 0;
 }
 
@@ -415,12 +438,15 @@ int reverse_step(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t)
                           p->dzp, p->stream);
         p->cur = nxt;
     } else {
+        const int64_t fps = fields_per_shot(p);
         {
             Timed tm(p, PYFWI_K_REV_STR);
-            launch_str(g, -1, ng, dsh, t, p->cur, p->lam.p, p->mu.p, p->dxp, p->dzp, p->stream);
+            launch_str(g, -1, ng, dsh, t, p->cur, fps, p->cur, fps, p->lam.p, p->mu.p, p->dxp,
+                       p->dzp, p->stream);
         }
         Timed tm(p, PYFWI_K_REV_VEL);
-        launch_vel(g, -1, ng, dsh, t, p->cur, p->bu.p, p->dxp, p->dzp, p->stream);
+        launch_vel(g, -1, ng, dsh, t, p->cur, fps, p->cur, fps, p->bu.p, p->dxp, p->dzp,
+                   p->stream);
     }
     return 0;
 }
@@ -465,20 +491,36 @@ int forward_pass(pyfwi_plan *p, int g0, int ng, const float *d_src, float *d_sei
     const int nt = max_nt(sh);
     const int nchp = (int)cl.size();
     if (int e = prepare_fields(p, ng)) return e;
-    // every step kept (chpr=100): the two-launch kernels write the state
-    // straight into its slot
-    const bool direct = snaps && keeps_every_step(cl, nt) && !p->tune_fused;
+    // every step kept (chpr=100): the two-launch kernels step from one slot
+    // into the next, so the kept states cost no extra traffic
+    if (snaps && keeps_every_step(cl, nt) && !p->tune_fused) {
+        if (int e = ensure(p->src_saved, (size_t)ng * 4, p->stream, "source cells")) return e;
+        const int64_t ss = (int64_t)nchp * fps;
+        float *in = p->fields.p, *prev = nullptr;
+        int64_t in_stride = fps, prev_stride = 0;
+        for (int t = 0; t < nt; ++t) {
+            if (ckpts && t % K == 0)
+                launch_copy_shots(ckpts + (int64_t)(t / K) * fps, (int64_t)ncp * fps, in,
+                                  in_stride, fps, ng, p->stream);
+            float *out = snaps + (int64_t)t * fps;
+            forward_step_kept(p, dsh, ng, t, d_src, d_seis, in, in_stride, prev, prev_stride, out,
+                              ss);
+            prev = in;
+            prev_stride = in_stride;
+            in = out;
+            in_stride = ss;
+            if (int e = maybe_harvest(p)) return e;
+        }
+        // take the last injection out of its slot
+        launch_inject_record_kept(p->dg, ng, dsh, nt, nullptr, 0, prev, prev_stride,
+                                  p->src_saved.p, d_src, d_seis, p->stream);
+        return 0;
+    }
     int chpc = 0;
     for (int t = 0; t < nt; ++t) {
         if (ckpts && t % K == 0)
             launch_copy_shots(ckpts + (int64_t)(t / K) * fps, (int64_t)ncp * fps, p->cur, fps, fps,
                               ng, p->stream);
-        if (direct) {
-            forward_step(p, dsh, ng, t, d_src, d_seis, snaps + (int64_t)t * fps,
-                         (int64_t)nchp * fps);
-            if (int e = maybe_harvest(p)) return e;
-            continue;
-        }
         forward_step(p, dsh, ng, t, d_src, d_seis);
         if (snaps && std::find(cl.begin(), cl.end(), t) != cl.end()) {
             if (chpc < nchp)
@@ -627,14 +669,15 @@ int gradient_exact(pyfwi_plan *p, int g0, int ng, const float *d_src, const floa
             launch_inject_record(g, ng, dsh, t, p->fields.p, d_src, p->seis_stage.p, p->stream);
             {
                 Timed tm(p, PYFWI_K_FWD_VEL);
-                launch_vel(g, +1, ng, dsh, t, p->fields.p, p->bu.p, p->dxp, p->dzp, p->stream);
+                launch_vel(g, +1, ng, dsh, t, p->fields.p, fps, p->fields.p, fps, p->bu.p, p->dxp,
+                           p->dzp, p->stream);
             }
             launch_copy_shots(p->seg.p + (int64_t)(t - t0) * fps, seg_stride, p->fields.p, fps,
                               fps, ng, p->stream);
             if (t + 1 < t1) {
                 Timed tm(p, PYFWI_K_FWD_STR);
-                launch_str(g, +1, ng, dsh, t, p->fields.p, p->lam.p, p->mu.p, p->dxp, p->dzp,
-                           p->stream);
+                launch_str(g, +1, ng, dsh, t, p->fields.p, fps, p->fields.p, fps, p->lam.p,
+                           p->mu.p, p->dxp, p->dzp, p->stream);
             }
         }
         for (int t = t1 - 1; t >= t0; --t) {

@@ -126,11 +126,15 @@ def test_gradient_energy_balancing(gpu):
         assert rel(g[k], gr[k]) < GRAD_TOL, k
 
 
-def test_snapshots_match_oracle(gpu):
-    w = _small(nt=200)
-    W = WavePropagator(w.inpa, w.src, w.rec_loc, w.model_shape, components=2, chpr=5)
+@pytest.mark.parametrize("chpr,nt", [(5, 200), (100, 120)])
+def test_snapshots_match_oracle(gpu, chpr, nt):
+    """Kept states: copies at the checkpoint steps (chpr=5), or every step
+    written slot to slot by the forward kernels with the source injection
+    taken out of each slot again (chpr=100)."""
+    w = _small(nt=nt)
+    W = WavePropagator(w.inpa, w.src, w.rec_loc, w.model_shape, components=2, chpr=chpr)
     W.forward_modeling(w.model)
-    st = oracle_setup(w, chpr=5, components=2)
+    st = oracle_setup(w, chpr=chpr, components=2)
     _, states = O.forward(st, w.model, np.float32)
     Wd = W.W
     for f, name in enumerate(O.FIELDS):
