@@ -18,7 +18,10 @@ constexpr int TARGET_WAVES = 8192;  // wavefronts per launch to aim for (32 per 
 constexpr int NUM_XCD = 8;          // MI355X: 8 XCDs, each with its own L2
 constexpr int FUSED_WAVES = 4;      // fused two-stage kernels: wavefronts per block
 #ifndef MARCH_UNROLL
-#define MARCH_UNROLL 1              // row-loop unrolling (register pressure outweighs it on gfx950)
+#define MARCH_UNROLL 2              // rows per unrolled iteration: two rows of loads in flight per wave
+#endif
+#ifndef ADJ_VEL_UNROLL
+#define ADJ_VEL_UNROLL 1            // the adjoint velocity + imaging kernel: registers limit it
 #endif
 
 // Constants every kernel takes by value.

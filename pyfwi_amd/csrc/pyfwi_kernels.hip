@@ -2,10 +2,11 @@
 // gfx950 (MI355X).  FP32 throughout, no MFMA: the path is HBM-bandwidth bound.
 //
 // Device layout (see pyfwi_plan.hip): every field and model array is a padded
-// row-major grid with row pitch P (a multiple of 64 floats, P >= tnx + 4) and
+// row-major grid with row pitch P (a multiple of 64 floats, P >= tnx + 8) and
 // HALO zero rows above and below; cell (i, j) of the reference's tnz x tnx
-// buffer lives at origin + i*P + j.  Columns tnx..P-1 of one row double as the
-// left halo of the next row, so every stencil tap of an active cell reads
+// buffer lives at origin + i*P + j, with origin = HALO*P + 64 - sdo so that
+// column sdo starts a 256-byte line.  The P - tnx zero columns after a row
+// double as the left halo of the next row, so every stencil tap of an active cell reads
 // either a real cell or a zero, which is what the reference's flat-index taps
 // see inside its never-updated outer ring (src/PyFWI/elastic.cl:163-166).
 #include <hip/hip_runtime.h>
@@ -178,6 +179,71 @@ struct ZWin {
     }
 };
 
+// A field seen through a byte offset: `base` is wave-uniform (scalar
+// registers) and the per-lane part is one 32-bit byte offset shared by all
+// fields with the same layout, so each extra field costs no vector registers
+// for its address.  at(o, k) reads k columns to the right of offset o.
+struct CellRef {
+    const float *base;
+    __device__ __forceinline__ float operator()(uint32_t o) const
+    {
+        return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + o);
+    }
+    __device__ __forceinline__ float at(uint32_t o, int k) const
+    {
+        return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base + k) + o);
+    }
+};
+
+__device__ __forceinline__ float &cell_ref(float *base, uint32_t o)
+{
+    return *reinterpret_cast<float *>(reinterpret_cast<char *>(base) + o);
+}
+
+// Staggered x-derivatives of a quantity given per column offset k.
+template <int SDO, class V>
+__device__ __forceinline__ float xminus(V v, const float c[4])
+{
+    float r = c[0] * (v(0) - v(-1));
+    r = fmaf(c[1], v(1) - v(-2), r);
+    if constexpr (SDO == 4) {
+        r = fmaf(c[2], v(2) - v(-3), r);
+        r = fmaf(c[3], v(3) - v(-4), r);
+    }
+    return r;
+}
+
+template <int SDO, class V>
+__device__ __forceinline__ float xplus(V v, const float c[4])
+{
+    float r = c[0] * (v(1) - v(0));
+    r = fmaf(c[1], v(2) - v(-1), r);
+    if constexpr (SDO == 4) {
+        r = fmaf(c[2], v(3) - v(-2), r);
+        r = fmaf(c[3], v(4) - v(-3), r);
+    }
+    return r;
+}
+
+// Delay line of N values: pop() returns the oldest value and appends the
+// newest, so a value comes back N calls after it went in.  The marching
+// kernels take the x-derivatives of a windowed field on the row entering its
+// z-window, while that row's cache lines are fresh, and delay them SDO rows
+// to the centre row where they are used; taking them at the centre row
+// instead re-fetches the row from memory.
+template <int N>
+struct Fifo {
+    float v[N];
+    __device__ __forceinline__ float pop(float x)
+    {
+        const float out = v[0];
+#pragma unroll
+        for (int k = 0; k < N - 1; ++k) v[k] = v[k + 1];
+        v[N - 1] = x;
+        return out;
+    }
+};
+
 // Tile of this block and column / row strip of this thread.
 //
 // Launches are 1-D.  The tiles (column block, row strip, shot) are numbered
@@ -233,26 +299,35 @@ __global__ __launch_bounds__(MARCH_X) void k_vel(DevGrid g, const pyfwi_shot *sh
     if (!tile<SDO>(g, s, j, i0, i1)) return;
     if (DIR > 0 ? !fwd_active(shots, s, t) : !bwd_active(shots, s, t)) return;
     const int64_t fs = g.fstride;
-    const int P = g.pitch;
+    const uint32_t RB = (uint32_t)g.pitch * 4u;
     const float *F = in + (int64_t)s * in_stride + g.origin;
     float *O = out + (int64_t)s * out_stride + g.origin;
-    const float *vx = F, *vz = F + fs;
-    float *ox = O, *oz = O + fs;
-    const float *txx = F + 2 * fs, *tzz = F + 3 * fs, *txz = F + 4 * fs;
-    const float *b = bu + g.origin;
+    const CellRef vx{F}, vz{F + fs}, txx{F + 2 * fs}, tzz{F + 3 * fs}, txz{F + 4 * fs};
+    const CellRef B{bu + g.origin};
     ZWin<SDO> wxz, wzz;  // txz for d/dz-, tzz for d/dz+
-    int64_t c = (int64_t)i0 * P + j;
-    prime(wxz, c, P, [&](int64_t k) { return txz[k]; });
-    prime(wzz, c, P, [&](int64_t k) { return tzz[k]; });
+    Fifo<SDO> fxz;   // d/dx+ txz
+    auto dx_txz = [&](uint32_t o) { return xplus<SDO>([&](int k) { return txz.at(o, k); }, g.cx); };
+    uint32_t cb = ((uint32_t)i0 * (uint32_t)g.pitch + (uint32_t)j) * 4u;
+#pragma unroll
+    for (int k = 0; k < 2 * SDO; ++k) {
+        const uint32_t o = cb + (uint32_t)(k - SDO) * RB;
+        wxz.v[k + 1] = txz(o);
+        wzz.v[k + 1] = tzz(o);
+        if (k >= SDO) fxz.v[k - SDO] = dx_txz(o);
+    }
     const float ddx = dxp[j];
 #pragma unroll MARCH_UNROLL
-    for (int i = i0; i <= i1; ++i, c += P) {
-        wxz.push(txz[c + (int64_t)SDO * P]);
-        wzz.push(tzz[c + (int64_t)SDO * P]);
-        const float ax = b[c] * (dminus<SDO>(txx + c, 1, g.cx) + wxz.dminus(g.cz));
-        const float az = b[c] * (wzz.dplus(g.cz) + dplus<SDO>(txz + c, 1, g.cx));
+    for (int i = i0; i <= i1; ++i, cb += RB) {
+        const uint32_t n = cb + (uint32_t)SDO * RB;
+        wxz.push(txz(n));
+        wzz.push(tzz(n));
+        const float dxp_txz = fxz.pop(dx_txz(n));
+        const float bc = B(cb);
+        const float ax = bc * (xminus<SDO>([&](int k) { return txx.at(cb, k); }, g.cx)
+                               + wxz.dminus(g.cz));
+        const float az = bc * (wzz.dplus(g.cz) + dxp_txz);
         const float damp = ddx + dzp[i];
-        float ux = vx[c], uz = vz[c];
+        float ux = vx(cb), uz = vz(cb);
         if (DIR > 0) {
             ux = fmaf(g.dt, ax, ux);
             ux = fmaf(g.dt, -damp * ux, ux);
@@ -264,8 +339,8 @@ __global__ __launch_bounds__(MARCH_X) void k_vel(DevGrid g, const pyfwi_shot *sh
             uz = fmaf(-g.dt, damp * uz, uz);
             uz = fmaf(-g.dt, az, uz);
         }
-        ox[c] = ux;
-        oz[c] = uz;
+        cell_ref(O, cb) = ux;
+        cell_ref(O + fs, cb) = uz;
     }
 }
 
@@ -285,32 +360,43 @@ __global__ __launch_bounds__(MARCH_X) void k_str(DevGrid g, const pyfwi_shot *sh
     if (!tile<SDO>(g, s, j, i0, i1)) return;
     if (DIR > 0 ? !fwd_active(shots, s, t) : !bwd_active(shots, s, t)) return;
     const int64_t fs = g.fstride;
-    const int P = g.pitch;
+    const uint32_t RB = (uint32_t)g.pitch * 4u;
     const float *F = in + (int64_t)s * in_stride + g.origin;
     float *O = out + (int64_t)s * out_stride + g.origin;
-    const float *vx = O, *vz = O + fs;
-    const float *txx = F + 2 * fs, *tzz = F + 3 * fs, *txz = F + 4 * fs;
-    float *oxx = O + 2 * fs, *ozz = O + 3 * fs, *oxz = O + 4 * fs;
-    const float *L = lam + g.origin, *M = mu + g.origin;
-    ZWin<SDO> wvz, wvx;  // vz for d/dz-, vx for d/dz+
-    int64_t c = (int64_t)i0 * P + j;
-    prime(wvz, c, P, [&](int64_t k) { return vz[k]; });
-    prime(wvx, c, P, [&](int64_t k) { return vx[k]; });
+    const CellRef vx{O}, vz{O + fs}, txx{F + 2 * fs}, tzz{F + 3 * fs}, txz{F + 4 * fs};
+    const CellRef L{lam + g.origin}, M{mu + g.origin};
+    ZWin<SDO> wvz, wvx;      // vz for d/dz-, vx for d/dz+
+    Fifo<SDO> fxx, fxz;  // d/dx+ vx, d/dx- vz
+    auto dx_vx = [&](uint32_t o) { return xplus<SDO>([&](int k) { return vx.at(o, k); }, g.cx); };
+    auto dx_vz = [&](uint32_t o) { return xminus<SDO>([&](int k) { return vz.at(o, k); }, g.cx); };
+    uint32_t cb = ((uint32_t)i0 * (uint32_t)g.pitch + (uint32_t)j) * 4u;
+#pragma unroll
+    for (int k = 0; k < 2 * SDO; ++k) {
+        const uint32_t o = cb + (uint32_t)(k - SDO) * RB;
+        wvz.v[k + 1] = vz(o);
+        wvx.v[k + 1] = vx(o);
+        if (k >= SDO) {
+            fxx.v[k - SDO] = dx_vx(o);
+            fxz.v[k - SDO] = dx_vz(o);
+        }
+    }
     const float ddx = dxp[j];
 #pragma unroll MARCH_UNROLL
-    for (int i = i0; i <= i1; ++i, c += P) {
-        wvz.push(vz[c + (int64_t)SDO * P]);
-        wvx.push(vx[c + (int64_t)SDO * P]);
-        const float exx = dplus<SDO>(vx + c, 1, g.cx);
+    for (int i = i0; i <= i1; ++i, cb += RB) {
+        const uint32_t n = cb + (uint32_t)SDO * RB;
+        wvz.push(vz(n));
+        wvx.push(vx(n));
+        const float exx = fxx.pop(dx_vx(n));
+        const float dxm_vz = fxz.pop(dx_vz(n));
         const float ezz = wvz.dminus(g.cz);
-        const float gxz = dminus<SDO>(vz + c, 1, g.cx) + wvx.dplus(g.cz);
-        const float l = L[c], m = M[c];
+        const float gxz = dxm_vz + wvx.dplus(g.cz);
+        const float l = L(cb), m = M(cb);
         const float lm = l + 2.0f * m;
         const float sxx = lm * exx + l * ezz;
         const float szz = lm * ezz + l * exx;
         const float sxz = m * gxz;
         const float damp = ddx + dzp[i];
-        float a = txx[c], bb = tzz[c], d = txz[c];
+        float a = txx(cb), bb = tzz(cb), d = txz(cb);
         if (DIR > 0) {
             a = fmaf(g.dt, sxx, a);
             bb = fmaf(g.dt, szz, bb);
@@ -326,9 +412,9 @@ __global__ __launch_bounds__(MARCH_X) void k_str(DevGrid g, const pyfwi_shot *sh
             bb = fmaf(-g.dt, szz, bb);
             d = fmaf(-g.dt, sxz, d);
         }
-        oxx[c] = a;
-        ozz[c] = bb;
-        oxz[c] = d;
+        cell_ref(O + 2 * fs, cb) = a;
+        cell_ref(O + 3 * fs, cb) = bb;
+        cell_ref(O + 4 * fs, cb) = d;
     }
 }
 
@@ -669,30 +755,43 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot
     if (!tile<SDO>(g, s, j, i0, i1)) return;
     if (!bwd_active(shots, s, t)) return;
     const int64_t fs = g.fstride;
-    const int P = g.pitch;
+    const uint32_t RB = (uint32_t)g.pitch * 4u;
     float *A = adj + (int64_t)s * 5 * fs + g.origin;
-    const float *avx = A, *avz = A + fs;
-    const float *b = bu + g.origin;
-    float *atxx = A + 2 * fs, *atzz = A + 3 * fs, *atxz = A + 4 * fs;
-    ZWin<SDO> wbz, wbx;  // b*avz for d/dz-, b*avx for d/dz+
-    int64_t c = (int64_t)i0 * P + j;
-    prime(wbz, c, P, [&](int64_t k) { return avz[k] * b[k]; });
-    prime(wbx, c, P, [&](int64_t k) { return avx[k] * b[k]; });
+    const CellRef avx{A}, avz{A + fs}, B{bu + g.origin};
+    ZWin<SDO> wbz, wbx;      // b*avz for d/dz-, b*avx for d/dz+
+    Fifo<SDO> fxp, fxm;  // d/dx+ (b*avx), d/dx- (b*avz)
+    auto dx_bvx = [&](uint32_t o) {
+        return xplus<SDO>([&](int k) { return avx.at(o, k) * B.at(o, k); }, g.cx); };
+    auto dx_bvz = [&](uint32_t o) {
+        return xminus<SDO>([&](int k) { return avz.at(o, k) * B.at(o, k); }, g.cx); };
+    uint32_t cb = ((uint32_t)i0 * (uint32_t)g.pitch + (uint32_t)j) * 4u;
+#pragma unroll
+    for (int k = 0; k < 2 * SDO; ++k) {
+        const uint32_t o = cb + (uint32_t)(k - SDO) * RB;
+        wbz.v[k + 1] = avz(o) * B(o);
+        wbx.v[k + 1] = avx(o) * B(o);
+        if (k >= SDO) {
+            fxp.v[k - SDO] = dx_bvx(o);
+            fxm.v[k - SDO] = dx_bvz(o);
+        }
+    }
     const float ex = dxp[j];
 #pragma unroll MARCH_UNROLL
-    for (int i = i0; i <= i1; ++i, c += P) {
-        const int64_t n = c + (int64_t)SDO * P;
-        wbz.push(avz[n] * b[n]);
-        wbx.push(avx[n] * b[n]);
-        const float dxpb = dplus2<SDO>(avx + c, b + c, 1, g.cx);
+    for (int i = i0; i <= i1; ++i, cb += RB) {
+        const uint32_t n = cb + (uint32_t)SDO * RB;
+        wbz.push(avz(n) * B(n));
+        wbx.push(avx(n) * B(n));
+        const float dxpb = fxp.pop(dx_bvx(n));
+        const float dxmb = fxm.pop(dx_bvz(n));
         const float dzmb = wbz.dminus(g.cz);
         const float dzpb = wbx.dplus(g.cz);
-        const float dxmb = dminus2<SDO>(avz + c, b + c, 1, g.cx);
         const float ez = dzp[i];
-        const float a = atxx[c], bb = atzz[c], d = atxz[c];
-        atxx[c] = fmaf(-g.dt, ex * a + dxpb, a);
-        atzz[c] = fmaf(-g.dt, ez * bb + dzmb, bb);
-        atxz[c] = fmaf(-g.dt, (ex + ez) * d + (dzpb + dxmb), d);
+        float &ra = cell_ref(A + 2 * fs, cb), &rb = cell_ref(A + 3 * fs, cb),
+              &rd = cell_ref(A + 4 * fs, cb);
+        const float a = ra, bb = rb, d = rd;
+        ra = fmaf(-g.dt, ex * a + dxpb, a);
+        rb = fmaf(-g.dt, ez * bb + dzmb, bb);
+        rd = fmaf(-g.dt, (ex + ez) * d + (dzpb + dxmb), d);
     }
 }
 
@@ -712,78 +811,112 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
     if (!tile<SDO>(g, s, j, i0, i1)) return;
     if (!bwd_active(shots, s, t)) return;
     const int64_t fs = g.fstride;
-    const int P = g.pitch;
+    const uint32_t RB = (uint32_t)g.pitch * 4u;
     float *A = adj + (int64_t)s * 5 * fs + g.origin;
-    const float *atxx = A + 2 * fs, *atzz = A + 3 * fs, *atxz = A + 4 * fs;
-    const float *L = lam + g.origin, *M = mu + g.origin;
+    const CellRef atxx{A + 2 * fs}, atzz{A + 3 * fs}, atxz{A + 4 * fs};
+    const CellRef L{lam + g.origin}, M{mu + g.origin}, B{bu + g.origin};
     // imaging condition: interior rows lo..hi and columns lo..hix only
     const int lo = g.npml + SDO, hi = g.nz - SDO - g.npml, hix = g.nx - SDO - g.npml;
     const bool gcol = j >= lo && j <= hix;
     const bool grow_strip = i1 >= lo && i0 <= hi;
     const bool img = gcol && grow_strip;
     const float *F = fields + (int64_t)s * fields_stride + g.origin;
-    const float *vx = F, *vz = F + fs, *txx = F + 2 * fs, *tzz = F + 3 * fs, *txz = F + 4 * fs;
+    const CellRef vx{F}, vz{F + fs}, txx{F + 2 * fs}, tzz{F + 3 * fs}, txz{F + 4 * fs};
     float *G = gacc + (int64_t)s * 6 * fs + g.origin;
-    const float *b = bu + g.origin;
 
     ZWin<SDO> wa3, wb1, wb2;  // mu*atxz (d/dz-), (lam+2mu)*atzz, lam*atxx (d/dz+)
     ZWin<SDO> fvz, ftzz, ftxz;  // forward vz, tzz, txz for the imaging condition
-    int64_t c = (int64_t)i0 * P + j;
-    prime(wa3, c, P, [&](int64_t k) { return M[k] * atxz[k]; });
-    prime(wb1, c, P, [&](int64_t k) { return (L[k] + 2.0f * M[k]) * atzz[k]; });
-    prime(wb2, c, P, [&](int64_t k) { return L[k] * atxx[k]; });
-    if (img) {
-        prime(fvz, c, P, [&](int64_t k) { return vz[k]; });
-        prime(ftzz, c, P, [&](int64_t k) { return tzz[k]; });
-        prime(ftxz, c, P, [&](int64_t k) { return txz[k]; });
+    // delayed to the centre row: x-derivatives of the windowed fields and the
+    // centre adjoint stresses
+    Fifo<SDO> fa1, fa2, fb3, faa, fab, fad, fdvz, fdtxz;
+    auto q_a3 = [&](uint32_t o) { return M(o) * atxz(o); };
+    auto q_b1 = [&](uint32_t o) { return (L(o) + 2.0f * M(o)) * atzz(o); };
+    auto q_b2 = [&](uint32_t o) { return L(o) * atxx(o); };
+    auto x_a1 = [&](uint32_t o) {
+        return xminus<SDO>([&](int k) {
+            return (L.at(o, k) + 2.0f * M.at(o, k)) * atxx.at(o, k); }, g.cx); };
+    auto x_a2 = [&](uint32_t o) {
+        return xminus<SDO>([&](int k) { return L.at(o, k) * atzz.at(o, k); }, g.cx); };
+    auto x_b3 = [&](uint32_t o) {
+        return xplus<SDO>([&](int k) { return M.at(o, k) * atxz.at(o, k); }, g.cx); };
+    auto x_vz = [&](uint32_t o) { return xminus<SDO>([&](int k) { return vz.at(o, k); }, g.cx); };
+    auto x_txz = [&](uint32_t o) { return xplus<SDO>([&](int k) { return txz.at(o, k); }, g.cx); };
+    uint32_t cb = ((uint32_t)i0 * (uint32_t)g.pitch + (uint32_t)j) * 4u;
+#pragma unroll
+    for (int k = 0; k < 2 * SDO; ++k) {
+        const uint32_t o = cb + (uint32_t)(k - SDO) * RB;
+        wa3.v[k + 1] = q_a3(o);
+        wb1.v[k + 1] = q_b1(o);
+        wb2.v[k + 1] = q_b2(o);
+        if (k >= SDO) {
+            const int q = k - SDO;
+            fa1.v[q] = x_a1(o);
+            fa2.v[q] = x_a2(o);
+            fb3.v[q] = x_b3(o);
+            faa.v[q] = atxx(o);
+            fab.v[q] = atzz(o);
+            fad.v[q] = atxz(o);
+        }
+        if (img) {
+            fvz.v[k + 1] = vz(o);
+            ftzz.v[k + 1] = tzz(o);
+            ftxz.v[k + 1] = txz(o);
+            if (k >= SDO) {
+                fdvz.v[k - SDO] = x_vz(o);
+                fdtxz.v[k - SDO] = x_txz(o);
+            }
+        }
     }
     const float ex = dxp[j];
     const float dt = g.dt;
-#pragma unroll MARCH_UNROLL
-    for (int i = i0; i <= i1; ++i, c += P) {
-        const int64_t n = c + (int64_t)SDO * P;
-        wa3.push(M[n] * atxz[n]);
-        wb1.push((L[n] + 2.0f * M[n]) * atzz[n]);
-        wb2.push(L[n] * atxx[n]);
-        const float a1 = dminus_m<SDO>(atxx + c, L + c, M + c, 1, g.cx);
-        const float a2 = dminus2<SDO>(atzz + c, L + c, 1, g.cx);
+#pragma unroll ADJ_VEL_UNROLL
+    for (int i = i0; i <= i1; ++i, cb += RB) {
+        const uint32_t n = cb + (uint32_t)SDO * RB;
+        wa3.push(q_a3(n));
+        wb1.push(q_b1(n));
+        wb2.push(q_b2(n));
+        const float a1 = fa1.pop(x_a1(n)), a2 = fa2.pop(x_a2(n));
+        const float b3 = fb3.pop(x_b3(n));
+        const float aa = faa.pop(atxx(n)), ab = fab.pop(atzz(n)), ad = fad.pop(atxz(n));
         const float a3 = wa3.dminus(g.cz);
         const float b1 = wb1.dplus(g.cz);
         const float b2 = wb2.dplus(g.cz);
-        const float b3 = dplus2<SDO>(atxz + c, M + c, 1, g.cx);
         const float ez = dzp[i];
-        float ux = A[c], uz = A[fs + c];
+        float &rx = cell_ref(A, cb), &rz = cell_ref(A + fs, cb);
+        float ux = rx, uz = rz;
         ux = fmaf(-dt, (2.0f * ex + ez) * ux + (a1 + a2 + a3), ux);
         uz = fmaf(-dt, (ex + 2.0f * ez) * uz + (b2 + b1 + b3), uz);
-        A[c] = ux;
-        A[fs + c] = uz;
+        rx = ux;
+        rz = uz;
         if (!img) continue;
-        fvz.push(vz[n]);
-        ftzz.push(tzz[n]);
-        ftxz.push(txz[n]);
+        fvz.push(vz(n));
+        ftzz.push(tzz(n));
+        ftxz.push(txz(n));
+        const float dxm_vz = fdvz.pop(x_vz(n));
+        const float dxp_txz = fdtxz.pop(x_txz(n));
         if (i < lo || i > hi) continue;
         const float dzp_vz = fvz.dplus(g.cz);
-        const float dxm_vz = dminus<SDO>(vz + c, 1, g.cx);
-        const float dxp_vx = dplus<SDO>(vx + c, 1, g.cx);
+        const float dxp_vx = xplus<SDO>([&](int k) { return vx.at(cb, k); }, g.cx);
         const float dzm_vz = fvz.dminus(g.cz);
         const float dzp_tzz = ftzz.dplus(g.cz);
-        const float dxm_txx = dminus<SDO>(txx + c, 1, g.cx);
-        const float dxp_txz = dplus<SDO>(txz + c, 1, g.cx);
+        const float dxm_txx = xminus<SDO>([&](int k) { return txx.at(cb, k); }, g.cx);
         const float dzm_txz = ftxz.dminus(g.cz);
-        const float aa = atxx[c], ab = atzz[c], ad = atxz[c];
-        G[c] += 2.0f * dt * (aa * dxp_vx + ab * dzm_vz) + dt * ad * (dzp_vz + dxm_vz);
-        G[2 * fs + c] += dt * (aa + ab) * (dxp_vx + dzm_vz);
-        const float bb = b[c];
-        G[4 * fs + c] += -(dt * (bb * bb)) * (ux * (dxm_txx + dzm_txz) + uz * (dzp_tzz + dxp_txz));
+        cell_ref(G, cb) += 2.0f * dt * (aa * dxp_vx + ab * dzm_vz) + dt * ad * (dzp_vz + dxm_vz);
+        cell_ref(G + 2 * fs, cb) += dt * (aa + ab) * (dxp_vx + dzm_vz);
+        const float bb = B(cb);
+        cell_ref(G + 4 * fs, cb) +=
+            -(dt * (bb * bb)) * (ux * (dxm_txx + dzm_txz) + uz * (dzp_tzz + dxp_txz));
         if constexpr (PRECOND) {
-            const float sxx = txx[c], szz = ftzz.centre(), sxz = ftxz.centre();
-            const float wx = vx[c], wz = fvz.centre();
+            const float sxx = txx(cb), szz = ftzz.centre(), sxz = ftxz.centre();
+            const float wx = vx(cb), wz = fvz.centre();
             const float dt2 = dt * dt;
-            G[fs + c] += 4.0f * dt2 * (sxx * sxx * dxp_vx * dxp_vx + szz * szz * dzm_vz * dzm_vz)
-                         + dt2 * sxz * sxz * (dzp_vz * dzp_vz + dxm_vz * dxm_vz);
-            G[3 * fs + c] += dt2 * (sxx * sxx + szz * szz) * (dxp_vx * dxp_vx + dzm_vz * dzm_vz);
-            G[5 * fs + c] += dt2 * (wx * wx * (dxm_txx * dxm_txx + dzm_txz * dzm_txz)
-                                    + wz * wz * (dzp_tzz * dzp_tzz + dxp_txz * dxp_txz));
+            cell_ref(G + fs, cb) +=
+                4.0f * dt2 * (sxx * sxx * dxp_vx * dxp_vx + szz * szz * dzm_vz * dzm_vz)
+                + dt2 * sxz * sxz * (dzp_vz * dzp_vz + dxm_vz * dxm_vz);
+            cell_ref(G + 3 * fs, cb) +=
+                dt2 * (sxx * sxx + szz * szz) * (dxp_vx * dxp_vx + dzm_vz * dzm_vz);
+            cell_ref(G + 5 * fs, cb) += dt2 * (wx * wx * (dxm_txx * dxm_txx + dzm_txz * dzm_txz)
+                                              + wz * wz * (dzp_tzz * dzp_tzz + dxp_txz * dxp_txz));
         }
     }
 }

@@ -279,8 +279,11 @@ int32_t pyfwi_plan_create(const pyfwi_grid *grid, void *stream, pyfwi_plan **out
     d.pitch = ((g.tnx + HALO + 63) / 64) * 64;  // >= tnx + 8 zero columns
     d.sdo = g.sdo;
     d.npml = g.npml;
-    d.origin = (int64_t)HALO * d.pitch;
-    d.fstride = (int64_t)(g.tnz + 2 * HALO) * d.pitch + 64;  // tail for the last left halo
+    // column sdo (the first active one, where every column block starts)
+    // falls on a 256-byte boundary, so each wavefront's row segment covers
+    // whole cache lines
+    d.origin = (int64_t)HALO * d.pitch + (64 - g.sdo);
+    d.fstride = (int64_t)(g.tnz + 2 * HALO) * d.pitch + 64;  // tail for the shifted last row
     d.fstride = ((d.fstride + 63) / 64) * 64;
     for (int k = 0; k < 4; ++k) {
         d.cx[k] = (float)(g.c[k] / g.dx);
