@@ -4,14 +4,17 @@ Workload (default): plain_medium_500x1000 -- nz=500, nx=1000, dh=5 m,
 seeded 20-layer random model, 20 Hz Ricker, dt from CFL 0.45, nt=2000,
 20 surface shots, 400 receivers, npml=20 (540 x 1040 cells with the
 absorbing layer), order-4 stencil, FP32.  One step = one full gradient of the
-L2 misfit: batched forward modelling of all shots (records + reference
-checkpoints), residual against observed data on the device, and the
-reference-semantics adjoint (time-reversed reconstruction, adjoint sweeps,
-imaging condition) of all shots.  Inputs are resident in HBM before the
-timed region.
+L2 misfit, as pyfwi_misfit_gradient computes it: per pass of G shots
+(G bounded by the HBM the kept states take), batched forward modelling with
+every forward state kept (the reference's chpr = 100), residual and misfit
+against observed data on the device, and the reference-semantics adjoint
+(adjoint sweeps + imaging condition) of the pass.  `--mode reverse` instead
+keeps no states and rebuilds them by time reversal (chpr = 0).  Inputs are
+resident in HBM before the timed region.
 
 A gradient cell-update is one (cell incl. absorbing layer, time step, shot)
 taken through forward plus adjoint; value = shots * cells * nt / seconds.
+Forward modelling alone is reported beside it.
 
 Prints ONE JSON line.  --gpus above 1 is rejected: this build runs on one GPU.
 """
@@ -152,6 +155,23 @@ def per_shot_rate(W, b, torch, nsub, step):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     return nsub * cells * W.nt / dt / 1e9, dt
+
+
+def forward_rate(W, b, torch, tuning):
+    """Forward modelling alone (records, no kept states) of all shots in one
+    pass with the gradient's kernel shape: Gcell-updates/s."""
+    cells = int(W.tnz) * int(W.tnx)
+    plan = b["plan"]
+    plan.set_tuning(0, tuning[1], tuning[2])
+    no_chp = np.zeros(0, np.int32)
+    plan.forward(b["shots"], b["src"], b["seis"], no_chp, 0, _native.MEM_DEVICE)  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan.forward(b["shots"], b["src"], b["seis"], no_chp, 0, _native.MEM_DEVICE)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    plan.set_tuning(*tuning)
+    return int(W.ns) * cells * W.nt / dt / 1e9, dt
 
 
 def kernel_profile(W, b, torch, step):
@@ -315,6 +335,9 @@ def main():
                for k, v in prof.items()}
     log(f"bench: kernels {kernels}")
 
+    fwd_rate, _ = forward_rate(W, b, torch, tuning)
+    log(f"bench: forward-only {fwd_rate:.2f} Gcell/s")
+
     ps = None
     if args.per_shot_sample > 0:
         ps_rate, ps_time = per_shot_rate(W, b, torch, args.per_shot_sample, step)
@@ -361,6 +384,9 @@ def main():
                      "bytes_per_launch": per_launch_bytes, "avg_launch_us": round(avg_s * 1e6, 2),
                      "shots_per_launch": round(shots_per_launch, 3)},
         "gradient_roofline_frac": round(value * grad_bytes / HBM_PEAK_GBS, 4),
+        "forward_only": {"value": round(fwd_rate, 3), "unit": "Gcell-updates/s",
+                         "roofline_frac": round(fwd_rate * 52.0 / HBM_PEAK_GBS, 4)},
+        "misfit": misfit,
         "per_shot": ps,
         "kernels": kernels,
         "cpu_baseline": cpu,

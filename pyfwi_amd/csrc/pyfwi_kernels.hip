@@ -246,11 +246,13 @@ struct Fifo {
 
 // Tile of this block and column / row strip of this thread.
 //
-// Launches are 1-D.  The tiles (column block, row strip, shot) are numbered
-// column block fastest, then strip, then shot; workgroup w runs tile
+// Launches are 1-D.  The tiles (shot, column block, row strip) are numbered
+// shot fastest, then column block, then strip; workgroup w runs tile
 // (w % 8) * ceil(T/8) + w / 8.  Workgroups are handed to the 8 XCDs in turn,
-// so each XCD receives one contiguous run of tiles and neighbouring tiles,
-// which share halo rows and cache lines, meet in the same L2.
+// so each XCD receives one contiguous run of tiles: the same cells of every
+// shot of the launch meet in one L2, which then serves the model arrays
+// (shared by all shots) once, and neighbouring column blocks, which share
+// halo columns, run side by side.
 template <int SDO>
 __device__ __forceinline__ bool tile(const DevGrid &g, int &s, int &j, int &i0, int &i1)
 {
@@ -259,10 +261,10 @@ __device__ __forceinline__ bool tile(const DevGrid &g, int &s, int &j, int &i0, 
     const int w = (int)blockIdx.x;
     const int logical = (w % NUM_XCD) * per + w / NUM_XCD;
     if (logical >= T) return false;
-    const int cb = logical % g.ncb;
-    const int rest = logical / g.ncb;
-    const int st = rest % g.nstrips;
-    s = rest / g.nstrips;
+    s = logical % g.nbatch;
+    const int rest = logical / g.nbatch;
+    const int cb = rest % g.ncb;
+    const int st = rest / g.ncb;
     j = SDO + cb * MARCH_X + (int)threadIdx.x;
     i0 = SDO + st * g.mrows;
     i1 = min(i0 + g.mrows - 1, g.nz - SDO);
