@@ -156,7 +156,10 @@ int32_t pyfwi_gradient(pyfwi_plan *plan, const pyfwi_shot *shots,
  * __kernel_gradient (wave_propagation.py:751-856) with the checkpoint list
  * `chp` (all shots must share nt).  When chp is every step 0..nt-1
  * (chpr = 100) the forward states are kept in HBM for the pass and read
- * directly by the adjoint, so no time reversal is needed.
+ * directly by the adjoint, so no time reversal is needed.  Shots per pass:
+ * the plan's shots_per_pass when set, else as many as the kept states
+ * (nchp * 5 fields per shot) of 90 % of the free device memory allow;
+ * PYFWI_ENOMEM when not even one shot fits.
  *   obs        observed records, layout of `seis` (rec_offset per shot)
  *   seis       optional output of the modelled records (may be NULL)
  *   grad       output [6][tnz*tnx] as pyfwi_gradient
@@ -173,8 +176,9 @@ int32_t pyfwi_misfit_gradient(pyfwi_plan *plan, const pyfwi_shot *shots, int32_t
  * that many shots (0: all in one pass; smaller passes need less memory and
  * can keep the working set in the 256 MB on-package cache); fused != 0 does
  * each time step as one fused launch (ping-pong buffers) instead of two
- * in-place half-step launches; strip_rows > 0 fixes the rows each thread
- * walks (0: chosen per launch).  Results do not depend on these choices. */
+ * half-step launches (the default, 0); strip_rows > 0 fixes the rows each
+ * thread walks (0: chosen per launch).  Results do not depend on these
+ * choices. */
 int32_t pyfwi_plan_set_tuning(pyfwi_plan *plan, int32_t shots_per_pass, int32_t fused,
                               int32_t strip_rows);
 

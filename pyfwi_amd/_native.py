@@ -164,7 +164,7 @@ class Plan:
             int(chp.size), int(comp_mask), int(flags), mem))
         return out.value
 
-    def set_tuning(self, shots_per_pass=0, fused=True, strip_rows=0):
+    def set_tuning(self, shots_per_pass=0, fused=False, strip_rows=0):
         check(self.lib.pyfwi_plan_set_tuning(self.h, int(shots_per_pass), 1 if fused else 0,
                                              int(strip_rows)))
 

@@ -86,7 +86,7 @@ struct pyfwi_plan {
     DevBuf fields, fields2, adj, av0, av1, gacc, snaps, ckpts, seg;
     float *cur = nullptr;  // buffer (fields or fields2) holding the current forward state
     // tuning: shots per pass (0: all), fused steps, strip height (0: automatic)
-    int tune_group = 0, tune_fused = 1, tune_rows = 0;
+    int tune_group = 0, tune_fused = 0, tune_rows = 0;
     bool ckpts_all = false;  // ckpts holds every shot of the last forward
     DevBuf src_stage, seis_stage, res_stage, grad_stage, shot_tab_f, resid, src_saved;
     double *d_misfit = nullptr;
@@ -821,7 +821,21 @@ int32_t pyfwi_misfit_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t ns
     p->chp = cl;
     p->last_shots = sh;
     p->have_forward = false;  // kept states belong to the last pass only
-    const int G = pass_size(p, nshots);
+    // shots per pass: the tuned value, else as many as the kept states of
+    // free device memory allow (the buffer held now counts as free)
+    int G = pass_size(p, nshots);
+    if (p->tune_group <= 0) {
+        size_t free_b = 0, total_b = 0;
+        HIPCHK(hipMemGetInfo(&free_b, &total_b));
+        const double per_shot = (double)nchp * fps * sizeof(float);
+        const double avail = 0.9 * ((double)free_b + (double)p->snaps.n * sizeof(float));
+        const int fit = (int)std::min<double>(nshots, avail / per_shot);
+        if (fit < 1)
+            return fail(PYFWI_ENOMEM, "the kept states of one shot (" +
+                                          std::to_string((long long)per_shot) +
+                                          " bytes) do not fit in device memory");
+        G = fit;
+    }
     if (int e = ensure(p->snaps, (size_t)G * nchp * fps, p->stream, "checkpoints")) return e;
     for (int g0 = 0; g0 < nshots; g0 += G) {
         const int ng = std::min(G, nshots - g0);
