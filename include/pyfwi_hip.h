@@ -122,7 +122,9 @@ int32_t pyfwi_set_model(pyfwi_plan *plan, const float *lam, const float *mu,
  *         rows that are never sampled are written as zero
  *   chp, nchp  steps after which the state is kept on the device for a later
  *         PYFWI_MODE_PYFWI gradient (the reference's checkpoint list); may be
- *         NULL/0.  All shots of one call must share nt when nchp > 0.
+ *         NULL/0.  A list of every step 0..nt-1 (nt the largest shot's) keeps
+ *         each shot's own steps 0..nt_s-1, so shots may differ in nt; any
+ *         other list needs equal nt for all shots.
  *   ckpt_every  > 0: also keep the state at the start of every ckpt_every-th
  *         step for a later PYFWI_MODE_EXACT gradient (0: none) */
 int32_t pyfwi_forward(pyfwi_plan *plan, const pyfwi_shot *shots, int32_t nshots,
@@ -154,7 +156,8 @@ int32_t pyfwi_gradient(pyfwi_plan *plan, const pyfwi_shot *shots,
  * modelling, residual res = seis - obs with misfit 0.5*sum(res^2)
  * (CostFunction.l2, src/PyFWI/fwi_tools.py:1402-1422), then the adjoint of
  * __kernel_gradient (wave_propagation.py:751-856) with the checkpoint list
- * `chp` (all shots must share nt).  When chp is every step 0..nt-1
+ * `chp` (as in pyfwi_forward: shots may differ in nt only when chp is every
+ * step).  When chp is every step 0..nt-1
  * (chpr = 100) the forward states are kept in HBM for the pass and read
  * directly by the adjoint, so no time reversal is needed.  Shots per pass:
  * the plan's shots_per_pass when set, else as many as the kept states

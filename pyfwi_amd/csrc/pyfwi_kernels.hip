@@ -682,9 +682,10 @@ __global__ void k_inject_record(DevGrid g, const pyfwi_shot *shots, int t,
     const int64_t fs = g.fstride;
     const int64_t cs = (int64_t)sh.src_z * g.pitch + sh.src_x;
     // kept-state runs inject into the previous step's slot: put back the
-    // source cell of the slot injected one step earlier, and remember the
-    // clean values of this one
-    if (saved && prev && threadIdx.x == 0) {
+    // source cell of the slot injected one step earlier (when this shot was
+    // still stepping then, t - 1 < nt), and remember the clean values of this
+    // one
+    if (saved && prev && t <= sh.nt && threadIdx.x == 0) {
         float *Q = prev + (int64_t)s * prev_stride + g.origin;
 #pragma unroll
         for (int f = 0; f < 4; ++f) Q[f * fs + cs] = saved[s * 4 + f];

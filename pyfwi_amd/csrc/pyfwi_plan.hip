@@ -464,6 +464,19 @@ bool keeps_every_step(const std::vector<int32_t> &cl, int nt)
     return true;
 }
 
+// A checkpoint list other than every step is a list of the reference's
+// time indices, which only has a meaning when all shots share nt; "every
+// step" keeps each shot's own steps 0..nt_s-1.
+int check_chp_nt(const std::vector<pyfwi_shot> &sh, const std::vector<int32_t> &cl, int nt)
+{
+    if (cl.empty() || keeps_every_step(cl, nt)) return 0;
+    for (auto &x : sh)
+        if (x.nt != nt)
+            return fail(PYFWI_EINVAL, "a checkpoint list other than every step 0..nt-1 needs "
+                                      "equal nt for all shots");
+    return 0;
+}
+
 int pass_size(const pyfwi_plan *p, int nshots)
 {
     return p->tune_group > 0 ? std::min(p->tune_group, nshots) : nshots;
@@ -549,9 +562,8 @@ int32_t pyfwi_forward(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, co
     if (ckpt_every < 0) return fail(PYFWI_EINVAL, "ckpt_every must be >= 0");
     std::vector<pyfwi_shot> sh(shots, shots + nshots);
     const int nt = max_nt(sh);
-    if (nchp > 0)
-        for (auto &x : sh)
-            if (x.nt != nt) return fail(PYFWI_EINVAL, "checkpoint lists need equal nt for all shots");
+    std::vector<int32_t> cl(chp, chp + std::max(nchp, 0));
+    if (int e = check_chp_nt(sh, cl, nt)) return e;
     const int64_t fps = fields_per_shot(p);
     p->have_forward = false;
     p->ckpts_all = false;
@@ -572,7 +584,6 @@ int32_t pyfwi_forward(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, co
         d_seis = p->seis_stage.p;
     }
     HIPCHK(hipMemsetAsync(d_seis, 0, nrec * sizeof(float), p->stream));
-    std::vector<int32_t> cl(chp, chp + std::max(nchp, 0));
     const int G = pass_size(p, nshots);
     for (int g0 = 0; g0 < nshots; g0 += G) {
         const int ng = std::min(G, nshots - g0);
@@ -793,8 +804,8 @@ int32_t pyfwi_misfit_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t ns
     if (comp_mask < 0 || comp_mask > 31) return fail(PYFWI_EINVAL, "comp_mask must be 0..31");
     std::vector<pyfwi_shot> sh(shots, shots + nshots);
     const int nt = max_nt(sh);
-    for (auto &x : sh)
-        if (x.nt != nt) return fail(PYFWI_EINVAL, "checkpoint lists need equal nt for all shots");
+    std::vector<int32_t> cl(chp, chp + nchp);
+    if (int e = check_chp_nt(sh, cl, nt)) return e;
     const DevGrid &g = p->dg;
     const int64_t fps = fields_per_shot(p);
     const size_t nrec = rec_extent(shots, nshots), nsrc = src_extent(shots, nshots);
@@ -817,7 +828,6 @@ int32_t pyfwi_misfit_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t ns
     if (!p->d_misfit) HIPCHK(hipMalloc(&p->d_misfit, sizeof(double)));
     HIPCHK(hipMemsetAsync(p->d_misfit, 0, sizeof(double), p->stream));
     HIPCHK(hipMemsetAsync(d_seis, 0, nrec * sizeof(float), p->stream));
-    std::vector<int32_t> cl(chp, chp + nchp);
     p->chp = cl;
     p->last_shots = sh;
     p->have_forward = false;  // kept states belong to the last pass only

@@ -16,7 +16,7 @@ from .acquisition import Source
 
 __all__ = ["Workload", "layered3", "random_layered", "gaussian_anomaly",
            "plain_small", "plain_medium", "differentiation", "large_step",
-           "t_for_nt", "gardner"]
+           "t_for_nt", "gardner", "UnevenWorkload", "uneven_batch", "uneven_small"]
 
 
 def gardner(vp):
@@ -171,3 +171,70 @@ def large_step(seed=5, ns=32, nt=6000):
     src.Ricker(15)
     return Workload("large_step_2000x6000", inpa, src, rec_loc, model, None, components=0,
                     chpr=0, meta={"nt": nt, "shots": ns, "seed": seed})
+
+
+@dataclass
+class UnevenWorkload:
+    """Shots that differ in duration, aperture and wavelet (for
+    ``pyfwi_amd.batch.UnevenBatch``)."""
+    name: str
+    inpa: dict
+    model: dict
+    model_obs: dict
+    specs: list
+    components: int = 2
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def model_shape(self):
+        return self.model["vp"].shape
+
+
+def _uneven_specs(rng, ns, nx, dh, dt, nt_range, aperture, fdom, rec_dis_cells,
+                  src_z_cells=4, rec_z_cells=2):
+    from .batch import ShotSpec
+    specs = []
+    for _ in range(ns):
+        nt = int(rng.integers(nt_range[0], nt_range[1] + 1))
+        nrec = int(rng.integers(aperture[0], aperture[1] + 1))
+        sx = int(rng.integers(4, nx - 9))
+        span = (nrec - 1) * rec_dis_cells
+        x0 = int(np.clip(sx - span // 2, 1, nx - span - 2))
+        rec = np.zeros((nrec, 2), np.float64)
+        rec[:, 0] = (x0 + rec_dis_cells * np.arange(nrec)) * dh
+        rec[:, 1] = rec_z_cells * dh
+        specs.append(ShotSpec((sx * dh, src_z_cells * dh), rec, t_for_nt(nt, dt),
+                              float(rng.uniform(*fdom))))
+    return specs
+
+
+def uneven_batch(seed=4, ns=64, nz=1000, nx=2000, dh=4.0, nt_range=(1500, 4000),
+                 aperture=(200, 1600), fdom=(10.0, 25.0)):
+    """nz=1000, nx=2000, dh=4 m, 20-layer random model, dt from CFL 0.45; 64
+    shots with source x ~ U, nt ~ U[1500, 4000], receivers U{200..1600}
+    (one-cell spacing, centred on the source, clipped to the grid) and a
+    Ricker of fdom ~ U(10, 25) Hz; vx/vz records; observed data from a +5 %
+    Gaussian vp anomaly (sigma 60 cells) at the centre."""
+    model = random_layered(nz, nx, 20, seed)
+    dt = _cfl_dt(model, dh)
+    inpa = _base_inpa(dh, dt, nt_range[1])
+    rng = np.random.default_rng(seed + 1000)
+    specs = _uneven_specs(rng, ns, nx, dh, dt, nt_range, aperture, fdom, 1)
+    obs = gaussian_anomaly(model, nz / 2, nx / 2, 60.0, 0.05)
+    return UnevenWorkload(f"uneven_{nz}x{nx}_{ns}shots", inpa, model, obs, specs,
+                          meta={"seed": seed, "shots": ns, "nt_range": nt_range,
+                                "aperture": aperture, "fdom": fdom})
+
+
+def uneven_small(seed=5, ns=5, nt_range=(150, 320), aperture=(15, 40), fdom=(20.0, 40.0)):
+    """The uneven batch at the size of plain_small (100x100 cells, dh=1 m,
+    dt=0.1 ms, 3 layers): per-shot nt, aperture (two-cell spacing) and
+    Ricker frequency."""
+    dh, dt = 1.0, 1e-4
+    model = layered3(100, 100)
+    inpa = _base_inpa(dh, dt, nt_range[1])
+    rng = np.random.default_rng(seed)
+    specs = _uneven_specs(rng, ns, 100, dh, dt, nt_range, aperture, fdom, 2)
+    obs = gaussian_anomaly(model, 50, 50, 8.0, 0.05)
+    return UnevenWorkload("uneven_small_100x100", inpa, model, obs, specs,
+                          meta={"seed": seed, "shots": ns})
