@@ -21,13 +21,10 @@ import numpy as np
 from . import _native
 from .acquisition import Source
 from .fwi_tools import expand_model
+from .processing import COMPONENT_MASK
 from .wave_propagation import WavePropagator, checkpoint_steps
 
-__all__ = ["ShotSpec", "UnevenBatch", "COMPONENT_MASK"]
-
-# components -> bits of pyfwi_misfit_gradient's comp_mask (vx, vz, txx, tzz, txz);
-# 0 compares the pressure (acquisition.py:313-333)
-COMPONENT_MASK = {0: 0, 1: 0b00100, 2: 0b00011, 3: 0b11100, 4: 0b11111}
+__all__ = ["ShotSpec", "UnevenBatch"]
 
 
 @dataclass

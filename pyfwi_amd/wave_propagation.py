@@ -312,6 +312,12 @@ class WavePropagator(WavePreparation):
         if mode == "exact":
             res = _section_vjp(res, self.components)
         G = self.raw_gradient(res, mode=mode)
+        return self._finish_gradient(G, parameterization, mode)
+
+    def _finish_gradient(self, G, parameterization="dv", mode="pyfwi"):
+        """Read-out of the accumulators G [6, tnz, tnx]: mask and balancing
+        (pyfwi mode) or padding fold (exact mode), chain rule, smoothing
+        (wave_propagation.py:900-938)."""
         if mode == "pyfwi":
             glam, gmu, grho0 = self.gradient_reading(G)
         else:

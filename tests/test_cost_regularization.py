@@ -1,0 +1,216 @@
+"""Host-side inversion tools: derivative operators, regularisation terms,
+multiscale lowpass and cost_seismic (src/PyFWI/fwi_tools.py:17-447,
+1153-1345), model vectors (:1066-1097).
+
+Each is pinned by closed forms or invariants: exact derivatives of linear and
+quadratic fields (including the reference's dx/dz scaling of Dz/Dx), l1 and
+Tikhonov values on known images, the digital Butterworth magnitude
+1/(1 + (tan(w/2)/tan(wc/2))^(2N)), zero phase, lowpass/adjoint duality and
+central differences of the misfit against the adjoint source.
+"""
+import numpy as np
+import pytest
+
+from pyfwi_amd import fwi_tools as T
+from pyfwi_amd.fwi_tools import Regularization
+
+
+def test_derivative_first_order_exact_on_linear_fields():
+    nz, nx, dx, dz = 7, 9, 2.0, 0.5
+    z, x = np.mgrid[0:nz, 0:nx].astype(np.float64)
+    Dx, Dz = T.derivative(nx, nz, dx, dz, 1)
+    assert Dx.shape == (nz * nx, nz * nx) and Dx.nnz == 2 * nz * nx and Dz.nnz == 2 * nz * nx
+    # one-sided on the edges and centred inside are all exact on linear fields;
+    # the reference scales Dx by 1/dz and Dz by 1/dx
+    np.testing.assert_allclose(Dx @ (3 * x).ravel(), 3 / dz, rtol=1e-14)
+    np.testing.assert_allclose(Dz @ (5 * z).ravel(), 5 / dx, rtol=1e-14)
+    np.testing.assert_allclose(Dx @ z.ravel(), 0, atol=1e-14)
+    np.testing.assert_allclose(Dz @ x.ravel(), 0, atol=1e-14)
+    # centred inside: d/dx of x^2 is 2x exactly there, one-sided error on the edges
+    q = (Dx @ (x ** 2).ravel()).reshape(nz, nx) * dz
+    np.testing.assert_allclose(q[:, 1:-1], 2 * x[:, 1:-1], rtol=1e-14)
+    np.testing.assert_allclose(q[:, 0], 1.0)            # forward: (1 - 0)
+    np.testing.assert_allclose(q[:, -1], 2 * (nx - 1) - 1)  # backward
+
+
+def test_derivative_second_order_exact_on_quadratics():
+    nz, nx, dx, dz = 6, 8, 2.0, 0.5
+    z, x = np.mgrid[0:nz, 0:nx].astype(np.float64)
+    Dx, Dz = T.derivative(nx, nz, dx, dz, 2)
+    assert Dx.nnz == 3 * nz * nx and Dz.nnz == 3 * nz * nx
+    np.testing.assert_allclose(Dx @ (x ** 2).ravel(), 2 / dz ** 2, rtol=1e-13)
+    np.testing.assert_allclose(Dz @ (z ** 2).ravel(), 2 / dx ** 2, rtol=1e-13)
+    np.testing.assert_allclose(Dx @ (7 * x + z).ravel(), 0, atol=1e-12)
+
+
+def test_l1_and_tv_values():
+    x = np.array([3.0, -2.0, 0.5, 1e-9, -1e-9, 0.0])
+    rms, g = Regularization.l1(x, eps=1e-7)
+    big = np.abs(x) > 1e-7
+    # sum x^2/max(|x|, eps) and its reweighted gradient 2 x / max(|x|, eps)
+    assert rms == pytest.approx(np.abs(x[big]).sum() + (x[~big] ** 2).sum() / 1e-7, rel=1e-14)
+    np.testing.assert_allclose(g[big], 2 * np.sign(x[big]), rtol=1e-14)
+    np.testing.assert_allclose(g[~big], 2 * x[~big] / 1e-7, rtol=1e-14)
+    # tv of a linear model: |ax * a + az * b| on every cell below row 17
+    nz, nx = 30, 20
+    R = Regularization(nx, nz, 1.0, 1.0)
+    z, x2 = np.mgrid[0:nz, 0:nx].astype(np.float64)
+    m = (0.3 * x2 + 2.0 * z).ravel()
+    rms, g = R.tv(m, 1e-7, alpha_z=0.5, alpha_x=2.0)
+    img = np.full((nz, nx), 2.0 * 0.3 + 0.5 * 2.0)
+    img[:17] = 2.0 * 0.3
+    assert rms == pytest.approx(np.abs(img).sum(), rel=1e-12)
+    np.testing.assert_allclose(g, 2 * np.sign(img).ravel(), rtol=1e-12)
+
+
+def test_tikhonov_value_and_gradient():
+    nz, nx = 25, 12
+    R = Regularization(nx, nz, 1.0, 1.0)
+    z, x = np.mgrid[0:nz, 0:nx].astype(np.float64)
+    m = np.concatenate([(1.5 * x - 0.5 * z).ravel(), (0.2 * z).ravel()])
+    rms, g = R.tikhonov(m, alpha_z=3.0, alpha_x=0.5)
+    img1 = np.full((nz, nx), 0.5 * 1.5 + 3.0 * -0.5)
+    img1[:17] = 0.5 * 1.5
+    img2 = np.full((nz, nx), 3.0 * 0.2)
+    img2[:17] = 0.0
+    assert rms == pytest.approx((img1 ** 2).sum() + (img2 ** 2).sum(), rel=1e-12)
+    # the reference's gradient ax Dx'Dx m + az Dz'Dz m, from dense operators
+    Dx = R.Bx1.toarray()
+    Dz = R.Bz1.toarray()
+    n = nz * nx
+    for k in range(2):
+        blk = m[k * n:(k + 1) * n]
+        np.testing.assert_allclose(g[k * n:(k + 1) * n],
+                                   0.5 * Dx.T @ (Dx @ blk) + 3.0 * Dz.T @ (Dz @ blk),
+                                   rtol=1e-12, atol=1e-12)
+    # a constant model costs nothing
+    rms0, g0 = R.tikhonov(np.full(n, 4.2), alpha_z=1.0, alpha_x=1.0)
+    assert rms0 == pytest.approx(0, abs=1e-20) and np.abs(g0).max() < 1e-12
+
+
+def test_cost_regularization_combines_terms():
+    nz, nx = 20, 10
+    R = Regularization(nx, nz, 1.0, 1.0)
+    rng = np.random.default_rng(0)
+    m = rng.standard_normal(nz * nx)
+    tv = {"lambda_weight": 0.7, "az": 1.0, "ax": 0.5}
+    tk = {"lambda_weight": 0.2, "az": 0.3, "ax": 1.0}
+    rms, g = R.cost_regularization(m, tv_properties=tv, tikhonov_properties=tk)
+    f1, g1 = R.tv(m, 1e-7, 1.0, 0.5)
+    f2, g2 = R.tikhonov(m, 0.3, 1.0)
+    assert rms == pytest.approx(0.7 * f1 + 0.2 * f2, rel=1e-14)
+    np.testing.assert_allclose(g, 0.7 * g1 + 0.2 * g2, rtol=1e-14)
+    # order-0 Tikhonov adds its weight to the gradient, as the reference does
+    rms0, g0 = R.cost_regularization(m, tikhonov0_properties={"lambda_weight": 0.25})
+    assert rms0 == pytest.approx(0.25 * np.abs(m).sum(), rel=1e-12)
+    np.testing.assert_allclose(g0, 0.25)
+
+
+def test_parameter_relation_and_prior():
+    nz, nx = 8, 6
+    n = nz * nx
+    R = Regularization(nx, nz, 1.0, 1.0)
+    rng = np.random.default_rng(1)
+    m = rng.uniform(1, 2, 3 * n)
+    rel = {"12": {"model": np.poly1d([0.5, 0.1]), "lam": 2.0, "freqs": [10, 20]}}
+    rms, g = R.parameter_relation(m, rel, 0, 3, 20)
+    dm = m[n:2 * n] - (0.5 * m[:n] + 0.1)
+    assert rms == pytest.approx(0.5 * 2.0 * dm @ dm, rel=1e-14)
+    from scipy.ndimage import gaussian_filter
+    np.testing.assert_allclose(g[n:2 * n], gaussian_filter(2.0 * dm, 1), rtol=1e-14)
+    assert not g[:n].any() and not g[2 * n:].any()
+    assert R.parameter_relation(m, rel, 0, 3, 15)[0] == 0.0  # band not listed
+    # prior: 0.5 lam (m - mp)^2 / var(mp), gradient checked by central differences
+    mp = {"vp": rng.uniform(1, 2, (nz, nx)), "vs": rng.uniform(1, 2, (nz, nx))}
+    prior = {"lam": 3.0, "freqs": [5], "mp": mp}
+    blk = m[:2 * n].copy()
+    f, g = R.priori_regularization(blk, prior, 0, 2, 5)
+    mpv = np.concatenate([mp["vp"].ravel(), mp["vs"].ravel()])
+    assert f == pytest.approx(0.5 * 3.0 * ((blk - mpv) ** 2).sum() / np.var(mpv), rel=1e-12)
+    for i in (0, n + 3, 2 * n - 1):
+        e = np.zeros_like(blk)
+        e[i] = 1e-6
+        fd = (R.priori_regularization(blk + e, prior, 0, 2, 5)[0]
+              - R.priori_regularization(blk - e, prior, 0, 2, 5)[0]) / 2e-6
+        assert fd == pytest.approx(g[i], rel=1e-6)
+    assert R.priori_regularization(blk, None, 0, 2, 5)[0] == 0.0
+
+
+def test_butterworth_response_closed_form():
+    # digital Butterworth by the bilinear transform: |H|^2 = 1 / (1 + (tan(w/2)/tan(wc/2))^2N)
+    fn, highcut, order, n = 1 / 2 / 0.0006, 25.0, 3, 1024
+    h = T._butter_response(highcut, fn, order, n)
+    w = 2 * np.pi * np.arange(n) / n
+    wc = np.pi * 2 * highcut / fn
+    ratio = np.abs(np.tan(w / 2)) / np.tan(wc / 2)
+    expect = 1 / (1 + ratio ** (2 * order))
+    np.testing.assert_allclose(np.abs(h) ** 2, expect, atol=1e-10)
+
+
+def _records(rng, nt=300, ntr=7):
+    return rng.standard_normal((5, nt, ntr))
+
+
+def test_lowpass_adjoint_pair_zero_phase_and_band():
+    rng = np.random.default_rng(2)
+    fn, freq, order = 1 / 2 / 0.0006, 30.0, 3
+    x, y = _records(rng), _records(rng)
+    lhs = np.sum(T.lowpass(x, freq, fn, order) * y)
+    rhs = np.sum(x * T.adj_lowpass(y, freq, fn, order))
+    assert lhs == pytest.approx(rhs, rel=1e-12)
+    # zero phase: an impulse in the middle gives a response symmetric about it
+    imp = np.zeros((1, 801, 1))
+    imp[0, 400, 0] = 1.0
+    r = T.lowpass(imp, freq, fn, order)[0, :, 0]
+    np.testing.assert_allclose(r[400 - 150:400], r[401:401 + 150][::-1], atol=1e-12)
+    # a slow sinusoid passes, a fast one is removed (interior samples)
+    t = np.arange(2000)
+    wc = np.pi * 2 * freq / fn
+    slow = np.sin(0.1 * wc * t)[None, :, None]
+    fast = np.sin(4.0 * wc * t)[None, :, None]
+    ls = T.lowpass(slow, freq, fn, order)[0, 300:1700, 0]
+    lf = T.lowpass(fast, freq, fn, order)[0, 300:1700, 0]
+    np.testing.assert_allclose(ls, slow[0, 300:1700, 0], atol=1e-3)
+    assert np.abs(lf).max() < 1e-3
+
+
+def test_cost_seismic_l2_and_adjoint_source():
+    rng = np.random.default_rng(3)
+    names = ("vx", "vz", "taux", "tauz", "tauxz")
+    dpre = {k: rng.standard_normal((120, 12)).astype(np.float32) for k in names}
+    dobs = {k: rng.standard_normal((120, 12)).astype(np.float32) for k in names}
+    CF = T.CostFunction("l2")
+    rms, adj = T.cost_seismic(dpre, dobs, fun=CF)
+    rms_ref, res_ref = CF(dpre, dobs)
+    assert rms == pytest.approx(rms_ref, rel=1e-12)
+    for k in names:
+        np.testing.assert_array_equal(adj[k], res_ref[k])
+    # with a band: the adjoint source is the gradient of the misfit.  The
+    # misfit is quadratic in the prediction, so central differences are exact
+    # at any step; a small residual keeps the float32 misfit precise.
+    fn, freq = 1 / 2 / 0.0006, 40.0
+    d64 = {k: v.astype(np.float64) for k, v in dpre.items()}
+    dobs = {k: (v + 0.01 * rng.standard_normal(v.shape)).astype(np.float32)
+            for k, v in dpre.items()}
+    rms, adj = T.cost_seismic(d64, dobs, fun=CF, fn=fn, freq=freq, order=3, axis=1)
+    for k, (i, j) in (("vx", (30, 2)), ("tauz", (77, 11)), ("tauxz", (5, 0))):
+        dp = {kk: v.copy() for kk, v in d64.items()}
+        dm = {kk: v.copy() for kk, v in d64.items()}
+        dp[k][i, j] += 1.0
+        dm[k][i, j] -= 1.0
+        fd = (T.cost_seismic(dp, dobs, fun=CF, fn=fn, freq=freq, order=3, axis=1)[0]
+              - T.cost_seismic(dm, dobs, fun=CF, fn=fn, freq=freq, order=3, axis=1)[0]) / 2.0
+        assert fd == pytest.approx(float(adj[k][i, j]), rel=1e-3, abs=1e-7)
+
+
+def test_source_weighting_and_model_vectors():
+    rng = np.random.default_rng(4)
+    d = rng.standard_normal((50, 3 * 4))
+    a = T.source_weighting(d, 2.5 * d, 3, 4)
+    np.testing.assert_allclose(a, 2.5, rtol=1e-12)
+    m = {k: rng.standard_normal((5, 7)) for k in ("vp", "vs", "rho")}
+    v = T.vel_dict2vec(m)
+    assert v.shape == (105,)
+    back = T.vec2vel_dict(v, 5, 7)
+    for k in m:
+        np.testing.assert_array_equal(back[k], m[k])

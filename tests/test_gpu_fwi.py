@@ -1,0 +1,95 @@
+"""The inversion driver on the GPU (src/PyFWI/fwi.py:12-282).
+
+fprime (forward, cost_seismic, gradient, chain rule) is compared with the same
+sequence built from the oracle's records and reference-semantics gradient,
+with and without the multiscale lowpass; the one-call device path
+(pyfwi_misfit_gradient) with the reference's sequence; and a short L-BFGS run
+must lower the misfit.  Tolerances: misfit 1e-4 relative against the oracle;
+gradient 1e-4 plus the records' parity bound (1e-5 of |d|) amplified by
+|d|/|r|, the ratio of records to residual, since the gradient is linear in the
+residual; 1e-6 between the two GPU paths.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pyfwi_amd import fwi_tools as T
+from pyfwi_amd import workloads
+from pyfwi_amd.fwi import FWI
+from pyfwi_amd.processing import prepare_residual
+
+from _helpers import oracle_setup, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(nt=600, amp=1.0):
+    """plain_small with observed data from a +20 % vp anomaly at 50 m depth,
+    long enough (nt=600) for the reflections to be recorded."""
+    w = workloads.plain_small()
+    w.src.w = np.float32(w.src.w * amp)
+    w.inpa["t"] = workloads.t_for_nt(nt, w.inpa["dt"])
+    w.model_obs = workloads.gaussian_anomaly(w.model, 50, 50, 8.0, 0.2)
+    st = oracle_setup(w, chpr=100, components=2)
+    seis_obs, _ = O.forward(st, w.model_obs, np.float32, keep_states=False)
+    d_obs = O.section(seis_obs, 2)
+    return w, st, d_obs
+
+
+@pytest.mark.parametrize("freq", [None, 40.0])
+def test_fprime_matches_oracle(gpu, freq):
+    w, st, d_obs = _setup()
+    fwi = FWI(d_obs, w.inpa, w.src, w.rec_loc, w.model_shape, components=2, chpr=100)
+    m = np.float32(T.vel_dict2vec(w.model))
+    rms, grad = fwi.fprime(m, freq)
+    seis, states = O.forward(st, w.model, np.float32)
+    d_est = prepare_residual(O.section(seis, 2), 1)
+    rms_ref, adj = T.cost_seismic(d_est, prepare_residual(d_obs, 1), fun=T.CostFunction("l2"),
+                                  fn=fwi.fn, freq=freq, order=3, axis=1)
+    g_ref = O.gradient(st, w.model, adj, states, parameterization="dv")
+    grad_ref = T.vel_dict2vec(g_ref)
+    e_rms = abs(rms - rms_ref) / abs(rms_ref)
+    n = w.model["vp"].size
+    errs = {k: rel(grad[i * n:(i + 1) * n], grad_ref[i * n:(i + 1) * n])
+            for i, k in enumerate(("vp", "vs", "rho"))}
+    print(f"fprime freq={freq}: misfit {e_rms:.2e}, gradient " +
+          ", ".join(f"{k} {e:.2e}" for k, e in errs.items()))
+    # the gradient is linear in the residual r = d - d_obs: record errors
+    # within their parity bound (1e-5 of |d|) reach it amplified by |d|/|r|
+    dv = np.concatenate([d_est[k].ravel() for k in ("vx", "vz")]).astype(np.float64)
+    rv = dv - np.concatenate([d_obs[k].ravel() for k in ("vx", "vz")])
+    ratio = np.linalg.norm(dv) / np.linalg.norm(rv)
+    tol = 1e-4 + 1e-5 * ratio
+    print(f"  |d|/|r| = {ratio:.1f}, gradient bound {tol:.2e}")
+    assert e_rms < 1e-4, e_rms
+    for k, e in errs.items():
+        assert e < tol, (k, e, tol)
+
+
+def test_device_path_equals_reference_sequence(gpu):
+    w, st, d_obs = _setup()
+    fwi = FWI(d_obs, w.inpa, w.src, w.rec_loc, w.model_shape, components=2, chpr=100)
+    m = np.float32(T.vel_dict2vec(w.model))
+    assert fwi._device_path_applies(None) and not fwi._device_path_applies(30.0)
+    rms_d, g_d = fwi.fprime(m, None)
+    fwi.device_misfit = False
+    rms_h, g_h = fwi.fprime(m, None)
+    print(f"device vs host path: misfit {abs(rms_d - rms_h) / rms_h:.2e}, "
+          f"gradient {rel(g_d, g_h):.2e}")
+    assert abs(rms_d - rms_h) <= 1e-6 * abs(rms_h)
+    assert rel(g_d, g_h) < 1e-6
+
+
+def test_lbfgs_lowers_misfit(gpu):
+    # the reference's L-BFGS settings stop at a projected gradient of 1e-8:
+    # a source of amplitude 1e8 puts this small problem's gradient above that
+    w, st, d_obs = _setup(nt=500, amp=1e8)
+    fwi = FWI(d_obs, w.inpa, w.src, w.rec_loc, w.model_shape, components=2, chpr=100)
+    rms0, _ = fwi.fprime(np.float32(T.vel_dict2vec(w.model)), None)
+    m_est, hist = fwi(w.model, method="lbfgs", iter=[3], freqs=[None], n_params=1, k_0=1,
+                      k_end=2)
+    rms1 = float(np.asarray(hist).ravel()[-1])
+    print(f"L-BFGS, 3 iterations on vp: misfit {rms0:.4e} -> {rms1:.4e}")
+    assert rms1 < 0.9 * rms0
+    assert set(m_est) == {"vp", "vs", "rho"}
+    np.testing.assert_array_equal(m_est["vs"], w.model["vs"])  # only vp was inverted
