@@ -17,17 +17,16 @@ import numpy as np
 from scipy.optimize import fmin_cg, fmin_l_bfgs_b
 from scipy.optimize._optimize import MemoizeJac
 
-from . import _native
 from . import fwi_tools as tools
-from .fwi_tools import Regularization, expand_model
-from .processing import COMPONENT_MASK, prepare_residual
+from .fwi_tools import Regularization
+from .processing import prepare_residual
 from .wave_propagation import WavePropagator
 
 __all__ = ["FWI"]
 
 
 class FWI(WavePropagator):
-    """Parameters as the reference (fwi.py:12-104): observed data ``d_obs``
+    """Parameters as the reference (fwi.py:12-107): observed data ``d_obs``
     (a seismic section dict), ``inpa`` (with optional 'sd', 'param_relation',
     'prior_model', 'tv', 'tikhonov', 'cost_function_type', 'grad_coeff'),
     source, receivers, model shape, components, chpr, n_well_rec and optional
@@ -60,14 +59,13 @@ class FWI(WavePropagator):
         self.CF = tools.CostFunction(inpa.get("cost_function_type", "l2"))
         self.grad_coeff = inpa.get("grad_coeff", [1.0, 1.0, 1.0])
         self.n_elements = self.nz * self.nx
-        self._obs_records = None
         self.device_misfit = True  # use pyfwi_misfit_gradient where it applies
 
     # ------------------------------------------------------------ driver
     def __call__(self, m0, method, iter, freqs, n_params, k_0, k_end):
         """Invert from ``m0`` ({'vp','vs','rho'}) with `method` ('lbfgs'/2 or
         'cg'/1), iter[i] iterations at frequency freqs[i], over parameters
-        k_0..k_end (1-based) in groups of n_params (fwi.py:106-164).  Returns
+        k_0..k_end (1-based) in groups of n_params (fwi.py:109-158).  Returns
         (model dict, misfit history)."""
         m = self.dict2vec(m0)
         run = self._method(method)
@@ -82,7 +80,7 @@ class FWI(WavePropagator):
         return self.vec2dict(m, self.nz, self.nx), np.array(rms)
 
     def _method(self, user_method):
-        """fwi.py:166-178."""
+        """fwi.py:160-172."""
         if user_method in [0, "SD", "sd"]:
             raise TypeError("Steepest descent is not provided yet.")
         if user_method in [1, "CG", "cg"]:
@@ -98,7 +96,7 @@ class FWI(WavePropagator):
             yield k, n
 
     def cg(self, m0, ITER, freq, n_params=1, k0=1, k_end=2):
-        """Nonlinear CG per parameter group (fwi.py:180-205)."""
+        """Nonlinear CG per parameter group (fwi.py:174-199)."""
         mtotal = np.copy(m0)
         hist_all = []
         fun = MemoizeJac(self.fprime_single)
@@ -113,7 +111,7 @@ class FWI(WavePropagator):
 
     def lbfgs(self, m0, ITER, freq, n_params=1, k0=1, k_end=2):
         """L-BFGS-B per parameter group, memory 10, at most 10 line-search
-        steps (fwi.py:207-232)."""
+        steps (fwi.py:201-227)."""
         mtotal = np.copy(m0)
         hist_all = []
         fun = MemoizeJac(self.fprime_single)
@@ -130,7 +128,7 @@ class FWI(WavePropagator):
     # ------------------------------------------------------------ misfit and gradient
     def fprime(self, m0, freq):
         """Data misfit and its gradient over the whole model vector
-        (fwi.py:234-255)."""
+        (fwi.py:229-252)."""
         mtotal = np.copy(m0)
         m_old = self.vec2dict(mtotal, self.nz, self.nx)
         m_new = self.model_to_dv(m_old, self.param_functions_args)
@@ -150,7 +148,7 @@ class FWI(WavePropagator):
 
     def fprime_single(self, m0, m_1, m1, freq):
         """Misfit (data + regularisation) and gradient of one parameter group
-        m0 between the fixed parts m_1 and m1 (fwi.py:257-282)."""
+        m0 between the fixed parts m_1 and m1 (fwi.py:254-282)."""
         mtotal = np.float32(np.hstack((m_1, m0, m1)))
         shape_1, shape0 = np.shape(m_1)[0], np.shape(m0)[0]
         k0 = np.int32(shape_1 / self.n_elements)
@@ -172,32 +170,8 @@ class FWI(WavePropagator):
 
     # ------------------------------------------------------------ one-call path
     def _device_path_applies(self, freq):
-        if not self.device_misfit or freq or self.per_shot:
-            return False
-        if self.CF.cost_function_type != "l2" or np.ndim(self.sd) or float(self.sd) != 1.0:
-            return False
-        if self.components == 0 and not np.array_equal(self.d_obs["taux"], self.d_obs["tauz"]):
-            return False  # pyfwi_misfit_gradient compares the pressure with one record
-        return True
+        return self.device_misfit and self.device_misfit_applies(self.d_obs, freq, self.sd,
+                                                                 self.CF)
 
     def _misfit_gradient_device(self, m_new):
-        model = {k: expand_model(np.asarray(v, np.float32), int(self.tnz), int(self.tnx),
-                                 self.npml) for k, v in m_new.items()}
-        self.pml_preparation(model["vp"].max())
-        self.elastic_buffers(model)
-        ids = list(range(int(self.ns)))
-        if self._obs_records is None:
-            self._obs_records = np.ascontiguousarray(self._residual_records(self.d_obs, ids))
-        G = np.zeros((6, int(self.tnz), int(self.tnx)), np.float32)
-        seis = np.zeros_like(self._obs_records)
-        flags = _native.F_PRECOND if self.energy_balancing else 0
-        rms = self._plan.misfit_gradient(self._shots(ids), self._source_table(),
-                                         self._obs_records, seis, G, self.chp,
-                                         COMPONENT_MASK[self.components], flags,
-                                         _native.MEM_HOST)
-        for f, name in enumerate(("vx", "vz", "taux", "tauz", "tauxz")):
-            self.seismogram[name][:] = seis[:, f].transpose(1, 2, 0)
-        # the plan keeps only the last pass's states: no stand-alone gradient
-        # may follow this call
-        self._forward_done = False
-        return rms, self._finish_gradient(G, "dv")
+        return self.misfit_gradient_device(m_new, self.d_obs)

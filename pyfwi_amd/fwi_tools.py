@@ -18,7 +18,8 @@ __all__ = ["Fdm", "expand_model", "pml_counstruction", "pml_construction",
            "pml_delta_calculation", "pml_profiles_1d", "CostFunction",
            "fd_coefficients", "kernel_constant", "derivative", "Regularization",
            "vel_dict2vec", "vec2vel_dict", "cost_preparation", "lowpass", "adj_lowpass",
-           "adj_cost_preparation", "source_weighting", "cost_seismic"]
+           "adj_cost_preparation", "source_weighting", "cost_seismic", "dict_diff",
+           "dict_summation"]
 
 
 def fd_coefficients(order):
@@ -494,3 +495,21 @@ def cost_seismic(d_pre0, d_obs0, fun, fn=None, freq=None, order=None, axis=None,
     adj = adj_cost_preparation(res, fn, freq=freq, order=order, axis=axis, params_oh=params_oh)
     names = ("vx", "vz", "taux", "tauz", "tauxz")
     return rms, {k: adj[i].astype(np.float32) for i, k in enumerate(names)}
+
+
+# ------------------------------------------------------------------ model dictionaries
+
+def dict_diff(dict1, dict2, positivity=False):
+    """dict1 - dict2 per key of dict1; negative differences set to zero when
+    `positivity` (src/PyFWI/fwi_tools.py:1537-1558)."""
+    out = {}
+    for k in dict1:
+        out[k] = dict1[k] - dict2[k]
+        if positivity:
+            out[k][out[k] < 0] = 0
+    return out
+
+
+def dict_summation(dict1, dict2, division=1.0):
+    """(dict1 + dict2) / division per key of dict1 (fwi_tools.py:1561-1580)."""
+    return {k: (dict1[k] + dict2[k]) / division for k in dict1}

@@ -21,7 +21,7 @@ from .acquisition import seismic_section
 from .fwi_tools import (Fdm, expand_model, fd_coefficients, kernel_constant,
                         pml_counstruction, pml_profiles_1d)
 from .grad_switcher import grad_lmd_to_vd
-from .processing import COMPONENTS, prepare_residual
+from .processing import COMPONENT_MASK, COMPONENTS, prepare_residual
 
 __all__ = ["WavePreparation", "WavePropagator"]
 
@@ -334,6 +334,56 @@ class WavePropagator(WavePreparation):
             from scipy.ndimage import gaussian_filter
             out = {k: gaussian_filter(v, self.g_smooth) for k, v in out.items()}
         return out
+
+    # ------------------------------------------------ one-call misfit + gradient
+    def device_misfit_applies(self, d_obs, freq=None, sd=1.0, cost=None):
+        """True when ``cost_seismic`` followed by ``gradient`` reduces to
+        pyfwi_misfit_gradient: no lowpass band, L2 cost, unit data weight,
+        all shots in one batch, and for the pressure (components=0) equal
+        'taux' and 'tauz' observations (the plan compares the pressure with
+        one record)."""
+        if freq or self.per_shot:
+            return False
+        if cost is not None and cost.cost_function_type != "l2":
+            return False
+        if np.ndim(sd) or float(sd) != 1.0:
+            return False
+        if self.components == 0:
+            return bool(np.array_equal(d_obs["taux"], d_obs["tauz"]))
+        return True
+
+    def _obs_records(self, d_obs):
+        """[ns, 5, nt_ext, nr] records of an observed-data dict, converted
+        once per dict."""
+        cache = self.__dict__.setdefault("_obs_cache", {})
+        hit = cache.get(id(d_obs))
+        if hit is None or hit[0] is not d_obs:
+            rec = self._residual_records(prepare_residual(d_obs, 1), list(range(int(self.ns))))
+            hit = cache[id(d_obs)] = (d_obs, np.ascontiguousarray(rec))
+        return hit[1]
+
+    def misfit_gradient_device(self, model0, d_obs, parameterization="dv"):
+        """L2 misfit of ``model0`` against ``d_obs`` and its gradient in one
+        pyfwi_misfit_gradient call; the same values as forward_modeling,
+        CostFunction.l2 and gradient (fwi.py:229-252).  Updates
+        ``seismogram``; no stand-alone ``gradient`` may follow."""
+        model = {k: expand_model(np.asarray(v, np.float32), int(self.tnz), int(self.tnx),
+                                 self.npml) for k, v in model0.items()}
+        self.pml_preparation(model["vp"].max())
+        self.elastic_buffers(model)
+        obs = self._obs_records(d_obs)
+        ids = list(range(int(self.ns)))
+        G = np.zeros((6, int(self.tnz), int(self.tnx)), np.float32)
+        seis = np.zeros_like(obs)
+        flags = _native.F_PRECOND if self.energy_balancing else 0
+        rms = self._plan.misfit_gradient(self._shots(ids), self._source_table(), obs, seis, G,
+                                         self.chp, COMPONENT_MASK[self.components], flags,
+                                         _native.MEM_HOST)
+        for f, name in enumerate(COMPONENTS):
+            self.seismogram[name][:] = seis[:, f].transpose(1, 2, 0)
+        # the plan keeps only the last pass's states
+        self._forward_done = False
+        return rms, self._finish_gradient(G, parameterization)
 
     # ------------------------------------------------ device-resident path
     # Used by pyfwi_amd.torchfwi: model, records and gradients stay in torch
