@@ -80,10 +80,15 @@ def measured_traffic(workload_name, kernel):
     return d.get("bytes_per_dispatch", {}).get(kernel)
 
 
-def build_problem(workload, nt=None):
+def build_problem(workload, nt=None, shots=None):
     from pyfwi_amd import workloads
     from pyfwi_amd.wave_propagation import WavePropagator
-    w = getattr(workloads, workload)(**({"nt": nt} if nt else {}))
+    kw = {}
+    if nt:
+        kw["nt"] = nt
+    if shots:
+        kw["ns"] = shots
+    w = getattr(workloads, workload)(**kw)
     W = WavePropagator(w.inpa, w.src, w.rec_loc, w.model_shape, components=w.components,
                        chpr=w.chpr)
     return w, W
@@ -236,6 +241,58 @@ def copy_calibration(torch, nbytes=1 << 30):
     log(f"bench: copy calibration {nbytes} bytes read + {nbytes} bytes written, 3 launches")
 
 
+def bench_uneven(args, torch):
+    """Uneven shot batch (uneven_batch: 1000x2000 cells, 64 shots with their
+    own nt, aperture and Ricker frequency): one misfit + reference gradient
+    of all shots, every state kept, passes sized to the free HBM."""
+    from pyfwi_amd import workloads
+    from pyfwi_amd.batch import UnevenBatch
+    wl = workloads.uneven_batch(ns=args.shots or 64)
+    B = UnevenBatch(wl.inpa, wl.specs, wl.model_shape, components=wl.components)
+    plan = B.plan
+    plan.set_stream(torch.cuda.current_stream().cuda_stream)
+    dev = torch.device("cuda:0")
+    src = torch.from_numpy(B.src_table).to(dev)
+    obs = torch.zeros(B.n_records, dtype=torch.float32, device=dev)
+    seis = torch.zeros_like(obs)
+    grad = torch.zeros(6 * B.tnz * B.tnx, dtype=torch.float32, device=dev)
+    log(f"bench: {wl.name}: {B.ns} shots, nt {min(B.nt)}..{max(B.nt)}, receivers "
+        f"{min(B.nr)}..{max(B.nr)}, {B.cell_updates() / 1e9:.1f} G cell-updates per gradient")
+    B.set_model(wl.model_obs)
+    B.forward(seis=obs, mem=_native.MEM_DEVICE, src_table=src)
+    B.set_model(wl.model)
+    torch.cuda.synchronize()
+
+    def step():
+        return B.misfit_gradient(obs, grad=grad, seis=seis, mem=_native.MEM_DEVICE,
+                                 src_table=src)[0]
+    for i in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        log(f"bench: warmup {i + 1}/{args.warmup} done")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        misfit = step()
+    torch.cuda.synchronize()
+    elapsed = (time.perf_counter() - t0) / args.steps
+    value = B.cell_updates() / elapsed / 1e9
+    log(f"bench: {elapsed * 1e3:.0f} ms per gradient, {value:.2f} Gcell/s")
+    print(json.dumps({
+        "metric": "fwd+adj gradient Gcell-updates/s, batched vs per-shot, % HBM roofline",
+        "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(elapsed * 1e3, 1), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded random-layered model, per-shot Ricker sources)",
+        "config": {"workload": wl.name, "nz": 1000, "nx": 2000, "cells": B.tnz * B.tnx,
+                   "shots": B.ns, "nt_range": [min(B.nt), max(B.nt)],
+                   "receivers_range": [min(B.nr), max(B.nr)],
+                   "gradient": "reference adjoint, every state kept (chpr=100), passes sized "
+                               "to free HBM",
+                   "cell_updates": B.cell_updates()},
+        "gradient_roofline_frac": round(value * 148.0 / HBM_PEAK_GBS, 4),
+        "misfit": misfit}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -243,6 +300,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="plain_medium")
     ap.add_argument("--nt", type=int, default=0, help="override time steps (profiling runs)")
+    ap.add_argument("--shots", type=int, default=0,
+                    help="override the number of shots (e.g. one GPU's share of large_step)")
     ap.add_argument("--per-shot-sample", type=int, default=2, help="0 skips the per-shot pass")
     ap.add_argument("--cpu-steps", type=int, default=2000)
     ap.add_argument("--cpu-steps-1t", type=int, default=150)
@@ -266,7 +325,9 @@ def main():
     log("bench: building problem")
     if args.copy_calibration:
         copy_calibration(torch)
-    w, W = build_problem(args.workload, args.nt or None)
+    if args.workload == "uneven_batch":
+        return bench_uneven(args, torch)
+    w, W = build_problem(args.workload, args.nt or None, args.shots or None)
     b = device_setup(w, W, torch)
     cells = int(W.tnz) * int(W.tnx)
     sdo, npml = int(W.sdo), int(W.npml)

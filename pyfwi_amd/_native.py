@@ -103,12 +103,25 @@ def check(code):
         raise NativeError(code, msg)
 
 
-def ptr(a):
-    """Raw address of a contiguous numpy array or torch tensor."""
+def ptr(a, mem=None):
+    """Raw address of a contiguous numpy array or torch tensor.  With `mem`
+    given, the buffer must live where `mem` says: host memory (a numpy
+    array or a CPU tensor) for MEM_HOST, a GPU tensor for MEM_DEVICE -- the
+    library takes the address as it is, and a kernel that reads host memory
+    through it faults the GPU."""
     if isinstance(a, np.ndarray):
+        if mem == MEM_DEVICE:
+            raise ValueError("a host (numpy) array was passed with MEM_DEVICE")
         if not a.flags["C_CONTIGUOUS"]:
             raise ValueError("array must be C-contiguous")
         return a.ctypes.data
+    on_gpu = bool(getattr(a, "is_cuda", False))
+    if mem == MEM_DEVICE and not on_gpu:
+        raise ValueError("a host tensor was passed with MEM_DEVICE")
+    if mem == MEM_HOST and on_gpu:
+        raise ValueError("a device tensor was passed with MEM_HOST")
+    if hasattr(a, "is_contiguous") and not a.is_contiguous():
+        raise ValueError("tensor must be contiguous")
     return a.data_ptr()
 
 
@@ -137,21 +150,22 @@ class Plan:
         check(self.lib.pyfwi_plan_set_stream(self.h, stream))
 
     def set_model(self, lam, mu, buoy, dxp, dzp, mem):
-        check(self.lib.pyfwi_set_model(self.h, ptr(lam), ptr(mu), ptr(buoy), ptr(dxp),
-                                       ptr(dzp), mem))
+        check(self.lib.pyfwi_set_model(self.h, ptr(lam, mem), ptr(mu, mem), ptr(buoy, mem),
+                                       ptr(dxp, mem), ptr(dzp, mem), mem))
 
     def forward(self, shots, src, seis, chp, ckpt_every, mem):
         arr = (Shot * len(shots))(*shots)
         chp = np.ascontiguousarray(chp, np.int32) if chp is not None else np.zeros(0, np.int32)
         cp = chp.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)) if chp.size else None
-        check(self.lib.pyfwi_forward(self.h, arr, len(shots), ptr(src), ptr(seis), cp,
+        check(self.lib.pyfwi_forward(self.h, arr, len(shots), ptr(src, mem), ptr(seis, mem), cp,
                                      int(chp.size), int(ckpt_every), mem))
 
     def gradient(self, shots, src, res, grad, mode, ckpt_every, flags, mem):
         arr = (Shot * len(shots))(*shots)
         check(self.lib.pyfwi_gradient(self.h, arr, len(shots),
-                                      ptr(src) if src is not None else None, ptr(res),
-                                      ptr(grad), mode, int(ckpt_every), flags, mem))
+                                      ptr(src, mem) if src is not None else None,
+                                      ptr(res, mem), ptr(grad, mem), mode, int(ckpt_every), flags,
+                                      mem))
 
     def misfit_gradient(self, shots, src, obs, seis, grad, chp, comp_mask, flags, mem):
         """Returns the L2 misfit; see pyfwi_misfit_gradient."""
@@ -159,8 +173,8 @@ class Plan:
         chp = np.ascontiguousarray(chp, np.int32)
         out = ctypes.c_double()
         check(self.lib.pyfwi_misfit_gradient(
-            self.h, arr, len(shots), ptr(src), ptr(obs), ptr(seis) if seis is not None else None,
-            ptr(grad), ctypes.byref(out), chp.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+            self.h, arr, len(shots), ptr(src, mem), ptr(obs, mem),
+            ptr(seis, mem) if seis is not None else None, ptr(grad, mem), ctypes.byref(out), chp.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
             int(chp.size), int(comp_mask), int(flags), mem))
         return out.value
 
@@ -169,7 +183,7 @@ class Plan:
                                              int(strip_rows)))
 
     def snapshot(self, shot, slot, out, mem):
-        check(self.lib.pyfwi_get_snapshot(self.h, shot, slot, ptr(out), mem))
+        check(self.lib.pyfwi_get_snapshot(self.h, shot, slot, ptr(out, mem), mem))
 
     def synchronize(self):
         check(self.lib.pyfwi_synchronize(self.h))

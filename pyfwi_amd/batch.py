@@ -121,11 +121,14 @@ class UnevenBatch:
         self.base.pml_preparation(model["vp"].max())
         self.base.elastic_buffers(model)
 
-    def forward(self, seis=None, mem=_native.MEM_HOST):
-        """Records of all shots into `seis` (flat, n_records floats)."""
+    def forward(self, seis=None, mem=_native.MEM_HOST, src_table=None):
+        """Records of all shots into `seis` (flat, n_records floats).  With
+        mem = MEM_DEVICE, `seis` and `src_table` (a device copy of
+        ``self.src_table``) are device buffers."""
         if seis is None:
             seis = np.zeros(self.n_records, np.float32)
-        self.plan.forward(self.shots(), self.src_table, seis, np.zeros(0, np.int32), 0, mem)
+        src = self.src_table if src_table is None else src_table
+        self.plan.forward(self.shots(), src, seis, np.zeros(0, np.int32), 0, mem)
         return seis
 
     def misfit_gradient(self, obs, grad=None, seis=None, flags=0, mem=_native.MEM_HOST,
@@ -133,7 +136,8 @@ class UnevenBatch:
         """L2 misfit of the components selected by ``components`` against
         `obs` (flat records) and its reference-semantics gradient with every
         forward state kept (chpr = 100, each shot its own nt).  Returns
-        (misfit, grad[6, tnz, tnx]) as pyfwi_misfit_gradient."""
+        (misfit, grad[6, tnz, tnx]) as pyfwi_misfit_gradient.  With mem =
+        MEM_DEVICE all buffers, `src_table` included, are device buffers."""
         if grad is None:
             grad = np.zeros((6, self.tnz, self.tnx), np.float32)
         chp = checkpoint_steps(self.nt_max, 100)

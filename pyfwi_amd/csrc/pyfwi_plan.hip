@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <initializer_list>
 #include <vector>
 
 #include "../../include/pyfwi_hip.h"
@@ -234,6 +235,25 @@ int upload_shots(pyfwi_plan *p, const pyfwi_shot *shots, int nshots)
 
 // Device pointer for a caller array: the pointer itself for device memory,
 // otherwise a staged copy (direction given by `in`).
+// With PYFWI_MEM_DEVICE every caller pointer must be memory HIP knows
+// (device, managed or pinned host): the kernels dereference it as it is, and
+// a plain host allocation read by a kernel faults the GPU.  Null passes.
+int check_device_ptrs(int mem, std::initializer_list<std::pair<const void *, const char *>> ptrs)
+{
+    if (mem != PYFWI_MEM_DEVICE) return 0;
+    for (const auto &pw : ptrs) {
+        if (!pw.first) continue;
+        hipPointerAttribute_t a{};
+        const hipError_t e = hipPointerGetAttributes(&a, pw.first);
+        if (e != hipSuccess) (void)hipGetLastError();
+        if (e != hipSuccess || a.type == hipMemoryTypeUnregistered)
+            return fail(PYFWI_EINVAL, std::string(pw.second) +
+                                          " is not memory the GPU can address (PYFWI_MEM_DEVICE "
+                                          "needs a device, managed or pinned pointer)");
+    }
+    return 0;
+}
+
 int stage_in(pyfwi_plan *p, DevBuf &stage, const float *ptr, size_t n, int mem,
              const float **out, const char *what)
 {
@@ -337,6 +357,9 @@ int32_t pyfwi_set_model(pyfwi_plan *p, const float *lam, const float *mu, const 
                         const float *dx_prof, const float *dz_prof, int32_t mem)
 {
     if (!p || !lam || !mu || !buoy || !dx_prof || !dz_prof) return fail(PYFWI_EINVAL, "null argument");
+    if (int e = check_device_ptrs(mem, {{lam, "lam"}, {mu, "mu"}, {buoy, "buoyancy"},
+                                        {dx_prof, "dx profile"}, {dz_prof, "dz profile"}}))
+        return e;
     const DevGrid &g = p->dg;
     const size_t n = (size_t)g.nz * g.nx;
     if (mem == PYFWI_MEM_HOST) {
@@ -556,6 +579,7 @@ int32_t pyfwi_forward(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, co
                       int32_t mem)
 {
     if (!p || !src || !seis) return fail(PYFWI_EINVAL, "null argument");
+    if (int e = check_device_ptrs(mem, {{src, "source table"}, {seis, "records"}})) return e;
     if (!p->model_set) return fail(PYFWI_ESTATE, "pyfwi_set_model must come first");
     if (int e = check_shots(p, shots, nshots)) return e;
     if (nchp < 0 || (nchp > 0 && !chp)) return fail(PYFWI_EINVAL, "bad checkpoint list");
@@ -721,6 +745,9 @@ int32_t pyfwi_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, c
                        int32_t flags, int32_t mem)
 {
     if (!p || !res || !grad) return fail(PYFWI_EINVAL, "null argument");
+    if (int e = check_device_ptrs(mem, {{src, "source table"}, {res, "residuals"},
+                                        {grad, "gradient"}}))
+        return e;
     if (!p->model_set) return fail(PYFWI_ESTATE, "pyfwi_set_model must come first");
     if (mode != PYFWI_MODE_PYFWI && mode != PYFWI_MODE_EXACT) return fail(PYFWI_EINVAL, "unknown mode");
     if (int e = check_shots(p, shots, nshots)) return e;
@@ -799,6 +826,9 @@ int32_t pyfwi_misfit_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t ns
 {
     if (!p || !src || !obs || !grad || !misfit || !chp || nchp <= 0)
         return fail(PYFWI_EINVAL, "null argument or empty checkpoint list");
+    if (int e = check_device_ptrs(mem, {{src, "source table"}, {obs, "observed records"},
+                                        {seis, "records"}, {grad, "gradient"}}))
+        return e;
     if (!p->model_set) return fail(PYFWI_ESTATE, "pyfwi_set_model must come first");
     if (int e = check_shots(p, shots, nshots)) return e;
     if (comp_mask < 0 || comp_mask > 31) return fail(PYFWI_EINVAL, "comp_mask must be 0..31");
@@ -890,6 +920,7 @@ int32_t pyfwi_plan_set_tuning(pyfwi_plan *p, int32_t shots_per_pass, int32_t fus
 int32_t pyfwi_get_snapshot(pyfwi_plan *p, int32_t shot, int32_t slot, float *out, int32_t mem)
 {
     if (!p || !out) return fail(PYFWI_EINVAL, "null argument");
+    if (int e = check_device_ptrs(mem, {{out, "snapshot output"}})) return e;
     if (!p->have_forward) return fail(PYFWI_ESTATE, "no forward run yet");
     const int nchp = (int)p->chp.size();
     if (shot < 0 || shot >= (int)p->last_shots.size() || slot < 0 || slot >= nchp)

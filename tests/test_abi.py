@@ -5,6 +5,9 @@ import glob
 import os
 import re
 
+import numpy as np
+import pytest
+
 from pyfwi_amd import _native
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -46,3 +49,21 @@ def test_invalid_grid_is_rejected_before_any_device_call():
     assert rc == 1 and b"sdo" in lib.pyfwi_last_error()
     g.sdo, g.dt = 2, -1.0
     assert lib.pyfwi_plan_create(ctypes.byref(g), None, ctypes.byref(h)) == 1
+
+
+def test_pointer_placement_is_checked():
+    """The library takes caller addresses as they are; the binding refuses a
+    host buffer passed as device memory (which a kernel would fault on) and a
+    device tensor passed as host memory."""
+    import torch
+    from pyfwi_amd import _native
+    a = np.zeros(4, np.float32)
+    assert _native.ptr(a, _native.MEM_HOST) == a.ctypes.data
+    with pytest.raises(ValueError):
+        _native.ptr(a, _native.MEM_DEVICE)
+    t = torch.zeros(4)
+    assert _native.ptr(t, _native.MEM_HOST) == t.data_ptr()
+    with pytest.raises(ValueError):
+        _native.ptr(t, _native.MEM_DEVICE)
+    with pytest.raises(ValueError):
+        _native.ptr(torch.zeros(4, 2).t(), _native.MEM_HOST)

@@ -68,3 +68,28 @@ def test_uneven_batch_equals_one_shot_per_pass(gpu):
     np.testing.assert_array_equal(s1, s2)
     np.testing.assert_array_equal(G1, G2)
     assert abs(m1 - m2) <= 1e-12 * abs(m1)
+
+
+def test_uneven_batch_device_buffers(gpu):
+    """All buffers on the device (the source table included) give the same
+    records and gradient as host buffers."""
+    import torch
+    from pyfwi_amd import _native
+    wl, B, obs = _batch()
+    seis_h = np.zeros(B.n_records, np.float32)
+    m_h, G_h = B.misfit_gradient(obs, seis=seis_h)
+    dev = torch.device("cuda:0")
+    B.plan.set_stream(torch.cuda.current_stream().cuda_stream)
+    src = torch.from_numpy(B.src_table).to(dev)
+    obs_d = torch.from_numpy(obs).to(dev)
+    fwd_d = torch.zeros(B.n_records, dtype=torch.float32, device=dev)
+    B.forward(seis=fwd_d, mem=_native.MEM_DEVICE, src_table=src)
+    seis_d = torch.zeros_like(fwd_d)
+    grad_d = torch.zeros(6 * B.tnz * B.tnx, dtype=torch.float32, device=dev)
+    m_d, _ = B.misfit_gradient(obs_d, grad=grad_d, seis=seis_d, mem=_native.MEM_DEVICE,
+                               src_table=src)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(seis_d.cpu().numpy(), seis_h)
+    np.testing.assert_array_equal(fwd_d.cpu().numpy(), seis_h)
+    np.testing.assert_array_equal(grad_d.cpu().numpy().reshape(G_h.shape), G_h)
+    assert abs(m_d - m_h) <= 1e-12 * abs(m_h)
