@@ -271,9 +271,11 @@ def bench_uneven(args, torch):
         torch.cuda.synchronize()
         log(f"bench: warmup {i + 1}/{args.warmup} done")
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for i in range(args.steps):
+        ts = time.perf_counter()
         misfit = step()
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        log(f"bench: step {i + 1}/{args.steps}: {time.perf_counter() - ts:.2f} s")
     elapsed = (time.perf_counter() - t0) / args.steps
     value = B.cell_updates() / elapsed / 1e9
     log(f"bench: {elapsed * 1e3:.0f} ms per gradient, {value:.2f} Gcell/s")

@@ -1159,7 +1159,10 @@ __global__ void k_copy_shots(float *dst, int64_t dst_stride, const float *src,
 }
 
 // Sum per-shot accumulators into the unpadded output [nslot][tnz*tnx];
-// exact mode converts dJ/db (slot 4) to dJ/drho = -b^2 dJ/db.
+// exact mode converts dJ/db (slot 4) to dJ/drho = -b^2 dJ/db.  With
+// `accumulate` the shots are added one by one onto the running output, so
+// the sum is the same, bit for bit, however the shots were split into
+// passes (the converted slot adds its pass total instead).
 __global__ void k_grad_sum(DevGrid g, int nshots, int nslot, const float *gacc,
                            float *out, const float *__restrict__ bu, int b_to_rho, int accumulate)
 {
@@ -1171,10 +1174,14 @@ __global__ void k_grad_sum(DevGrid g, int nshots, int nslot, const float *gacc,
         const int64_t cell = k - (int64_t)slot * n;
         const int i = (int)(cell / g.nx), j = (int)(cell - (int64_t)i * g.nx);
         const int64_t p = g.origin + (int64_t)i * g.pitch + j;
-        float acc = 0.0f;
+        const bool conv = b_to_rho && slot == 4;
+        float acc = accumulate && !conv ? out[k] : 0.0f;
         for (int s = 0; s < nshots; ++s) acc += gacc[((int64_t)s * 6 + slot) * fs + p];
-        if (b_to_rho && slot == 4) acc = -(bu[p] * bu[p]) * acc;
-        out[k] = accumulate ? out[k] + acc : acc;
+        if (conv) {
+            acc = -(bu[p] * bu[p]) * acc;
+            if (accumulate) acc += out[k];
+        }
+        out[k] = acc;
     }
 }
 

@@ -477,14 +477,20 @@ int reverse_step(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t)
     return 0;
 }
 
-// True when the checkpoint list is every step 0..nt-1 (slot t = state after
-// step t).
+// True when the checkpoint list is every step 0..n-1 with n >= nt (slot t =
+// state after step t), so it keeps every step of shots of up to nt steps.
 bool keeps_every_step(const std::vector<int32_t> &cl, int nt)
 {
-    if ((int)cl.size() != nt) return false;
-    for (int k = 0; k < nt; ++k)
+    if ((int)cl.size() < nt || cl.empty()) return false;
+    for (int k = 0; k < (int)cl.size(); ++k)
         if (cl[k] != k) return false;
     return true;
+}
+
+void every_step(int nt, std::vector<int32_t> &cl)
+{
+    cl.resize(nt);
+    for (int k = 0; k < nt; ++k) cl[k] = k;
 }
 
 // A checkpoint list other than every step is a list of the reference's
@@ -633,19 +639,22 @@ int32_t pyfwi_forward(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, co
 namespace {
 
 // Reference adjoint of one pass (shots g0 .. g0+ng-1) into gacc[ng].
+// `snaps` holds the pass's kept states, per-shot stride cl.size() states, at
+// the steps of `cl`.
 int gradient_pyfwi(pyfwi_plan *p, int g0, int ng, const float *d_res, bool precond,
-                   const float *snaps)
+                   const float *snaps, const std::vector<int32_t> &cl)
 {
     const DevGrid &g = p->dg;
     const int64_t fps = fields_per_shot(p);
     const pyfwi_shot *dsh = p->d_shots + g0;
-    const int nt = max_nt(p->last_shots);
-    const int nchp = (int)p->chp.size();
+    const int nt = max_nt(std::vector<pyfwi_shot>(p->h_shots.begin() + g0,
+                                                  p->h_shots.begin() + g0 + ng));
+    const int nchp = (int)cl.size();
     if (int e = ensure(p->adj, (size_t)ng * fps, p->stream, "adjoint fields")) return e;
     HIPCHK(hipMemsetAsync(p->adj.p, 0, (size_t)ng * fps * sizeof(float), p->stream));
     if (int e = prepare_fields(p, ng)) return e;
     // every step kept: read the forward state of step t from its slot
-    const bool direct = keeps_every_step(p->chp, nt);
+    const bool direct = keeps_every_step(cl, nt);
     int chpc = nchp - 1;
     for (int t = nt - 1; t > 0; --t) {
         const float *fwd = p->cur;
@@ -653,7 +662,7 @@ int gradient_pyfwi(pyfwi_plan *p, int g0, int ng, const float *d_res, bool preco
         if (direct) {
             fwd = snaps + (int64_t)t * fps;
             fwd_stride = (int64_t)nchp * fps;
-        } else if (chpc >= 0 && t == p->chp[chpc]) {
+        } else if (chpc >= 0 && t == cl[chpc]) {
             launch_copy_shots(p->cur, fps, snaps + (int64_t)chpc * fps, (int64_t)nchp * fps, fps,
                               ng, p->stream);
             chpc--;
@@ -786,7 +795,8 @@ int32_t pyfwi_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, c
         HIPCHK(hipMemsetAsync(p->gacc.p, 0, (size_t)ng * 6 * g.fstride * sizeof(float), p->stream));
         if (mode == PYFWI_MODE_PYFWI) {
             const float *sn = p->snaps.p + (int64_t)g0 * p->chp.size() * fps;
-            if (int e = gradient_pyfwi(p, g0, ng, d_res, (flags & PYFWI_F_PRECOND) != 0, sn))
+            if (int e = gradient_pyfwi(p, g0, ng, d_res, (flags & PYFWI_F_PRECOND) != 0, sn,
+                                       p->chp))
                 return e;
         } else {
             std::vector<pyfwi_shot> shp(shots + g0, shots + g0 + ng);
@@ -836,11 +846,18 @@ int32_t pyfwi_misfit_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t ns
     const int nt = max_nt(sh);
     std::vector<int32_t> cl(chp, chp + nchp);
     if (int e = check_chp_nt(sh, cl, nt)) return e;
+    const bool every = keeps_every_step(cl, nt);
     const DevGrid &g = p->dg;
     const int64_t fps = fields_per_shot(p);
     const size_t nrec = rec_extent(shots, nshots), nsrc = src_extent(shots, nshots);
     const size_t n = (size_t)g.nz * g.nx;
-    if (int e = upload_shots(p, shots, nshots)) return e;
+    // passes run the shots longest first, so that each pass's kept states
+    // are sized by its own longest shot; outputs follow the shots' offsets
+    std::vector<pyfwi_shot> shs = sh;
+    if (every)
+        std::stable_sort(shs.begin(), shs.end(),
+                         [](const pyfwi_shot &a, const pyfwi_shot &b) { return a.nt > b.nt; });
+    if (int e = upload_shots(p, shs.data(), nshots)) return e;
     const float *d_src = nullptr, *d_obs = nullptr;
     if (int e = stage_in(p, p->src_stage, src, nsrc, mem, &d_src, "source table")) return e;
     if (int e = stage_in(p, p->res_stage, obs, nrec, mem, &d_obs, "observed records")) return e;
@@ -861,33 +878,48 @@ int32_t pyfwi_misfit_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t ns
     p->chp = cl;
     p->last_shots = sh;
     p->have_forward = false;  // kept states belong to the last pass only
-    // shots per pass: the tuned value, else as many as the kept states of
-    // free device memory allow (the buffer held now counts as free)
-    int G = pass_size(p, nshots);
-    if (p->tune_group <= 0) {
+    // kept-state slots: the tuned pass size times the longest shot; else the
+    // buffer held from an earlier call when it takes the longest shot (a
+    // reallocation of this size costs seconds), or as many states as 90 % of
+    // the free memory holds (the buffer held counts as free), never more
+    // than all shots need
+    size_t slots;
+    const size_t first = every ? (size_t)shs[0].nt : (size_t)nchp;
+    size_t need = 0;
+    for (auto &x : shs) need += every ? (size_t)x.nt : (size_t)nchp;
+    if (p->tune_group > 0) {
+        slots = (size_t)std::min(p->tune_group, nshots) * first;
+    } else if (p->snaps.n / (size_t)fps >= first) {
+        slots = std::min(need, p->snaps.n / (size_t)fps);
+    } else {
         size_t free_b = 0, total_b = 0;
         HIPCHK(hipMemGetInfo(&free_b, &total_b));
-        const double per_shot = (double)nchp * fps * sizeof(float);
         const double avail = 0.9 * ((double)free_b + (double)p->snaps.n * sizeof(float));
-        const int fit = (int)std::min<double>(nshots, avail / per_shot);
-        if (fit < 1)
+        slots = std::min(need, (size_t)(avail / ((double)fps * sizeof(float))));
+        if (slots < first)
             return fail(PYFWI_ENOMEM, "the kept states of one shot (" +
-                                          std::to_string((long long)per_shot) +
+                                          std::to_string((long long)(first * fps * sizeof(float))) +
                                           " bytes) do not fit in device memory");
-        G = fit;
     }
-    if (int e = ensure(p->snaps, (size_t)G * nchp * fps, p->stream, "checkpoints")) return e;
-    for (int g0 = 0; g0 < nshots; g0 += G) {
-        const int ng = std::min(G, nshots - g0);
-        if (int e = forward_pass(p, g0, ng, d_src, d_seis, cl, p->snaps.p, nullptr, 0, 1)) return e;
+    if (int e = ensure(p->snaps, slots * fps, p->stream, "checkpoints")) return e;
+    for (int g0 = 0; g0 < nshots;) {
+        const int nt_pass = every ? shs[g0].nt : nt;
+        const int stride = every ? nt_pass : nchp;
+        int ng = p->tune_group > 0 ? p->tune_group : (int)(slots / (size_t)stride);
+        ng = std::max(1, std::min(ng, nshots - g0));
+        std::vector<int32_t> cl_pass = cl;
+        if (every) every_step(nt_pass, cl_pass);
+        if (int e = forward_pass(p, g0, ng, d_src, d_seis, cl_pass, p->snaps.p, nullptr, 0, 1))
+            return e;
         launch_l2_residual(ng, p->d_shots + g0, d_seis, d_obs, p->resid.p, comp_mask, p->d_misfit,
                            p->stream);
         if (int e = ensure(p->gacc, (size_t)ng * 6 * g.fstride, p->stream, "gradient")) return e;
         HIPCHK(hipMemsetAsync(p->gacc.p, 0, (size_t)ng * 6 * g.fstride * sizeof(float), p->stream));
         if (int e = gradient_pyfwi(p, g0, ng, p->resid.p, (flags & PYFWI_F_PRECOND) != 0,
-                                   p->snaps.p))
+                                   p->snaps.p, cl_pass))
             return e;
         launch_grad_sum(g, ng, 6, p->gacc.p, d_grad, p->bu.p, 0, g0 > 0, p->stream);
+        g0 += ng;
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(misfit, p->d_misfit, sizeof(double), hipMemcpyDeviceToHost, p->stream));

@@ -55,13 +55,15 @@ def test_uneven_batch_matches_per_shot_oracle(gpu):
         assert e < 1e-4, (k, e)
 
 
-def test_uneven_batch_equals_one_shot_per_pass(gpu):
-    """Batching changes only the schedule: one shot per pass gives the same
-    records and the same gradient, bit for bit."""
+@pytest.mark.parametrize("per_pass", [1, 2, 3])
+def test_uneven_batch_pass_split_is_bitwise(gpu, per_pass):
+    """Passes change only the schedule: any split of the shots into passes
+    gives the same records and the same gradient, bit for bit (each pass's
+    accumulators are added shot by shot onto the running sum)."""
     wl, B, obs = _batch()
     s1 = np.zeros(B.n_records, np.float32)
     m1, G1 = B.misfit_gradient(obs, seis=s1)
-    B.plan.set_tuning(1, False, 0)
+    B.plan.set_tuning(per_pass, False, 0)
     s2 = np.zeros(B.n_records, np.float32)
     m2, G2 = B.misfit_gradient(obs, seis=s2)
     B.plan.set_tuning(0, False, 0)
