@@ -41,32 +41,32 @@ struct DevGrid {
 
 // Velocity half step: state read from `in`, velocities written to `out`
 // (per-shot strides in floats; in == out with stride 5*fstride updates in
-// place, the only form the time-reversed direction takes).
+// place, the only form the time-reversed direction takes).  Forward with
+// `src`: the step's source term is added to the values loaded at the source
+// cell (memory is not changed); src == nullptr when the caller injected it.
 void launch_vel(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
                 const float *in, int64_t in_stride, float *out, int64_t out_stride,
-                const float *bu, const float *dxp, const float *dzp, hipStream_t st);
+                const float *bu, const float *dxp, const float *dzp, hipStream_t st,
+                const float *src = nullptr);
 // Fused velocity + stress step (forward, DIR=+1) or stress + velocity step
 // (time reversed, DIR=-1): reads `in`, writes `out` (both [nshots][5][fs]).
 void launch_fused_step(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
                        const float *in, float *out, const float *bu, const float *lam,
                        const float *mu, const float *dxp, const float *dzp, hipStream_t st);
 // Stress half step: forward reads this step's velocities from `out` and the
-// previous stresses from `in`, writes the stresses to `out`.
+// previous stresses from `in`, writes the stresses to `out`.  Forward with
+// `src`, the source term on txx, tzz at the source cell; with `seis`, the
+// record sample of step t+1 (this step's result) at the receivers.
 void launch_str(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
                 const float *in, int64_t in_stride, float *out, int64_t out_stride,
                 const float *lam, const float *mu, const float *dxp, const float *dzp,
-                hipStream_t st);
+                hipStream_t st, const float *src = nullptr, float *seis = nullptr);
+// Record the state before step t's source term, then add the term in memory
+// (the fused steps and the exact mode's recomputation, which keep the
+// injected stresses).
 void launch_inject_record(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
                           float *fields, const float *src, float *seis, hipStream_t st);
-// Kept-state form: record from and inject into `fields` (the slot of step
-// t-1, per-shot stride fields_stride) after restoring the source cell of
-// `prev` (the slot injected at step t-1) from saved[nshots][4], whose values
-// are then replaced by the clean source-cell values of `fields`.  fields ==
-// nullptr only restores `prev`.
-void launch_inject_record_kept(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
-                               float *fields, int64_t fields_stride, float *prev,
-                               int64_t prev_stride, float *saved, const float *src, float *seis,
-                               hipStream_t st);
+// Residual injection in memory, before the adjoint stress update.
 void launch_adj_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
                        float *adj, const float *res, hipStream_t st);
 void launch_adj_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,

@@ -280,6 +280,77 @@ __device__ __forceinline__ void prime(ZWin<SDO> &w, int64_t c0, int pitch, F val
     for (int k = 0; k < 2 * SDO; ++k) w.v[k + 1] = val(c0 + (int64_t)(k - SDO) * pitch);
 }
 
+// ------------------------------------------- sources, records, residuals
+//
+// The forward stencil kernels apply the sparse terms themselves instead of a
+// separate launch: the source of step t is added to the values they load at
+// the source cell (vx, vz, txx, tzz), and the record sample of step t+1 is
+// written by the stress update of step t at the receiver cells.  The sums are
+// the float additions the reference makes in memory (elastic_surface.cl:
+// 7-41), so the results are the same; the state in memory itself stays free
+// of source terms.  (The adjoint's residual injection stays a launch of its
+// own: carrying it in the adjoint kernels cost them more time than the
+// launch takes.)
+
+// Source of step t for one shot (injSrc, elastic_surface.cl:34-40); z = -1
+// when the step injects nothing.
+struct SrcTerm {
+    int z = -1, x = -1;
+    float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+};
+
+__device__ __forceinline__ SrcTerm src_term(const pyfwi_shot &sh, int t, const float *src)
+{
+    SrcTerm r;
+    if (src && t < sh.nt && (t % sh.dt_scale) == 0) {
+        const float *sv = src + sh.src_offset + (int64_t)(t / sh.dt_scale) * 5;
+        r.z = sh.src_z;
+        r.x = sh.src_x;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) r.v[f] = sv[f];
+    }
+    return r;
+}
+
+// A receiver line of one shot at one time sample: `base` points at component
+// 0, receiver 0 of the sample's row; component f is `fs` floats further.  z
+// = -1 when nothing is recorded at this step.
+struct RecLine {
+    int z = -1, x0 = 0, dx = 1, n = 0;
+    int64_t fs = 0;
+    const float *base = nullptr;
+    // receiver index of column `col` on the line, or -1
+    __device__ __forceinline__ int index(int col) const
+    {
+        const int d = col - x0;
+        if (d < 0 || d % dx != 0) return -1;
+        const int r = d / dx;
+        return r < n ? r : -1;
+    }
+    __device__ __forceinline__ float value(int col, int f) const
+    {
+        const int r = index(col);
+        return r >= 0 ? base[f * fs + r] : 0.0f;
+    }
+};
+
+// Sample of step t1 = t + 1 written by the forward step t (the reference
+// records before step t1's injection, which is the state after step t).
+__device__ __forceinline__ RecLine record_line(const pyfwi_shot &sh, int t1, float *seis)
+{
+    RecLine q;
+    if (!seis || t1 >= sh.nt || (t1 % sh.dt_scale) != 0) return q;
+    const int k = t1 / sh.dt_scale;
+    if (k >= sh.n_samples) return q;
+    q.z = sh.rec_z;
+    q.x0 = sh.rec_x0;
+    q.dx = sh.rec_dx;
+    q.n = sh.n_rec;
+    q.fs = (int64_t)sh.n_samples * sh.rec_stride;
+    q.base = seis + sh.rec_offset + (int64_t)k * sh.rec_stride;
+    return q;
+}
+
 // ---------------------------------------------------------- forward sweeps
 
 // Velocity half step of every active shot (src/PyFWI/elastic.cl:155-226).
@@ -295,7 +366,8 @@ __global__ __launch_bounds__(MARCH_X) void k_vel(DevGrid g, const pyfwi_shot *sh
                                                  const float *in, int64_t in_stride, float *out,
                                                  int64_t out_stride, const float *__restrict__ bu,
                                                  const float *__restrict__ dxp,
-                                                 const float *__restrict__ dzp)
+                                                 const float *__restrict__ dzp,
+                                                 const float *__restrict__ src)
 {
     int s, j, i0, i1;
     if (!tile<SDO>(g, s, j, i0, i1)) return;
@@ -306,15 +378,31 @@ __global__ __launch_bounds__(MARCH_X) void k_vel(DevGrid g, const pyfwi_shot *sh
     float *O = out + (int64_t)s * out_stride + g.origin;
     const CellRef vx{F}, vz{F + fs}, txx{F + 2 * fs}, tzz{F + 3 * fs}, txz{F + 4 * fs};
     const CellRef B{bu + g.origin};
+    // the step's source term (forward only): vx, vz at the centre, txx in the
+    // x-taps and tzz in the z-window at the source cell.  The terms this
+    // thread adds are fixed here; the loop only adds registers.
+    SrcTerm st = DIR > 0 ? src_term(shots[s], t, src) : SrcTerm{};
+    if (st.z < i0 - SDO || st.z > i1 + SDO || st.x < j - SDO || st.x > j + SDO) st.z = -1;
+    float s_txx[2 * SDO + 1];  // txx term at tap offset k (index k + SDO)
+#pragma unroll
+    for (int k = -SDO; k <= SDO; ++k) s_txx[k + SDO] = j + k == st.x ? st.v[2] : 0.0f;
+    const bool src_col = j == st.x;
+    const float s_vx = src_col ? st.v[0] : 0.0f, s_vz = src_col ? st.v[1] : 0.0f;
+    const float s_tzz = src_col ? st.v[3] : 0.0f;
+    auto tzz_row = [&](uint32_t o, int row) {
+        float v = tzz(o);
+        if (row == st.z) v += s_tzz;
+        return v;
+    };
     ZWin<SDO> wxz, wzz;  // txz for d/dz-, tzz for d/dz+
-    Fifo<SDO> fxz;   // d/dx+ txz
+    Fifo<SDO> fxz;       // d/dx+ txz
     auto dx_txz = [&](uint32_t o) { return xplus<SDO>([&](int k) { return txz.at(o, k); }, g.cx); };
     uint32_t cb = ((uint32_t)i0 * (uint32_t)g.pitch + (uint32_t)j) * 4u;
 #pragma unroll
     for (int k = 0; k < 2 * SDO; ++k) {
         const uint32_t o = cb + (uint32_t)(k - SDO) * RB;
         wxz.v[k + 1] = txz(o);
-        wzz.v[k + 1] = tzz(o);
+        wzz.v[k + 1] = tzz_row(o, i0 + k - SDO);
         if (k >= SDO) fxz.v[k - SDO] = dx_txz(o);
     }
     const float ddx = dxp[j];
@@ -322,14 +410,21 @@ __global__ __launch_bounds__(MARCH_X) void k_vel(DevGrid g, const pyfwi_shot *sh
     for (int i = i0; i <= i1; ++i, cb += RB) {
         const uint32_t n = cb + (uint32_t)SDO * RB;
         wxz.push(txz(n));
-        wzz.push(tzz(n));
+        wzz.push(tzz_row(n, i + SDO));
         const float dxp_txz = fxz.pop(dx_txz(n));
         const float bc = B(cb);
-        const float ax = bc * (xminus<SDO>([&](int k) { return txx.at(cb, k); }, g.cx)
-                               + wxz.dminus(g.cz));
+        float dxm_txx;
+        float ux = vx(cb), uz = vz(cb);
+        if (i == st.z) {  // the source row
+            dxm_txx = xminus<SDO>([&](int k) { return txx.at(cb, k) + s_txx[k + SDO]; }, g.cx);
+            ux += s_vx;
+            uz += s_vz;
+        } else {
+            dxm_txx = xminus<SDO>([&](int k) { return txx.at(cb, k); }, g.cx);
+        }
+        const float ax = bc * (dxm_txx + wxz.dminus(g.cz));
         const float az = bc * (wzz.dplus(g.cz) + dxp_txz);
         const float damp = ddx + dzp[i];
-        float ux = vx(cb), uz = vz(cb);
         if (DIR > 0) {
             ux = fmaf(g.dt, ax, ux);
             ux = fmaf(g.dt, -damp * ux, ux);
@@ -356,7 +451,8 @@ __global__ __launch_bounds__(MARCH_X) void k_str(DevGrid g, const pyfwi_shot *sh
                                                  int64_t out_stride, const float *__restrict__ lam,
                                                  const float *__restrict__ mu,
                                                  const float *__restrict__ dxp,
-                                                 const float *__restrict__ dzp)
+                                                 const float *__restrict__ dzp,
+                                                 const float *__restrict__ src, float *seis)
 {
     int s, j, i0, i1;
     if (!tile<SDO>(g, s, j, i0, i1)) return;
@@ -367,7 +463,14 @@ __global__ __launch_bounds__(MARCH_X) void k_str(DevGrid g, const pyfwi_shot *sh
     float *O = out + (int64_t)s * out_stride + g.origin;
     const CellRef vx{O}, vz{O + fs}, txx{F + 2 * fs}, tzz{F + 3 * fs}, txz{F + 4 * fs};
     const CellRef L{lam + g.origin}, M{mu + g.origin};
-    ZWin<SDO> wvz, wvx;      // vz for d/dz-, vx for d/dz+
+    // forward only: the source term on txx, tzz at the source cell, and the
+    // record sample of step t+1 (this step's result) at the receivers
+    SrcTerm st = DIR > 0 ? src_term(shots[s], t, src) : SrcTerm{};
+    if (st.x != j || st.z < i0 || st.z > i1) st.z = -1;
+    RecLine rl = DIR > 0 ? record_line(shots[s], t + 1, seis) : RecLine{};
+    const int rec_r = rl.z >= i0 && rl.z <= i1 ? rl.index(j) : -1;
+    if (rec_r < 0) rl.z = -1;
+    ZWin<SDO> wvz, wvx;  // vz for d/dz-, vx for d/dz+
     Fifo<SDO> fxx, fxz;  // d/dx+ vx, d/dx- vz
     auto dx_vx = [&](uint32_t o) { return xplus<SDO>([&](int k) { return vx.at(o, k); }, g.cx); };
     auto dx_vz = [&](uint32_t o) { return xminus<SDO>([&](int k) { return vz.at(o, k); }, g.cx); };
@@ -399,6 +502,10 @@ __global__ __launch_bounds__(MARCH_X) void k_str(DevGrid g, const pyfwi_shot *sh
         const float sxz = m * gxz;
         const float damp = ddx + dzp[i];
         float a = txx(cb), bb = tzz(cb), d = txz(cb);
+        if (i == st.z) {
+            a += st.v[2];
+            bb += st.v[3];
+        }
         if (DIR > 0) {
             a = fmaf(g.dt, sxx, a);
             bb = fmaf(g.dt, szz, bb);
@@ -417,6 +524,14 @@ __global__ __launch_bounds__(MARCH_X) void k_str(DevGrid g, const pyfwi_shot *sh
         cell_ref(O + 2 * fs, cb) = a;
         cell_ref(O + 3 * fs, cb) = bb;
         cell_ref(O + 4 * fs, cb) = d;
+        if (i == rl.z) {  // this thread's receiver
+            float *w = const_cast<float *>(rl.base) + rec_r;
+            w[0] = wvx.centre();
+            w[rl.fs] = wvz.centre();
+            w[2 * rl.fs] = a;
+            w[3 * rl.fs] = bb;
+            w[4 * rl.fs] = d;
+        }
     }
 }
 
@@ -673,28 +788,15 @@ __global__ __launch_bounds__(64 * FUSED_WAVES) void k_fused_rev(
 // holds the state after t steps; the source sample k is added to vx, vz, txx,
 // tzz at the source cell afterwards.
 __global__ void k_inject_record(DevGrid g, const pyfwi_shot *shots, int t,
-                                float *fields, int64_t fields_stride, float *prev,
-                                int64_t prev_stride, float *saved,
-                                const float *__restrict__ src, float *seis)
+                                float *fields, const float *__restrict__ src, float *seis)
 {
     const int s = blockIdx.x;
     const pyfwi_shot sh = shots[s];
-    const int64_t fs = g.fstride;
-    const int64_t cs = (int64_t)sh.src_z * g.pitch + sh.src_x;
-    // kept-state runs inject into the previous step's slot: put back the
-    // source cell of the slot injected one step earlier (when this shot was
-    // still stepping then, t - 1 < nt), and remember the clean values of this
-    // one
-    if (saved && prev && t <= sh.nt && threadIdx.x == 0) {
-        float *Q = prev + (int64_t)s * prev_stride + g.origin;
-#pragma unroll
-        for (int f = 0; f < 4; ++f) Q[f * fs + cs] = saved[s * 4 + f];
-    }
-    if (!fields || t >= sh.nt) return;
-    float *F = fields + (int64_t)s * fields_stride + g.origin;
-    const bool on = (t % sh.dt_scale) == 0;
+    if (t >= sh.nt || (t % sh.dt_scale) != 0) return;
     const int k = t / sh.dt_scale;
-    if (on && k < sh.n_samples) {
+    const int64_t fs = g.fstride;
+    float *F = fields + (int64_t)s * 5 * fs + g.origin;
+    if (k < sh.n_samples) {
         float *rec = seis + sh.rec_offset;
         const int64_t c0 = (int64_t)sh.rec_z * g.pitch + sh.rec_x0;
         for (int r = threadIdx.x; r < sh.n_rec; r += blockDim.x) {
@@ -706,17 +808,12 @@ __global__ void k_inject_record(DevGrid g, const pyfwi_shot *shots, int t,
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        if (saved) {
-#pragma unroll
-            for (int f = 0; f < 4; ++f) saved[s * 4 + f] = F[f * fs + cs];
-        }
-        if (on) {
-            const float *sv = src + sh.src_offset + (int64_t)k * 5;
-            F[cs] += sv[0];
-            F[fs + cs] += sv[1];
-            F[2 * fs + cs] += sv[2];
-            F[3 * fs + cs] += sv[3];
-        }
+        const float *sv = src + sh.src_offset + (int64_t)k * 5;
+        const int64_t c = (int64_t)sh.src_z * g.pitch + sh.src_x;
+        F[c] += sv[0];
+        F[fs + c] += sv[1];
+        F[2 * fs + c] += sv[2];
+        F[3 * fs + c] += sv[3];
     }
 }
 
@@ -1245,11 +1342,12 @@ static dim3 march_grid(DevGrid &g, int sdo, int nshots)
 
 void launch_vel(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
                 const float *in, int64_t in_stride, float *out, int64_t out_stride,
-                const float *bu, const float *dxp, const float *dzp, hipStream_t st)
+                const float *bu, const float *dxp, const float *dzp, hipStream_t st,
+                const float *src)
 {
     DevGrid gm = g;
     dim3 grid = march_grid(gm, g.sdo, nshots), block(MARCH_X);
-#define ARGS gm, shots, t, in, in_stride, out, out_stride, bu, dxp, dzp
+#define ARGS gm, shots, t, in, in_stride, out, out_stride, bu, dxp, dzp, src
     if (dir > 0) DISPATCH_SDO(g.sdo, k_vel, , 1);
     else DISPATCH_SDO(g.sdo, k_vel, , -1);
 #undef ARGS
@@ -1258,11 +1356,11 @@ void launch_vel(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, 
 void launch_str(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
                 const float *in, int64_t in_stride, float *out, int64_t out_stride,
                 const float *lam, const float *mu, const float *dxp, const float *dzp,
-                hipStream_t st)
+                hipStream_t st, const float *src, float *seis)
 {
     DevGrid gm = g;
     dim3 grid = march_grid(gm, g.sdo, nshots), block(MARCH_X);
-#define ARGS gm, shots, t, in, in_stride, out, out_stride, lam, mu, dxp, dzp
+#define ARGS gm, shots, t, in, in_stride, out, out_stride, lam, mu, dxp, dzp, src, seis
     if (dir > 0) DISPATCH_SDO(g.sdo, k_str, , 1);
     else DISPATCH_SDO(g.sdo, k_str, , -1);
 #undef ARGS
@@ -1302,17 +1400,8 @@ void launch_fused_step(const DevGrid &g, int dir, int nshots, const pyfwi_shot *
 void launch_inject_record(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
                           float *fields, const float *src, float *seis, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_inject_record, dim3(nshots), dim3(256), 0, st, g, shots, t, fields,
-                       (int64_t)5 * g.fstride, nullptr, (int64_t)0, nullptr, src, seis);
-}
-
-void launch_inject_record_kept(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
-                               float *fields, int64_t fields_stride, float *prev,
-                               int64_t prev_stride, float *saved, const float *src, float *seis,
-                               hipStream_t st)
-{
-    hipLaunchKernelGGL(k_inject_record, dim3(nshots), dim3(256), 0, st, g, shots, t, fields,
-                       fields_stride, prev, prev_stride, saved, src, seis);
+    hipLaunchKernelGGL(k_inject_record, dim3(nshots), dim3(256), 0, st, g, shots, t, fields, src,
+                       seis);
 }
 
 void launch_adj_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,

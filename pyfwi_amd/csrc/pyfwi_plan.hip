@@ -89,7 +89,7 @@ struct pyfwi_plan {
     // tuning: shots per pass (0: all), fused steps, strip height (0: automatic)
     int tune_group = 0, tune_fused = 0, tune_rows = 0;
     bool ckpts_all = false;  // ckpts holds every shot of the last forward
-    DevBuf src_stage, seis_stage, res_stage, grad_stage, shot_tab_f, resid, src_saved;
+    DevBuf src_stage, seis_stage, res_stage, grad_stage, shot_tab_f, resid;
     double *d_misfit = nullptr;
     pyfwi_shot *d_shots = nullptr;
     int shots_cap = 0;
@@ -100,6 +100,12 @@ struct pyfwi_plan {
     int ckpt_every = 0;
     int ncp = 0;
     bool have_forward = false;
+    // every source and receiver of the uploaded shots lies on an active cell
+    // (rows and columns sdo..N-sdo): the forward stencil launches then apply
+    // the source and record terms themselves.  Otherwise they are applied in
+    // memory, where on a never-updated cell they accumulate from step to
+    // step, as the reference's in-place additions do.
+    bool sparse_inside = true;
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -220,6 +226,16 @@ size_t rec_extent(const pyfwi_shot *shots, int nshots)
 int upload_shots(pyfwi_plan *p, const pyfwi_shot *shots, int nshots)
 {
     p->h_shots.assign(shots, shots + nshots);
+    const int sdo = p->grid.sdo, nz = p->grid.tnz, nx = p->grid.tnx;
+    auto inside = [&](int z, int x) { return z >= sdo && z <= nz - sdo && x >= sdo && x <= nx - sdo; };
+    p->sparse_inside = true;
+    for (int k = 0; k < nshots; ++k) {
+        const pyfwi_shot &sh = shots[k];
+        const int last = sh.rec_x0 + (sh.n_rec > 0 ? sh.n_rec - 1 : 0) * sh.rec_dx;
+        if (!inside(sh.src_z, sh.src_x) ||
+            (sh.n_rec > 0 && (!inside(sh.rec_z, sh.rec_x0) || !inside(sh.rec_z, last))))
+            p->sparse_inside = false;
+    }
     if (nshots > p->shots_cap) {
         if (p->d_shots) (void)hipFree(p->d_shots);
         p->d_shots = nullptr;
@@ -331,7 +347,7 @@ int32_t pyfwi_plan_destroy(pyfwi_plan *p)
     for (DevBuf *b : {&p->lam, &p->mu, &p->bu, &p->dxp_buf, &p->dzp_buf, &p->scratch_in, &p->fields, &p->fields2,
                       &p->adj, &p->av0, &p->av1, &p->gacc, &p->snaps, &p->ckpts, &p->seg,
                       &p->src_stage, &p->seis_stage, &p->res_stage, &p->grad_stage,
-                      &p->shot_tab_f, &p->resid, &p->src_saved})
+                      &p->shot_tab_f, &p->resid})
         b->release();
     if (p->d_misfit) (void)hipFree(p->d_misfit);
     if (p->d_shots) (void)hipFree(p->d_shots);
@@ -404,15 +420,26 @@ namespace {
 
 // One forward step of the shots of a pass, the reference's loop body at
 // wave_propagation.py:704-708: gather + inject on the current state, then the
-// velocity and stress half steps, either as one fused launch into the other
-// buffer or as two in-place launches.
+// velocity and stress half steps.  The two-launch form applies the source
+// term to the values it loads and writes the next step's record sample from
+// the stress update; the fused form records and injects in memory first.
 int forward_step(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t, const float *src,
                  float *seis)
 {
     const DevGrid &g = p->dg;
     const int64_t fps = fields_per_shot(p);
-    launch_inject_record(g, ng, dsh, t, p->cur, src, seis, p->stream);
-    if (p->tune_fused) {
+    if (!p->tune_fused && !p->sparse_inside) {
+        launch_inject_record(g, ng, dsh, t, p->cur, src, seis, p->stream);
+        {
+            Timed tm(p, PYFWI_K_FWD_VEL);
+            launch_vel(g, +1, ng, dsh, t, p->cur, fps, p->cur, fps, p->bu.p, p->dxp, p->dzp,
+                       p->stream);
+        }
+        Timed tm(p, PYFWI_K_FWD_STR);
+        launch_str(g, +1, ng, dsh, t, p->cur, fps, p->cur, fps, p->lam.p, p->mu.p, p->dxp, p->dzp,
+                   p->stream);
+    } else if (p->tune_fused) {
+        launch_inject_record(g, ng, dsh, t, p->cur, src, seis, p->stream);
         float *nxt = p->cur == p->fields.p ? p->fields2.p : p->fields.p;
         Timed tm(p, PYFWI_K_FWD_STEP);
         launch_fused_step(g, +1, ng, dsh, t, p->cur, nxt, p->bu.p, p->lam.p, p->mu.p, p->dxp,
@@ -422,34 +449,31 @@ int forward_step(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t, const floa
         {
             Timed tm(p, PYFWI_K_FWD_VEL);
             launch_vel(g, +1, ng, dsh, t, p->cur, fps, p->cur, fps, p->bu.p, p->dxp, p->dzp,
-                       p->stream);
+                       p->stream, src);
         }
         Timed tm(p, PYFWI_K_FWD_STR);
         launch_str(g, +1, ng, dsh, t, p->cur, fps, p->cur, fps, p->lam.p, p->mu.p, p->dxp, p->dzp,
-                   p->stream);
+                   p->stream, src, seis);
     }
     return 0;
 }
 
 // Forward step of a kept-state run: the state of step t-1 is read from `in`
-// (per-shot stride in_stride) and step t written to `out`.  The source is
-// injected into `in` and taken out again by the next step's injection launch
-// (or finish_kept), so every slot ends up holding the clean state.
+// (per-shot stride in_stride) and step t written to `out`; the source term
+// never enters memory, so every slot holds the reference's snapshot.
 int forward_step_kept(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t, const float *src,
-                      float *seis, float *in, int64_t in_stride, float *prev, int64_t prev_stride,
-                      float *out, int64_t out_stride)
+                      float *seis, const float *in, int64_t in_stride, float *out,
+                      int64_t out_stride)
 {
     const DevGrid &g = p->dg;
-    launch_inject_record_kept(g, ng, dsh, t, in, in_stride, prev, prev_stride, p->src_saved.p,
-                              src, seis, p->stream);
     {
         Timed tm(p, PYFWI_K_FWD_VEL);
         launch_vel(g, +1, ng, dsh, t, in, in_stride, out, out_stride, p->bu.p, p->dxp, p->dzp,
-                   p->stream);
+                   p->stream, src);
     }
     Timed tm(p, PYFWI_K_FWD_STR);
     launch_str(g, +1, ng, dsh, t, in, in_stride, out, out_stride, p->lam.p, p->mu.p, p->dxp,
-               p->dzp, p->stream);
+               p->dzp, p->stream, src, seis);
     return 0;
 }
 
@@ -538,27 +562,20 @@ int forward_pass(pyfwi_plan *p, int g0, int ng, const float *d_src, float *d_sei
     if (int e = prepare_fields(p, ng)) return e;
     // every step kept (chpr=100): the two-launch kernels step from one slot
     // into the next, so the kept states cost no extra traffic
-    if (snaps && keeps_every_step(cl, nt) && !p->tune_fused) {
-        if (int e = ensure(p->src_saved, (size_t)ng * 4, p->stream, "source cells")) return e;
+    if (snaps && keeps_every_step(cl, nt) && !p->tune_fused && p->sparse_inside) {
         const int64_t ss = (int64_t)nchp * fps;
-        float *in = p->fields.p, *prev = nullptr;
-        int64_t in_stride = fps, prev_stride = 0;
+        const float *in = p->fields.p;
+        int64_t in_stride = fps;
         for (int t = 0; t < nt; ++t) {
             if (ckpts && t % K == 0)
                 launch_copy_shots(ckpts + (int64_t)(t / K) * fps, (int64_t)ncp * fps, in,
                                   in_stride, fps, ng, p->stream);
             float *out = snaps + (int64_t)t * fps;
-            forward_step_kept(p, dsh, ng, t, d_src, d_seis, in, in_stride, prev, prev_stride, out,
-                              ss);
-            prev = in;
-            prev_stride = in_stride;
+            forward_step_kept(p, dsh, ng, t, d_src, d_seis, in, in_stride, out, ss);
             in = out;
             in_stride = ss;
             if (int e = maybe_harvest(p)) return e;
         }
-        // take the last injection out of its slot
-        launch_inject_record_kept(p->dg, ng, dsh, nt, nullptr, 0, prev, prev_stride,
-                                  p->src_saved.p, d_src, d_seis, p->stream);
         return 0;
     }
     int chpc = 0;
