@@ -42,8 +42,8 @@ KERNEL_BYTES = {
     "fwd_str": (40, 0),       # R vx vz txx tzz txz lam mu, W txx tzz txz
     "rev_str": (40, 0),
     "rev_vel": (32, 0),
-    "adj_str": (36, 0),       # R avx avz b atxx atzz atxz, W atxx atzz atxz
-    "adj_vel_grad": (36, 48),  # all: R 3 atau lam mu 2 av, W 2 av; interior: R 5 fwd b 3 acc, W 3 acc
+    "adj_str": (36, 24),      # all: R avx avz b 3 atau, W 3 atau; interior: R fwd vx vz, RW g_mu g_lam
+    "adj_vel_grad": (36, 24),  # all: R 3 atau lam mu 2 av, W 2 av; interior: R fwd 3 tau, b, RW g_rho
     "exact_str": (48, 0),
     "exact_vel": (52, 0),
     "fwd_step": (52, 0),       # R 5 fields lam mu b, W 5 fields

@@ -69,8 +69,12 @@ void launch_inject_record(const DevGrid &g, int nshots, const pyfwi_shot *shots,
 // Residual injection in memory, before the adjoint stress update.
 void launch_adj_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
                        float *adj, const float *res, hipStream_t st);
+// Adjoint stresses + the lambda / mu imaging terms; adjoint velocities + the
+// rho imaging term.  `fields`: the forward state of the step (per-shot
+// stride fields_stride), gacc: [nshots][6][fstride].
 void launch_adj_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
-                    float *adj, const float *bu, const float *dxp, const float *dzp,
+                    const float *fields, int64_t fields_stride, float *adj, const float *bu,
+                    const float *dxp, const float *dzp, float *gacc, bool precond,
                     hipStream_t st);
 void launch_adj_vel_grad(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
                          const float *fields, int64_t fields_stride, float *adj,

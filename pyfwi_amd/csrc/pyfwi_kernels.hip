@@ -844,12 +844,24 @@ __global__ void k_adj_inject(DevGrid g, const pyfwi_shot *shots, int t,
     }
 }
 
-// Adjoint stresses from adjoint velocities (src/PyFWI/elastic.cl:372-411).
-template <int SDO>
+// The imaging condition of a step (elastic.cl:470-526) reads the adjoint
+// fields only at the cell itself, so its terms go where those fields are
+// made: the lambda and mu parts, which need the new adjoint stresses and the
+// forward velocities' derivatives, in the adjoint stress update; the rho
+// part, which needs the new adjoint velocities and the forward stresses'
+// derivatives, in the adjoint velocity update.  Each kernel then reads about
+// the same number of arrays.  The shear term uses the z-derivative of vz as
+// the reference does (elastic.cl:485-486).  gacc: [nshots][6][fstride] =
+// gmu, gmu_pre, glam, glam_pre, grho, grho_pre; interior cells only.
+
+// Adjoint stresses from adjoint velocities (src/PyFWI/elastic.cl:372-411) and
+// the lambda / mu imaging terms.
+template <int SDO, bool PRECOND>
 __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot *shots, int t,
+                                                     const float *fields, int64_t fields_stride,
                                                      float *adj, const float *__restrict__ bu,
                                                      const float *__restrict__ dxp,
-                                                     const float *__restrict__ dzp)
+                                                     const float *__restrict__ dzp, float *gacc)
 {
     int s, j, i0, i1;
     if (!tile<SDO>(g, s, j, i0, i1)) return;
@@ -858,12 +870,20 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot
     const uint32_t RB = (uint32_t)g.pitch * 4u;
     float *A = adj + (int64_t)s * 5 * fs + g.origin;
     const CellRef avx{A}, avz{A + fs}, B{bu + g.origin};
-    ZWin<SDO> wbz, wbx;      // b*avz for d/dz-, b*avx for d/dz+
+    const int lo = g.npml + SDO, hi = g.nz - SDO - g.npml, hix = g.nx - SDO - g.npml;
+    const bool img = j >= lo && j <= hix && i1 >= lo && i0 <= hi;
+    const float *F = fields + (int64_t)s * fields_stride + g.origin;
+    const CellRef vx{F}, vz{F + fs}, txx{F + 2 * fs}, tzz{F + 3 * fs}, txz{F + 4 * fs};
+    float *G = gacc + (int64_t)s * 6 * fs + g.origin;
+    ZWin<SDO> wbz, wbx;  // b*avz for d/dz-, b*avx for d/dz+
     Fifo<SDO> fxp, fxm;  // d/dx+ (b*avx), d/dx- (b*avz)
+    ZWin<SDO> fvz;       // forward vz (d/dz+ and d/dz-)
+    Fifo<SDO> fdvz;      // d/dx- of forward vz
     auto dx_bvx = [&](uint32_t o) {
         return xplus<SDO>([&](int k) { return avx.at(o, k) * B.at(o, k); }, g.cx); };
     auto dx_bvz = [&](uint32_t o) {
         return xminus<SDO>([&](int k) { return avz.at(o, k) * B.at(o, k); }, g.cx); };
+    auto x_vz = [&](uint32_t o) { return xminus<SDO>([&](int k) { return vz.at(o, k); }, g.cx); };
     uint32_t cb = ((uint32_t)i0 * (uint32_t)g.pitch + (uint32_t)j) * 4u;
 #pragma unroll
     for (int k = 0; k < 2 * SDO; ++k) {
@@ -874,8 +894,13 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot
             fxp.v[k - SDO] = dx_bvx(o);
             fxm.v[k - SDO] = dx_bvz(o);
         }
+        if (img) {
+            fvz.v[k + 1] = vz(o);
+            if (k >= SDO) fdvz.v[k - SDO] = x_vz(o);
+        }
     }
     const float ex = dxp[j];
+    const float dt = g.dt;
 #pragma unroll MARCH_UNROLL
     for (int i = i0; i <= i1; ++i, cb += RB) {
         const uint32_t n = cb + (uint32_t)SDO * RB;
@@ -889,17 +914,34 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot
         float &ra = cell_ref(A + 2 * fs, cb), &rb = cell_ref(A + 3 * fs, cb),
               &rd = cell_ref(A + 4 * fs, cb);
         const float a = ra, bb = rb, d = rd;
-        ra = fmaf(-g.dt, ex * a + dxpb, a);
-        rb = fmaf(-g.dt, ez * bb + dzmb, bb);
-        rd = fmaf(-g.dt, (ex + ez) * d + (dzpb + dxmb), d);
+        const float aa = fmaf(-dt, ex * a + dxpb, a);
+        const float ab = fmaf(-dt, ez * bb + dzmb, bb);
+        const float ad = fmaf(-dt, (ex + ez) * d + (dzpb + dxmb), d);
+        ra = aa;
+        rb = ab;
+        rd = ad;
+        if (!img) continue;
+        fvz.push(vz(n));
+        const float dxm_vz = fdvz.pop(x_vz(n));
+        if (i < lo || i > hi) continue;
+        const float dzp_vz = fvz.dplus(g.cz);
+        const float dxp_vx = xplus<SDO>([&](int k) { return vx.at(cb, k); }, g.cx);
+        const float dzm_vz = fvz.dminus(g.cz);
+        cell_ref(G, cb) += 2.0f * dt * (aa * dxp_vx + ab * dzm_vz) + dt * ad * (dzp_vz + dxm_vz);
+        cell_ref(G + 2 * fs, cb) += dt * (aa + ab) * (dxp_vx + dzm_vz);
+        if constexpr (PRECOND) {
+            const float sxx = txx(cb), szz = tzz(cb), sxz = txz(cb);
+            const float dt2 = dt * dt;
+            cell_ref(G + fs, cb) +=
+                4.0f * dt2 * (sxx * sxx * dxp_vx * dxp_vx + szz * szz * dzm_vz * dzm_vz)
+                + dt2 * sxz * sxz * (dzp_vz * dzp_vz + dxm_vz * dxm_vz);
+            cell_ref(G + 3 * fs, cb) +=
+                dt2 * (sxx * sxx + szz * szz) * (dxp_vx * dxp_vx + dzm_vz * dzm_vz);
+        }
     }
 }
 
-// Adjoint velocities (src/PyFWI/elastic.cl:419-468) fused with the imaging
-// condition of the same step (elastic.cl:470-526), which reads the adjoint
-// fields only at the cell itself.  The shear term uses the z-derivative of vz
-// as the reference does (elastic.cl:485-486).  gacc: [nshots][6][fstride]
-// = gmu, gmu_pre, glam, glam_pre, grho, grho_pre.
+// Adjoint velocities (src/PyFWI/elastic.cl:419-468) and the rho imaging term.
 template <int SDO, bool PRECOND>
 __global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
     DevGrid g, const pyfwi_shot *shots, int t, const float *fields, int64_t fields_stride,
@@ -915,20 +957,16 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
     float *A = adj + (int64_t)s * 5 * fs + g.origin;
     const CellRef atxx{A + 2 * fs}, atzz{A + 3 * fs}, atxz{A + 4 * fs};
     const CellRef L{lam + g.origin}, M{mu + g.origin}, B{bu + g.origin};
-    // imaging condition: interior rows lo..hi and columns lo..hix only
     const int lo = g.npml + SDO, hi = g.nz - SDO - g.npml, hix = g.nx - SDO - g.npml;
-    const bool gcol = j >= lo && j <= hix;
-    const bool grow_strip = i1 >= lo && i0 <= hi;
-    const bool img = gcol && grow_strip;
+    const bool img = j >= lo && j <= hix && i1 >= lo && i0 <= hi;
     const float *F = fields + (int64_t)s * fields_stride + g.origin;
     const CellRef vx{F}, vz{F + fs}, txx{F + 2 * fs}, tzz{F + 3 * fs}, txz{F + 4 * fs};
     float *G = gacc + (int64_t)s * 6 * fs + g.origin;
 
     ZWin<SDO> wa3, wb1, wb2;  // mu*atxz (d/dz-), (lam+2mu)*atzz, lam*atxx (d/dz+)
-    ZWin<SDO> fvz, ftzz, ftxz;  // forward vz, tzz, txz for the imaging condition
-    // delayed to the centre row: x-derivatives of the windowed fields and the
-    // centre adjoint stresses
-    Fifo<SDO> fa1, fa2, fb3, faa, fab, fad, fdvz, fdtxz;
+    ZWin<SDO> ftzz, ftxz;     // forward tzz (d/dz+), txz (d/dz-)
+    // x-derivatives of the windowed fields, delayed to the centre row
+    Fifo<SDO> fa1, fa2, fb3, fdtxz;
     auto q_a3 = [&](uint32_t o) { return M(o) * atxz(o); };
     auto q_b1 = [&](uint32_t o) { return (L(o) + 2.0f * M(o)) * atzz(o); };
     auto q_b2 = [&](uint32_t o) { return L(o) * atxx(o); };
@@ -939,7 +977,6 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
         return xminus<SDO>([&](int k) { return L.at(o, k) * atzz.at(o, k); }, g.cx); };
     auto x_b3 = [&](uint32_t o) {
         return xplus<SDO>([&](int k) { return M.at(o, k) * atxz.at(o, k); }, g.cx); };
-    auto x_vz = [&](uint32_t o) { return xminus<SDO>([&](int k) { return vz.at(o, k); }, g.cx); };
     auto x_txz = [&](uint32_t o) { return xplus<SDO>([&](int k) { return txz.at(o, k); }, g.cx); };
     uint32_t cb = ((uint32_t)i0 * (uint32_t)g.pitch + (uint32_t)j) * 4u;
 #pragma unroll
@@ -953,18 +990,11 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
             fa1.v[q] = x_a1(o);
             fa2.v[q] = x_a2(o);
             fb3.v[q] = x_b3(o);
-            faa.v[q] = atxx(o);
-            fab.v[q] = atzz(o);
-            fad.v[q] = atxz(o);
         }
         if (img) {
-            fvz.v[k + 1] = vz(o);
             ftzz.v[k + 1] = tzz(o);
             ftxz.v[k + 1] = txz(o);
-            if (k >= SDO) {
-                fdvz.v[k - SDO] = x_vz(o);
-                fdtxz.v[k - SDO] = x_txz(o);
-            }
+            if (k >= SDO) fdtxz.v[k - SDO] = x_txz(o);
         }
     }
     const float ex = dxp[j];
@@ -977,7 +1007,6 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
         wb2.push(q_b2(n));
         const float a1 = fa1.pop(x_a1(n)), a2 = fa2.pop(x_a2(n));
         const float b3 = fb3.pop(x_b3(n));
-        const float aa = faa.pop(atxx(n)), ab = fab.pop(atzz(n)), ad = fad.pop(atxz(n));
         const float a3 = wa3.dminus(g.cz);
         const float b1 = wb1.dplus(g.cz);
         const float b2 = wb2.dplus(g.cz);
@@ -989,32 +1018,19 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
         rx = ux;
         rz = uz;
         if (!img) continue;
-        fvz.push(vz(n));
         ftzz.push(tzz(n));
         ftxz.push(txz(n));
-        const float dxm_vz = fdvz.pop(x_vz(n));
         const float dxp_txz = fdtxz.pop(x_txz(n));
         if (i < lo || i > hi) continue;
-        const float dzp_vz = fvz.dplus(g.cz);
-        const float dxp_vx = xplus<SDO>([&](int k) { return vx.at(cb, k); }, g.cx);
-        const float dzm_vz = fvz.dminus(g.cz);
         const float dzp_tzz = ftzz.dplus(g.cz);
         const float dxm_txx = xminus<SDO>([&](int k) { return txx.at(cb, k); }, g.cx);
         const float dzm_txz = ftxz.dminus(g.cz);
-        cell_ref(G, cb) += 2.0f * dt * (aa * dxp_vx + ab * dzm_vz) + dt * ad * (dzp_vz + dxm_vz);
-        cell_ref(G + 2 * fs, cb) += dt * (aa + ab) * (dxp_vx + dzm_vz);
         const float bb = B(cb);
         cell_ref(G + 4 * fs, cb) +=
             -(dt * (bb * bb)) * (ux * (dxm_txx + dzm_txz) + uz * (dzp_tzz + dxp_txz));
         if constexpr (PRECOND) {
-            const float sxx = txx(cb), szz = ftzz.centre(), sxz = ftxz.centre();
-            const float wx = vx(cb), wz = fvz.centre();
+            const float wx = vx(cb), wz = vz(cb);
             const float dt2 = dt * dt;
-            cell_ref(G + fs, cb) +=
-                4.0f * dt2 * (sxx * sxx * dxp_vx * dxp_vx + szz * szz * dzm_vz * dzm_vz)
-                + dt2 * sxz * sxz * (dzp_vz * dzp_vz + dxm_vz * dxm_vz);
-            cell_ref(G + 3 * fs, cb) +=
-                dt2 * (sxx * sxx + szz * szz) * (dxp_vx * dxp_vx + dzm_vz * dzm_vz);
             cell_ref(G + 5 * fs, cb) += dt2 * (wx * wx * (dxm_txx * dxm_txx + dzm_txz * dzm_txz)
                                               + wz * wz * (dzp_tzz * dzp_tzz + dxp_txz * dxp_txz));
         }
@@ -1411,13 +1427,15 @@ void launch_adj_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, in
 }
 
 void launch_adj_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
-                    float *adj, const float *bu, const float *dxp, const float *dzp,
+                    const float *fields, int64_t fields_stride, float *adj, const float *bu,
+                    const float *dxp, const float *dzp, float *gacc, bool precond,
                     hipStream_t st)
 {
     DevGrid gm = g;
     dim3 grid = march_grid(gm, g.sdo, nshots), block(MARCH_X);
-#define ARGS gm, shots, t, adj, bu, dxp, dzp
-    DISPATCH_SDO(g.sdo, k_adj_str, );
+#define ARGS gm, shots, t, fields, fields_stride, adj, bu, dxp, dzp, gacc
+    if (precond) DISPATCH_SDO(g.sdo, k_adj_str, , true);
+    else DISPATCH_SDO(g.sdo, k_adj_str, , false);
 #undef ARGS
 }
 

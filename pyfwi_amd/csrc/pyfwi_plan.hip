@@ -691,7 +691,8 @@ int gradient_pyfwi(pyfwi_plan *p, int g0, int ng, const float *d_res, bool preco
         launch_adj_inject(g, ng, dsh, t, p->adj.p, d_res, p->stream);
         {
             Timed tm(p, PYFWI_K_ADJ_STR);
-            launch_adj_str(g, ng, dsh, t, p->adj.p, p->bu.p, p->dxp, p->dzp, p->stream);
+            launch_adj_str(g, ng, dsh, t, fwd, fwd_stride, p->adj.p, p->bu.p, p->dxp, p->dzp,
+                           p->gacc.p, precond, p->stream);
         }
         {
             Timed tm(p, PYFWI_K_ADJ_VEL);
