@@ -66,21 +66,28 @@ void launch_str(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, 
 // injected stresses).
 void launch_inject_record(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
                           float *fields, const float *src, float *seis, hipStream_t st);
-// Residual injection in memory, before the adjoint stress update.
+// Residual injection in memory at backward step t, components f0..f1-1 (all
+// five by default); with first_only, only for shots whose first backward
+// step t is.
 void launch_adj_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
-                       float *adj, const float *res, hipStream_t st);
+                       float *adj, const float *res, hipStream_t st, int f0 = 0, int f1 = 5,
+                       bool first_only = false);
 // Adjoint stresses + the lambda / mu imaging terms; adjoint velocities + the
 // rho imaging term.  `fields`: the forward state of the step (per-shot
-// stride fields_stride), gacc: [nshots][6][fstride].
+// stride fields_stride), gacc: [nshots][6][fstride].  With `res`, the stress
+// update adds step t's residual to the adjoint stresses at the receivers and
+// the velocity update writes its adjoint velocities with step t-1's residual
+// added (launch_adj_inject(..., 0, 2, true) covers each shot's first step);
+// res == nullptr when the caller injects every step in memory.
 void launch_adj_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
                     const float *fields, int64_t fields_stride, float *adj, const float *bu,
                     const float *dxp, const float *dzp, float *gacc, bool precond,
-                    hipStream_t st);
+                    hipStream_t st, const float *res = nullptr);
 void launch_adj_vel_grad(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
                          const float *fields, int64_t fields_stride, float *adj,
                          const float *lam, const float *mu, const float *bu,
                          const float *dxp, const float *dzp, float *gacc, bool precond,
-                         hipStream_t st);
+                         hipStream_t st, const float *res = nullptr);
 void launch_l2_residual(int nshots, const pyfwi_shot *shots, const float *seis, const float *obs,
                         float *res, int comp_mask, double *misfit, hipStream_t st);
 void launch_exact_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,

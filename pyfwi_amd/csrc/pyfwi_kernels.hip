@@ -288,9 +288,11 @@ __device__ __forceinline__ void prime(ZWin<SDO> &w, int64_t c0, int pitch, F val
 // written by the stress update of step t at the receiver cells.  The sums are
 // the float additions the reference makes in memory (elastic_surface.cl:
 // 7-41), so the results are the same; the state in memory itself stays free
-// of source terms.  (The adjoint's residual injection stays a launch of its
-// own: carrying it in the adjoint kernels cost them more time than the
-// launch takes.)
+// of source terms.  The adjoint's residual enters memory as in the reference,
+// but written by the adjoint kernels themselves: the adjoint stresses by the
+// stress update of the step, the adjoint velocities by the velocity update of
+// the step before (in time reversal), after it used them for its own imaging
+// term; only a shot's first adjoint step needs a launch of its own.
 
 // Source of step t for one shot (injSrc, elastic_surface.cl:34-40); z = -1
 // when the step injects nothing.
@@ -314,7 +316,7 @@ __device__ __forceinline__ SrcTerm src_term(const pyfwi_shot &sh, int t, const f
 
 // A receiver line of one shot at one time sample: `base` points at component
 // 0, receiver 0 of the sample's row; component f is `fs` floats further.  z
-// = -1 when nothing is recorded at this step.
+// = -1 when nothing is recorded or injected at this step.
 struct RecLine {
     int z = -1, x0 = 0, dx = 1, n = 0;
     int64_t fs = 0;
@@ -348,6 +350,27 @@ __device__ __forceinline__ RecLine record_line(const pyfwi_shot &sh, int t1, flo
     q.n = sh.n_rec;
     q.fs = (int64_t)sh.n_samples * sh.rec_stride;
     q.base = seis + sh.rec_offset + (int64_t)k * sh.rec_stride;
+    return q;
+}
+
+// Residual row injected at backward step t (Adj_injSrc, elastic_surface.cl:
+// 48-72; wave_propagation.py:815-824): the reference counts injections down
+// from nt_ext, so step t uses row nt_ext-1-(floor((nt-1)/dt_scale) -
+// t/dt_scale), a negative row wrapping like a Python index.  z = -1 when the
+// step injects nothing.
+__device__ __forceinline__ RecLine residual_line(const pyfwi_shot &sh, int t, const float *res)
+{
+    RecLine q;
+    if (!res || t < 1 || t >= sh.nt || (t % sh.dt_scale) != 0) return q;
+    int row = sh.n_samples - 1 - ((sh.nt - 1) / sh.dt_scale - t / sh.dt_scale);
+    if (row < 0) row += sh.n_samples;
+    if (row < 0) return q;
+    q.z = sh.rec_z;
+    q.x0 = sh.rec_x0;
+    q.dx = sh.rec_dx;
+    q.n = sh.n_rec;
+    q.fs = (int64_t)sh.n_samples * sh.rec_stride;
+    q.base = res + sh.rec_offset + (int64_t)row * sh.rec_stride;
     return q;
 }
 
@@ -819,28 +842,26 @@ __global__ void k_inject_record(DevGrid g, const pyfwi_shot *shots, int t,
 
 // ---------------------------------------------------- adjoint, pyfwi mode
 
-// Residual injection at backward step t (src/PyFWI/elastic_surface.cl:48-72,
-// wave_propagation.py:815-824): the reference counts injections down from
-// nt_ext, so step t uses row nt_ext-1-(floor((nt-1)/dt_scale) - t/dt_scale);
-// a negative row wraps like a Python index.
+// Residual injection in memory at backward step t (Adj_injSrc,
+// elastic_surface.cl:48-72), components f0..f1-1; with `first_only`, only for
+// the shots whose first backward step t is (t == nt - 1), the adjoint
+// velocities of later steps being written with their residual by the
+// velocity update of the step before.
 __global__ void k_adj_inject(DevGrid g, const pyfwi_shot *shots, int t,
-                             float *adj, const float *__restrict__ res)
+                             float *adj, const float *__restrict__ res, int f0, int f1,
+                             int first_only)
 {
     const int s = blockIdx.x;
     const pyfwi_shot sh = shots[s];
-    if (t < 1 || t >= sh.nt || (t % sh.dt_scale) != 0) return;
-    int row = sh.n_samples - 1 - ((sh.nt - 1) / sh.dt_scale - t / sh.dt_scale);
-    if (row < 0) row += sh.n_samples;
-    if (row < 0) return;
+    if (first_only && t != sh.nt - 1) return;
+    const RecLine rq = residual_line(sh, t, res);
+    if (rq.z < 0) return;
     const int64_t fs = g.fstride;
     float *A = adj + (int64_t)s * 5 * fs + g.origin;
-    const float *rr = res + sh.rec_offset;
     const int64_t c0 = (int64_t)sh.rec_z * g.pitch + sh.rec_x0;
     for (int r = threadIdx.x; r < sh.n_rec; r += blockDim.x) {
         const int64_t c = c0 + (int64_t)r * sh.rec_dx;
-#pragma unroll
-        for (int f = 0; f < 5; ++f)
-            A[f * fs + c] += rr[((int64_t)f * sh.n_samples + row) * sh.rec_stride + r];
+        for (int f = f0; f < f1; ++f) A[f * fs + c] += rq.base[f * rq.fs + r];
     }
 }
 
@@ -861,7 +882,8 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot
                                                      const float *fields, int64_t fields_stride,
                                                      float *adj, const float *__restrict__ bu,
                                                      const float *__restrict__ dxp,
-                                                     const float *__restrict__ dzp, float *gacc)
+                                                     const float *__restrict__ dzp, float *gacc,
+                                                     const float *__restrict__ res)
 {
     int s, j, i0, i1;
     if (!tile<SDO>(g, s, j, i0, i1)) return;
@@ -870,6 +892,14 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot
     const uint32_t RB = (uint32_t)g.pitch * 4u;
     float *A = adj + (int64_t)s * 5 * fs + g.origin;
     const CellRef avx{A}, avz{A + fs}, B{bu + g.origin};
+    // the step's residual on the adjoint stresses of this thread's receiver
+    // (zero elsewhere), added before the update
+    const RecLine rq = residual_line(shots[s], t, res);
+    const int rr = rq.z >= i0 && rq.z <= i1 ? rq.index(j) : -1;
+    const int rrow = rr >= 0 ? rq.z : -1;
+    const float r_txx = rr >= 0 ? rq.base[2 * rq.fs + rr] : 0.0f;
+    const float r_tzz = rr >= 0 ? rq.base[3 * rq.fs + rr] : 0.0f;
+    const float r_txz = rr >= 0 ? rq.base[4 * rq.fs + rr] : 0.0f;
     const int lo = g.npml + SDO, hi = g.nz - SDO - g.npml, hix = g.nx - SDO - g.npml;
     const bool img = j >= lo && j <= hix && i1 >= lo && i0 <= hi;
     const float *F = fields + (int64_t)s * fields_stride + g.origin;
@@ -913,7 +943,10 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot
         const float ez = dzp[i];
         float &ra = cell_ref(A + 2 * fs, cb), &rb = cell_ref(A + 3 * fs, cb),
               &rd = cell_ref(A + 4 * fs, cb);
-        const float a = ra, bb = rb, d = rd;
+        const bool at_rec = i == rrow;
+        const float a = at_rec ? ra + r_txx : ra;
+        const float bb = at_rec ? rb + r_tzz : rb;
+        const float d = at_rec ? rd + r_txz : rd;
         const float aa = fmaf(-dt, ex * a + dxpb, a);
         const float ab = fmaf(-dt, ez * bb + dzmb, bb);
         const float ad = fmaf(-dt, (ex + ez) * d + (dzpb + dxmb), d);
@@ -947,7 +980,7 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
     DevGrid g, const pyfwi_shot *shots, int t, const float *fields, int64_t fields_stride,
     float *adj, const float *__restrict__ lam, const float *__restrict__ mu,
     const float *__restrict__ bu, const float *__restrict__ dxp,
-    const float *__restrict__ dzp, float *gacc)
+    const float *__restrict__ dzp, float *gacc, const float *__restrict__ res)
 {
     int s, j, i0, i1;
     if (!tile<SDO>(g, s, j, i0, i1)) return;
@@ -956,6 +989,14 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
     const uint32_t RB = (uint32_t)g.pitch * 4u;
     float *A = adj + (int64_t)s * 5 * fs + g.origin;
     const CellRef atxx{A + 2 * fs}, atzz{A + 3 * fs}, atxz{A + 4 * fs};
+    // the residual of the next backward step (t-1) on the adjoint velocities
+    // of this thread's receiver, added to what is written (this step's
+    // imaging uses the velocities without it)
+    const RecLine rq = residual_line(shots[s], t - 1, res);
+    const int rr = rq.z >= i0 && rq.z <= i1 ? rq.index(j) : -1;
+    const int rrow = rr >= 0 ? rq.z : -1;
+    const float r_vx = rr >= 0 ? rq.base[rr] : 0.0f;
+    const float r_vz = rr >= 0 ? rq.base[rq.fs + rr] : 0.0f;
     const CellRef L{lam + g.origin}, M{mu + g.origin}, B{bu + g.origin};
     const int lo = g.npml + SDO, hi = g.nz - SDO - g.npml, hix = g.nx - SDO - g.npml;
     const bool img = j >= lo && j <= hix && i1 >= lo && i0 <= hi;
@@ -1015,8 +1056,9 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
         float ux = rx, uz = rz;
         ux = fmaf(-dt, (2.0f * ex + ez) * ux + (a1 + a2 + a3), ux);
         uz = fmaf(-dt, (ex + 2.0f * ez) * uz + (b2 + b1 + b3), uz);
-        rx = ux;
-        rz = uz;
+        const bool at_rec = i == rrow;
+        rx = at_rec ? ux + r_vx : ux;
+        rz = at_rec ? uz + r_vz : uz;
         if (!img) continue;
         ftzz.push(tzz(n));
         ftxz.push(txz(n));
@@ -1421,19 +1463,21 @@ void launch_inject_record(const DevGrid &g, int nshots, const pyfwi_shot *shots,
 }
 
 void launch_adj_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
-                       float *adj, const float *res, hipStream_t st)
+                       float *adj, const float *res, hipStream_t st, int f0, int f1,
+                       bool first_only)
 {
-    hipLaunchKernelGGL(k_adj_inject, dim3(nshots), dim3(256), 0, st, g, shots, t, adj, res);
+    hipLaunchKernelGGL(k_adj_inject, dim3(nshots), dim3(256), 0, st, g, shots, t, adj, res, f0,
+                       f1, first_only ? 1 : 0);
 }
 
 void launch_adj_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
                     const float *fields, int64_t fields_stride, float *adj, const float *bu,
                     const float *dxp, const float *dzp, float *gacc, bool precond,
-                    hipStream_t st)
+                    hipStream_t st, const float *res)
 {
     DevGrid gm = g;
     dim3 grid = march_grid(gm, g.sdo, nshots), block(MARCH_X);
-#define ARGS gm, shots, t, fields, fields_stride, adj, bu, dxp, dzp, gacc
+#define ARGS gm, shots, t, fields, fields_stride, adj, bu, dxp, dzp, gacc, res
     if (precond) DISPATCH_SDO(g.sdo, k_adj_str, , true);
     else DISPATCH_SDO(g.sdo, k_adj_str, , false);
 #undef ARGS
@@ -1443,11 +1487,11 @@ void launch_adj_vel_grad(const DevGrid &g, int nshots, const pyfwi_shot *shots, 
                          const float *fields, int64_t fields_stride, float *adj,
                          const float *lam, const float *mu, const float *bu,
                          const float *dxp, const float *dzp, float *gacc, bool precond,
-                         hipStream_t st)
+                         hipStream_t st, const float *res)
 {
     DevGrid gm = g;
     dim3 grid = march_grid(gm, g.sdo, nshots), block(MARCH_X);
-#define ARGS gm, shots, t, fields, fields_stride, adj, lam, mu, bu, dxp, dzp, gacc
+#define ARGS gm, shots, t, fields, fields_stride, adj, lam, mu, bu, dxp, dzp, gacc, res
     if (precond) DISPATCH_SDO(g.sdo, k_adj_vel_grad, , true);
     else DISPATCH_SDO(g.sdo, k_adj_vel_grad, , false);
 #undef ARGS

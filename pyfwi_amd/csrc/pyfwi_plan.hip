@@ -670,6 +670,12 @@ int gradient_pyfwi(pyfwi_plan *p, int g0, int ng, const float *d_res, bool preco
     if (int e = ensure(p->adj, (size_t)ng * fps, p->stream, "adjoint fields")) return e;
     HIPCHK(hipMemsetAsync(p->adj.p, 0, (size_t)ng * fps * sizeof(float), p->stream));
     if (int e = prepare_fields(p, ng)) return e;
+    // steps at which a shot of the pass starts its backward sweep
+    std::vector<char> first_step(nt + 1, 0);
+    for (int k = 0; k < ng; ++k) {
+        const int ft = p->h_shots[g0 + k].nt - 1;
+        if (ft >= 1 && ft <= nt) first_step[ft] = 1;
+    }
     // every step kept: read the forward state of step t from its slot
     const bool direct = keeps_every_step(cl, nt);
     int chpc = nchp - 1;
@@ -688,16 +694,24 @@ int gradient_pyfwi(pyfwi_plan *p, int g0, int ng, const float *d_res, bool preco
             reverse_step(p, dsh, ng, t);
             fwd = p->cur;
         }
-        launch_adj_inject(g, ng, dsh, t, p->adj.p, d_res, p->stream);
+        // the residual of step t: the adjoint kernels add it (the velocity
+        // update of step t+1 already wrote the adjoint velocities with it,
+        // except for shots starting here); with receivers off the active
+        // cells, in memory every step
+        const float *r_on = p->sparse_inside ? d_res : nullptr;
+        if (!p->sparse_inside)
+            launch_adj_inject(g, ng, dsh, t, p->adj.p, d_res, p->stream);
+        else if (first_step[t])
+            launch_adj_inject(g, ng, dsh, t, p->adj.p, d_res, p->stream, 0, 2, true);
         {
             Timed tm(p, PYFWI_K_ADJ_STR);
             launch_adj_str(g, ng, dsh, t, fwd, fwd_stride, p->adj.p, p->bu.p, p->dxp, p->dzp,
-                           p->gacc.p, precond, p->stream);
+                           p->gacc.p, precond, p->stream, r_on);
         }
         {
             Timed tm(p, PYFWI_K_ADJ_VEL);
             launch_adj_vel_grad(g, ng, dsh, t, fwd, fwd_stride, p->adj.p, p->lam.p, p->mu.p,
-                                p->bu.p, p->dxp, p->dzp, p->gacc.p, precond, p->stream);
+                                p->bu.p, p->dxp, p->dzp, p->gacc.p, precond, p->stream, r_on);
         }
         if (int e = maybe_harvest(p)) return e;
     }
