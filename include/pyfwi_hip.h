@@ -67,14 +67,34 @@ typedef struct pyfwi_grid {
     double dx, dz;      /* grid spacing as the kernels see it (dx, dz)           */
 } pyfwi_grid;
 
-/* One shot: a point source and a surface receiver line
- * (injSrc / Adj_injSrc of elastic_surface.cl:7-72). */
+/* Receiver layouts (pyfwi_shot.rec_layout). */
+enum {
+    /* One horizontal line: receiver r at row rec_z, column rec_x0 + r*rec_dx
+     * (the surface gather of elastic_surface.cl:7-72). */
+    PYFWI_REC_LINE = 0,
+    /* Any cells, from the plan's receiver table (pyfwi_set_receivers): the
+     * well gathers of elastic_crosswell.cl:1-58 and elastic.cl:43-98,
+     * 316-363. */
+    PYFWI_REC_TABLE = 1
+};
+
+/* Receiver flags (pyfwi_shot.rec_flags). */
+enum {
+    /* The vz record reads vx, as the crosswell gather does
+     * (elastic_crosswell.cl:27); the residual of vz still goes to the
+     * adjoint vz in PYFWI_MODE_PYFWI (elastic_crosswell.cl:51). */
+    PYFWI_REC_VZ_FROM_VX = 1
+};
+
+/* One shot: a point source and its receivers
+ * (injSrc / Adj_injSrc of elastic_surface.cl:7-72, elastic_crosswell.cl:1-58,
+ * elastic.cl:43-98, 316-363). */
 typedef struct pyfwi_shot {
     int32_t src_x, src_z;   /* source cell (column, row), absorbing layer incl.   */
-    int32_t rec_z;          /* receiver row (rec_surface_const)                   */
-    int32_t rec_x0;         /* first receiver column (rec_surface_var)            */
-    int32_t rec_dx;         /* receiver spacing in cells (dxr), >= 1              */
-    int32_t n_rec;          /* receivers recorded; all must lie inside the grid   */
+    int32_t rec_z;          /* PYFWI_REC_LINE: receiver row (rec_surface_const)   */
+    int32_t rec_x0;         /* PYFWI_REC_LINE: first receiver column (rec_surface_var) */
+    int32_t rec_dx;         /* PYFWI_REC_LINE: receiver spacing in cells (dxr), >= 1 */
+    int32_t n_rec;          /* record columns filled; all receivers inside the grid */
     int32_t rec_stride;     /* floats per record row (>= n_rec; the reference's Nr) */
     int32_t nt;             /* internal time steps                                 */
     int32_t dt_scale;       /* internal steps per record sample, >= 1              */
@@ -83,6 +103,16 @@ typedef struct pyfwi_shot {
                                [ceil(nt/dt_scale)][5] (vx, vz, txx, tzz, txz)     */
     int64_t rec_offset;     /* float offset of this shot's record / residual
                                [5][n_samples][rec_stride]                         */
+    int32_t rec_layout;     /* PYFWI_REC_LINE or PYFWI_REC_TABLE                  */
+    int32_t rec_flags;      /* PYFWI_REC_* flags                                   */
+    int64_t rec_table;      /* PYFWI_REC_TABLE: offset of this shot's entries in
+                               the plan's receiver table: [2][n_rec] cells
+                               (row*tnx + column) -- first the cell each record
+                               column samples, then the cell each residual column
+                               is injected at by PYFWI_MODE_PYFWI (the exact mode
+                               injects at the sampled cells); -1 for a column
+                               that no receiver fills (recorded as zero, never
+                               injected) */
 } pyfwi_shot;
 
 typedef struct pyfwi_plan pyfwi_plan;
@@ -113,6 +143,14 @@ int32_t pyfwi_plan_set_stream(pyfwi_plan *plan, void *stream);
 int32_t pyfwi_set_model(pyfwi_plan *plan, const float *lam, const float *mu,
                         const float *buoy, const float *dx_prof,
                         const float *dz_prof, int32_t mem);
+
+/* Receiver table of the plan, read by the shots whose rec_layout is
+ * PYFWI_REC_TABLE; `table` is host memory, `count` int32 entries, each a cell
+ * row*tnx + column inside the grid or -1.  Replaces the receiver macros
+ * rec_top_left_const .. rec_surface_var that kernel_caller pastes into the
+ * program (wave_propagation.py:475-490, set at :116-143).  The table stays
+ * until the next call; a call with count 0 clears it. */
+int32_t pyfwi_set_receivers(pyfwi_plan *plan, const int32_t *table, int64_t count);
 
 /* Forward modelling of `nshots` shots in one batch (forward_propagator +
  * __kernel + make_seismogram, wave_propagation.py:625-733, 334-366).

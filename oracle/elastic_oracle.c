@@ -58,7 +58,73 @@ typedef struct orc_acq {
     int nt;              /* internal time steps                            */
     int dt_scale;        /* internal steps per recorded sample             */
     int nt_ext;          /* recorded samples                               */
+    /* receiver macros of kernel_caller (src/PyFWI/wave_propagation.py:475-480) */
+    int acq_type;        /* 0 crosswell, 1 surface, 2 wells + surface      */
+    int n_well_rec;      /* receivers per well (n_extera_rec)              */
+    int rec_left_col;    /* rec_top_left_const                             */
+    int rec_left_row0;   /* rec_top_left_var                               */
+    int rec_right_col;   /* rec_top_right_const                            */
+    int rec_right_row0;  /* rec_top_right_var                              */
 } orc_acq;
+
+#define ORC_NO_CELL (-1L - 0x7fffffffL)
+
+/* Flat cell read by each record column, in the order the reference's gathers
+ * write the columns, or ORC_NO_CELL for a column no gather reaches.  Cells
+ * may fall outside [0, nz*nx): the reference then reads outside the field.
+ *   surface   elastic_surface.cl:30-38: j % dxr == 0 over j in [0, Nx),
+ *             ir = j/dxr < n_main_rec, cell rec_row*Nx + j + rec_col0
+ *   crosswell elastic_crosswell.cl:24-31: i % dxr == 0 over i in [0, Nz),
+ *             ir = i/dxr, cell (i + rec_right_row0)*Nx + rec_right_col
+ *   combined  elastic.cl:62-98: left well (ir < n_extera_rec) in columns
+ *             [0, n_well), surface in [n_well, n_well + n_main_rec), right
+ *             well after them.
+ * The crosswell and right-well gathers do not bound ir: the reference writes
+ * columns past Nr into the next record row, which the next step's gather
+ * overwrites, and reads their cells below the grid.  Only columns < nr are
+ * restated. */
+static void orc_rec_cells(const orc_grid *g, const orc_acq *q, long *cell)
+{
+    for (int r = 0; r < q->nr; ++r) cell[r] = ORC_NO_CELL;
+    const long nx = g->nx;
+    if (q->acq_type == 0 || q->acq_type == 2) {
+        const int first = q->acq_type == 0 ? 0 : q->n_well_rec + q->n_main_rec;
+        for (int i = 0; i < g->nz; i += q->dxr) {
+            const int col = first + i / q->dxr;
+            if (col < q->nr) cell[col] = (long)(i + q->rec_right_row0) * nx + q->rec_right_col;
+        }
+    }
+    if (q->acq_type == 2) {
+        for (int i = 0; i < g->nz; i += q->dxr) {
+            const int ir = i / q->dxr;
+            if (ir < q->n_well_rec && ir < q->nr)
+                cell[ir] = (long)(i + q->rec_left_row0) * nx + q->rec_left_col;
+        }
+    }
+    if (q->acq_type == 1 || q->acq_type == 2) {
+        const int first = q->acq_type == 1 ? 0 : q->n_well_rec;
+        for (int j = 0; j < g->nx; j += q->dxr) {
+            const int ir = j / q->dxr;
+            if (ir < q->n_main_rec && first + ir < q->nr)
+                cell[first + ir] = (long)q->rec_row * nx + j + q->rec_col0;
+        }
+    }
+}
+
+/* Field a record component reads: the crosswell gather fills the vz record
+ * from vx (elastic_crosswell.cl:27). */
+static inline int orc_rec_field(const orc_acq *q, int f)
+{
+    return (q->acq_type == 0 && f == 1) ? 0 : f;
+}
+
+/* Exported for tests: the record columns' cells (see orc_rec_cells). */
+int orc_receiver_cells(const orc_grid *g, const orc_acq *q, long *cell)
+{
+    if (q->dxr < 1 || q->acq_type < 0 || q->acq_type > 2) return 1;
+    orc_rec_cells(g, q, cell);
+    return 0;
+}
 
 enum { TAP_F = 0, TAP_FA = 1, TAP_FM = 2 };
 
@@ -74,4 +140,4 @@ enum { TAP_F = 0, TAP_FA = 1, TAP_FM = 2 };
 #undef R
 #undef SFX
 
-int orc_abi_version(void) { return 1; }
+int orc_abi_version(void) { return 2; }

@@ -265,47 +265,69 @@ void SFX(orc_grad_acc)(const orc_grid *g, const R *F, const R *A, const R *bu,
 
 /* ------------------------------------------- sparse source and receivers */
 
-/* Surface receiver set: receiver ir sits at flat index
- * rec_row*nx + rec_col0 + ir*dxr and exists when ir < n_main_rec and
- * ir*dxr < nx (the reference tests j%dxr==0 over j in [0, nx),
- * src/PyFWI/elastic_surface.cl:30-32). */
-static inline int SFX(n_live_rec)(const orc_grid *g, const orc_acq *q)
-{
-    int lim = (g->nx + q->dxr - 1) / q->dxr;
-    return q->n_main_rec < lim ? q->n_main_rec : lim;
-}
-
 /* record the 5 fields at the receivers into row `row`, then add the source
- * sample to vx, vz, txx, tzz at the source cell (elastic_surface.cl:7-41).
- * seis: [5][nt_ext][nr] for this shot. */
-static void SFX(inj_rec)(const orc_grid *g, const orc_acq *q, R *F, int sx,
-                         int sz, const R *sv, R *seis, int row)
+ * sample to vx, vz, txx, tzz at the source cell (elastic_surface.cl:7-41,
+ * elastic_crosswell.cl:1-33, elastic.cl:43-98).  cell: orc_rec_cells; a cell
+ * outside the grid records 0.  seis: [5][nt_ext][nr] for this shot. */
+static void SFX(inj_rec)(const orc_grid *g, const orc_acq *q, const long *cell, R *F,
+                         int sx, int sz, const R *sv, R *seis, int row)
 {
     const long n = (long)g->nz * g->nx;
-    const int nl = row < q->nt_ext ? SFX(n_live_rec)(g, q) : 0;
-    for (int ir = 0; ir < nl; ++ir) {
-        long k = (long)q->rec_row * g->nx + q->rec_col0 + (long)ir * q->dxr;
+    for (int ir = 0; row < q->nt_ext && ir < q->nr; ++ir) {
+        const long k = cell[ir];
+        if (k == ORC_NO_CELL) continue;
         for (int f = 0; f < 5; ++f)
             seis[((long)f * q->nt_ext + row) * q->nr + ir] =
-                (k >= 0 && k < n) ? F[f * n + k] : (R)0;
+                (k >= 0 && k < n) ? F[orc_rec_field(q, f) * n + k] : (R)0;
     }
     long c = (long)sz * g->nx + sx;
     for (int f = 0; f < 4; ++f) F[f * n + c] += sv[f];
 }
 
-/* add residual row to the adjoint fields at the receivers
- * (elastic_surface.cl:48-72); r5: 5 pointers to nr values */
-static void SFX(adj_inj)(const orc_grid *g, const orc_acq *q, R *A,
+/* add residual row `row` to the adjoint fields at the receivers, component f
+ * into adjoint field f (elastic_surface.cl:48-72, elastic_crosswell.cl:35-58,
+ * elastic.cl:316-363).  res: [5][ns][nt_ext][nr]. */
+static void SFX(adj_inj)(const orc_grid *g, const orc_acq *q, const long *cell, R *A,
                          const R *res, int s, int row)
 {
     const long n = (long)g->nz * g->nx;
-    const int nl = SFX(n_live_rec)(g, q);
-    for (int ir = 0; ir < nl; ++ir) {
-        long k = (long)q->rec_row * g->nx + q->rec_col0 + (long)ir * q->dxr;
+    for (int ir = 0; ir < q->nr; ++ir) {
+        const long k = cell[ir];
         if (k < 0 || k >= n) continue;
         for (int f = 0; f < 5; ++f)
             A[f * n + k] += res[(((long)f * q->ns + s) * q->nt_ext + row) * q->nr + ir];
     }
+}
+
+/* transpose of the gather of inj_rec: component f's residual goes to the
+ * field that component recorded (the exact adjoint; differs from adj_inj
+ * only for the crosswell vz record). */
+static void SFX(rec_adj)(const orc_grid *g, const orc_acq *q, const long *cell, R *A,
+                         const R *res, int s, int row)
+{
+    const long n = (long)g->nz * g->nx;
+    for (int ir = 0; ir < q->nr; ++ir) {
+        const long k = cell[ir];
+        if (k < 0 || k >= n) continue;
+        for (int f = 0; f < 5; ++f)
+            A[orc_rec_field(q, f) * n + k] +=
+                res[(((long)f * q->ns + s) * q->nt_ext + row) * q->nr + ir];
+    }
+}
+
+/* Exported for tests: residual row `row` of shot s added to A [5][n] by the
+ * reference's injection (exact = 0) or by the transpose of the gather
+ * (exact = 1).  res: [5][ns][nt_ext][nr]. */
+int SFX(orc_inject_residual)(const orc_grid *g, const orc_acq *q, R *A, const R *res,
+                             int s, int row, int exact)
+{
+    long *cell = (long *)malloc((size_t)(q->nr > 0 ? q->nr : 1) * sizeof(long));
+    if (!cell) return 1;
+    orc_rec_cells(g, q, cell);
+    if (exact) SFX(rec_adj)(g, q, cell, A, res, s, row);
+    else SFX(adj_inj)(g, q, cell, A, res, s, row);
+    free(cell);
+    return 0;
 }
 
 /* ------------------------------------------------------------ host loops */
@@ -322,14 +344,16 @@ int SFX(orc_forward)(const orc_grid *g, const orc_acq *q, const R *lam,
     const int ninj = (q->nt + q->dt_scale - 1) / q->dt_scale;
     R *F = (R *)calloc(5 * n, sizeof(R));
     R *sb = (R *)calloc((size_t)5 * q->nt_ext * q->nr, sizeof(R));
-    if (!F || !sb) { free(F); free(sb); return 1; }
+    long *cell = (long *)malloc((size_t)(q->nr > 0 ? q->nr : 1) * sizeof(long));
+    if (!F || !sb || !cell) { free(F); free(sb); free(cell); return 1; }
+    orc_rec_cells(g, q, cell);
     for (int s = 0; s < q->ns; ++s) {
         memset(F, 0, 5 * n * sizeof(R));
         int chpc = 0, t_src = 0;
         for (int t = 0; t < q->nt; ++t) {
             if (t % q->dt_scale == 0) {
                 if (t_src < ninj)
-                    SFX(inj_rec)(g, q, F, q->srcx[s], q->srcz[s],
+                    SFX(inj_rec)(g, q, cell, F, q->srcx[s], q->srcz[s],
                                  src + ((long)s * ninj + t_src) * 5, sb, t_src);
                 t_src++;
             }
@@ -346,7 +370,7 @@ int SFX(orc_forward)(const orc_grid *g, const orc_acq *q, const R *lam,
                    sb + (long)f * q->nt_ext * q->nr,
                    (size_t)q->nt_ext * q->nr * sizeof(R));
     }
-    free(F); free(sb);
+    free(F); free(sb); free(cell);
     return 0;
 }
 
@@ -362,7 +386,9 @@ int SFX(orc_gradient)(const orc_grid *g, const orc_acq *q, const R *lam,
     const long n = (long)g->nz * g->nx;
     R *F = (R *)calloc(5 * n, sizeof(R));
     R *A = (R *)calloc(5 * n, sizeof(R));
-    if (!F || !A) { free(F); free(A); return 1; }
+    long *cell = (long *)malloc((size_t)(q->nr > 0 ? q->nr : 1) * sizeof(long));
+    if (!F || !A || !cell) { free(F); free(A); free(cell); return 1; }
+    orc_rec_cells(g, q, cell);
     for (int s = 0; s < q->ns; ++s) {
         memset(F, 0, 5 * n * sizeof(R));
         memset(A, 0, 5 * n * sizeof(R));
@@ -376,14 +402,14 @@ int SFX(orc_gradient)(const orc_grid *g, const orc_acq *q, const R *lam,
                 SFX(orc_rev_step)(g, F, lam, mu, bu, dxp, dzp);
             }
             if (t % q->dt_scale == 0) {
-                if (t_src - 1 >= 0) SFX(adj_inj)(g, q, A, res, s, t_src - 1);
+                if (t_src - 1 >= 0) SFX(adj_inj)(g, q, cell, A, res, s, t_src - 1);
                 t_src--;
             }
             SFX(orc_adj_step)(g, A, lam, mu, bu, dxp, dzp);
             SFX(grad)(g, F, A, bu, G);
         }
     }
-    free(F); free(A);
+    free(F); free(A); free(cell);
     return 0;
 }
 
@@ -412,9 +438,11 @@ int SFX(orc_exact_gradient)(const orc_grid *g, const orc_acq *q, const R *lam,
     R *A = (R *)calloc(5 * n, sizeof(R));
     R *Q = (R *)calloc(5 * n, sizeof(R));
     R *sb = (R *)calloc((size_t)5 * q->nt_ext * q->nr, sizeof(R));
-    if (!P || !V || !F || !A || !Q || !sb) {
-        free(P); free(V); free(F); free(A); free(Q); free(sb); return 1;
+    long *cell = (long *)malloc((size_t)(q->nr > 0 ? q->nr : 1) * sizeof(long));
+    if (!P || !V || !F || !A || !Q || !sb || !cell) {
+        free(P); free(V); free(F); free(A); free(Q); free(sb); free(cell); return 1;
     }
+    orc_rec_cells(g, q, cell);
     R *glam = g3, *gmu = g3 + n, *gb = g3 + 2 * n;
     const int lo = g->sdo, hiz = g->nz - g->sdo, hix = g->nx - g->sdo;
     for (int s = 0; s < q->ns; ++s) {
@@ -423,7 +451,7 @@ int SFX(orc_exact_gradient)(const orc_grid *g, const orc_acq *q, const R *lam,
         for (int t = 0; t < nt; ++t) {
             if (t % q->dt_scale == 0) {
                 if (t_src < ninj)
-                    SFX(inj_rec)(g, q, F, q->srcx[s], q->srcz[s],
+                    SFX(inj_rec)(g, q, cell, F, q->srcx[s], q->srcz[s],
                                  src + ((long)s * ninj + t_src) * 5, sb, t_src);
                 t_src++;
             }
@@ -485,11 +513,11 @@ int SFX(orc_exact_gradient)(const orc_grid *g, const orc_acq *q, const R *lam,
             /* adjoint of the receiver sampling of S_t */
             if (t % q->dt_scale == 0) {
                 int row = t / q->dt_scale;
-                if (row < q->nt_ext) SFX(adj_inj)(g, q, A, res, s, row);
+                if (row < q->nt_ext) SFX(rec_adj)(g, q, cell, A, res, s, row);
             }
         }
     }
-    free(P); free(V); free(F); free(A); free(Q); free(sb);
+    free(P); free(V); free(F); free(A); free(Q); free(sb); free(cell);
     return 0;
 }
 

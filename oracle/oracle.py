@@ -8,6 +8,7 @@ It restates, independently of ``pyfwi_amd``, the host logic of PyFWI v0.1.10
 around the kernels and drives the C restatement of the kernels in
 ``liborc.so`` (elastic_oracle.c):
   geometry and timing       src/PyFWI/wave_propagation.py:28-144
+  left-well reordering      src/PyFWI/wave_propagation.py:647-649
   damping profiles          src/PyFWI/fwi_tools.py:943-1063
   model padding             src/PyFWI/fwi_tools.py:776-811
   Lame parameters           src/PyFWI/wave_propagation.py:570-591
@@ -41,7 +42,10 @@ class OAcq(ctypes.Structure):
     _fields_ = [("ns", ctypes.c_int), ("nr", ctypes.c_int), ("n_main_rec", ctypes.c_int),
                 ("dxr", ctypes.c_int), ("rec_row", ctypes.c_int), ("rec_col0", ctypes.c_int),
                 ("srcx", ctypes.POINTER(ctypes.c_int)), ("srcz", ctypes.POINTER(ctypes.c_int)),
-                ("nt", ctypes.c_int), ("dt_scale", ctypes.c_int), ("nt_ext", ctypes.c_int)]
+                ("nt", ctypes.c_int), ("dt_scale", ctypes.c_int), ("nt_ext", ctypes.c_int),
+                ("acq_type", ctypes.c_int), ("n_well_rec", ctypes.c_int),
+                ("rec_left_col", ctypes.c_int), ("rec_left_row0", ctypes.c_int),
+                ("rec_right_col", ctypes.c_int), ("rec_right_row0", ctypes.c_int)]
 
 
 _lib = None
@@ -116,10 +120,11 @@ def pad_edge(a, npml):
 
 
 class Setup:
-    """The reference's WavePreparation state for a surface acquisition."""
+    """The reference's WavePreparation state (wave_propagation.py:18-144) for
+    surface (acq_type 1), crosswell (0) and wells + surface (2) acquisition."""
 
     def __init__(self, inpa, src_loc, rec_loc, model_shape, wavelet, component,
-                 chpr=0, components=0, src_dh=None):
+                 chpr=0, components=0, src_dh=None, n_well_rec=0):
         self.inpa = dict(inpa)
         self.dt_scale = int(math.ceil(inpa["dt"] / 0.0006))
         self.dt = inpa["dt"] / self.dt_scale
@@ -146,9 +151,31 @@ class Setup:
             rec_dis = (rec_loc[1, :] - rec_loc[0, :]).max()
         self.dxr = int(np.int32(float(rec_dis) / float(dh32)))
         self.nr = rec_loc.shape[0]
-        self.rec_row = int(np.int32(float(rec_loc[0, 1]) / float(dh32) + self.npml))
-        self.rec_col0 = int(np.int32(float(rec_loc[0, 0]) / float(dh32) + self.npml))
-        self.src_cts = int(self.src_j[0])
+        self.acq_type = int(inpa.get("acq_type", 1))
+        self.n_well_rec = int(n_well_rec)
+        self.n_main_rec = self.nr - 2 * self.n_well_rec
+        if self.acq_type == 2 and self.n_well_rec == 0:
+            raise ValueError("acq_type 2 needs n_well_rec")
+
+        def cell(k):  # (column, row) of receiver k incl. the absorbing layer
+            return tuple(int(np.int32(float(rec_loc[k, a]) / float(dh32) + self.npml))
+                         for a in (0, 1))
+
+        # receiver macros (wave_propagation.py:116-143)
+        self.rec_row = self.rec_col0 = 0
+        self.rec_left_col = self.rec_left_row0 = 0
+        self.rec_right_col = self.rec_right_row0 = 0
+        if self.acq_type == 0:
+            self.rec_right_col, self.rec_right_row0 = cell(0)
+            self.src_cts = int(self.src_i[0])
+        elif self.acq_type == 1:
+            self.rec_col0, self.rec_row = cell(0)
+            self.src_cts = int(self.src_j[0])
+        else:
+            self.rec_right_col, self.rec_right_row0 = cell(self.nr - self.n_well_rec)
+            self.rec_col0, self.rec_row = cell(self.n_well_rec)
+            self.rec_left_col, self.rec_left_row0 = cell(0)
+            self.src_cts = int(self.src_j[0])
         n = int(chpr * self.nt / 100)
         chp = np.linspace(0, self.nt - 1, n, dtype=np.int32)
         if len(chp) < 2:
@@ -174,10 +201,32 @@ class Setup:
         ids = list(range(self.ns)) if shots is None else list(shots)
         self._sx = np.ascontiguousarray(self.srcx[ids], np.int32)
         self._sz = np.ascontiguousarray(self.srcz[ids], np.int32)
-        return OAcq(len(ids), self.nr, self.nr, self.dxr, self.rec_row, self.rec_col0,
+        n_main = self.nr if self.acq_type == 1 else self.n_main_rec
+        return OAcq(len(ids), self.nr, n_main, self.dxr, self.rec_row, self.rec_col0,
                     self._sx.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                     self._sz.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-                    self.nt, self.dt_scale, self.nt_ext)
+                    self.nt, self.dt_scale, self.nt_ext, self.acq_type, self.n_well_rec,
+                    self.rec_left_col, self.rec_left_row0, self.rec_right_col,
+                    self.rec_right_row0)
+
+    def receiver_cells(self):
+        """Flat cell each record column reads (None where no gather reaches),
+        from the C restatement of the gathers."""
+        out = np.zeros(self.nr, np.int64)
+        rc = lib().orc_receiver_cells(ctypes.byref(self.grid), ctypes.byref(self.acq()),
+                                      _p(out))
+        assert rc == 0
+        no_cell = -(2 ** 31)
+        return [None if c == no_cell else int(c) for c in out]
+
+    def flip_wells(self, seis):
+        """forward_propagator's reordering of the left well's receivers
+        (wave_propagation.py:647-649) on records [ns, 5, nt, nr]."""
+        if self.acq_type != 2:
+            return seis
+        out = seis.copy()
+        out[..., :self.n_well_rec] = seis[..., :self.n_well_rec][..., ::-1]
+        return out
 
     def medium(self, model, dtype=np.float32):
         """Padded lam, mu, b and damping grids in `dtype` (float32 host
@@ -286,9 +335,13 @@ def read_gradient(setup, G):
     def factor(p):
         f = np.zeros((setup.nz, setup.nx), np.float32)
         r0 = sdo + setup.src_cts + 2
-        d = np.sqrt(p)[npml + r0:tnz - npml - sdo, npml + sdo:tnx - npml - sdo]
         with np.errstate(divide="ignore", invalid="ignore"):
-            f[r0:setup.nz - sdo, sdo:setup.nx - sdo] = 1 / d
+            if setup.acq_type == 0:  # columns right of the source well
+                d = np.sqrt(p)[npml + sdo:tnz - npml - sdo, npml + r0:tnx - npml - sdo]
+                f[sdo:setup.nz - sdo, r0:setup.nx - sdo] = 1 / d
+            else:
+                d = np.sqrt(p)[npml + r0:tnz - npml - sdo, npml + sdo:tnx - npml - sdo]
+                f[r0:setup.nz - sdo, sdo:setup.nx - sdo] = 1 / d
             return f / np.abs(f).max()
 
     crop = (slice(npml, tnz - npml), slice(npml, tnx - npml))

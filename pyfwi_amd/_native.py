@@ -16,6 +16,9 @@ MEM_DEVICE = 1
 MODE_PYFWI = 0
 MODE_EXACT = 1
 F_PRECOND = 1
+REC_LINE = 0
+REC_TABLE = 1
+REC_VZ_FROM_VX = 1
 
 (K_FWD_VEL, K_FWD_STR, K_REV_STR, K_REV_VEL, K_ADJ_STR, K_ADJ_VEL, K_EXA_STR, K_EXA_VEL,
  K_FWD_STEP, K_REV_STEP) = range(10)
@@ -25,7 +28,8 @@ KERNEL_NAMES = ["fwd_vel", "fwd_str", "rev_str", "rev_vel", "adj_str", "adj_vel_
 # exported symbols, in header order (checked by the CPU test suite)
 EXPORTS = [
     "pyfwi_version", "pyfwi_last_error", "pyfwi_plan_create", "pyfwi_plan_destroy",
-    "pyfwi_plan_set_stream", "pyfwi_set_model", "pyfwi_forward", "pyfwi_gradient",
+    "pyfwi_plan_set_stream", "pyfwi_set_receivers", "pyfwi_set_model", "pyfwi_forward",
+    "pyfwi_gradient",
     "pyfwi_misfit_gradient", "pyfwi_plan_set_tuning", "pyfwi_get_snapshot", "pyfwi_synchronize", "pyfwi_plan_set_timing",
     "pyfwi_kernel_timing",
 ]
@@ -44,7 +48,9 @@ class Shot(ctypes.Structure):
                 ("rec_dx", ctypes.c_int32), ("n_rec", ctypes.c_int32),
                 ("rec_stride", ctypes.c_int32), ("nt", ctypes.c_int32),
                 ("dt_scale", ctypes.c_int32), ("n_samples", ctypes.c_int32),
-                ("src_offset", ctypes.c_int64), ("rec_offset", ctypes.c_int64)]
+                ("src_offset", ctypes.c_int64), ("rec_offset", ctypes.c_int64),
+                ("rec_layout", ctypes.c_int32), ("rec_flags", ctypes.c_int32),
+                ("rec_table", ctypes.c_int64)]
 
 
 class NativeError(RuntimeError):
@@ -77,6 +83,7 @@ def load(path=None):
     lib.pyfwi_plan_create.argtypes = [ctypes.POINTER(Grid), P, ctypes.POINTER(P)]
     lib.pyfwi_plan_destroy.argtypes = [P]
     lib.pyfwi_plan_set_stream.argtypes = [P, P]
+    lib.pyfwi_set_receivers.argtypes = [P, ctypes.POINTER(ctypes.c_int32), ctypes.c_int64]
     lib.pyfwi_set_model.argtypes = [P, fp, fp, fp, fp, fp, i32]
     lib.pyfwi_forward.argtypes = [P, ctypes.POINTER(Shot), i32, fp, fp,
                                   ctypes.POINTER(ctypes.c_int32), i32, i32, i32]
@@ -148,6 +155,14 @@ class Plan:
 
     def set_stream(self, stream):
         check(self.lib.pyfwi_plan_set_stream(self.h, stream))
+
+    def set_receivers(self, table):
+        """Receiver table (host int32 cells row*tnx + column, or -1) read by
+        REC_TABLE shots; see pyfwi_set_receivers."""
+        t = np.ascontiguousarray(table, np.int32)
+        check(self.lib.pyfwi_set_receivers(
+            self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)) if t.size else None,
+            int(t.size)))
 
     def set_model(self, lam, mu, buoy, dxp, dzp, mem):
         check(self.lib.pyfwi_set_model(self.h, ptr(lam, mem), ptr(mu, mem), ptr(buoy, mem),

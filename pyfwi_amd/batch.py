@@ -43,6 +43,8 @@ class UnevenBatch:
     def __init__(self, inpa, specs, model_shape, components=2):
         if not specs:
             raise ValueError("no shots")
+        if inpa.get("acq_type", 1) != 1:
+            raise ValueError("UnevenBatch takes surface shots (acq_type 1)")
         self.inpa = dict(inpa)
         self.specs = list(specs)
         self.components = components

@@ -62,16 +62,18 @@ void launch_str(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, 
                 const float *lam, const float *mu, const float *dxp, const float *dzp,
                 hipStream_t st, const float *src = nullptr, float *seis = nullptr);
 // Record the state before step t's source term, then add the term in memory
-// (the fused steps and the exact mode's recomputation, which keep the
-// injected stresses).
+// (the fused steps, the exact mode's recomputation and receivers off the
+// surface line or sources off the active cells).  rtab: the plan's receiver
+// table (device), read by PYFWI_REC_TABLE shots.
 void launch_inject_record(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
-                          float *fields, const float *src, float *seis, hipStream_t st);
+                          float *fields, const float *src, float *seis, const int32_t *rtab,
+                          hipStream_t st);
 // Residual injection in memory at backward step t, components f0..f1-1 (all
 // five by default); with first_only, only for shots whose first backward
 // step t is.
 void launch_adj_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
-                       float *adj, const float *res, hipStream_t st, int f0 = 0, int f1 = 5,
-                       bool first_only = false);
+                       float *adj, const float *res, const int32_t *rtab, hipStream_t st,
+                       int f0 = 0, int f1 = 5, bool first_only = false);
 // Adjoint stresses + the lambda / mu imaging terms; adjoint velocities + the
 // rho imaging term.  `fields`: the forward state of the step (per-shot
 // stride fields_stride), gacc: [nshots][6][fstride].  With `res`, the stress
@@ -99,7 +101,8 @@ void launch_exact_vel(const DevGrid &g, int nshots, const pyfwi_shot *shots, int
                       float *av_out, const float *bu, const float *dxp, const float *dzp,
                       float *gacc, hipStream_t st);
 void launch_exact_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
-                         float *adj, float *av, const float *res, hipStream_t st);
+                         float *adj, float *av, const float *res, const int32_t *rtab,
+                         hipStream_t st);
 void launch_copy_shots(float *dst, int64_t dst_stride, const float *src, int64_t src_stride,
                        int64_t count, int nshots, hipStream_t st);
 void launch_grad_sum(const DevGrid &g, int nshots, int nslot, const float *gacc, float *out,

@@ -806,12 +806,29 @@ __global__ __launch_bounds__(64 * FUSED_WAVES) void k_fused_rev(
     }
 }
 
+// Offset from cell (0, 0) of a padded field of receiver-table entry `e`
+// (row*nx + column of the unpadded grid).
+__device__ __forceinline__ int64_t table_cell(const DevGrid &g, int32_t e)
+{
+    const int row = e / g.nx;
+    return (int64_t)row * g.pitch + (e - row * g.nx);
+}
+
+// Field a record component samples: the crosswell gather fills the vz
+// record from vx (src/PyFWI/elastic_crosswell.cl:27).
+__device__ __forceinline__ int recorded_field(const pyfwi_shot &sh, int f)
+{
+    return (f == 1 && (sh.rec_flags & PYFWI_REC_VZ_FROM_VX)) ? 0 : f;
+}
+
 // Receiver gather then source injection of step t for every shot, one block
-// per shot (src/PyFWI/elastic_surface.cl:7-41).  Record row k = t/dt_scale
-// holds the state after t steps; the source sample k is added to vx, vz, txx,
-// tzz at the source cell afterwards.
+// per shot (src/PyFWI/elastic_surface.cl:7-41; the well gathers of
+// elastic_crosswell.cl:1-33 and elastic.cl:43-98 through the receiver
+// table).  Record row k = t/dt_scale holds the state after t steps; the
+// source sample k is added to vx, vz, txx, tzz at the source cell afterwards.
 __global__ void k_inject_record(DevGrid g, const pyfwi_shot *shots, int t,
-                                float *fields, const float *__restrict__ src, float *seis)
+                                float *fields, const float *__restrict__ src, float *seis,
+                                const int32_t *__restrict__ rtab)
 {
     const int s = blockIdx.x;
     const pyfwi_shot sh = shots[s];
@@ -821,12 +838,20 @@ __global__ void k_inject_record(DevGrid g, const pyfwi_shot *shots, int t,
     float *F = fields + (int64_t)s * 5 * fs + g.origin;
     if (k < sh.n_samples) {
         float *rec = seis + sh.rec_offset;
+        const int32_t *cells = sh.rec_layout == PYFWI_REC_TABLE ? rtab + sh.rec_table : nullptr;
         const int64_t c0 = (int64_t)sh.rec_z * g.pitch + sh.rec_x0;
         for (int r = threadIdx.x; r < sh.n_rec; r += blockDim.x) {
-            const int64_t c = c0 + (int64_t)r * sh.rec_dx;
+            int64_t c = c0 + (int64_t)r * sh.rec_dx;
+            bool live = true;
+            if (cells) {
+                const int32_t e = cells[r];
+                live = e >= 0;
+                c = live ? table_cell(g, e) : 0;
+            }
 #pragma unroll
             for (int f = 0; f < 5; ++f)
-                rec[((int64_t)f * sh.n_samples + k) * sh.rec_stride + r] = F[f * fs + c];
+                rec[((int64_t)f * sh.n_samples + k) * sh.rec_stride + r] =
+                    live ? F[recorded_field(sh, f) * fs + c] : 0.0f;
         }
     }
     __syncthreads();
@@ -843,21 +868,33 @@ __global__ void k_inject_record(DevGrid g, const pyfwi_shot *shots, int t,
 // ---------------------------------------------------- adjoint, pyfwi mode
 
 // Residual injection in memory at backward step t (Adj_injSrc,
-// elastic_surface.cl:48-72), components f0..f1-1; with `first_only`, only for
-// the shots whose first backward step t is (t == nt - 1), the adjoint
-// velocities of later steps being written with their residual by the
-// velocity update of the step before.
+// elastic_surface.cl:48-72, elastic_crosswell.cl:35-58, elastic.cl:316-363),
+// components f0..f1-1, component f into adjoint field f; with `first_only`,
+// only for the shots whose first backward step t is (t == nt - 1), the
+// adjoint velocities of later steps being written with their residual by the
+// velocity update of the step before.  Table receivers may share a cell (a
+// well's top receiver on the surface line): their residuals are summed.
 __global__ void k_adj_inject(DevGrid g, const pyfwi_shot *shots, int t,
                              float *adj, const float *__restrict__ res, int f0, int f1,
-                             int first_only)
+                             int first_only, const int32_t *__restrict__ rtab)
 {
     const int s = blockIdx.x;
     const pyfwi_shot sh = shots[s];
     if (first_only && t != sh.nt - 1) return;
     const RecLine rq = residual_line(sh, t, res);
-    if (rq.z < 0) return;
+    if (!rq.base) return;
     const int64_t fs = g.fstride;
     float *A = adj + (int64_t)s * 5 * fs + g.origin;
+    if (sh.rec_layout == PYFWI_REC_TABLE) {
+        const int32_t *cells = rtab + sh.rec_table + sh.n_rec;
+        for (int r = threadIdx.x; r < sh.n_rec; r += blockDim.x) {
+            const int32_t e = cells[r];
+            if (e < 0) continue;
+            const int64_t c = table_cell(g, e);
+            for (int f = f0; f < f1; ++f) atomicAdd(&A[f * fs + c], rq.base[f * rq.fs + r]);
+        }
+        return;
+    }
     const int64_t c0 = (int64_t)sh.rec_z * g.pitch + sh.rec_x0;
     for (int r = threadIdx.x; r < sh.n_rec; r += blockDim.x) {
         const int64_t c = c0 + (int64_t)r * sh.rec_dx;
@@ -1211,10 +1248,12 @@ __global__ __launch_bounds__(TILE_X *TILE_Y) void k_exact_vel(
 }
 
 // Residual of sample t/dt_scale added at the receivers after backward step t
-// (adjoint of the gather of S_t).  Velocity components go to the ping-pong
-// buffer `av`, stresses to adj[2..4].
+// (adjoint of the gather of S_t: component f goes to the field it sampled).
+// Velocity components go to the ping-pong buffer `av`, stresses to
+// adj[2..4].
 __global__ void k_exact_inject(DevGrid g, const pyfwi_shot *shots, int t, float *adj,
-                               float *av, const float *__restrict__ res)
+                               float *av, const float *__restrict__ res,
+                               const int32_t *__restrict__ rtab)
 {
     const int s = blockIdx.x;
     const pyfwi_shot sh = shots[s];
@@ -1225,13 +1264,26 @@ __global__ void k_exact_inject(DevGrid g, const pyfwi_shot *shots, int t, float 
     float *A = adj + (int64_t)s * 5 * fs + g.origin;
     float *V = av + (int64_t)s * 2 * fs + g.origin;
     const float *rr = res + sh.rec_offset;
+    auto target = [&](int f, int64_t c) {
+        const int d = recorded_field(sh, f);
+        return d < 2 ? &V[d * fs + c] : &A[d * fs + c];
+    };
+    if (sh.rec_layout == PYFWI_REC_TABLE) {
+        const int32_t *cells = rtab + sh.rec_table;
+        for (int r = threadIdx.x; r < sh.n_rec; r += blockDim.x) {
+            const int32_t e = cells[r];
+            if (e < 0) continue;
+            const int64_t c = table_cell(g, e);
+            for (int f = 0; f < 5; ++f)
+                atomicAdd(target(f, c), rr[((int64_t)f * sh.n_samples + row) * sh.rec_stride + r]);
+        }
+        return;
+    }
     const int64_t c0 = (int64_t)sh.rec_z * g.pitch + sh.rec_x0;
     for (int r = threadIdx.x; r < sh.n_rec; r += blockDim.x) {
         const int64_t c = c0 + (int64_t)r * sh.rec_dx;
-        V[c] += rr[((int64_t)0 * sh.n_samples + row) * sh.rec_stride + r];
-        V[fs + c] += rr[((int64_t)1 * sh.n_samples + row) * sh.rec_stride + r];
-        for (int f = 2; f < 5; ++f)
-            A[f * fs + c] += rr[((int64_t)f * sh.n_samples + row) * sh.rec_stride + r];
+        for (int f = 0; f < 5; ++f)
+            *target(f, c) += rr[((int64_t)f * sh.n_samples + row) * sh.rec_stride + r];
     }
 }
 
@@ -1456,18 +1508,19 @@ void launch_fused_step(const DevGrid &g, int dir, int nshots, const pyfwi_shot *
 }
 
 void launch_inject_record(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
-                          float *fields, const float *src, float *seis, hipStream_t st)
+                          float *fields, const float *src, float *seis, const int32_t *rtab,
+                          hipStream_t st)
 {
     hipLaunchKernelGGL(k_inject_record, dim3(nshots), dim3(256), 0, st, g, shots, t, fields, src,
-                       seis);
+                       seis, rtab);
 }
 
 void launch_adj_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
-                       float *adj, const float *res, hipStream_t st, int f0, int f1,
-                       bool first_only)
+                       float *adj, const float *res, const int32_t *rtab, hipStream_t st, int f0,
+                       int f1, bool first_only)
 {
     hipLaunchKernelGGL(k_adj_inject, dim3(nshots), dim3(256), 0, st, g, shots, t, adj, res, f0,
-                       f1, first_only ? 1 : 0);
+                       f1, first_only ? 1 : 0, rtab);
 }
 
 void launch_adj_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
@@ -1520,10 +1573,11 @@ void launch_exact_vel(const DevGrid &g, int nshots, const pyfwi_shot *shots, int
 }
 
 void launch_exact_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
-                         float *adj, float *av, const float *res, hipStream_t st)
+                         float *adj, float *av, const float *res, const int32_t *rtab,
+                         hipStream_t st)
 {
     hipLaunchKernelGGL(k_exact_inject, dim3(nshots), dim3(256), 0, st, g, shots, t, adj,
-                       av, res);
+                       av, res, rtab);
 }
 
 void launch_copy_shots(float *dst, int64_t dst_stride, const float *src, int64_t src_stride,

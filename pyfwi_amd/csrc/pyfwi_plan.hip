@@ -93,6 +93,10 @@ struct pyfwi_plan {
     double *d_misfit = nullptr;
     pyfwi_shot *d_shots = nullptr;
     int shots_cap = 0;
+    // receiver table (pyfwi_set_receivers): host copy for the shot checks
+    std::vector<int32_t> h_rtab;
+    int32_t *d_rtab = nullptr;
+    int64_t rtab_cap = 0;
     // state of the last forward
     std::vector<pyfwi_shot> last_shots;
     std::vector<pyfwi_shot> h_shots;  // host copy of the table on the device
@@ -101,10 +105,11 @@ struct pyfwi_plan {
     int ncp = 0;
     bool have_forward = false;
     // every source and receiver of the uploaded shots lies on an active cell
-    // (rows and columns sdo..N-sdo): the forward stencil launches then apply
-    // the source and record terms themselves.  Otherwise they are applied in
-    // memory, where on a never-updated cell they accumulate from step to
-    // step, as the reference's in-place additions do.
+    // (rows and columns sdo..N-sdo) and every shot records a plain receiver
+    // line: the forward stencil launches then apply the source and record
+    // terms themselves.  Otherwise they are applied in memory, where on a
+    // never-updated cell they accumulate from step to step, as the
+    // reference's in-place additions do.
     bool sparse_inside = true;
     // timing
     bool timing = false;
@@ -193,7 +198,14 @@ int check_shots(const pyfwi_plan *p, const pyfwi_shot *shots, int nshots)
             return fail(PYFWI_EINVAL, w + "source cell outside the grid");
         if (h.n_rec < 0 || h.rec_stride < h.n_rec)
             return fail(PYFWI_EINVAL, w + "need 0 <= n_rec <= rec_stride");
-        if (h.n_rec > 0) {
+        if (h.rec_flags & ~PYFWI_REC_VZ_FROM_VX) return fail(PYFWI_EINVAL, w + "unknown receiver flag");
+        if (h.rec_layout == PYFWI_REC_TABLE) {
+            if (h.rec_table < 0 ||
+                h.rec_table + 2 * (int64_t)h.n_rec > (int64_t)p->h_rtab.size())
+                return fail(PYFWI_EINVAL, w + "receiver entries outside the plan's receiver table");
+        } else if (h.rec_layout != PYFWI_REC_LINE) {
+            return fail(PYFWI_EINVAL, w + "unknown receiver layout");
+        } else if (h.n_rec > 0) {
             if (h.rec_dx < 1) return fail(PYFWI_EINVAL, w + "receiver spacing must be >= 1 cell");
             int64_t last = (int64_t)h.rec_x0 + (int64_t)(h.n_rec - 1) * h.rec_dx;
             if (h.rec_z < 0 || h.rec_z >= g.tnz || h.rec_x0 < 0 || last >= g.tnx)
@@ -231,6 +243,10 @@ int upload_shots(pyfwi_plan *p, const pyfwi_shot *shots, int nshots)
     p->sparse_inside = true;
     for (int k = 0; k < nshots; ++k) {
         const pyfwi_shot &sh = shots[k];
+        if (sh.rec_layout != PYFWI_REC_LINE || sh.rec_flags != 0) {
+            p->sparse_inside = false;
+            continue;
+        }
         const int last = sh.rec_x0 + (sh.n_rec > 0 ? sh.n_rec - 1 : 0) * sh.rec_dx;
         if (!inside(sh.src_z, sh.src_x) ||
             (sh.n_rec > 0 && (!inside(sh.rec_z, sh.rec_x0) || !inside(sh.rec_z, last))))
@@ -351,6 +367,7 @@ int32_t pyfwi_plan_destroy(pyfwi_plan *p)
         b->release();
     if (p->d_misfit) (void)hipFree(p->d_misfit);
     if (p->d_shots) (void)hipFree(p->d_shots);
+    if (p->d_rtab) (void)hipFree(p->d_rtab);
     for (auto e : p->ev_pool) (void)hipEventDestroy(e);
     if (p->own_stream) (void)hipStreamDestroy(p->own_stream);
     delete p;
@@ -366,6 +383,34 @@ int32_t pyfwi_plan_set_stream(pyfwi_plan *p, void *stream)
         HIPCHK(hipStreamCreate(&p->own_stream));
         p->stream = p->own_stream;
     }
+    return 0;
+}
+
+int32_t pyfwi_set_receivers(pyfwi_plan *p, const int32_t *table, int64_t count)
+{
+    if (!p) return fail(PYFWI_EINVAL, "null plan");
+    if (count < 0 || (count > 0 && !table)) return fail(PYFWI_EINVAL, "bad receiver table");
+    const int64_t ncell = (int64_t)p->grid.tnz * p->grid.tnx;
+    for (int64_t k = 0; k < count; ++k)
+        if (table[k] < -1 || table[k] >= ncell)
+            return fail(PYFWI_EINVAL, "receiver table entry " + std::to_string(k) +
+                                          " is not a cell of the grid (or -1)");
+    // kernels of earlier calls may still read the old table
+    HIPCHK(hipStreamSynchronize(p->stream));
+    if (count > p->rtab_cap) {
+        if (p->d_rtab) HIPCHK(hipFree(p->d_rtab));
+        p->d_rtab = nullptr;
+        p->rtab_cap = 0;
+        if (hipMalloc(&p->d_rtab, (size_t)count * sizeof(int32_t)) != hipSuccess) {
+            p->d_rtab = nullptr;
+            p->h_rtab.clear();
+            return fail(PYFWI_ENOMEM, "hipMalloc failed for the receiver table");
+        }
+        p->rtab_cap = count;
+    }
+    p->h_rtab.assign(table, table + count);
+    if (count > 0)
+        HIPCHK(hipMemcpy(p->d_rtab, table, (size_t)count * sizeof(int32_t), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -429,7 +474,7 @@ int forward_step(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t, const floa
     const DevGrid &g = p->dg;
     const int64_t fps = fields_per_shot(p);
     if (!p->tune_fused && !p->sparse_inside) {
-        launch_inject_record(g, ng, dsh, t, p->cur, src, seis, p->stream);
+        launch_inject_record(g, ng, dsh, t, p->cur, src, seis, p->d_rtab, p->stream);
         {
             Timed tm(p, PYFWI_K_FWD_VEL);
             launch_vel(g, +1, ng, dsh, t, p->cur, fps, p->cur, fps, p->bu.p, p->dxp, p->dzp,
@@ -439,7 +484,7 @@ int forward_step(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t, const floa
         launch_str(g, +1, ng, dsh, t, p->cur, fps, p->cur, fps, p->lam.p, p->mu.p, p->dxp, p->dzp,
                    p->stream);
     } else if (p->tune_fused) {
-        launch_inject_record(g, ng, dsh, t, p->cur, src, seis, p->stream);
+        launch_inject_record(g, ng, dsh, t, p->cur, src, seis, p->d_rtab, p->stream);
         float *nxt = p->cur == p->fields.p ? p->fields2.p : p->fields.p;
         Timed tm(p, PYFWI_K_FWD_STEP);
         launch_fused_step(g, +1, ng, dsh, t, p->cur, nxt, p->bu.p, p->lam.p, p->mu.p, p->dxp,
@@ -700,9 +745,9 @@ int gradient_pyfwi(pyfwi_plan *p, int g0, int ng, const float *d_res, bool preco
         // cells, in memory every step
         const float *r_on = p->sparse_inside ? d_res : nullptr;
         if (!p->sparse_inside)
-            launch_adj_inject(g, ng, dsh, t, p->adj.p, d_res, p->stream);
+            launch_adj_inject(g, ng, dsh, t, p->adj.p, d_res, p->d_rtab, p->stream);
         else if (first_step[t])
-            launch_adj_inject(g, ng, dsh, t, p->adj.p, d_res, p->stream, 0, 2, true);
+            launch_adj_inject(g, ng, dsh, t, p->adj.p, d_res, p->d_rtab, p->stream, 0, 2, true);
         {
             Timed tm(p, PYFWI_K_ADJ_STR);
             launch_adj_str(g, ng, dsh, t, fwd, fwd_stride, p->adj.p, p->bu.p, p->dxp, p->dzp,
@@ -745,7 +790,8 @@ int gradient_exact(pyfwi_plan *p, int g0, int ng, const float *d_src, const floa
         launch_copy_shots(p->fields.p, fps, ckpts + (int64_t)c * fps, (int64_t)ncp * fps, fps, ng,
                           p->stream);
         for (int t = t0; t < t1; ++t) {
-            launch_inject_record(g, ng, dsh, t, p->fields.p, d_src, p->seis_stage.p, p->stream);
+            launch_inject_record(g, ng, dsh, t, p->fields.p, d_src, p->seis_stage.p, p->d_rtab,
+                                 p->stream);
             {
                 Timed tm(p, PYFWI_K_FWD_VEL);
                 launch_vel(g, +1, ng, dsh, t, p->fields.p, fps, p->fields.p, fps, p->bu.p, p->dxp,
@@ -772,7 +818,7 @@ int gradient_exact(pyfwi_plan *p, int g0, int ng, const float *d_src, const floa
                                  av_nxt, p->bu.p, p->dxp, p->dzp, p->gacc.p, p->stream);
             }
             std::swap(av_cur, av_nxt);
-            launch_exact_inject(g, ng, dsh, t, p->adj.p, av_cur, d_res, p->stream);
+            launch_exact_inject(g, ng, dsh, t, p->adj.p, av_cur, d_res, p->d_rtab, p->stream);
             if (int e = maybe_harvest(p)) return e;
         }
     }

@@ -7,10 +7,10 @@ OpenCL program build, buffers and per-step kernel enqueues of the reference
 are replaced by one native plan (libpyfwi_hip.so) that runs all shots of a
 call as one batch on the GPU.
 
-Supported: surface acquisition (``acq_type`` 1), orders 4 and 8, every
-``pml_dir``, ``components`` 0-4, ``'2d'``/``'3d'`` records, checkpoint ratios
-``chpr`` 0-100, ``energy_balancing``, ``g_smooth``.  Crosswell / combined
-acquisition (``acq_type`` 0, 2) raises ``NotImplementedError``.
+Supported: surface, crosswell and wells + surface acquisition (``acq_type``
+1, 0, 2), orders 4 and 8, every ``pml_dir``, ``components`` 0-4,
+``'2d'``/``'3d'`` records, checkpoint ratios ``chpr`` 0-100,
+``energy_balancing``, ``g_smooth``.
 """
 import math
 
@@ -69,9 +69,8 @@ class WavePreparation:
             rec_dis = (rec_loc[1, :] - rec_loc[0, :]).max()
         self.acq_type = inpa.get("acq_type", 1)
         self.energy_balancing = inpa.get("energy_balancing", False)
-        if self.acq_type != 1:
-            raise NotImplementedError(
-                "only surface acquisition (acq_type=1) is propagated by pyfwi_amd")
+        if self.acq_type not in (0, 1, 2):
+            raise ValueError("acq_type must be 0 (crosswell), 1 (surface) or 2 (both)")
 
         self.tnx = np.int32(self.nx + 2 * self.npml)
         self.tnz = np.int32(self.nz + 2 * self.npml)
@@ -92,10 +91,29 @@ class WavePreparation:
         self.nr = rec_loc.shape[0]
         self.n_well_rec = n_well_rec
         self.n_surface_rec = self.nr - 2 * n_well_rec
+        if n_well_rec == 0 and self.acq_type == 2:
+            raise Exception(" Number of geophons in the wells is not defined")
 
-        self.rec_surface_const = np.int32(float(rec_loc[0, 1]) / float(self.dh) + self.npml)
-        self.rec_surface_var = np.int32(float(rec_loc[0, 0]) / float(self.dh) + self.npml)
-        self.src_cts = src.j[0]
+        def cell(k):  # (column, row) of receiver k, absorbing layer included
+            return (np.int32(float(rec_loc[k, 0]) / float(self.dh) + self.npml),
+                    np.int32(float(rec_loc[k, 1]) / float(self.dh) + self.npml))
+
+        # receiver macros (wave_propagation.py:116-143)
+        zero = np.int32(0)
+        self.rec_top_left_const = self.rec_top_left_var = zero
+        self.rec_top_right_const = self.rec_top_right_var = zero
+        self.rec_surface_const = self.rec_surface_var = zero
+        if self.acq_type == 0:
+            self.rec_top_right_const, self.rec_top_right_var = cell(0)
+            self.src_cts = src.i[0]
+        elif self.acq_type == 1:
+            self.rec_surface_var, self.rec_surface_const = cell(0)
+            self.src_cts = src.j[0]
+        else:
+            self.rec_top_right_const, self.rec_top_right_var = cell(self.nr - n_well_rec)
+            self.rec_surface_var, self.rec_surface_const = cell(n_well_rec)
+            self.rec_top_left_const, self.rec_top_left_var = cell(0)
+            self.src_cts = src.j[0]
 
         self._pml_dir = pml_dir
         self._pmlR = pmlR
@@ -104,14 +122,19 @@ class WavePreparation:
         self.D = Fdm(order=int(self.sdo) * 2)
         self.components = components
 
-        # receivers the reference's gather reaches: j % dxr == 0 for j < tnx
-        # (src/PyFWI/elastic_surface.cl:30-32)
-        n_live = min(int(self.n_surface_rec), -(-int(self.tnx) // int(self.dxr)))
-        last = int(self.rec_surface_var) + (n_live - 1) * int(self.dxr)
-        if n_live > 0 and (last >= self.tnx or self.rec_surface_const >= self.tnz
-                           or self.rec_surface_var < 0 or self.rec_surface_const < 0):
-            raise ValueError("receiver line leaves the grid")
-        self._n_live = n_live
+        self._rec_table = None
+        if self.acq_type == 1:
+            # receivers the reference's gather reaches: j % dxr == 0 for j < tnx
+            # (src/PyFWI/elastic_surface.cl:30-32)
+            n_live = min(int(self.n_surface_rec), -(-int(self.tnx) // int(self.dxr)))
+            last = int(self.rec_surface_var) + (n_live - 1) * int(self.dxr)
+            if n_live > 0 and (last >= self.tnx or self.rec_surface_const >= self.tnz
+                               or self.rec_surface_var < 0 or self.rec_surface_const < 0):
+                raise ValueError("receiver line leaves the grid")
+            self._n_live = n_live
+        else:
+            self._n_live = int(self.nr)
+            self._rec_table = self.receiver_table()
 
         grid = _native.Grid()
         grid.tnz, grid.tnx = int(self.tnz), int(self.tnx)
@@ -130,12 +153,61 @@ class WavePreparation:
         self.seismogram = {f: np.zeros((self.nt_ext, self.nr, self.ns), np.float32)
                            for f in COMPONENTS}
 
+    def receiver_table(self):
+        """Receiver cells ``[2, nr]`` (row*tnx + column, -1 where no gather
+        reaches) of a well acquisition: row 0 the cell each record column
+        samples, row 1 the cell each residual column is injected at.
+
+        The gathers (elastic_crosswell.cl:24-31; elastic.cl:62-98, 316-363)
+        test ``i % dxr == 0`` over the rows i < tnz (wells) or ``j % dxr == 0``
+        over the columns j < tnx (surface); receiver ir = i/dxr of a well
+        sits at row rec_top_*_var + i of column rec_top_*_const, column
+        order left well, surface, right well.  Two reference rules shape the
+        table:
+          - forward_propagator reverses the left well's columns of the records
+            (wave_propagation.py:647-649), while gradient injects residual
+            column k at left-well receiver k (:911-913 reverse only the stored
+            records), so row 0 holds the left well reversed and row 1 not;
+          - the crosswell and right-well gathers do not bound ir: columns past
+            nr land in the next record row, which the next step overwrites,
+            and the adjoint reads past the residual row.  Only the nr columns
+            are kept here.
+        """
+        tnz, tnx, dxr, nr = int(self.tnz), int(self.tnx), int(self.dxr), int(self.nr)
+        cells = np.full(nr, -1, np.int64)
+
+        def well(first, count, col, row0):
+            ir = np.arange(min(count, -(-tnz // dxr), max(nr - first, 0)))
+            cells[first + ir] = (ir * dxr + int(row0)) * tnx + int(col)
+
+        def surface(first, count):
+            ir = np.arange(min(count, -(-tnx // dxr), max(nr - first, 0)))
+            cells[first + ir] = int(self.rec_surface_const) * tnx + ir * dxr \
+                + int(self.rec_surface_var)
+
+        nw, nsurf = int(self.n_well_rec), int(self.n_surface_rec)
+        if self.acq_type == 0:
+            well(0, nr, self.rec_top_right_const, self.rec_top_right_var)
+        else:
+            well(0, nw, self.rec_top_left_const, self.rec_top_left_var)
+            surface(nw, nsurf)
+            well(nw + nsurf, nw, self.rec_top_right_const, self.rec_top_right_var)
+        if np.any((cells != -1) & ((cells < 0) | (cells >= tnz * tnx))):
+            raise ValueError("a receiver lies outside the grid")
+        record = cells.copy()
+        if self.acq_type == 2:
+            record[:nw] = cells[:nw][::-1]
+        return np.stack([record, cells]).astype(np.int32)
+
     # ------------------------------------------------------------ helpers
     @property
     def _plan(self):
         # created on first use so that the geometry can be inspected without a GPU
         if self._plan_obj is None:
-            self._plan_obj = _native.Plan(self._grid)
+            plan = _native.Plan(self._grid)
+            if self._rec_table is not None:
+                plan.set_receivers(self._rec_table.ravel())
+            self._plan_obj = plan
         return self._plan_obj
 
     @property
@@ -149,6 +221,9 @@ class WavePreparation:
             sh.src_x, sh.src_z = int(self.srcx[s]), int(self.srcz[s])
             sh.rec_z, sh.rec_x0 = int(self.rec_surface_const), int(self.rec_surface_var)
             sh.rec_dx, sh.n_rec, sh.rec_stride = int(self.dxr), self._n_live, int(self.nr)
+            if self._rec_table is not None:
+                sh.rec_layout, sh.rec_table = _native.REC_TABLE, 0
+                sh.rec_flags = _native.REC_VZ_FROM_VX if self.acq_type == 0 else 0
             sh.nt, sh.dt_scale, sh.n_samples = int(self.nt), int(self.dt_scale), int(self.nt_ext)
             sh.src_offset = 0
             sh.rec_offset = k * 5 * int(self.nt_ext) * int(self.nr)
@@ -219,9 +294,13 @@ class WavePreparation:
             factor = np.zeros((int(self.nz), int(self.nx)), np.float32)
             denom = np.sqrt(precond)
             r0 = sdo + int(self.src_cts) + 2
-            denom = denom[npml + r0:tnz - npml - sdo, npml + sdo:tnx - npml - sdo]
             with np.errstate(divide="ignore", invalid="ignore"):
-                factor[r0:-sdo, sdo:-sdo] = 1 / denom
+                if self.acq_type == 0:  # columns right of the source well
+                    denom = denom[npml + sdo:tnz - npml - sdo, npml + r0:tnx - npml - sdo]
+                    factor[sdo:-sdo, r0:-sdo] = 1 / denom
+                else:
+                    denom = denom[npml + r0:tnz - npml - sdo, npml + sdo:tnx - npml - sdo]
+                    factor[r0:-sdo, sdo:-sdo] = 1 / denom
                 factor = factor / np.abs(factor).max()
             return factor
 
@@ -309,10 +388,20 @@ class WavePropagator(WavePreparation):
         (no mask), returned as {'lam','mu','rho'} or {'vp','vs','rho'}."""
         if not self._forward_done:
             raise RuntimeError("forward_modeling must run before gradient")
+        self._unflip_left_well()
         if mode == "exact":
             res = _section_vjp(res, self.components)
         G = self.raw_gradient(res, mode=mode)
         return self._finish_gradient(G, parameterization, mode)
+
+    def _unflip_left_well(self):
+        """The reference's gradient puts the left well's records of
+        ``seismogram`` back in receiver order (wave_propagation.py:911-913);
+        the residual it is given keeps the reversed order."""
+        if self.acq_type == 2:
+            nw = int(self.n_well_rec)
+            for name in self.seismogram:
+                self.seismogram[name][:, :nw] = np.flip(self.seismogram[name][:, :nw], axis=1)
 
     def _finish_gradient(self, G, parameterization="dv", mode="pyfwi"):
         """Read-out of the accumulators G [6, tnz, tnx]: mask and balancing
@@ -381,6 +470,7 @@ class WavePropagator(WavePreparation):
                                          _native.MEM_HOST)
         for f, name in enumerate(COMPONENTS):
             self.seismogram[name][:] = seis[:, f].transpose(1, 2, 0)
+        self._unflip_left_well()  # as forward_modeling + gradient leave it
         # the plan keeps only the last pass's states
         self._forward_done = False
         return rms, self._finish_gradient(G, parameterization)

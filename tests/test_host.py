@@ -210,8 +210,8 @@ def test_propagator_geometry_without_gpu():
     m = workloads.plain_medium()
     Wm = WavePropagator(m.inpa, m.src, m.rec_loc, m.model_shape, components=0)
     assert Wm.nt == 2000 and list(Wm.chp) == [1, 1999]
-    with pytest.raises(NotImplementedError):
-        WavePropagator(dict(w.inpa, acq_type=0), w.src, w.rec_loc, w.model_shape)
+    with pytest.raises(ValueError, match="acq_type"):
+        WavePropagator(dict(w.inpa, acq_type=3), w.src, w.rec_loc, w.model_shape)
 
 
 def test_workloads_are_seeded():
