@@ -90,7 +90,7 @@ def build_problem(workload, nt=None, shots=None):
         kw["ns"] = shots
     w = getattr(workloads, workload)(**kw)
     W = WavePropagator(w.inpa, w.src, w.rec_loc, w.model_shape, components=w.components,
-                       chpr=w.chpr)
+                       n_well_rec=w.meta.get("n_well_rec", 0), chpr=w.chpr)
     return w, W
 
 
@@ -214,7 +214,7 @@ def cpu_baseline(w, steps_cap, threads):
     inpa["t"] = (nt - 1) * inpa["dt"] + inpa["dt"] / 2
     src_loc = np.c_[w.src.i, w.src.j].astype(np.float64) * inpa["dh"]
     st = O.Setup(inpa, src_loc[:1], w.rec_loc, w.model_shape, w.src.w, w.src.component,
-                 chpr=0, components=w.components)
+                 chpr=0, components=w.components, n_well_rec=w.meta.get("n_well_rec", 0))
     O.lib()
     _set_omp_threads(threads)
     t0 = time.perf_counter()

@@ -62,12 +62,14 @@ void launch_str(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, 
                 const float *lam, const float *mu, const float *dxp, const float *dzp,
                 hipStream_t st, const float *src = nullptr, float *seis = nullptr);
 // Record the state before step t's source term, then add the term in memory
-// (the fused steps, the exact mode's recomputation and receivers off the
-// surface line or sources off the active cells).  rtab: the plan's receiver
-// table (device), read by PYFWI_REC_TABLE shots.
+// (the fused steps, the exact mode's recomputation and sources off the
+// active cells); with src == nullptr only record (receiver tables, whose
+// gather the stencil kernels do not do).  `fields`: per-shot stride
+// fields_stride.  rtab: the plan's receiver table (device), read by
+// PYFWI_REC_TABLE shots.
 void launch_inject_record(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
-                          float *fields, const float *src, float *seis, const int32_t *rtab,
-                          hipStream_t st);
+                          float *fields, int64_t fields_stride, const float *src, float *seis,
+                          const int32_t *rtab, hipStream_t st);
 // Residual injection in memory at backward step t, components f0..f1-1 (all
 // five by default); with first_only, only for shots whose first backward
 // step t is.

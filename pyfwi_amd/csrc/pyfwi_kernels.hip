@@ -825,9 +825,12 @@ __device__ __forceinline__ int recorded_field(const pyfwi_shot &sh, int f)
 // per shot (src/PyFWI/elastic_surface.cl:7-41; the well gathers of
 // elastic_crosswell.cl:1-33 and elastic.cl:43-98 through the receiver
 // table).  Record row k = t/dt_scale holds the state after t steps; the
-// source sample k is added to vx, vz, txx, tzz at the source cell afterwards.
+// source sample k is added to vx, vz, txx, tzz at the source cell afterwards,
+// unless src == nullptr (gather only, the stencil kernels then apply the
+// source term).  `fields`: per-shot stride fields_stride.
 __global__ void k_inject_record(DevGrid g, const pyfwi_shot *shots, int t,
-                                float *fields, const float *__restrict__ src, float *seis,
+                                float *fields, int64_t fields_stride,
+                                const float *__restrict__ src, float *seis,
                                 const int32_t *__restrict__ rtab)
 {
     const int s = blockIdx.x;
@@ -835,7 +838,7 @@ __global__ void k_inject_record(DevGrid g, const pyfwi_shot *shots, int t,
     if (t >= sh.nt || (t % sh.dt_scale) != 0) return;
     const int k = t / sh.dt_scale;
     const int64_t fs = g.fstride;
-    float *F = fields + (int64_t)s * 5 * fs + g.origin;
+    float *F = fields + (int64_t)s * fields_stride + g.origin;
     if (k < sh.n_samples) {
         float *rec = seis + sh.rec_offset;
         const int32_t *cells = sh.rec_layout == PYFWI_REC_TABLE ? rtab + sh.rec_table : nullptr;
@@ -854,6 +857,7 @@ __global__ void k_inject_record(DevGrid g, const pyfwi_shot *shots, int t,
                     live ? F[recorded_field(sh, f) * fs + c] : 0.0f;
         }
     }
+    if (!src) return;
     __syncthreads();
     if (threadIdx.x == 0) {
         const float *sv = src + sh.src_offset + (int64_t)k * 5;
@@ -1508,11 +1512,11 @@ void launch_fused_step(const DevGrid &g, int dir, int nshots, const pyfwi_shot *
 }
 
 void launch_inject_record(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
-                          float *fields, const float *src, float *seis, const int32_t *rtab,
-                          hipStream_t st)
+                          float *fields, int64_t fields_stride, const float *src, float *seis,
+                          const int32_t *rtab, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_inject_record, dim3(nshots), dim3(256), 0, st, g, shots, t, fields, src,
-                       seis, rtab);
+    hipLaunchKernelGGL(k_inject_record, dim3(nshots), dim3(256), 0, st, g, shots, t, fields,
+                       fields_stride, src, seis, rtab);
 }
 
 void launch_adj_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,

@@ -104,13 +104,16 @@ struct pyfwi_plan {
     int ckpt_every = 0;
     int ncp = 0;
     bool have_forward = false;
-    // every source and receiver of the uploaded shots lies on an active cell
-    // (rows and columns sdo..N-sdo) and every shot records a plain receiver
-    // line: the forward stencil launches then apply the source and record
-    // terms themselves.  Otherwise they are applied in memory, where on a
-    // never-updated cell they accumulate from step to step, as the
+    // every source of the uploaded shots lies on an active cell (rows and
+    // columns sdo..N-sdo): the forward stencil launches then apply the source
+    // term to the values they load.  Otherwise it is added in memory, where on
+    // a never-updated cell it accumulates from step to step, as the
     // reference's in-place additions do.
-    bool sparse_inside = true;
+    bool src_inside = true;
+    // every shot records a receiver line on active cells: the stress update
+    // writes the record samples and the adjoint kernels add the residual.
+    // Otherwise a gather launch records and the residual is added in memory.
+    bool rec_inline = true;
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
@@ -240,17 +243,17 @@ int upload_shots(pyfwi_plan *p, const pyfwi_shot *shots, int nshots)
     p->h_shots.assign(shots, shots + nshots);
     const int sdo = p->grid.sdo, nz = p->grid.tnz, nx = p->grid.tnx;
     auto inside = [&](int z, int x) { return z >= sdo && z <= nz - sdo && x >= sdo && x <= nx - sdo; };
-    p->sparse_inside = true;
+    p->src_inside = p->rec_inline = true;
     for (int k = 0; k < nshots; ++k) {
         const pyfwi_shot &sh = shots[k];
+        if (!inside(sh.src_z, sh.src_x)) p->src_inside = false;
         if (sh.rec_layout != PYFWI_REC_LINE || sh.rec_flags != 0) {
-            p->sparse_inside = false;
+            p->rec_inline = false;
             continue;
         }
         const int last = sh.rec_x0 + (sh.n_rec > 0 ? sh.n_rec - 1 : 0) * sh.rec_dx;
-        if (!inside(sh.src_z, sh.src_x) ||
-            (sh.n_rec > 0 && (!inside(sh.rec_z, sh.rec_x0) || !inside(sh.rec_z, last))))
-            p->sparse_inside = false;
+        if (sh.n_rec > 0 && (!inside(sh.rec_z, sh.rec_x0) || !inside(sh.rec_z, last)))
+            p->rec_inline = false;
     }
     if (nshots > p->shots_cap) {
         if (p->d_shots) (void)hipFree(p->d_shots);
@@ -467,14 +470,15 @@ namespace {
 // wave_propagation.py:704-708: gather + inject on the current state, then the
 // velocity and stress half steps.  The two-launch form applies the source
 // term to the values it loads and writes the next step's record sample from
-// the stress update; the fused form records and injects in memory first.
+// the stress update (receiver tables: a gather launch first); the fused form,
+// and sources off the active cells, record and inject in memory first.
 int forward_step(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t, const float *src,
                  float *seis)
 {
     const DevGrid &g = p->dg;
     const int64_t fps = fields_per_shot(p);
-    if (!p->tune_fused && !p->sparse_inside) {
-        launch_inject_record(g, ng, dsh, t, p->cur, src, seis, p->d_rtab, p->stream);
+    if (!p->tune_fused && !p->src_inside) {
+        launch_inject_record(g, ng, dsh, t, p->cur, fps, src, seis, p->d_rtab, p->stream);
         {
             Timed tm(p, PYFWI_K_FWD_VEL);
             launch_vel(g, +1, ng, dsh, t, p->cur, fps, p->cur, fps, p->bu.p, p->dxp, p->dzp,
@@ -484,13 +488,15 @@ int forward_step(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t, const floa
         launch_str(g, +1, ng, dsh, t, p->cur, fps, p->cur, fps, p->lam.p, p->mu.p, p->dxp, p->dzp,
                    p->stream);
     } else if (p->tune_fused) {
-        launch_inject_record(g, ng, dsh, t, p->cur, src, seis, p->d_rtab, p->stream);
+        launch_inject_record(g, ng, dsh, t, p->cur, fps, src, seis, p->d_rtab, p->stream);
         float *nxt = p->cur == p->fields.p ? p->fields2.p : p->fields.p;
         Timed tm(p, PYFWI_K_FWD_STEP);
         launch_fused_step(g, +1, ng, dsh, t, p->cur, nxt, p->bu.p, p->lam.p, p->mu.p, p->dxp,
                           p->dzp, p->stream);
         p->cur = nxt;
     } else {
+        if (!p->rec_inline)
+            launch_inject_record(g, ng, dsh, t, p->cur, fps, nullptr, seis, p->d_rtab, p->stream);
         {
             Timed tm(p, PYFWI_K_FWD_VEL);
             launch_vel(g, +1, ng, dsh, t, p->cur, fps, p->cur, fps, p->bu.p, p->dxp, p->dzp,
@@ -498,7 +504,7 @@ int forward_step(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t, const floa
         }
         Timed tm(p, PYFWI_K_FWD_STR);
         launch_str(g, +1, ng, dsh, t, p->cur, fps, p->cur, fps, p->lam.p, p->mu.p, p->dxp, p->dzp,
-                   p->stream, src, seis);
+                   p->stream, src, p->rec_inline ? seis : nullptr);
     }
     return 0;
 }
@@ -511,6 +517,11 @@ int forward_step_kept(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t, const
                       int64_t out_stride)
 {
     const DevGrid &g = p->dg;
+    if (!p->rec_inline) {
+        launch_inject_record(g, ng, dsh, t, const_cast<float *>(in), in_stride, nullptr, seis,
+                             p->d_rtab, p->stream);
+        seis = nullptr;
+    }
     {
         Timed tm(p, PYFWI_K_FWD_VEL);
         launch_vel(g, +1, ng, dsh, t, in, in_stride, out, out_stride, p->bu.p, p->dxp, p->dzp,
@@ -607,7 +618,7 @@ int forward_pass(pyfwi_plan *p, int g0, int ng, const float *d_src, float *d_sei
     if (int e = prepare_fields(p, ng)) return e;
     // every step kept (chpr=100): the two-launch kernels step from one slot
     // into the next, so the kept states cost no extra traffic
-    if (snaps && keeps_every_step(cl, nt) && !p->tune_fused && p->sparse_inside) {
+    if (snaps && keeps_every_step(cl, nt) && !p->tune_fused && p->src_inside) {
         const int64_t ss = (int64_t)nchp * fps;
         const float *in = p->fields.p;
         int64_t in_stride = fps;
@@ -742,9 +753,9 @@ int gradient_pyfwi(pyfwi_plan *p, int g0, int ng, const float *d_res, bool preco
         // the residual of step t: the adjoint kernels add it (the velocity
         // update of step t+1 already wrote the adjoint velocities with it,
         // except for shots starting here); with receivers off the active
-        // cells, in memory every step
-        const float *r_on = p->sparse_inside ? d_res : nullptr;
-        if (!p->sparse_inside)
+        // cells or from a table, in memory every step
+        const float *r_on = p->rec_inline ? d_res : nullptr;
+        if (!p->rec_inline)
             launch_adj_inject(g, ng, dsh, t, p->adj.p, d_res, p->d_rtab, p->stream);
         else if (first_step[t])
             launch_adj_inject(g, ng, dsh, t, p->adj.p, d_res, p->d_rtab, p->stream, 0, 2, true);
@@ -790,8 +801,8 @@ int gradient_exact(pyfwi_plan *p, int g0, int ng, const float *d_src, const floa
         launch_copy_shots(p->fields.p, fps, ckpts + (int64_t)c * fps, (int64_t)ncp * fps, fps, ng,
                           p->stream);
         for (int t = t0; t < t1; ++t) {
-            launch_inject_record(g, ng, dsh, t, p->fields.p, d_src, p->seis_stage.p, p->d_rtab,
-                                 p->stream);
+            launch_inject_record(g, ng, dsh, t, p->fields.p, fps, d_src, p->seis_stage.p,
+                                 p->d_rtab, p->stream);
             {
                 Timed tm(p, PYFWI_K_FWD_VEL);
                 launch_vel(g, +1, ng, dsh, t, p->fields.p, fps, p->fields.p, fps, p->bu.p, p->dxp,

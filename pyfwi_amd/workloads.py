@@ -15,7 +15,8 @@ import numpy as np
 from .acquisition import Source
 
 __all__ = ["Workload", "layered3", "random_layered", "gaussian_anomaly",
-           "plain_small", "plain_medium", "differentiation", "large_step",
+           "plain_small", "plain_medium", "crosswell_medium", "wells_surface_medium",
+           "differentiation", "large_step",
            "t_for_nt", "gardner", "UnevenWorkload", "uneven_batch", "uneven_small"]
 
 
@@ -132,6 +133,38 @@ def plain_medium(seed=2, ns=20, nt=2000):
     src.Ricker(20)
     return Workload("plain_medium_500x1000", inpa, src, rec_loc, model, None, components=0,
                     chpr=0, meta={"nt": nt, "shots": ns, "receivers": 400, "seed": seed})
+
+
+def _wells_medium(acq_type, name, seed, ns, nt, nz, nx):
+    from .acquisition import acq_parameters
+    dh = 5.0
+    model = random_layered(nz, nx, 20, seed)
+    dt = _cfl_dt(model, dh)
+    inpa = _base_inpa(dh, dt, nt)
+    inpa.update(rec_dis=2 * dh, acq_type=acq_type)
+    src_loc, rec_loc, n_surface, n_well = acq_parameters(ns, 2 * dh, nx * dh, nz * dh, dh,
+                                                         inpa["sdo"], acq_type)
+    src = Source(src_loc, dh, dt)
+    src.Ricker(20)
+    return Workload(name, inpa, src, rec_loc.astype(np.float64), model, None, components=0,
+                    chpr=0, meta={"nt": nt, "shots": ns, "receivers": int(rec_loc.shape[0]),
+                                  "n_well_rec": int(n_well) if acq_type != 1 else 0,
+                                  "seed": seed})
+
+
+def crosswell_medium(seed=2, ns=20, nt=2000):
+    """nz=1000, nx=500 (the plain_medium cell count, upright), dh=5 m,
+    20-layer random model, 20 Hz, nt=2000, with the reference's crosswell
+    geometry (acq_type 0, acquisition.py:62-124): 20 sources down the left
+    well, 496 receivers 10 m apart down the right one, 2.46 km away."""
+    return _wells_medium(0, "crosswell_medium_1000x500", seed, ns, nt, 1000, 500)
+
+
+def wells_surface_medium(seed=2, ns=20, nt=2000):
+    """The plain_medium model with both wells and the surface line
+    (acq_type 2, acquisition.py:46-57): 246 receivers in each well and 496
+    on the surface line, surface sources."""
+    return _wells_medium(2, "wells_surface_medium_500x1000", seed, ns, nt, 500, 1000)
 
 
 def differentiation(seed=3, ns=10, nt=2500):
