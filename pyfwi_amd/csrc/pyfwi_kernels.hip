@@ -225,6 +225,23 @@ __device__ __forceinline__ float xplus(V v, const float c[4])
     return r;
 }
 
+// The 2*SDO values a staggered x-derivative reads, loaded into registers:
+// t[k] = field at column offset k - SDO + 1 - M, with M = 1 for the
+// backward-staggered (xminus, offsets -SDO..SDO-1) and M = 0 for the
+// forward-staggered form (xplus, offsets -SDO+1..SDO).  Loading the taps
+// apart from the arithmetic lets a conditional load be issued with the
+// other loads of a row instead of waiting inside its branch.
+template <int SDO, int M>
+struct Taps {
+    float t[2 * SDO] = {};
+    __device__ __forceinline__ void load(const CellRef &f, uint32_t o)
+    {
+#pragma unroll
+        for (int k = 0; k < 2 * SDO; ++k) t[k] = f.at(o, k - SDO + 1 - M);
+    }
+    __device__ __forceinline__ float operator()(int k) const { return t[k + SDO - 1 + M]; }
+};
+
 // Delay line of N values: pop() returns the oldest value and appends the
 // newest, so a value comes back N calls after it went in.  The marching
 // kernels take the x-derivatives of a windowed field on the row entering its
@@ -972,6 +989,9 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot
     }
     const float ex = dxp[j];
     const float dt = g.dt;
+    // Every load of a row is issued before the row's stores: the compiler
+    // does not move a load across a store to memory it cannot prove
+    // distinct, and a load placed after one waits out a full memory latency.
 #pragma unroll MARCH_UNROLL
     for (int i = i0; i <= i1; ++i, cb += RB) {
         const uint32_t n = cb + (uint32_t)SDO * RB;
@@ -979,37 +999,58 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot
         wbx.push(avx(n) * B(n));
         const float dxpb = fxp.pop(dx_bvx(n));
         const float dxmb = fxm.pop(dx_bvz(n));
+        const float ez = dzp[i];
+        float *pa = &cell_ref(A + 2 * fs, cb), *pb = &cell_ref(A + 3 * fs, cb),
+              *pd = &cell_ref(A + 4 * fs, cb);
+        const float a0 = *pa, b0 = *pb, d0 = *pd;
+        const bool row_img = img && i >= lo && i <= hi;
+        float vz_n = 0.0f, g_mu = 0.0f, g_lam = 0.0f;
+        float sxx = 0.0f, szz = 0.0f, sxz = 0.0f, p_mu = 0.0f, p_lam = 0.0f;
+        Taps<SDO, 1> tvz;  // forward vz on the entering row (d/dx-)
+        Taps<SDO, 0> tvx;  // forward vx on the centre row (d/dx+)
+        if (img) {
+            vz_n = vz(n);
+            tvz.load(vz, n);
+        }
+        if (row_img) {
+            tvx.load(vx, cb);
+            g_mu = cell_ref(G, cb);
+            g_lam = cell_ref(G + 2 * fs, cb);
+            if constexpr (PRECOND) {
+                sxx = txx(cb);
+                szz = tzz(cb);
+                sxz = txz(cb);
+                p_mu = cell_ref(G + fs, cb);
+                p_lam = cell_ref(G + 3 * fs, cb);
+            }
+        }
+        // (lanes outside the imaging region push values they never use)
+        fvz.push(vz_n);
+        const float dxm_vz = fdvz.pop(xminus<SDO>(tvz, g.cx));
+        const float dxp_vx = xplus<SDO>(tvx, g.cx);
         const float dzmb = wbz.dminus(g.cz);
         const float dzpb = wbx.dplus(g.cz);
-        const float ez = dzp[i];
-        float &ra = cell_ref(A + 2 * fs, cb), &rb = cell_ref(A + 3 * fs, cb),
-              &rd = cell_ref(A + 4 * fs, cb);
         const bool at_rec = i == rrow;
-        const float a = at_rec ? ra + r_txx : ra;
-        const float bb = at_rec ? rb + r_tzz : rb;
-        const float d = at_rec ? rd + r_txz : rd;
+        const float a = at_rec ? a0 + r_txx : a0;
+        const float bb = at_rec ? b0 + r_tzz : b0;
+        const float d = at_rec ? d0 + r_txz : d0;
         const float aa = fmaf(-dt, ex * a + dxpb, a);
         const float ab = fmaf(-dt, ez * bb + dzmb, bb);
         const float ad = fmaf(-dt, (ex + ez) * d + (dzpb + dxmb), d);
-        ra = aa;
-        rb = ab;
-        rd = ad;
-        if (!img) continue;
-        fvz.push(vz(n));
-        const float dxm_vz = fdvz.pop(x_vz(n));
-        if (i < lo || i > hi) continue;
+        *pa = aa;
+        *pb = ab;
+        *pd = ad;
+        if (!row_img) continue;
         const float dzp_vz = fvz.dplus(g.cz);
-        const float dxp_vx = xplus<SDO>([&](int k) { return vx.at(cb, k); }, g.cx);
         const float dzm_vz = fvz.dminus(g.cz);
-        cell_ref(G, cb) += 2.0f * dt * (aa * dxp_vx + ab * dzm_vz) + dt * ad * (dzp_vz + dxm_vz);
-        cell_ref(G + 2 * fs, cb) += dt * (aa + ab) * (dxp_vx + dzm_vz);
+        cell_ref(G, cb) = g_mu + (2.0f * dt * (aa * dxp_vx + ab * dzm_vz) + dt * ad * (dzp_vz + dxm_vz));
+        cell_ref(G + 2 * fs, cb) = g_lam + dt * (aa + ab) * (dxp_vx + dzm_vz);
         if constexpr (PRECOND) {
-            const float sxx = txx(cb), szz = tzz(cb), sxz = txz(cb);
             const float dt2 = dt * dt;
-            cell_ref(G + fs, cb) +=
-                4.0f * dt2 * (sxx * sxx * dxp_vx * dxp_vx + szz * szz * dzm_vz * dzm_vz)
-                + dt2 * sxz * sxz * (dzp_vz * dzp_vz + dxm_vz * dxm_vz);
-            cell_ref(G + 3 * fs, cb) +=
+            cell_ref(G + fs, cb) = p_mu +
+                (4.0f * dt2 * (sxx * sxx * dxp_vx * dxp_vx + szz * szz * dzm_vz * dzm_vz)
+                 + dt2 * sxz * sxz * (dzp_vz * dzp_vz + dxm_vz * dxm_vz));
+            cell_ref(G + 3 * fs, cb) = p_lam +
                 dt2 * (sxx * sxx + szz * szz) * (dxp_vx * dxp_vx + dzm_vz * dzm_vz);
         }
     }
@@ -1081,6 +1122,7 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
     }
     const float ex = dxp[j];
     const float dt = g.dt;
+    // every load of a row before the row's stores (see k_adj_str)
 #pragma unroll ADJ_VEL_UNROLL
     for (int i = i0; i <= i1; ++i, cb += RB) {
         const uint32_t n = cb + (uint32_t)SDO * RB;
@@ -1089,33 +1131,51 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
         wb2.push(q_b2(n));
         const float a1 = fa1.pop(x_a1(n)), a2 = fa2.pop(x_a2(n));
         const float b3 = fb3.pop(x_b3(n));
+        const float ez = dzp[i];
+        float *px = &cell_ref(A, cb), *pz = &cell_ref(A + fs, cb);
+        float ux = *px, uz = *pz;
+        const bool row_img = img && i >= lo && i <= hi;
+        float tzz_n = 0.0f, txz_n = 0.0f, bb = 0.0f, g_rho = 0.0f;
+        float wx = 0.0f, wz = 0.0f, p_rho = 0.0f;
+        Taps<SDO, 0> ttxz;  // forward txz on the entering row (d/dx+)
+        Taps<SDO, 1> ttxx;  // forward txx on the centre row (d/dx-)
+        if (img) {
+            tzz_n = tzz(n);
+            txz_n = txz(n);
+            ttxz.load(txz, n);
+        }
+        if (row_img) {
+            ttxx.load(txx, cb);
+            bb = B(cb);
+            g_rho = cell_ref(G + 4 * fs, cb);
+            if constexpr (PRECOND) {
+                wx = vx(cb);
+                wz = vz(cb);
+                p_rho = cell_ref(G + 5 * fs, cb);
+            }
+        }
+        // (lanes outside the imaging region push values they never use)
+        ftzz.push(tzz_n);
+        ftxz.push(txz_n);
+        const float dxp_txz = fdtxz.pop(xplus<SDO>(ttxz, g.cx));
+        const float dxm_txx = xminus<SDO>(ttxx, g.cx);
         const float a3 = wa3.dminus(g.cz);
         const float b1 = wb1.dplus(g.cz);
         const float b2 = wb2.dplus(g.cz);
-        const float ez = dzp[i];
-        float &rx = cell_ref(A, cb), &rz = cell_ref(A + fs, cb);
-        float ux = rx, uz = rz;
         ux = fmaf(-dt, (2.0f * ex + ez) * ux + (a1 + a2 + a3), ux);
         uz = fmaf(-dt, (ex + 2.0f * ez) * uz + (b2 + b1 + b3), uz);
         const bool at_rec = i == rrow;
-        rx = at_rec ? ux + r_vx : ux;
-        rz = at_rec ? uz + r_vz : uz;
-        if (!img) continue;
-        ftzz.push(tzz(n));
-        ftxz.push(txz(n));
-        const float dxp_txz = fdtxz.pop(x_txz(n));
-        if (i < lo || i > hi) continue;
+        *px = at_rec ? ux + r_vx : ux;
+        *pz = at_rec ? uz + r_vz : uz;
+        if (!row_img) continue;
         const float dzp_tzz = ftzz.dplus(g.cz);
-        const float dxm_txx = xminus<SDO>([&](int k) { return txx.at(cb, k); }, g.cx);
         const float dzm_txz = ftxz.dminus(g.cz);
-        const float bb = B(cb);
-        cell_ref(G + 4 * fs, cb) +=
+        cell_ref(G + 4 * fs, cb) = g_rho +
             -(dt * (bb * bb)) * (ux * (dxm_txx + dzm_txz) + uz * (dzp_tzz + dxp_txz));
         if constexpr (PRECOND) {
-            const float wx = vx(cb), wz = vz(cb);
             const float dt2 = dt * dt;
-            cell_ref(G + 5 * fs, cb) += dt2 * (wx * wx * (dxm_txx * dxm_txx + dzm_txz * dzm_txz)
-                                              + wz * wz * (dzp_tzz * dzp_tzz + dxp_txz * dxp_txz));
+            cell_ref(G + 5 * fs, cb) = p_rho + dt2 * (wx * wx * (dxm_txx * dxm_txx + dzm_txz * dzm_txz)
+                                                      + wz * wz * (dzp_tzz * dzp_tzz + dxp_txz * dxp_txz));
         }
     }
 }
