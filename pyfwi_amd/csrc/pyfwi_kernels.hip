@@ -453,15 +453,16 @@ __global__ __launch_bounds__(MARCH_X) void k_vel(DevGrid g, const pyfwi_shot *sh
         wzz.push(tzz_row(n, i + SDO));
         const float dxp_txz = fxz.pop(dx_txz(n));
         const float bc = B(cb);
-        float dxm_txx;
+        Taps<SDO, 1> ttxx;  // loaded outside the branch below, which then holds no load
+        ttxx.load(txx, cb);
         float ux = vx(cb), uz = vz(cb);
         if (i == st.z) {  // the source row
-            dxm_txx = xminus<SDO>([&](int k) { return txx.at(cb, k) + s_txx[k + SDO]; }, g.cx);
+#pragma unroll
+            for (int k = 0; k < 2 * SDO; ++k) ttxx.t[k] += s_txx[k];
             ux += s_vx;
             uz += s_vz;
-        } else {
-            dxm_txx = xminus<SDO>([&](int k) { return txx.at(cb, k); }, g.cx);
         }
+        const float dxm_txx = xminus<SDO>(ttxx, g.cx);
         const float ax = bc * (dxm_txx + wxz.dminus(g.cz));
         const float az = bc * (wzz.dplus(g.cz) + dxp_txz);
         const float damp = ddx + dzp[i];
