@@ -80,7 +80,7 @@ def measured_traffic(workload_name, kernel):
     return d.get("bytes_per_dispatch", {}).get(kernel)
 
 
-def build_problem(workload, nt=None, shots=None):
+def build_problem(workload, nt=None, shots=None, order=0):
     from pyfwi_amd import workloads
     from pyfwi_amd.wave_propagation import WavePropagator
     kw = {}
@@ -89,6 +89,9 @@ def build_problem(workload, nt=None, shots=None):
     if shots:
         kw["ns"] = shots
     w = getattr(workloads, workload)(**kw)
+    if order:
+        w.inpa["sdo"] = order
+        w.name += f"_o{order}"
     W = WavePropagator(w.inpa, w.src, w.rec_loc, w.model_shape, components=w.components,
                        n_well_rec=w.meta.get("n_well_rec", 0), chpr=w.chpr)
     return w, W
@@ -302,6 +305,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="plain_medium")
     ap.add_argument("--nt", type=int, default=0, help="override time steps (profiling runs)")
+    ap.add_argument("--order", type=int, default=0, choices=[0, 4, 8],
+                    help="stencil order (0: the workload's own, 4)")
     ap.add_argument("--shots", type=int, default=0,
                     help="override the number of shots (e.g. one GPU's share of large_step)")
     ap.add_argument("--per-shot-sample", type=int, default=2, help="0 skips the per-shot pass")
@@ -329,7 +334,7 @@ def main():
         copy_calibration(torch)
     if args.workload == "uneven_batch":
         return bench_uneven(args, torch)
-    w, W = build_problem(args.workload, args.nt or None, args.shots or None)
+    w, W = build_problem(args.workload, args.nt or None, args.shots or None, args.order)
     b = device_setup(w, W, torch)
     cells = int(W.tnz) * int(W.tnx)
     sdo, npml = int(W.sdo), int(W.npml)
