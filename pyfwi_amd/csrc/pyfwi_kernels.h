@@ -20,6 +20,9 @@ constexpr int FUSED_WAVES = 4;      // fused two-stage kernels: wavefronts per b
 #ifndef MARCH_UNROLL
 #define MARCH_UNROLL 2              // rows per unrolled iteration: two rows of loads in flight per wave
 #endif
+#ifndef MARCH_SHOTS
+#define MARCH_SHOTS 4               // column-marching blocks: shots per block (one wave each)
+#endif
 #ifndef ADJ_VEL_UNROLL
 #define ADJ_VEL_UNROLL 1            // the adjoint velocity + imaging kernel: registers limit it
 #endif
@@ -35,7 +38,7 @@ struct DevGrid {
     float cx[4], cz[4];  // c_k / dx, c_k / dz
     float dt;
     // set per launch by the column-marching launchers
-    int mrows = 0, ncb = 0, nstrips = 0, nbatch = 0;
+    int mrows = 0, ncb = 0, nstrips = 0, nbatch = 0, nshot = 0;
     int rows_override = 0;  // strip height chosen by the caller (0: automatic)
 };
 

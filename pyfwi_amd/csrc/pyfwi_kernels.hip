@@ -261,31 +261,37 @@ struct Fifo {
     }
 };
 
-// Tile of this block and column / row strip of this thread.
+// Tile of this block and shot / column / row strip of this thread.
 //
-// Launches are 1-D.  The tiles (shot, column block, row strip) are numbered
-// shot fastest, then column block, then strip; workgroup w runs tile
+// A block is MARCH_X threads: MARCH_SHOTS shots side by side, each over
+// MARCH_X / MARCH_SHOTS columns, on one row strip.  With one wave per shot,
+// the block's waves read the same model cells (lambda, mu, b) at about the
+// same time, so the compute unit's L1 serves them to every shot after the
+// first.  Launches are 1-D.  The tiles (shot group, column block, row
+// strip) are numbered shot group fastest; workgroup w runs tile
 // (w % 8) * ceil(T/8) + w / 8.  Workgroups are handed to the 8 XCDs in turn,
 // so each XCD receives one contiguous run of tiles: the same cells of every
-// shot of the launch meet in one L2, which then serves the model arrays
-// (shared by all shots) once, and neighbouring column blocks, which share
-// halo columns, run side by side.
+// shot of the launch meet in one L2, and neighbouring column blocks, which
+// share halo columns, run side by side.
 template <int SDO>
 __device__ __forceinline__ bool tile(const DevGrid &g, int &s, int &j, int &i0, int &i1)
 {
+    constexpr int WX = MARCH_X / 64 / MARCH_SHOTS;  // waves of one shot along x
     const int T = g.ncb * g.nstrips * g.nbatch;
     const int per = (T + NUM_XCD - 1) / NUM_XCD;
     const int w = (int)blockIdx.x;
     const int logical = (w % NUM_XCD) * per + w / NUM_XCD;
     if (logical >= T) return false;
-    s = logical % g.nbatch;
+    const int grp = logical % g.nbatch;
     const int rest = logical / g.nbatch;
     const int cb = rest % g.ncb;
     const int st = rest / g.ncb;
-    j = SDO + cb * MARCH_X + (int)threadIdx.x;
+    const int wave = (int)threadIdx.x / 64, lane = (int)threadIdx.x % 64;
+    s = grp * MARCH_SHOTS + wave / WX;
+    j = SDO + (cb * WX + wave % WX) * 64 + lane;
     i0 = SDO + st * g.mrows;
     i1 = min(i0 + g.mrows - 1, g.nz - SDO);
-    return j <= g.nx - SDO;
+    return j <= g.nx - SDO && s < g.nshot;
 }
 
 // Prime a window for the first row i0 of a strip: rows i0-SDO .. i0+SDO-1.
@@ -1496,15 +1502,17 @@ static dim3 stencil_grid(const DevGrid &g, int sdo, int nshots)
 static dim3 march_grid(DevGrid &g, int sdo, int nshots)
 {
     const int nx_act = g.nx - 2 * sdo + 1, nz_act = g.nz - 2 * sdo + 1;
-    g.ncb = (nx_act + MARCH_X - 1) / MARCH_X;
+    constexpr int cols = MARCH_X / MARCH_SHOTS;  // columns of a block
+    g.ncb = (nx_act + cols - 1) / cols;
     const int64_t waves_per_row = (int64_t)((nx_act + 63) / 64) * nshots;
     int rows = (int)((waves_per_row * nz_act + TARGET_WAVES - 1) / TARGET_WAVES);
     rows = std::max(MARCH_ROWS_MIN, std::min(MARCH_ROWS_MAX, rows));
     if (g.rows_override > 0) rows = g.rows_override;
     g.mrows = rows;
     g.nstrips = (nz_act + rows - 1) / rows;
-    g.nbatch = nshots;
-    const int T = g.ncb * g.nstrips * nshots;
+    g.nbatch = (nshots + MARCH_SHOTS - 1) / MARCH_SHOTS;
+    g.nshot = nshots;
+    const int T = g.ncb * g.nstrips * g.nbatch;
     const int per = (T + NUM_XCD - 1) / NUM_XCD;
     return dim3(per * NUM_XCD);
 }
@@ -1554,8 +1562,9 @@ static dim3 fused_grid(DevGrid &g, int sdo, int nshots)
     if (g.rows_override > 0) rows = g.rows_override;
     g.mrows = rows;
     g.nstrips = (nz_act + rows - 1) / rows;
-    g.nbatch = nshots;
-    const int T = g.ncb * g.nstrips * nshots;
+    g.nbatch = nshots;  // fused tiles: one shot per block
+    g.nshot = nshots;
+    const int T = g.ncb * g.nstrips * g.nbatch;
     const int per = (T + NUM_XCD - 1) / NUM_XCD;
     return dim3(per * NUM_XCD);
 }
