@@ -11,7 +11,10 @@ namespace pyfwi {
 constexpr int HALO = 8;      // zero rows above and below each padded field (>= 2*SDO)
 constexpr int TILE_X = 64;   // one wavefront along x
 constexpr int TILE_Y = 4;    // 4 wavefronts per block along z
-constexpr int MARCH_X = 256;        // column-marching kernels: columns per block (4 waves)
+#ifndef MARCH_THREADS
+#define MARCH_THREADS 256           // column-marching kernels: threads per block (4 waves)
+#endif
+constexpr int MARCH_X = MARCH_THREADS;
 constexpr int MARCH_ROWS_MIN = 8;   // strip height bounds (rows walked by a thread)
 constexpr int MARCH_ROWS_MAX = 64;
 constexpr int TARGET_WAVES = 8192;  // wavefronts per launch to aim for (32 per CU)
@@ -21,7 +24,7 @@ constexpr int FUSED_WAVES = 4;      // fused two-stage kernels: wavefronts per b
 #define MARCH_UNROLL 2              // rows per unrolled iteration: two rows of loads in flight per wave
 #endif
 #ifndef MARCH_SHOTS
-#define MARCH_SHOTS 4               // column-marching blocks: shots per block (one wave each)
+#define MARCH_SHOTS 4               // column-marching blocks: shots per block (one wave each at 256 threads)
 #endif
 #ifndef ADJ_VEL_UNROLL
 #define ADJ_VEL_UNROLL 1            // the adjoint velocity + imaging kernel: registers limit it

@@ -264,10 +264,9 @@ struct Fifo {
 // Tile of this block and shot / column / row strip of this thread.
 //
 // A block is MARCH_X threads: MARCH_SHOTS shots side by side, each over
-// MARCH_X / MARCH_SHOTS columns, on one row strip.  With one wave per shot,
-// the block's waves read the same model cells (lambda, mu, b) at about the
-// same time, so the compute unit's L1 serves them to every shot after the
-// first.  Launches are 1-D.  The tiles (shot group, column block, row
+// MARCH_X / MARCH_SHOTS columns, on one row strip (one wave per shot by
+// default, so a block column is 64 cells wide and a narrow grid leaves few
+// lanes idle in its last column block).  Launches are 1-D.  The tiles (shot group, column block, row
 // strip) are numbered shot group fastest; workgroup w runs tile
 // (w % 8) * ceil(T/8) + w / 8.  Workgroups are handed to the 8 XCDs in turn,
 // so each XCD receives one contiguous run of tiles: the same cells of every
