@@ -183,19 +183,39 @@ def forward_rate(W, b, torch, tuning):
 
 
 def kernel_profile(W, b, torch, step):
-    """Average launch duration of each stencil kernel over one step, from HIP
-    events on the plan's stream."""
+    """In-sequence launch durations (total ms, launches) of the stencil kernels
+    over one gradient step, from HIP events on the plan's stream.
+
+    A recorded event stalls the stream for several microseconds, part of which
+    falls inside a bracket around one launch.  So one run brackets every launch
+    only to split each time loop between its kernels, and a second run
+    brackets only the forward and backward time loops, inside which the
+    kernels run as in an untimed call; each kernel gets its share of its
+    loop's time."""
     from pyfwi_amd import _native
     plan = b["plan"]
-    plan.set_timing(True)
+    plan.set_timing(_native.TIMING_LAUNCHES)
     step(W, b)
     torch.cuda.synchronize()
-    out = {}
+    per = {}
     for k, name in enumerate(_native.KERNEL_NAMES):
         ms, n = plan.kernel_timing(k)
         if n:
-            out[name] = (ms, n)
-    plan.set_timing(False)
+            per[name] = (ms, n)
+    plan.set_timing(_native.TIMING_LOOPS)
+    step(W, b)
+    torch.cuda.synchronize()
+    loops = {"fwd": plan.kernel_timing(_native.K_FWD_LOOP)[0],
+             "bwd": plan.kernel_timing(_native.K_BWD_LOOP)[0]}
+    plan.set_timing(0)
+    groups = {"fwd": ("fwd_vel", "fwd_str", "fwd_step"),
+              "bwd": ("rev_str", "rev_vel", "rev_step", "adj_str", "adj_vel_grad")}
+    out = {}
+    for loop, kinds in groups.items():
+        ks = [k for k in kinds if k in per]
+        total = sum(per[k][0] for k in ks)
+        for k in ks:
+            out[k] = (loops[loop] * per[k][0] / total, per[k][1])
     return out
 
 
@@ -445,6 +465,8 @@ def main():
                    "parallelism": "single GPU, shots batched per launch",
                    "tuning": {"shots_per_pass": tuning[0], "fused_steps": bool(tuning[1]),
                               "strip_rows": tuning[2]}},
+        "kernel_timing": "per-step time of each time loop (HIP events around the loops) split "
+                         "between its kernels by event-bracketed launches",
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -232,10 +232,15 @@ int32_t pyfwi_get_snapshot(pyfwi_plan *plan, int32_t shot, int32_t slot,
 /* Wait for all work queued on the plan's stream. */
 int32_t pyfwi_synchronize(pyfwi_plan *plan);
 
-/* Per-kernel timing with HIP events on the plan's stream.  While enabled,
- * every stencil launch is bracketed by an event pair; pyfwi_kernel_timing
- * returns the summed duration and launch count of one kernel kind since the
- * last enable.  Kinds: see PYFWI_K_* below. */
+/* Timing with HIP events on the plan's stream.  enable = 1: every stencil
+ * launch is bracketed by an event pair.  A recorded event stalls the stream
+ * for several microseconds, part of which falls inside the bracket, so these
+ * durations serve to split a loop's time between its kernels.  enable = 2:
+ * only the forward and backward time loops of each pass are bracketed
+ * (PYFWI_K_FWD_LOOP, PYFWI_K_BWD_LOOP, counted in time steps), and the kernels
+ * inside run as in an untimed call.  pyfwi_kernel_timing returns the summed
+ * duration and count of one kind since the last enable.  Kinds: see PYFWI_K_*
+ * below. */
 enum {
     PYFWI_K_FWD_VEL = 0,   /* velocity half step, forward             */
     PYFWI_K_FWD_STR = 1,   /* stress half step, forward               */
@@ -247,7 +252,9 @@ enum {
     PYFWI_K_EXA_VEL = 7,   /* exact adjoint of the velocity half step */
     PYFWI_K_FWD_STEP = 8,  /* fused forward step (velocities + stresses) */
     PYFWI_K_REV_STEP = 9,  /* fused time-reversed step                   */
-    PYFWI_K_COUNT = 10
+    PYFWI_K_FWD_LOOP = 10, /* forward time loop of a pass (enable = 2)   */
+    PYFWI_K_BWD_LOOP = 11, /* backward time loop of a pass (enable = 2)  */
+    PYFWI_K_COUNT = 12
 };
 int32_t pyfwi_plan_set_timing(pyfwi_plan *plan, int32_t enable);
 int32_t pyfwi_kernel_timing(pyfwi_plan *plan, int32_t kind, double *total_ms,

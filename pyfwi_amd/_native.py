@@ -21,9 +21,10 @@ REC_TABLE = 1
 REC_VZ_FROM_VX = 1
 
 (K_FWD_VEL, K_FWD_STR, K_REV_STR, K_REV_VEL, K_ADJ_STR, K_ADJ_VEL, K_EXA_STR, K_EXA_VEL,
- K_FWD_STEP, K_REV_STEP) = range(10)
+ K_FWD_STEP, K_REV_STEP, K_FWD_LOOP, K_BWD_LOOP) = range(12)
 KERNEL_NAMES = ["fwd_vel", "fwd_str", "rev_str", "rev_vel", "adj_str", "adj_vel_grad",
                 "exact_str", "exact_vel", "fwd_step", "rev_step"]
+TIMING_LAUNCHES, TIMING_LOOPS = 1, 2
 
 # exported symbols, in header order (checked by the CPU test suite)
 EXPORTS = [
@@ -203,8 +204,10 @@ class Plan:
     def synchronize(self):
         check(self.lib.pyfwi_synchronize(self.h))
 
-    def set_timing(self, enable):
-        check(self.lib.pyfwi_plan_set_timing(self.h, 1 if enable else 0))
+    def set_timing(self, mode):
+        """0 off, TIMING_LAUNCHES (or True) every stencil launch, TIMING_LOOPS
+        the time loops (see pyfwi_plan_set_timing)."""
+        check(self.lib.pyfwi_plan_set_timing(self.h, int(mode)))
 
     def kernel_timing(self, kind):
         ms = ctypes.c_double()

@@ -115,9 +115,10 @@ struct pyfwi_plan {
     // Otherwise a gather launch records and the residual is added in memory.
     bool rec_inline = true;
     // timing
-    bool timing = false;
+    int timing = 0;  // 0 off, 1 every stencil launch, 2 the time loops
     std::vector<hipEvent_t> ev_pool;
     std::vector<int> ev_kind;
+    std::vector<int> ev_count;  // steps a bracket covers (1 for a launch)
     size_t ev_used = 0;
     double kt_ms[PYFWI_K_COUNT] = {0};
     int64_t kt_n[PYFWI_K_COUNT] = {0};
@@ -134,13 +135,15 @@ int max_nt(const std::vector<pyfwi_shot> &s)
     return m;
 }
 
-// Bracket a launch with an event pair when timing is on.
+// Bracket a stencil launch (timing mode 1) or a time loop of `count` steps
+// (mode 2) with an event pair.
 struct Timed {
     pyfwi_plan *p;
     int kind;
     size_t slot = 0;
     bool on;
-    Timed(pyfwi_plan *p_, int k) : p(p_), kind(k), on(p_->timing)
+    Timed(pyfwi_plan *p_, int k, int mode = 1, int count = 1)
+        : p(p_), kind(k), on(p_->timing == mode)
     {
         if (!on) return;
         if (p->ev_used + 2 > p->ev_pool.size()) {
@@ -149,11 +152,13 @@ struct Timed {
                 if (hipEventCreate(&e) != hipSuccess) { on = false; return; }
                 p->ev_pool.push_back(e);
                 p->ev_kind.push_back(-1);
+                p->ev_count.push_back(1);
             }
         }
         slot = p->ev_used;
         p->ev_used += 2;
         p->ev_kind[slot] = kind;
+        p->ev_count[slot] = count;
         (void)hipEventRecord(p->ev_pool[slot], p->stream);
     }
     ~Timed()
@@ -173,7 +178,7 @@ int harvest_timing(pyfwi_plan *p)
         int kind = p->ev_kind[k];
         if (kind >= 0 && kind < PYFWI_K_COUNT) {
             p->kt_ms[kind] += ms;
-            p->kt_n[kind] += 1;
+            p->kt_n[kind] += p->ev_count[k];
         }
     }
     p->ev_used = 0;
@@ -616,6 +621,7 @@ int forward_pass(pyfwi_plan *p, int g0, int ng, const float *d_src, float *d_sei
     const int nt = max_nt(sh);
     const int nchp = (int)cl.size();
     if (int e = prepare_fields(p, ng)) return e;
+    Timed loop_tm(p, PYFWI_K_FWD_LOOP, 2, nt);
     // every step kept (chpr=100): the two-launch kernels step from one slot
     // into the next, so the kept states cost no extra traffic
     if (snaps && keeps_every_step(cl, nt) && !p->tune_fused && p->src_inside) {
@@ -735,6 +741,7 @@ int gradient_pyfwi(pyfwi_plan *p, int g0, int ng, const float *d_res, bool preco
     // every step kept: read the forward state of step t from its slot
     const bool direct = keeps_every_step(cl, nt);
     int chpc = nchp - 1;
+    Timed loop_tm(p, PYFWI_K_BWD_LOOP, 2, nt - 1);
     for (int t = nt - 1; t > 0; --t) {
         const float *fwd = p->cur;
         int64_t fwd_stride = fps;
@@ -1074,7 +1081,8 @@ int32_t pyfwi_plan_set_timing(pyfwi_plan *p, int32_t enable)
 {
     if (!p) return fail(PYFWI_EINVAL, "null plan");
     if (int e = harvest_timing(p)) return e;
-    p->timing = enable != 0;
+    if (enable < 0 || enable > 2) return fail(PYFWI_EINVAL, "timing mode must be 0, 1 or 2");
+    p->timing = enable;
     for (int k = 0; k < PYFWI_K_COUNT; ++k) {
         p->kt_ms[k] = 0;
         p->kt_n[k] = 0;

@@ -240,3 +240,24 @@ def test_sparse_terms_off_active_cells(gpu):
     gr = O.gradient(st, w.model, res, states, parameterization="lmd")
     for k in gr:
         assert rel(g[k], gr[k]) < GRAD_TOL, k
+
+
+def test_loop_timing(gpu):
+    """Timing mode 2 brackets only the time loops: one forward and one
+    backward bracket per pass, counted in steps, and no launch brackets."""
+    from pyfwi_amd import _native
+    w = _small(nt=120)
+    W = WavePropagator(w.inpa, w.src, w.rec_loc, w.model_shape, components=2, chpr=100)
+    d_obs = WavePropagator(w.inpa, w.src, w.rec_loc, w.model_shape,
+                           components=2).forward_modeling(w.model_obs)
+    W._plan.set_tuning(3, False, 0)   # passes of 3 and 2 shots
+    W._plan.set_timing(_native.TIMING_LOOPS)
+    W.misfit_gradient_device(w.model, d_obs, parameterization="lmd")
+    fms, fn = W._plan.kernel_timing(_native.K_FWD_LOOP)
+    bms, bn = W._plan.kernel_timing(_native.K_BWD_LOOP)
+    assert (fn, bn) == (2 * int(W.nt), 2 * (int(W.nt) - 1))
+    assert fms > 0 and bms > 0
+    assert W._plan.kernel_timing(_native.K_ADJ_VEL)[1] == 0
+    with pytest.raises(_native.NativeError):
+        W._plan.set_timing(3)
+    W._plan.set_timing(0)
