@@ -199,7 +199,7 @@ def kernel_profile(W, b, torch, step):
     torch.cuda.synchronize()
     per = {}
     for k, name in enumerate(_native.KERNEL_NAMES):
-        ms, n = plan.kernel_timing(k)
+        ms, n = plan.kernel_timing(k) if name else (0.0, 0)
         if n:
             per[name] = (ms, n)
     plan.set_timing(_native.TIMING_LOOPS)
@@ -208,14 +208,22 @@ def kernel_profile(W, b, torch, step):
     loops = {"fwd": plan.kernel_timing(_native.K_FWD_LOOP)[0],
              "bwd": plan.kernel_timing(_native.K_BWD_LOOP)[0]}
     plan.set_timing(0)
-    groups = {"fwd": ("fwd_vel", "fwd_str", "fwd_step"),
-              "bwd": ("rev_str", "rev_vel", "rev_step", "adj_str", "adj_vel_grad")}
+    groups = {"fwd": ("fwd_vel", "fwd_str", "fwd_step", "fwd_sparse"),
+              "bwd": ("rev_str", "rev_vel", "rev_step", "adj_str", "adj_vel_grad", "bwd_sparse")}
     out = {}
     for loop, kinds in groups.items():
         ks = [k for k in kinds if k in per]
-        total = sum(per[k][0] for k in ks)
+        if not ks:
+            continue
+        # the stall a bracket adds is about the same for every launch: take it
+        # out of each kind (per launch), then share the loop's time
+        stall = max(0.0, (sum(per[k][0] for k in ks) - loops[loop])
+                    / sum(per[k][1] for k in ks))
+        net = {k: max(per[k][0] - per[k][1] * stall, 0.0) for k in ks}
+        total = sum(net.values()) or 1.0
         for k in ks:
-            out[k] = (loops[loop] * per[k][0] / total, per[k][1])
+            if not k.endswith("_sparse"):  # the small gather / injection launches
+                out[k] = (loops[loop] * net[k] / total, per[k][1])
     return out
 
 

@@ -254,7 +254,9 @@ enum {
     PYFWI_K_REV_STEP = 9,  /* fused time-reversed step                   */
     PYFWI_K_FWD_LOOP = 10, /* forward time loop of a pass (enable = 2)   */
     PYFWI_K_BWD_LOOP = 11, /* backward time loop of a pass (enable = 2)  */
-    PYFWI_K_COUNT = 12
+    PYFWI_K_FWD_SPARSE = 12, /* record gather / source injection launches  */
+    PYFWI_K_BWD_SPARSE = 13, /* residual injection launches                */
+    PYFWI_K_COUNT = 14
 };
 int32_t pyfwi_plan_set_timing(pyfwi_plan *plan, int32_t enable);
 int32_t pyfwi_kernel_timing(pyfwi_plan *plan, int32_t kind, double *total_ms,

@@ -21,9 +21,10 @@ REC_TABLE = 1
 REC_VZ_FROM_VX = 1
 
 (K_FWD_VEL, K_FWD_STR, K_REV_STR, K_REV_VEL, K_ADJ_STR, K_ADJ_VEL, K_EXA_STR, K_EXA_VEL,
- K_FWD_STEP, K_REV_STEP, K_FWD_LOOP, K_BWD_LOOP) = range(12)
+ K_FWD_STEP, K_REV_STEP, K_FWD_LOOP, K_BWD_LOOP, K_FWD_SPARSE, K_BWD_SPARSE) = range(14)
 KERNEL_NAMES = ["fwd_vel", "fwd_str", "rev_str", "rev_vel", "adj_str", "adj_vel_grad",
-                "exact_str", "exact_vel", "fwd_step", "rev_step"]
+                "exact_str", "exact_vel", "fwd_step", "rev_step", None, None,
+                "fwd_sparse", "bwd_sparse"]
 TIMING_LAUNCHES, TIMING_LOOPS = 1, 2
 
 # exported symbols, in header order (checked by the CPU test suite)

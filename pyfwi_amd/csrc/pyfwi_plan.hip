@@ -483,7 +483,10 @@ int forward_step(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t, const floa
     const DevGrid &g = p->dg;
     const int64_t fps = fields_per_shot(p);
     if (!p->tune_fused && !p->src_inside) {
-        launch_inject_record(g, ng, dsh, t, p->cur, fps, src, seis, p->d_rtab, p->stream);
+        {
+            Timed ts(p, PYFWI_K_FWD_SPARSE);
+            launch_inject_record(g, ng, dsh, t, p->cur, fps, src, seis, p->d_rtab, p->stream);
+        }
         {
             Timed tm(p, PYFWI_K_FWD_VEL);
             launch_vel(g, +1, ng, dsh, t, p->cur, fps, p->cur, fps, p->bu.p, p->dxp, p->dzp,
@@ -493,15 +496,20 @@ int forward_step(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t, const floa
         launch_str(g, +1, ng, dsh, t, p->cur, fps, p->cur, fps, p->lam.p, p->mu.p, p->dxp, p->dzp,
                    p->stream);
     } else if (p->tune_fused) {
-        launch_inject_record(g, ng, dsh, t, p->cur, fps, src, seis, p->d_rtab, p->stream);
+        {
+            Timed ts(p, PYFWI_K_FWD_SPARSE);
+            launch_inject_record(g, ng, dsh, t, p->cur, fps, src, seis, p->d_rtab, p->stream);
+        }
         float *nxt = p->cur == p->fields.p ? p->fields2.p : p->fields.p;
         Timed tm(p, PYFWI_K_FWD_STEP);
         launch_fused_step(g, +1, ng, dsh, t, p->cur, nxt, p->bu.p, p->lam.p, p->mu.p, p->dxp,
                           p->dzp, p->stream);
         p->cur = nxt;
     } else {
-        if (!p->rec_inline)
+        if (!p->rec_inline) {
+            Timed ts(p, PYFWI_K_FWD_SPARSE);
             launch_inject_record(g, ng, dsh, t, p->cur, fps, nullptr, seis, p->d_rtab, p->stream);
+        }
         {
             Timed tm(p, PYFWI_K_FWD_VEL);
             launch_vel(g, +1, ng, dsh, t, p->cur, fps, p->cur, fps, p->bu.p, p->dxp, p->dzp,
@@ -523,6 +531,7 @@ int forward_step_kept(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t, const
 {
     const DevGrid &g = p->dg;
     if (!p->rec_inline) {
+        Timed ts(p, PYFWI_K_FWD_SPARSE);
         launch_inject_record(g, ng, dsh, t, const_cast<float *>(in), in_stride, nullptr, seis,
                              p->d_rtab, p->stream);
         seis = nullptr;
@@ -762,10 +771,13 @@ int gradient_pyfwi(pyfwi_plan *p, int g0, int ng, const float *d_res, bool preco
         // except for shots starting here); with receivers off the active
         // cells or from a table, in memory every step
         const float *r_on = p->rec_inline ? d_res : nullptr;
-        if (!p->rec_inline)
+        if (!p->rec_inline) {
+            Timed ts(p, PYFWI_K_BWD_SPARSE);
             launch_adj_inject(g, ng, dsh, t, p->adj.p, d_res, p->d_rtab, p->stream);
-        else if (first_step[t])
+        } else if (first_step[t]) {
+            Timed ts(p, PYFWI_K_BWD_SPARSE);
             launch_adj_inject(g, ng, dsh, t, p->adj.p, d_res, p->d_rtab, p->stream, 0, 2, true);
+        }
         {
             Timed tm(p, PYFWI_K_ADJ_STR);
             launch_adj_str(g, ng, dsh, t, fwd, fwd_stride, p->adj.p, p->bu.p, p->dxp, p->dzp,
