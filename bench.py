@@ -61,10 +61,11 @@ def log(msg):
 
 
 def kernel_source_sha1():
-    import glob
+    """SHA-1 of the kernel sources, which alone set the HBM bytes a launch
+    moves (the plan code around them does not)."""
     import hashlib
-    files = sorted(glob.glob(os.path.join(ROOT, "pyfwi_amd", "csrc", "*.hip"))
-                   + glob.glob(os.path.join(ROOT, "pyfwi_amd", "csrc", "*.h")))
+    files = [os.path.join(ROOT, "pyfwi_amd", "csrc", n)
+             for n in ("pyfwi_kernels.hip", "pyfwi_kernels.h")]
     return hashlib.sha1(b"".join(open(f, "rb").read() for f in files)).hexdigest()
 
 
