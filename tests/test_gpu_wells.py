@@ -130,3 +130,25 @@ def test_misfit_gradient_wells(gpu, acq_type):
         assert e < 1e-5, (k, e)
     for name in W.seismogram:
         assert rel(W.seismogram[name], W2.seismogram[name]) < SEIS_TOL
+
+
+def test_receiver_table_checks(gpu):
+    """The plan rejects table entries outside the grid and shots whose
+    entries run past the table, before any kernel runs."""
+    from pyfwi_amd import _native
+    W, st, model, _ = wells(0, nt=50, ns=1)
+    plan = W._plan
+    ncell = int(W.tnz) * int(W.tnx)
+    for bad in ([0, ncell], [-2, 5]):
+        with pytest.raises(_native.NativeError):
+            plan.set_receivers(np.array(bad, np.int32))
+    W.forward_modeling(model)                      # the valid table is still in place
+    plan.set_receivers(np.zeros(0, np.int32))      # cleared: table shots now fail
+    with pytest.raises(_native.NativeError):
+        W.forward_modeling(model)
+    plan.set_receivers(W.receiver_table().ravel())
+    sh = W._shots([0])
+    sh[0].rec_table = 1                            # one entry past the end
+    with pytest.raises(_native.NativeError):
+        plan.forward(sh, W._source_table(), np.zeros(5 * int(W.nt_ext) * int(W.nr), np.float32),
+                     np.zeros(0, np.int32), 0, _native.MEM_HOST)
