@@ -27,8 +27,16 @@ constexpr int FUSED_WAVES = 4;      // fused two-stage kernels: wavefronts per b
 #define MARCH_SHOTS 4               // column-marching blocks: shots per block (one wave each at 256 threads)
 #endif
 #ifndef ADJ_VEL_UNROLL
-#define ADJ_VEL_UNROLL 1            // the adjoint velocity + imaging kernel: registers limit it
+#define ADJ_VEL_UNROLL 2            // rows per unrolled iteration of the adjoint velocity + imaging kernel
 #endif
+#ifndef ADJ_GROUP_ROWS
+#define ADJ_GROUP_ROWS 2            // adjoint kernels: rows per block-wide sum of the imaging terms
+#endif
+
+// Gradient accumulator sets that the reference-mode adjoint kernels fill for
+// `nshots` shots of one pass: one per group of MARCH_SHOTS shots (the shots
+// of a block sum their imaging terms before they reach memory).
+inline int adj_acc_sets(int nshots) { return (nshots + MARCH_SHOTS - 1) / MARCH_SHOTS; }
 
 // Constants every kernel takes by value.
 struct DevGrid {
@@ -84,7 +92,7 @@ void launch_adj_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, in
                        int f0 = 0, int f1 = 5, bool first_only = false);
 // Adjoint stresses + the lambda / mu imaging terms; adjoint velocities + the
 // rho imaging term.  `fields`: the forward state of the step (per-shot
-// stride fields_stride), gacc: [nshots][6][fstride].  With `res`, the stress
+// stride fields_stride), gacc: [adj_acc_sets(nshots)][6][fstride].  With `res`, the stress
 // update adds step t's residual to the adjoint stresses at the receivers and
 // the velocity update writes its adjoint velocities with step t-1's residual
 // added (launch_adj_inject(..., 0, 2, true) covers each shot's first step);
@@ -113,7 +121,7 @@ void launch_exact_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, 
                          hipStream_t st);
 void launch_copy_shots(float *dst, int64_t dst_stride, const float *src, int64_t src_stride,
                        int64_t count, int nshots, hipStream_t st);
-void launch_grad_sum(const DevGrid &g, int nshots, int nslot, const float *gacc, float *out,
+void launch_grad_sum(const DevGrid &g, int nsets, int nslot, const float *gacc, float *out,
                      const float *bu, int b_to_rho, int accumulate, hipStream_t st);
 void launch_pad(const DevGrid &g, const float *src, float *dst, hipStream_t st);
 void launch_unpad(const DevGrid &g, const float *src, float *dst, int nf, hipStream_t st);

@@ -272,6 +272,15 @@ struct Fifo {
 // so each XCD receives one contiguous run of tiles: the same cells of every
 // shot of the launch meet in one L2, and neighbouring column blocks, which
 // share halo columns, run side by side.
+// Index of the calling wave in its block, in a scalar register: the compiler
+// cannot tell on its own that threadIdx.x / 64 is the same for a whole wave,
+// and would otherwise keep the shot index and every pointer derived from it
+// in vector registers.
+__device__ __forceinline__ int wave_id()
+{
+    return __builtin_amdgcn_readfirstlane((int)threadIdx.x / 64);
+}
+
 template <int SDO>
 __device__ __forceinline__ bool tile(const DevGrid &g, int &s, int &j, int &i0, int &i1)
 {
@@ -285,7 +294,7 @@ __device__ __forceinline__ bool tile(const DevGrid &g, int &s, int &j, int &i0, 
     const int rest = logical / g.nbatch;
     const int cb = rest % g.ncb;
     const int st = rest / g.ncb;
-    const int wave = (int)threadIdx.x / 64, lane = (int)threadIdx.x % 64;
+    const int wave = wave_id(), lane = (int)threadIdx.x % 64;
     s = grp * MARCH_SHOTS + wave / WX;
     j = SDO + (cb * WX + wave % WX) * 64 + lane;
     i0 = SDO + st * g.mrows;
@@ -613,7 +622,7 @@ __device__ __forceinline__ bool fused_tile(const DevGrid &g, FusedTile<SDO> &ft)
     const int rest = logical / g.ncb;
     const int st = rest % g.nstrips;
     ft.s = rest / g.nstrips;
-    const int wave = (int)threadIdx.x / 64, lane = (int)threadIdx.x % 64;
+    const int wave = wave_id(), lane = (int)threadIdx.x % 64;
     const int span = 64 - 2 * SDO;
     const int x0 = SDO + (cb * FUSED_WAVES + wave) * span;  // first owned column
     if (x0 > g.nx - SDO) return false;                      // whole wavefront idle
@@ -936,8 +945,70 @@ __global__ void k_adj_inject(DevGrid g, const pyfwi_shot *shots, int t,
 // part, which needs the new adjoint velocities and the forward stresses'
 // derivatives, in the adjoint velocity update.  Each kernel then reads about
 // the same number of arrays.  The shear term uses the z-derivative of vz as
-// the reference does (elastic.cl:485-486).  gacc: [nshots][6][fstride] =
-// gmu, gmu_pre, glam, glam_pre, grho, grho_pre; interior cells only.
+// the reference does (elastic.cl:485-486).
+//
+// The shots of a block (one wave each, same cells) add their terms up before
+// they reach memory: each wave stages a shot's terms for ADJ_GROUP_ROWS rows
+// in LDS, and after a barrier the live waves share those rows, each adding
+// the shots' terms in shot order onto the group's accumulators.  The
+// accumulators then move once per group instead of once per shot.  gacc:
+// [adj_acc_sets(nshots)][6][fstride] = gmu, gmu_pre, glam, glam_pre, grho,
+// grho_pre; interior cells only.
+
+// The shots of the calling wave's group that take part in step t (waves of
+// the others have left the kernel), and the wave's rank among them.
+struct GroupLive {
+    uint32_t mask = 0;
+    int count = 0, rank = 0;
+};
+
+__device__ __forceinline__ GroupLive group_live(const DevGrid &g, const pyfwi_shot *shots, int s,
+                                                int t)
+{
+    static_assert(MARCH_X == 64 * MARCH_SHOTS, "the shot-group sum takes one wave per shot");
+    const int wave = wave_id(), s0 = s - wave;
+    GroupLive r;
+#pragma unroll
+    for (int w = 0; w < MARCH_SHOTS; ++w)
+        if (s0 + w < g.nshot && bwd_active(shots, s0 + w, t)) r.mask |= 1u << w;
+    r.count = __popc(r.mask);
+    r.rank = __popc(r.mask & ((1u << wave) - 1u));
+    return r;
+}
+
+// Adds the staged terms of rows ic .. ic+ADJ_GROUP_ROWS-1 (part: [row][shot]
+// [term][lane]) onto the group accumulators G + slot[v]*fs; rows are dealt
+// out to the live waves by rank.  `c0`: byte offset of (row ic, the lane's
+// column).  Called by every live wave after the barrier that follows the
+// staging.
+template <int NV>
+__device__ __forceinline__ void add_group_terms(const float *part, const GroupLive &gl, int ic,
+                                                int i1, bool img, int lo, int hi, float *G,
+                                                const int *slot, int64_t fs, uint32_t c0,
+                                                uint32_t RB)
+{
+    const int lane = (int)threadIdx.x % 64;
+    for (int r = gl.rank; r < ADJ_GROUP_ROWS; r += gl.count) {
+        const int i = ic + r;
+        if (i > i1) break;
+        if (!img || i < lo || i > hi) continue;
+        const uint32_t o = c0 + (uint32_t)r * RB;
+        float cur[NV], acc[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            cur[v] = cell_ref(G + slot[v] * fs, o);
+            acc[v] = 0.0f;
+        }
+#pragma unroll
+        for (int w = 0; w < MARCH_SHOTS; ++w) {
+            if (!(gl.mask >> w & 1u)) continue;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[v] += part[((r * MARCH_SHOTS + w) * NV + v) * 64 + lane];
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) cell_ref(G + slot[v] * fs, o) = cur[v] + acc[v];
+    }
+}
 
 // Adjoint stresses from adjoint velocities (src/PyFWI/elastic.cl:372-411) and
 // the lambda / mu imaging terms.
@@ -949,9 +1020,13 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot
                                                      const float *__restrict__ dzp, float *gacc,
                                                      const float *__restrict__ res)
 {
+    constexpr int NV = PRECOND ? 4 : 2;  // staged terms: mu, lam (, mu_pre, lam_pre)
+    __shared__ float part[2][ADJ_GROUP_ROWS * MARCH_SHOTS * NV * 64];
     int s, j, i0, i1;
     if (!tile<SDO>(g, s, j, i0, i1)) return;
     if (!bwd_active(shots, s, t)) return;
+    const GroupLive gl = group_live(g, shots, s, t);
+    const int wave = wave_id(), lane = (int)threadIdx.x % 64;
     const int64_t fs = g.fstride;
     const uint32_t RB = (uint32_t)g.pitch * 4u;
     float *A = adj + (int64_t)s * 5 * fs + g.origin;
@@ -968,7 +1043,8 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot
     const bool img = j >= lo && j <= hix && i1 >= lo && i0 <= hi;
     const float *F = fields + (int64_t)s * fields_stride + g.origin;
     const CellRef vx{F}, vz{F + fs}, txx{F + 2 * fs}, tzz{F + 3 * fs}, txz{F + 4 * fs};
-    float *G = gacc + (int64_t)s * 6 * fs + g.origin;
+    float *G = gacc + (int64_t)(s / MARCH_SHOTS) * 6 * fs + g.origin;
+    constexpr int slots[4] = {0, 2, 1, 3};  // accumulator of each staged term
     ZWin<SDO> wbz, wbx;  // b*avz for d/dz-, b*avx for d/dz+
     Fifo<SDO> fxp, fxm;  // d/dx+ (b*avx), d/dx- (b*avz)
     ZWin<SDO> fvz;       // forward vz (d/dz+ and d/dz-)
@@ -995,84 +1071,95 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot
     }
     const float ex = dxp[j];
     const float dt = g.dt;
-    // Every load of a row is issued before the row's stores: the compiler
-    // does not move a load across a store to memory it cannot prove
-    // distinct, and a load placed after one waits out a full memory latency.
+    int buf = 0;
+    for (int ic = i0; ic <= i1; ic += ADJ_GROUP_ROWS, buf ^= 1) {
+        const uint32_t c0 = cb;
+        float *mine = &part[buf][wave * NV * 64 + lane];
+        // Every load of a row is issued before the row's stores: the compiler
+        // does not move a load across a store to memory it cannot prove
+        // distinct, and a load placed after one waits out a full memory latency.
 #pragma unroll MARCH_UNROLL
-    for (int i = i0; i <= i1; ++i, cb += RB) {
-        const uint32_t n = cb + (uint32_t)SDO * RB;
-        wbz.push(avz(n) * B(n));
-        wbx.push(avx(n) * B(n));
-        const float dxpb = fxp.pop(dx_bvx(n));
-        const float dxmb = fxm.pop(dx_bvz(n));
-        const float ez = dzp[i];
-        float *pa = &cell_ref(A + 2 * fs, cb), *pb = &cell_ref(A + 3 * fs, cb),
-              *pd = &cell_ref(A + 4 * fs, cb);
-        const float a0 = *pa, b0 = *pb, d0 = *pd;
-        const bool row_img = img && i >= lo && i <= hi;
-        float vz_n = 0.0f, g_mu = 0.0f, g_lam = 0.0f;
-        float sxx = 0.0f, szz = 0.0f, sxz = 0.0f, p_mu = 0.0f, p_lam = 0.0f;
-        Taps<SDO, 1> tvz;  // forward vz on the entering row (d/dx-)
-        Taps<SDO, 0> tvx;  // forward vx on the centre row (d/dx+)
-        if (img) {
-            vz_n = vz(n);
-            tvz.load(vz, n);
-        }
-        if (row_img) {
-            tvx.load(vx, cb);
-            g_mu = cell_ref(G, cb);
-            g_lam = cell_ref(G + 2 * fs, cb);
+        for (int r = 0; r < ADJ_GROUP_ROWS; ++r, cb += RB) {
+            const int i = ic + r;
+            if (i > i1) break;
+            const uint32_t n = cb + (uint32_t)SDO * RB;
+            wbz.push(avz(n) * B(n));
+            wbx.push(avx(n) * B(n));
+            const float dxpb = fxp.pop(dx_bvx(n));
+            const float dxmb = fxm.pop(dx_bvz(n));
+            const float ez = dzp[i];
+            float *pa = &cell_ref(A + 2 * fs, cb), *pb = &cell_ref(A + 3 * fs, cb),
+                  *pd = &cell_ref(A + 4 * fs, cb);
+            const float a0 = *pa, b0 = *pb, d0 = *pd;
+            const bool row_img = img && i >= lo && i <= hi;
+            float vz_n = 0.0f;
+            float sxx = 0.0f, szz = 0.0f, sxz = 0.0f;
+            Taps<SDO, 1> tvz;  // forward vz on the entering row (d/dx-)
+            Taps<SDO, 0> tvx;  // forward vx on the centre row (d/dx+)
+            if (img) {
+                vz_n = vz(n);
+                tvz.load(vz, n);
+            }
+            if (row_img) {
+                tvx.load(vx, cb);
+                if constexpr (PRECOND) {
+                    sxx = txx(cb);
+                    szz = tzz(cb);
+                    sxz = txz(cb);
+                }
+            }
+            // (lanes outside the imaging region push values they never use)
+            fvz.push(vz_n);
+            const float dxm_vz = fdvz.pop(xminus<SDO>(tvz, g.cx));
+            const float dxp_vx = xplus<SDO>(tvx, g.cx);
+            const float dzmb = wbz.dminus(g.cz);
+            const float dzpb = wbx.dplus(g.cz);
+            const bool at_rec = i == rrow;
+            const float a = at_rec ? a0 + r_txx : a0;
+            const float bb = at_rec ? b0 + r_tzz : b0;
+            const float d = at_rec ? d0 + r_txz : d0;
+            const float aa = fmaf(-dt, ex * a + dxpb, a);
+            const float ab = fmaf(-dt, ez * bb + dzmb, bb);
+            const float ad = fmaf(-dt, (ex + ez) * d + (dzpb + dxmb), d);
+            *pa = aa;
+            *pb = ab;
+            *pd = ad;
+            if (!row_img) continue;
+            const float dzp_vz = fvz.dplus(g.cz);
+            const float dzm_vz = fvz.dminus(g.cz);
+            float *q = mine + r * MARCH_SHOTS * NV * 64;
+            q[0] = 2.0f * dt * (aa * dxp_vx + ab * dzm_vz) + dt * ad * (dzp_vz + dxm_vz);
+            q[64] = dt * (aa + ab) * (dxp_vx + dzm_vz);
             if constexpr (PRECOND) {
-                sxx = txx(cb);
-                szz = tzz(cb);
-                sxz = txz(cb);
-                p_mu = cell_ref(G + fs, cb);
-                p_lam = cell_ref(G + 3 * fs, cb);
+                const float dt2 = dt * dt;
+                q[128] = 4.0f * dt2 * (sxx * sxx * dxp_vx * dxp_vx + szz * szz * dzm_vz * dzm_vz)
+                         + dt2 * sxz * sxz * (dzp_vz * dzp_vz + dxm_vz * dxm_vz);
+                q[192] = dt2 * (sxx * sxx + szz * szz) * (dxp_vx * dxp_vx + dzm_vz * dzm_vz);
             }
         }
-        // (lanes outside the imaging region push values they never use)
-        fvz.push(vz_n);
-        const float dxm_vz = fdvz.pop(xminus<SDO>(tvz, g.cx));
-        const float dxp_vx = xplus<SDO>(tvx, g.cx);
-        const float dzmb = wbz.dminus(g.cz);
-        const float dzpb = wbx.dplus(g.cz);
-        const bool at_rec = i == rrow;
-        const float a = at_rec ? a0 + r_txx : a0;
-        const float bb = at_rec ? b0 + r_tzz : b0;
-        const float d = at_rec ? d0 + r_txz : d0;
-        const float aa = fmaf(-dt, ex * a + dxpb, a);
-        const float ab = fmaf(-dt, ez * bb + dzmb, bb);
-        const float ad = fmaf(-dt, (ex + ez) * d + (dzpb + dxmb), d);
-        *pa = aa;
-        *pb = ab;
-        *pd = ad;
-        if (!row_img) continue;
-        const float dzp_vz = fvz.dplus(g.cz);
-        const float dzm_vz = fvz.dminus(g.cz);
-        cell_ref(G, cb) = g_mu + (2.0f * dt * (aa * dxp_vx + ab * dzm_vz) + dt * ad * (dzp_vz + dxm_vz));
-        cell_ref(G + 2 * fs, cb) = g_lam + dt * (aa + ab) * (dxp_vx + dzm_vz);
-        if constexpr (PRECOND) {
-            const float dt2 = dt * dt;
-            cell_ref(G + fs, cb) = p_mu +
-                (4.0f * dt2 * (sxx * sxx * dxp_vx * dxp_vx + szz * szz * dzm_vz * dzm_vz)
-                 + dt2 * sxz * sxz * (dzp_vz * dzp_vz + dxm_vz * dxm_vz));
-            cell_ref(G + 3 * fs, cb) = p_lam +
-                dt2 * (sxx * sxx + szz * szz) * (dxp_vx * dxp_vx + dzm_vz * dzm_vz);
-        }
+        __syncthreads();
+        add_group_terms<NV>(part[buf], gl, ic, i1, img, lo, hi, G, slots, fs, c0, RB);
     }
 }
 
 // Adjoint velocities (src/PyFWI/elastic.cl:419-468) and the rho imaging term.
+#ifndef ADJ_VEL_ATTR
+#define ADJ_VEL_ATTR
+#endif
 template <int SDO, bool PRECOND>
-__global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
+__global__ __launch_bounds__(MARCH_X) ADJ_VEL_ATTR void k_adj_vel_grad(
     DevGrid g, const pyfwi_shot *shots, int t, const float *fields, int64_t fields_stride,
     float *adj, const float *__restrict__ lam, const float *__restrict__ mu,
     const float *__restrict__ bu, const float *__restrict__ dxp,
     const float *__restrict__ dzp, float *gacc, const float *__restrict__ res)
 {
+    constexpr int NV = PRECOND ? 2 : 1;  // staged terms: rho (, rho_pre)
+    __shared__ float part[2][ADJ_GROUP_ROWS * MARCH_SHOTS * NV * 64];
     int s, j, i0, i1;
     if (!tile<SDO>(g, s, j, i0, i1)) return;
     if (!bwd_active(shots, s, t)) return;
+    const GroupLive gl = group_live(g, shots, s, t);
+    const int wave = wave_id(), lane = (int)threadIdx.x % 64;
     const int64_t fs = g.fstride;
     const uint32_t RB = (uint32_t)g.pitch * 4u;
     float *A = adj + (int64_t)s * 5 * fs + g.origin;
@@ -1090,7 +1177,8 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
     const bool img = j >= lo && j <= hix && i1 >= lo && i0 <= hi;
     const float *F = fields + (int64_t)s * fields_stride + g.origin;
     const CellRef vx{F}, vz{F + fs}, txx{F + 2 * fs}, tzz{F + 3 * fs}, txz{F + 4 * fs};
-    float *G = gacc + (int64_t)s * 6 * fs + g.origin;
+    float *G = gacc + (int64_t)(s / MARCH_SHOTS) * 6 * fs + g.origin;
+    constexpr int slots[2] = {4, 5};  // accumulator of each staged term
 
     ZWin<SDO> wa3, wb1, wb2;  // mu*atxz (d/dz-), (lam+2mu)*atzz, lam*atxx (d/dz+)
     ZWin<SDO> ftzz, ftxz;     // forward tzz (d/dz+), txz (d/dz-)
@@ -1128,61 +1216,68 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
     }
     const float ex = dxp[j];
     const float dt = g.dt;
-    // every load of a row before the row's stores (see k_adj_str)
+    int buf = 0;
+    for (int ic = i0; ic <= i1; ic += ADJ_GROUP_ROWS, buf ^= 1) {
+        const uint32_t c0 = cb;
+        float *mine = &part[buf][wave * NV * 64 + lane];
+        // every load of a row before the row's stores (see k_adj_str)
 #pragma unroll ADJ_VEL_UNROLL
-    for (int i = i0; i <= i1; ++i, cb += RB) {
-        const uint32_t n = cb + (uint32_t)SDO * RB;
-        wa3.push(q_a3(n));
-        wb1.push(q_b1(n));
-        wb2.push(q_b2(n));
-        const float a1 = fa1.pop(x_a1(n)), a2 = fa2.pop(x_a2(n));
-        const float b3 = fb3.pop(x_b3(n));
-        const float ez = dzp[i];
-        float *px = &cell_ref(A, cb), *pz = &cell_ref(A + fs, cb);
-        float ux = *px, uz = *pz;
-        const bool row_img = img && i >= lo && i <= hi;
-        float tzz_n = 0.0f, txz_n = 0.0f, bb = 0.0f, g_rho = 0.0f;
-        float wx = 0.0f, wz = 0.0f, p_rho = 0.0f;
-        Taps<SDO, 0> ttxz;  // forward txz on the entering row (d/dx+)
-        Taps<SDO, 1> ttxx;  // forward txx on the centre row (d/dx-)
-        if (img) {
-            tzz_n = tzz(n);
-            txz_n = txz(n);
-            ttxz.load(txz, n);
-        }
-        if (row_img) {
-            ttxx.load(txx, cb);
-            bb = B(cb);
-            g_rho = cell_ref(G + 4 * fs, cb);
+        for (int r = 0; r < ADJ_GROUP_ROWS; ++r, cb += RB) {
+            const int i = ic + r;
+            if (i > i1) break;
+            const uint32_t n = cb + (uint32_t)SDO * RB;
+            wa3.push(q_a3(n));
+            wb1.push(q_b1(n));
+            wb2.push(q_b2(n));
+            const float a1 = fa1.pop(x_a1(n)), a2 = fa2.pop(x_a2(n));
+            const float b3 = fb3.pop(x_b3(n));
+            const float ez = dzp[i];
+            float *px = &cell_ref(A, cb), *pz = &cell_ref(A + fs, cb);
+            float ux = *px, uz = *pz;
+            const bool row_img = img && i >= lo && i <= hi;
+            float tzz_n = 0.0f, txz_n = 0.0f, bb = 0.0f;
+            float wx = 0.0f, wz = 0.0f;
+            Taps<SDO, 0> ttxz;  // forward txz on the entering row (d/dx+)
+            Taps<SDO, 1> ttxx;  // forward txx on the centre row (d/dx-)
+            if (img) {
+                tzz_n = tzz(n);
+                txz_n = txz(n);
+                ttxz.load(txz, n);
+            }
+            if (row_img) {
+                ttxx.load(txx, cb);
+                bb = B(cb);
+                if constexpr (PRECOND) {
+                    wx = vx(cb);
+                    wz = vz(cb);
+                }
+            }
+            // (lanes outside the imaging region push values they never use)
+            ftzz.push(tzz_n);
+            ftxz.push(txz_n);
+            const float dxp_txz = fdtxz.pop(xplus<SDO>(ttxz, g.cx));
+            const float dxm_txx = xminus<SDO>(ttxx, g.cx);
+            const float a3 = wa3.dminus(g.cz);
+            const float b1 = wb1.dplus(g.cz);
+            const float b2 = wb2.dplus(g.cz);
+            ux = fmaf(-dt, (2.0f * ex + ez) * ux + (a1 + a2 + a3), ux);
+            uz = fmaf(-dt, (ex + 2.0f * ez) * uz + (b2 + b1 + b3), uz);
+            const bool at_rec = i == rrow;
+            *px = at_rec ? ux + r_vx : ux;
+            *pz = at_rec ? uz + r_vz : uz;
+            if (!row_img) continue;
+            const float dzp_tzz = ftzz.dplus(g.cz);
+            const float dzm_txz = ftxz.dminus(g.cz);
+            float *q = mine + r * MARCH_SHOTS * NV * 64;
+            q[0] = -(dt * (bb * bb)) * (ux * (dxm_txx + dzm_txz) + uz * (dzp_tzz + dxp_txz));
             if constexpr (PRECOND) {
-                wx = vx(cb);
-                wz = vz(cb);
-                p_rho = cell_ref(G + 5 * fs, cb);
+                const float dt2 = dt * dt;
+                q[64] = dt2 * (wx * wx * (dxm_txx * dxm_txx + dzm_txz * dzm_txz)
+                               + wz * wz * (dzp_tzz * dzp_tzz + dxp_txz * dxp_txz));
             }
         }
-        // (lanes outside the imaging region push values they never use)
-        ftzz.push(tzz_n);
-        ftxz.push(txz_n);
-        const float dxp_txz = fdtxz.pop(xplus<SDO>(ttxz, g.cx));
-        const float dxm_txx = xminus<SDO>(ttxx, g.cx);
-        const float a3 = wa3.dminus(g.cz);
-        const float b1 = wb1.dplus(g.cz);
-        const float b2 = wb2.dplus(g.cz);
-        ux = fmaf(-dt, (2.0f * ex + ez) * ux + (a1 + a2 + a3), ux);
-        uz = fmaf(-dt, (ex + 2.0f * ez) * uz + (b2 + b1 + b3), uz);
-        const bool at_rec = i == rrow;
-        *px = at_rec ? ux + r_vx : ux;
-        *pz = at_rec ? uz + r_vz : uz;
-        if (!row_img) continue;
-        const float dzp_tzz = ftzz.dplus(g.cz);
-        const float dzm_txz = ftxz.dminus(g.cz);
-        cell_ref(G + 4 * fs, cb) = g_rho +
-            -(dt * (bb * bb)) * (ux * (dxm_txx + dzm_txz) + uz * (dzp_tzz + dxp_txz));
-        if constexpr (PRECOND) {
-            const float dt2 = dt * dt;
-            cell_ref(G + 5 * fs, cb) = p_rho + dt2 * (wx * wx * (dxm_txx * dxm_txx + dzm_txz * dzm_txz)
-                                                      + wz * wz * (dzp_tzz * dzp_tzz + dxp_txz * dxp_txz));
-        }
+        __syncthreads();
+        add_group_terms<NV>(part[buf], gl, ic, i1, img, lo, hi, G, slots, fs, c0, RB);
     }
 }
 
@@ -1435,12 +1530,12 @@ __global__ void k_copy_shots(float *dst, int64_t dst_stride, const float *src,
         b[k] = a[k];
 }
 
-// Sum per-shot accumulators into the unpadded output [nslot][tnz*tnx];
-// exact mode converts dJ/db (slot 4) to dJ/drho = -b^2 dJ/db.  With
-// `accumulate` the shots are added one by one onto the running output, so
-// the sum is the same, bit for bit, however the shots were split into
-// passes (the converted slot adds its pass total instead).
-__global__ void k_grad_sum(DevGrid g, int nshots, int nslot, const float *gacc,
+// Sum `nsets` accumulator sets (one per shot in exact mode, one per shot
+// group in reference mode) into the unpadded output [nslot][tnz*tnx]; exact
+// mode converts dJ/db (slot 4) to dJ/drho = -b^2 dJ/db.  With `accumulate`
+// the sets are added one by one onto the running output (the converted slot
+// adds its pass total instead).
+__global__ void k_grad_sum(DevGrid g, int nsets, int nslot, const float *gacc,
                            float *out, const float *__restrict__ bu, int b_to_rho, int accumulate)
 {
     const int64_t n = (int64_t)g.nz * g.nx;
@@ -1453,7 +1548,7 @@ __global__ void k_grad_sum(DevGrid g, int nshots, int nslot, const float *gacc,
         const int64_t p = g.origin + (int64_t)i * g.pitch + j;
         const bool conv = b_to_rho && slot == 4;
         float acc = accumulate && !conv ? out[k] : 0.0f;
-        for (int s = 0; s < nshots; ++s) acc += gacc[((int64_t)s * 6 + slot) * fs + p];
+        for (int s = 0; s < nsets; ++s) acc += gacc[((int64_t)s * 6 + slot) * fs + p];
         if (conv) {
             acc = -(bu[p] * bu[p]) * acc;
             if (accumulate) acc += out[k];
@@ -1664,10 +1759,10 @@ void launch_copy_shots(float *dst, int64_t dst_stride, const float *src, int64_t
                        src, src_stride, count);
 }
 
-void launch_grad_sum(const DevGrid &g, int nshots, int nslot, const float *gacc, float *out,
+void launch_grad_sum(const DevGrid &g, int nsets, int nslot, const float *gacc, float *out,
                      const float *bu, int b_to_rho, int accumulate, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_grad_sum, dim3(1024), dim3(256), 0, st, g, nshots, nslot, gacc, out,
+    hipLaunchKernelGGL(k_grad_sum, dim3(1024), dim3(256), 0, st, g, nsets, nslot, gacc, out,
                        bu, b_to_rho, accumulate);
 }
 

@@ -5,7 +5,8 @@
 //   fields  [nshots][5][fstride]  forward state of every shot of the batch
 //   adj     [nshots][5][fstride]  adjoint state
 //   av      2 x [nshots][2][fstride]  exact mode velocity adjoint ping-pong
-//   gacc    [nshots][6][fstride]  per-shot gradient accumulators
+//   gacc    [nsets][6][fstride]  gradient accumulators: one set per shot (exact
+//           mode) or per group of MARCH_SHOTS shots (reference mode)
 //   snaps   [nshots][nchp][5][fstride]  reference checkpoints (pyfwi mode)
 //   ckpts   [nshots][ncp][5][fstride]   exact-mode checkpoints every K steps
 //   seg     [nshots][K][5][fstride]     exact-mode recomputed segment
@@ -726,7 +727,7 @@ int32_t pyfwi_forward(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, co
 
 namespace {
 
-// Reference adjoint of one pass (shots g0 .. g0+ng-1) into gacc[ng].
+// Reference adjoint of one pass (shots g0 .. g0+ng-1) into gacc[adj_acc_sets(ng)].
 // `snaps` holds the pass's kept states, per-shot stride cl.size() states, at
 // the steps of `cl`.
 int gradient_pyfwi(pyfwi_plan *p, int g0, int ng, const float *d_res, bool precond,
@@ -899,8 +900,11 @@ int32_t pyfwi_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, c
     const bool reuse = mode == PYFWI_MODE_EXACT && same && p->ckpts_all && p->ckpt_every == K;
     for (int g0 = 0; g0 < nshots; g0 += G) {
         const int ng = std::min(G, nshots - g0);
-        if (int e = ensure(p->gacc, (size_t)ng * 6 * g.fstride, p->stream, "gradient")) return e;
-        HIPCHK(hipMemsetAsync(p->gacc.p, 0, (size_t)ng * 6 * g.fstride * sizeof(float), p->stream));
+        const int nsets = mode == PYFWI_MODE_PYFWI ? adj_acc_sets(ng) : ng;
+        if (int e = ensure(p->gacc, (size_t)nsets * 6 * g.fstride, p->stream, "gradient"))
+            return e;
+        HIPCHK(hipMemsetAsync(p->gacc.p, 0, (size_t)nsets * 6 * g.fstride * sizeof(float),
+                              p->stream));
         if (mode == PYFWI_MODE_PYFWI) {
             const float *sn = p->snaps.p + (int64_t)g0 * p->chp.size() * fps;
             if (int e = gradient_pyfwi(p, g0, ng, d_res, (flags & PYFWI_F_PRECOND) != 0, sn,
@@ -925,7 +929,7 @@ int32_t pyfwi_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, c
                 if (int e = gradient_exact(p, g0, ng, d_src, d_res, p->ckpts.p, ncp, K)) return e;
             }
         }
-        launch_grad_sum(g, ng, 6, p->gacc.p, d_grad, p->bu.p, mode == PYFWI_MODE_EXACT,
+        launch_grad_sum(g, nsets, 6, p->gacc.p, d_grad, p->bu.p, mode == PYFWI_MODE_EXACT,
                         g0 > 0, p->stream);
     }
     HIPCHK(hipGetLastError());
@@ -1021,12 +1025,15 @@ int32_t pyfwi_misfit_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t ns
             return e;
         launch_l2_residual(ng, p->d_shots + g0, d_seis, d_obs, p->resid.p, comp_mask, p->d_misfit,
                            p->stream);
-        if (int e = ensure(p->gacc, (size_t)ng * 6 * g.fstride, p->stream, "gradient")) return e;
-        HIPCHK(hipMemsetAsync(p->gacc.p, 0, (size_t)ng * 6 * g.fstride * sizeof(float), p->stream));
+        const int nsets = adj_acc_sets(ng);
+        if (int e = ensure(p->gacc, (size_t)nsets * 6 * g.fstride, p->stream, "gradient"))
+            return e;
+        HIPCHK(hipMemsetAsync(p->gacc.p, 0, (size_t)nsets * 6 * g.fstride * sizeof(float),
+                              p->stream));
         if (int e = gradient_pyfwi(p, g0, ng, p->resid.p, (flags & PYFWI_F_PRECOND) != 0,
                                    p->snaps.p, cl_pass))
             return e;
-        launch_grad_sum(g, ng, 6, p->gacc.p, d_grad, p->bu.p, 0, g0 > 0, p->stream);
+        launch_grad_sum(g, nsets, 6, p->gacc.p, d_grad, p->bu.p, 0, g0 > 0, p->stream);
         g0 += ng;
     }
     HIPCHK(hipGetLastError());
