@@ -20,6 +20,7 @@ Prints ONE JSON line.  --gpus above 1 is rejected: this build runs on one GPU.
 """
 import argparse
 import json
+import math
 import os
 import subprocess
 import sys
@@ -35,25 +36,29 @@ from pyfwi_amd.wave_propagation import checkpoint_steps  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes of one launch (FP32 words read + written once per cell):
-# (bytes per cell of the whole grid, extra bytes per cell of the interior where
-# the imaging condition runs).
+# (bytes per cell of the whole grid and shot, extra bytes per cell of the
+# interior where the imaging condition runs and shot, bytes per interior cell
+# and shot group).  The reference-mode gradient accumulators are shared by
+# the SHOTS_PER_BLOCK shots of a block, which add their terms up first.
 KERNEL_BYTES = {
-    "fwd_vel": (32, 0),       # R txx tzz txz vx vz b, W vx vz
-    "fwd_str": (40, 0),       # R vx vz txx tzz txz lam mu, W txx tzz txz
-    "rev_str": (40, 0),
-    "rev_vel": (32, 0),
-    "adj_str": (36, 24),      # all: R avx avz b 3 atau, W 3 atau; interior: R fwd vx vz, RW g_mu g_lam
-    "adj_vel_grad": (36, 24),  # all: R 3 atau lam mu 2 av, W 2 av; interior: R fwd 3 tau, b, RW g_rho
-    "exact_str": (48, 0),
-    "exact_vel": (52, 0),
-    "fwd_step": (52, 0),       # R 5 fields lam mu b, W 5 fields
-    "rev_step": (52, 0),
+    "fwd_vel": (32, 0, 0),       # R txx tzz txz vx vz b, W vx vz
+    "fwd_str": (40, 0, 0),       # R vx vz txx tzz txz lam mu, W txx tzz txz
+    "rev_str": (40, 0, 0),
+    "rev_vel": (32, 0, 0),
+    "adj_str": (36, 8, 16),      # all: R avx avz b 3 atau, W 3 atau; interior: R fwd vx vz; group: RW g_mu g_lam
+    "adj_vel_grad": (36, 16, 8),  # all: R 3 atau lam mu 2 av, W 2 av; interior: R fwd 3 tau, b; group: RW g_rho
+    "exact_str": (48, 0, 0),
+    "exact_vel": (52, 0, 0),
+    "fwd_step": (52, 0, 0),       # R 5 fields lam mu b, W 5 fields
+    "rev_step": (52, 0, 0),
 }
+SHOTS_PER_BLOCK = 4
 
 
 def launch_bytes(kernel, cells, interior, shots):
-    a, b = KERNEL_BYTES[kernel]
-    return (a * cells + b * interior) * shots
+    a, b, c = KERNEL_BYTES[kernel]
+    groups = math.ceil(shots / SHOTS_PER_BLOCK)
+    return (a * cells + b * interior) * shots + c * interior * groups
 
 
 def log(msg):
