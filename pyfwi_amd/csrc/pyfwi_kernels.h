@@ -17,7 +17,12 @@ constexpr int TILE_Y = 4;    // 4 wavefronts per block along z
 constexpr int MARCH_X = MARCH_THREADS;
 constexpr int MARCH_ROWS_MIN = 8;   // strip height bounds (rows walked by a thread)
 constexpr int MARCH_ROWS_MAX = 64;
-constexpr int TARGET_WAVES = 8192;  // wavefronts per launch to aim for (32 per CU)
+// Wavefronts with work per CU that one launch aims for; the strip height is
+// chosen to match.  Measured on MI355X (540 x 1040 cells, 10 shots a launch,
+// all four stencil kernels): 20 per CU (18-row strips) runs 5-10 % faster
+// than 22-30 per CU (12-17-row strips), although the forward kernels'
+// registers allow 32; 13-17 per CU loses 2-6 % again.
+constexpr int MARCH_WAVES_PER_CU = 20;
 constexpr int NUM_XCD = 8;          // MI355X: 8 XCDs, each with its own L2
 constexpr int FUSED_WAVES = 4;      // fused two-stage kernels: wavefronts per block
 #ifndef MARCH_UNROLL
@@ -48,6 +53,7 @@ struct DevGrid {
     int64_t origin;      // offset of cell (0, 0) inside a padded field
     float cx[4], cz[4];  // c_k / dx, c_k / dz
     float dt;
+    int ncu = 256;       // compute units of the device
     // set per launch by the column-marching launchers
     int mrows = 0, ncb = 0, nstrips = 0, nbatch = 0, nshot = 0;
     int rows_override = 0;  // strip height chosen by the caller (0: automatic)

@@ -1012,7 +1012,7 @@ __device__ __forceinline__ void add_group_terms(const float *part, const GroupLi
 
 // Adjoint stresses from adjoint velocities (src/PyFWI/elastic.cl:372-411) and
 // the lambda / mu imaging terms.
-template <int SDO, bool PRECOND>
+template <int SDO, bool PRECOND, bool ALONE>
 __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot *shots, int t,
                                                      const float *fields, int64_t fields_stride,
                                                      float *adj, const float *__restrict__ bu,
@@ -1071,6 +1071,9 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot
     }
     const float ex = dxp[j];
     const float dt = g.dt;
+    // ALONE (launches of one shot): the shot adds its terms onto the
+    // accumulators itself, with no staging and no barrier; the sums are the
+    // same as a group of one gets through the staging.
     int buf = 0;
     for (int ic = i0; ic <= i1; ic += ADJ_GROUP_ROWS, buf ^= 1) {
         const uint32_t c0 = cb;
@@ -1100,8 +1103,13 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot
                 vz_n = vz(n);
                 tvz.load(vz, n);
             }
+            float gc[NV] = {};  // the row's accumulators (ALONE)
             if (row_img) {
                 tvx.load(vx, cb);
+                if constexpr (ALONE) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) gc[v] = cell_ref(G + slots[v] * fs, cb);
+                }
                 if constexpr (PRECOND) {
                     sxx = txx(cb);
                     szz = tzz(cb);
@@ -1127,18 +1135,26 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot
             if (!row_img) continue;
             const float dzp_vz = fvz.dplus(g.cz);
             const float dzm_vz = fvz.dminus(g.cz);
-            float *q = mine + r * MARCH_SHOTS * NV * 64;
-            q[0] = 2.0f * dt * (aa * dxp_vx + ab * dzm_vz) + dt * ad * (dzp_vz + dxm_vz);
-            q[64] = dt * (aa + ab) * (dxp_vx + dzm_vz);
+            float term[NV];
+            term[0] = 2.0f * dt * (aa * dxp_vx + ab * dzm_vz) + dt * ad * (dzp_vz + dxm_vz);
+            term[1] = dt * (aa + ab) * (dxp_vx + dzm_vz);
             if constexpr (PRECOND) {
                 const float dt2 = dt * dt;
-                q[128] = 4.0f * dt2 * (sxx * sxx * dxp_vx * dxp_vx + szz * szz * dzm_vz * dzm_vz)
-                         + dt2 * sxz * sxz * (dzp_vz * dzp_vz + dxm_vz * dxm_vz);
-                q[192] = dt2 * (sxx * sxx + szz * szz) * (dxp_vx * dxp_vx + dzm_vz * dzm_vz);
+                term[2] = 4.0f * dt2 * (sxx * sxx * dxp_vx * dxp_vx + szz * szz * dzm_vz * dzm_vz)
+                          + dt2 * sxz * sxz * (dzp_vz * dzp_vz + dxm_vz * dxm_vz);
+                term[3] = dt2 * (sxx * sxx + szz * szz) * (dxp_vx * dxp_vx + dzm_vz * dzm_vz);
+            }
+            float *q = mine + r * MARCH_SHOTS * NV * 64;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                if constexpr (ALONE) cell_ref(G + slots[v] * fs, cb) = gc[v] + term[v];
+                else q[v * 64] = term[v];
             }
         }
-        __syncthreads();
-        add_group_terms<NV>(part[buf], gl, ic, i1, img, lo, hi, G, slots, fs, c0, RB);
+        if constexpr (!ALONE) {
+            __syncthreads();
+            add_group_terms<NV>(part[buf], gl, ic, i1, img, lo, hi, G, slots, fs, c0, RB);
+        }
     }
 }
 
@@ -1146,7 +1162,7 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot
 #ifndef ADJ_VEL_ATTR
 #define ADJ_VEL_ATTR
 #endif
-template <int SDO, bool PRECOND>
+template <int SDO, bool PRECOND, bool ALONE>
 __global__ __launch_bounds__(MARCH_X) ADJ_VEL_ATTR void k_adj_vel_grad(
     DevGrid g, const pyfwi_shot *shots, int t, const float *fields, int64_t fields_stride,
     float *adj, const float *__restrict__ lam, const float *__restrict__ mu,
@@ -1216,6 +1232,7 @@ __global__ __launch_bounds__(MARCH_X) ADJ_VEL_ATTR void k_adj_vel_grad(
     }
     const float ex = dxp[j];
     const float dt = g.dt;
+    // ALONE: the shot adds its terms itself (see k_adj_str)
     int buf = 0;
     for (int ic = i0; ic <= i1; ic += ADJ_GROUP_ROWS, buf ^= 1) {
         const uint32_t c0 = cb;
@@ -1244,9 +1261,14 @@ __global__ __launch_bounds__(MARCH_X) ADJ_VEL_ATTR void k_adj_vel_grad(
                 txz_n = txz(n);
                 ttxz.load(txz, n);
             }
+            float gc[NV] = {};  // the row's accumulators (ALONE)
             if (row_img) {
                 ttxx.load(txx, cb);
                 bb = B(cb);
+                if constexpr (ALONE) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) gc[v] = cell_ref(G + slots[v] * fs, cb);
+                }
                 if constexpr (PRECOND) {
                     wx = vx(cb);
                     wz = vz(cb);
@@ -1268,16 +1290,24 @@ __global__ __launch_bounds__(MARCH_X) ADJ_VEL_ATTR void k_adj_vel_grad(
             if (!row_img) continue;
             const float dzp_tzz = ftzz.dplus(g.cz);
             const float dzm_txz = ftxz.dminus(g.cz);
-            float *q = mine + r * MARCH_SHOTS * NV * 64;
-            q[0] = -(dt * (bb * bb)) * (ux * (dxm_txx + dzm_txz) + uz * (dzp_tzz + dxp_txz));
+            float term[NV];
+            term[0] = -(dt * (bb * bb)) * (ux * (dxm_txx + dzm_txz) + uz * (dzp_tzz + dxp_txz));
             if constexpr (PRECOND) {
                 const float dt2 = dt * dt;
-                q[64] = dt2 * (wx * wx * (dxm_txx * dxm_txx + dzm_txz * dzm_txz)
-                               + wz * wz * (dzp_tzz * dzp_tzz + dxp_txz * dxp_txz));
+                term[1] = dt2 * (wx * wx * (dxm_txx * dxm_txx + dzm_txz * dzm_txz)
+                                 + wz * wz * (dzp_tzz * dzp_tzz + dxp_txz * dxp_txz));
+            }
+            float *q = mine + r * MARCH_SHOTS * NV * 64;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                if constexpr (ALONE) cell_ref(G + slots[v] * fs, cb) = gc[v] + term[v];
+                else q[v * 64] = term[v];
             }
         }
-        __syncthreads();
-        add_group_terms<NV>(part[buf], gl, ic, i1, img, lo, hi, G, slots, fs, c0, RB);
+        if constexpr (!ALONE) {
+            __syncthreads();
+            add_group_terms<NV>(part[buf], gl, ic, i1, img, lo, hi, G, slots, fs, c0, RB);
+        }
     }
 }
 
@@ -1589,23 +1619,32 @@ static dim3 stencil_grid(const DevGrid &g, int sdo, int nshots)
     return dim3((nx_act + TILE_X - 1) / TILE_X, (nz_act + TILE_Y - 1) / TILE_Y, nshots);
 }
 
-// 1-D grid of (column block, row strip, shot) tiles, XCD-interleaved (see
-// tile()).  The strip height is chosen per launch so that a launch carries
-// about TARGET_WAVES wavefronts: tall strips (fewer halo rows) when many shots
-// are batched, short strips when a single shot must still fill the GPU.
+// Strip height for a launch of `per_strip` working wavefronts per row strip:
+// as many strips as keep the launch within MARCH_WAVES_PER_CU such waves per
+// CU (tall strips, fewer halo rows, when many shots are batched; short strips
+// when a single shot must still fill the GPU), within the row bounds.  Waves
+// of a block without a shot leave at once and are not counted.
+static int strip_rows(const DevGrid &g, int nz_act, int64_t per_strip)
+{
+    if (g.rows_override > 0) return g.rows_override;
+    const int64_t room = (int64_t)MARCH_WAVES_PER_CU * g.ncu;
+    const int nstrips = (int)std::max<int64_t>(1, std::min<int64_t>(nz_act, room / per_strip));
+    const int rows = (nz_act + nstrips - 1) / nstrips;
+    return std::max(MARCH_ROWS_MIN, std::min(MARCH_ROWS_MAX, rows));
+}
+
+// 1-D grid of (column block, row strip, shot group) tiles, XCD-interleaved
+// (see tile()).
 static dim3 march_grid(DevGrid &g, int sdo, int nshots)
 {
     const int nx_act = g.nx - 2 * sdo + 1, nz_act = g.nz - 2 * sdo + 1;
     constexpr int cols = MARCH_X / MARCH_SHOTS;  // columns of a block
     g.ncb = (nx_act + cols - 1) / cols;
-    const int64_t waves_per_row = (int64_t)((nx_act + 63) / 64) * nshots;
-    int rows = (int)((waves_per_row * nz_act + TARGET_WAVES - 1) / TARGET_WAVES);
-    rows = std::max(MARCH_ROWS_MIN, std::min(MARCH_ROWS_MAX, rows));
-    if (g.rows_override > 0) rows = g.rows_override;
-    g.mrows = rows;
-    g.nstrips = (nz_act + rows - 1) / rows;
     g.nbatch = (nshots + MARCH_SHOTS - 1) / MARCH_SHOTS;
     g.nshot = nshots;
+    const int rows = strip_rows(g, nz_act, (int64_t)g.ncb * (MARCH_X / 64 / MARCH_SHOTS) * nshots);
+    g.mrows = rows;
+    g.nstrips = (nz_act + rows - 1) / rows;
     const int T = g.ncb * g.nstrips * g.nbatch;
     const int per = (T + NUM_XCD - 1) / NUM_XCD;
     return dim3(per * NUM_XCD);
@@ -1650,14 +1689,11 @@ static dim3 fused_grid(DevGrid &g, int sdo, int nshots)
     const int nx_act = g.nx - 2 * sdo + 1, nz_act = g.nz - 2 * sdo + 1;
     const int span = FUSED_WAVES * (64 - 2 * sdo);
     g.ncb = (nx_act + span - 1) / span;
-    const int64_t waves_per_row = (int64_t)g.ncb * FUSED_WAVES * nshots;
-    int rows = (int)((waves_per_row * nz_act + TARGET_WAVES - 1) / TARGET_WAVES);
-    rows = std::max(MARCH_ROWS_MIN, std::min(MARCH_ROWS_MAX, rows));
-    if (g.rows_override > 0) rows = g.rows_override;
-    g.mrows = rows;
-    g.nstrips = (nz_act + rows - 1) / rows;
     g.nbatch = nshots;  // fused tiles: one shot per block
     g.nshot = nshots;
+    const int rows = strip_rows(g, nz_act, (int64_t)g.ncb * FUSED_WAVES * nshots);
+    g.mrows = rows;
+    g.nstrips = (nz_act + rows - 1) / rows;
     const int T = g.ncb * g.nstrips * g.nbatch;
     const int per = (T + NUM_XCD - 1) / NUM_XCD;
     return dim3(per * NUM_XCD);
@@ -1698,9 +1734,12 @@ void launch_adj_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t
 {
     DevGrid gm = g;
     dim3 grid = march_grid(gm, g.sdo, nshots), block(MARCH_X);
+    const bool alone = nshots == 1;  // no other shot to share the accumulators with
 #define ARGS gm, shots, t, fields, fields_stride, adj, bu, dxp, dzp, gacc, res
-    if (precond) DISPATCH_SDO(g.sdo, k_adj_str, , true);
-    else DISPATCH_SDO(g.sdo, k_adj_str, , false);
+    if (precond && alone) DISPATCH_SDO(g.sdo, k_adj_str, , true, true);
+    else if (precond) DISPATCH_SDO(g.sdo, k_adj_str, , true, false);
+    else if (alone) DISPATCH_SDO(g.sdo, k_adj_str, , false, true);
+    else DISPATCH_SDO(g.sdo, k_adj_str, , false, false);
 #undef ARGS
 }
 
@@ -1712,9 +1751,12 @@ void launch_adj_vel_grad(const DevGrid &g, int nshots, const pyfwi_shot *shots, 
 {
     DevGrid gm = g;
     dim3 grid = march_grid(gm, g.sdo, nshots), block(MARCH_X);
+    const bool alone = nshots == 1;
 #define ARGS gm, shots, t, fields, fields_stride, adj, lam, mu, bu, dxp, dzp, gacc, res
-    if (precond) DISPATCH_SDO(g.sdo, k_adj_vel_grad, , true);
-    else DISPATCH_SDO(g.sdo, k_adj_vel_grad, , false);
+    if (precond && alone) DISPATCH_SDO(g.sdo, k_adj_vel_grad, , true, true);
+    else if (precond) DISPATCH_SDO(g.sdo, k_adj_vel_grad, , true, false);
+    else if (alone) DISPATCH_SDO(g.sdo, k_adj_vel_grad, , false, true);
+    else DISPATCH_SDO(g.sdo, k_adj_vel_grad, , false, false);
 #undef ARGS
 }
 

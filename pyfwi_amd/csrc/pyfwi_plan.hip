@@ -329,8 +329,9 @@ int32_t pyfwi_plan_create(const pyfwi_grid *grid, void *stream, pyfwi_plan **out
     if (g.npml < 0) return fail(PYFWI_EINVAL, "npml must be >= 0");
     if (!(g.dt > 0) || !(g.dx > 0) || !(g.dz > 0) || !std::isfinite(g.dt))
         return fail(PYFWI_EINVAL, "dt, dx, dz must be positive");
-    int dev = 0;
+    int dev = 0, ncu = 0;
     HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     pyfwi_plan *p = new pyfwi_plan();
     p->grid = g;
     p->device = dev;
@@ -351,6 +352,7 @@ int32_t pyfwi_plan_create(const pyfwi_grid *grid, void *stream, pyfwi_plan **out
         d.cz[k] = (float)(g.c[k] / g.dz);
     }
     d.dt = (float)g.dt;
+    if (ncu > 0) d.ncu = ncu;
     if (stream) {
         p->stream = (hipStream_t)stream;
     } else {
