@@ -125,6 +125,8 @@ void launch_exact_vel(const DevGrid &g, int nshots, const pyfwi_shot *shots, int
 void launch_exact_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
                          float *adj, float *av, const float *res, const int32_t *rtab,
                          hipStream_t st);
+// Zero n floats from p on stream st.
+void launch_zero(float *p, int64_t n, hipStream_t st);
 void launch_copy_shots(float *dst, int64_t dst_stride, const float *src, int64_t src_stride,
                        int64_t count, int nshots, hipStream_t st);
 void launch_grad_sum(const DevGrid &g, int nsets, int nslot, const float *gacc, float *out,

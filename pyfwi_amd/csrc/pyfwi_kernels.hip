@@ -1548,6 +1548,19 @@ void launch_l2_residual(int nshots, const pyfwi_shot *shots, const float *seis, 
 // ------------------------------------------------------------- utilities
 
 // Copy `count` floats per shot (count % 4 == 0, 16-byte aligned bases).
+// Zero n floats from p (p 16-byte aligned): a grid-stride fill of 16-byte
+// stores, which clears the large per-pass buffers at close to copy speed.
+__global__ void k_zero(float *p, int64_t n)
+{
+    const int64_t n4 = n / 4;
+    float4 *q = reinterpret_cast<float4 *>(p);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n4; k += stride)
+        q[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int64_t k = 4 * n4 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += stride)
+        p[k] = 0.0f;
+}
+
 __global__ void k_copy_shots(float *dst, int64_t dst_stride, const float *src,
                              int64_t src_stride, int64_t count)
 {
@@ -1788,6 +1801,17 @@ void launch_exact_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, 
 {
     hipLaunchKernelGGL(k_exact_inject, dim3(nshots), dim3(256), 0, st, g, shots, t, adj,
                        av, res, rtab);
+}
+
+void launch_zero(float *p, int64_t n, hipStream_t st)
+{
+    if (n <= 0) return;
+    if (reinterpret_cast<uintptr_t>(p) % 16) {  // a caller's buffer off the 16-byte grid
+        (void)hipMemsetAsync(p, 0, (size_t)n * sizeof(float), st);
+        return;
+    }
+    const int64_t blocks = std::min<int64_t>((n / 4 + 255) / 256 + 1, 8192);
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, st, p, n);
 }
 
 void launch_copy_shots(float *dst, int64_t dst_stride, const float *src, int64_t src_stride,

@@ -69,7 +69,7 @@ int ensure(DevBuf &b, size_t n, hipStream_t st, const char *what)
                                       std::to_string(n * sizeof(float)) + " bytes)");
     }
     b.n = n;
-    HIPCHK(hipMemsetAsync(b.p, 0, n * sizeof(float), st));
+    launch_zero(b.p, (int64_t)n, st);  // halos and padding must read as zero
     return 0;
 }
 
@@ -615,9 +615,9 @@ int prepare_fields(pyfwi_plan *p, int ng)
     if (int e = ensure(p->fields, (size_t)ng * fps, p->stream, "fields")) return e;
     if (p->tune_fused)
         if (int e = ensure(p->fields2, (size_t)ng * fps, p->stream, "fields")) return e;
-    HIPCHK(hipMemsetAsync(p->fields.p, 0, (size_t)ng * fps * sizeof(float), p->stream));
+    launch_zero(p->fields.p, (int64_t)ng * fps, p->stream);
     if (p->tune_fused)
-        HIPCHK(hipMemsetAsync(p->fields2.p, 0, (size_t)ng * fps * sizeof(float), p->stream));
+        launch_zero(p->fields2.p, (int64_t)ng * fps, p->stream);
     p->cur = p->fields.p;
     return 0;
 }
@@ -704,7 +704,7 @@ int32_t pyfwi_forward(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, co
         if (int e = ensure(p->seis_stage, nrec, p->stream, "records")) return e;
         d_seis = p->seis_stage.p;
     }
-    HIPCHK(hipMemsetAsync(d_seis, 0, nrec * sizeof(float), p->stream));
+    launch_zero(d_seis, (int64_t)nrec, p->stream);
     const int G = pass_size(p, nshots);
     for (int g0 = 0; g0 < nshots; g0 += G) {
         const int ng = std::min(G, nshots - g0);
@@ -742,7 +742,7 @@ int gradient_pyfwi(pyfwi_plan *p, int g0, int ng, const float *d_res, bool preco
                                                   p->h_shots.begin() + g0 + ng));
     const int nchp = (int)cl.size();
     if (int e = ensure(p->adj, (size_t)ng * fps, p->stream, "adjoint fields")) return e;
-    HIPCHK(hipMemsetAsync(p->adj.p, 0, (size_t)ng * fps * sizeof(float), p->stream));
+    launch_zero(p->adj.p, (int64_t)ng * fps, p->stream);
     if (int e = prepare_fields(p, ng)) return e;
     // steps at which a shot of the pass starts its backward sweep
     std::vector<char> first_step(nt + 1, 0);
@@ -811,9 +811,9 @@ int gradient_exact(pyfwi_plan *p, int g0, int ng, const float *d_src, const floa
     if (int e = ensure(p->av1, (size_t)ng * 2 * g.fstride, p->stream, "adjoint velocity")) return e;
     if (int e = ensure(p->seg, (size_t)ng * K * fps, p->stream, "segment")) return e;
     if (int e = ensure(p->fields, (size_t)ng * fps, p->stream, "fields")) return e;
-    HIPCHK(hipMemsetAsync(p->adj.p, 0, (size_t)ng * fps * sizeof(float), p->stream));
-    HIPCHK(hipMemsetAsync(p->av0.p, 0, (size_t)ng * 2 * g.fstride * sizeof(float), p->stream));
-    HIPCHK(hipMemsetAsync(p->av1.p, 0, (size_t)ng * 2 * g.fstride * sizeof(float), p->stream));
+    launch_zero(p->adj.p, (int64_t)ng * fps, p->stream);
+    launch_zero(p->av0.p, (int64_t)ng * 2 * g.fstride, p->stream);
+    launch_zero(p->av1.p, (int64_t)ng * 2 * g.fstride, p->stream);
     float *av_cur = p->av0.p, *av_nxt = p->av1.p;
     const int64_t seg_stride = (int64_t)K * fps;  // per shot
     for (int c = ncp - 1; c >= 0; --c) {
@@ -905,8 +905,7 @@ int32_t pyfwi_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, c
         const int nsets = mode == PYFWI_MODE_PYFWI ? adj_acc_sets(ng) : ng;
         if (int e = ensure(p->gacc, (size_t)nsets * 6 * g.fstride, p->stream, "gradient"))
             return e;
-        HIPCHK(hipMemsetAsync(p->gacc.p, 0, (size_t)nsets * 6 * g.fstride * sizeof(float),
-                              p->stream));
+        launch_zero(p->gacc.p, (int64_t)nsets * 6 * g.fstride, p->stream);
         if (mode == PYFWI_MODE_PYFWI) {
             const float *sn = p->snaps.p + (int64_t)g0 * p->chp.size() * fps;
             if (int e = gradient_pyfwi(p, g0, ng, d_res, (flags & PYFWI_F_PRECOND) != 0, sn,
@@ -988,7 +987,7 @@ int32_t pyfwi_misfit_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t ns
     if (int e = ensure(p->resid, nrec, p->stream, "residuals")) return e;
     if (!p->d_misfit) HIPCHK(hipMalloc(&p->d_misfit, sizeof(double)));
     HIPCHK(hipMemsetAsync(p->d_misfit, 0, sizeof(double), p->stream));
-    HIPCHK(hipMemsetAsync(d_seis, 0, nrec * sizeof(float), p->stream));
+    launch_zero(d_seis, (int64_t)nrec, p->stream);
     p->chp = cl;
     p->last_shots = sh;
     p->have_forward = false;  // kept states belong to the last pass only
@@ -1030,8 +1029,7 @@ int32_t pyfwi_misfit_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t ns
         const int nsets = adj_acc_sets(ng);
         if (int e = ensure(p->gacc, (size_t)nsets * 6 * g.fstride, p->stream, "gradient"))
             return e;
-        HIPCHK(hipMemsetAsync(p->gacc.p, 0, (size_t)nsets * 6 * g.fstride * sizeof(float),
-                              p->stream));
+        launch_zero(p->gacc.p, (int64_t)nsets * 6 * g.fstride, p->stream);
         if (int e = gradient_pyfwi(p, g0, ng, p->resid.p, (flags & PYFWI_F_PRECOND) != 0,
                                    p->snaps.p, cl_pass))
             return e;
