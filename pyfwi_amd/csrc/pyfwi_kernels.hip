@@ -261,6 +261,73 @@ struct Fifo {
     }
 };
 
+// ------------------------------------------------ x-taps from lane shifts
+//
+// At order 8 a staggered x-derivative reads 8 columns of a row; loading each
+// tap costs the adjoint kernels 8 loads and registers per array.  Instead a
+// lane loads its own column and one halo value, and the taps come from the
+// neighbouring lanes through DPP moves (GFX9 encodings: a wave shift by one
+// lane, a row shift by n lanes within the 16-lane rows; a lane whose source
+// is out of range keeps the `old` operand).  Every lane of the wave must be
+// active at these moves.
+constexpr int DPP_ROW_SHL = 0x100, DPP_ROW_SHR = 0x110;      // + n, 1 <= n <= 15
+constexpr int DPP_WAVE_SHL1 = 0x130, DPP_WAVE_SHR1 = 0x138;  // lane i <- i+1, i <- i-1
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float old, float v)
+{
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+
+// Byte offset of a lane's halo column: lanes 0..SDO-1 load columns j0-SDO ..
+// j0-1 and lanes 64-SDO..63 columns j0+64 .. j0+63+SDO (j0: lane 0's column);
+// the others load their own column again.
+template <int SDO>
+__device__ __forceinline__ uint32_t halo_offset(int lane)
+{
+    const int k = lane < SDO ? -SDO : (lane >= 64 - SDO ? SDO : 0);
+    return (uint32_t)(k * 4);
+}
+
+// The halo value that fills the wave's edge lanes at distance k: lane 0 gets
+// column j0-k (halo lane SDO-k), lane 63 gets column j0+63+k (halo lane
+// 63-SDO+k).
+template <int SDO, int K>
+__device__ __forceinline__ float halo_fill(float h)
+{
+    constexpr int N = SDO - K;
+    if constexpr (N == 0) {
+        return h;
+    } else {
+        const float right = dpp_mov<DPP_ROW_SHR + N>(h, h);  // lane 63 <- lane 63-N
+        return dpp_mov<DPP_ROW_SHL + N>(right, h);           // lane 0 <- lane N
+    }
+}
+
+// A quantity at column offsets -SDO..SDO of the lane, from its own value `c`
+// and its halo value `h` (see halo_offset); usable with xminus / xplus.
+template <int SDO>
+struct LaneRow {
+    float s[2 * SDO + 1];
+    __device__ __forceinline__ LaneRow(float c, float h)
+    {
+        s[SDO] = c;
+        fill<1>(c, c, h);
+    }
+    template <int K>
+    __device__ __forceinline__ void fill(float l, float r, float h)
+    {
+        const float e = halo_fill<SDO, K>(h);
+        l = dpp_mov<DPP_WAVE_SHR1>(e, l);
+        r = dpp_mov<DPP_WAVE_SHL1>(e, r);
+        s[SDO - K] = l;
+        s[SDO + K] = r;
+        if constexpr (K < SDO) fill<K + 1>(l, r, h);
+    }
+    __device__ __forceinline__ float operator()(int k) const { return s[SDO + k]; }
+};
+
 // Tile of this block and shot / column / row strip of this thread.
 //
 // A block is MARCH_X threads: MARCH_SHOTS shots side by side, each over
@@ -282,7 +349,8 @@ __device__ __forceinline__ int wave_id()
 }
 
 template <int SDO>
-__device__ __forceinline__ bool tile(const DevGrid &g, int &s, int &j, int &i0, int &i1)
+__device__ __forceinline__ bool tile_lane(const DevGrid &g, int &s, int &j, int &i0, int &i1,
+                                          bool &lane_ok)
 {
     constexpr int WX = MARCH_X / 64 / MARCH_SHOTS;  // waves of one shot along x
     const int T = g.ncb * g.nstrips * g.nbatch;
@@ -299,7 +367,16 @@ __device__ __forceinline__ bool tile(const DevGrid &g, int &s, int &j, int &i0, 
     j = SDO + (cb * WX + wave % WX) * 64 + lane;
     i0 = SDO + st * g.mrows;
     i1 = min(i0 + g.mrows - 1, g.nz - SDO);
-    return j <= g.nx - SDO && s < g.nshot;
+    lane_ok = j <= g.nx - SDO;
+    return s < g.nshot;
+}
+
+// As tile_lane, false also for a lane whose column is off the grid.
+template <int SDO>
+__device__ __forceinline__ bool tile(const DevGrid &g, int &s, int &j, int &i0, int &i1)
+{
+    bool lane_ok = false;
+    return tile_lane<SDO>(g, s, j, i0, i1, lane_ok) && lane_ok;
 }
 
 // Prime a window for the first row i0 of a strip: rows i0-SDO .. i0+SDO-1.
@@ -1171,8 +1248,18 @@ __global__ __launch_bounds__(MARCH_X) ADJ_VEL_ATTR void k_adj_vel_grad(
 {
     constexpr int NV = PRECOND ? 2 : 1;  // staged terms: rho (, rho_pre)
     __shared__ float part[2][ADJ_GROUP_ROWS * MARCH_SHOTS * NV * 64];
+    // order 8 takes its x-taps from lane shifts: every lane of a wave stays,
+    // and only those on the grid store.  (In k_adj_str, with three such
+    // derivatives a row, the same change was 18 % slower at order 8; it
+    // keeps its tap loads.)
+    constexpr bool LT = SDO == 4;
     int s, j, i0, i1;
-    if (!tile<SDO>(g, s, j, i0, i1)) return;
+    bool lane_ok = true;
+    if constexpr (LT) {
+        if (!tile_lane<SDO>(g, s, j, i0, i1, lane_ok)) return;
+    } else if (!tile<SDO>(g, s, j, i0, i1)) {
+        return;
+    }
     if (!bwd_active(shots, s, t)) return;
     const GroupLive gl = group_live(g, shots, s, t);
     const int wave = wave_id(), lane = (int)threadIdx.x % 64;
@@ -1184,13 +1271,17 @@ __global__ __launch_bounds__(MARCH_X) ADJ_VEL_ATTR void k_adj_vel_grad(
     // of this thread's receiver, added to what is written (this step's
     // imaging uses the velocities without it)
     const RecLine rq = residual_line(shots[s], t - 1, res);
-    const int rr = rq.z >= i0 && rq.z <= i1 ? rq.index(j) : -1;
+    const int rr = lane_ok && rq.z >= i0 && rq.z <= i1 ? rq.index(j) : -1;
     const int rrow = rr >= 0 ? rq.z : -1;
     const float r_vx = rr >= 0 ? rq.base[rr] : 0.0f;
     const float r_vz = rr >= 0 ? rq.base[rq.fs + rr] : 0.0f;
     const CellRef L{lam + g.origin}, M{mu + g.origin}, B{bu + g.origin};
     const int lo = g.npml + SDO, hi = g.nz - SDO - g.npml, hix = g.nx - SDO - g.npml;
     const bool img = j >= lo && j <= hix && i1 >= lo && i0 <= hi;
+    // forward-state loads: with lane shifts, on every lane of a wave that has
+    // an imaging lane
+    const bool imgl = LT ? __ballot(img) != 0 : img;
+    const uint32_t hoff = halo_offset<SDO>(lane);
     const float *F = fields + (int64_t)s * fields_stride + g.origin;
     const CellRef vx{F}, vz{F + fs}, txx{F + 2 * fs}, tzz{F + 3 * fs}, txz{F + 4 * fs};
     float *G = gacc + (int64_t)(s / MARCH_SHOTS) * 6 * fs + g.origin;
@@ -1204,13 +1295,32 @@ __global__ __launch_bounds__(MARCH_X) ADJ_VEL_ATTR void k_adj_vel_grad(
     auto q_b1 = [&](uint32_t o) { return (L(o) + 2.0f * M(o)) * atzz(o); };
     auto q_b2 = [&](uint32_t o) { return L(o) * atxx(o); };
     auto x_a1 = [&](uint32_t o) {
-        return xminus<SDO>([&](int k) {
-            return (L.at(o, k) + 2.0f * M.at(o, k)) * atxx.at(o, k); }, g.cx); };
+        if constexpr (LT) {
+            auto q = [&](uint32_t p) { return (L(p) + 2.0f * M(p)) * atxx(p); };
+            return xminus<SDO>(LaneRow<SDO>(q(o), q(o + hoff)), g.cx);
+        } else {
+            return xminus<SDO>([&](int k) {
+                return (L.at(o, k) + 2.0f * M.at(o, k)) * atxx.at(o, k); }, g.cx);
+        }
+    };
     auto x_a2 = [&](uint32_t o) {
-        return xminus<SDO>([&](int k) { return L.at(o, k) * atzz.at(o, k); }, g.cx); };
+        if constexpr (LT)
+            return xminus<SDO>(LaneRow<SDO>(L(o) * atzz(o), L(o + hoff) * atzz(o + hoff)), g.cx);
+        else
+            return xminus<SDO>([&](int k) { return L.at(o, k) * atzz.at(o, k); }, g.cx);
+    };
     auto x_b3 = [&](uint32_t o) {
-        return xplus<SDO>([&](int k) { return M.at(o, k) * atxz.at(o, k); }, g.cx); };
-    auto x_txz = [&](uint32_t o) { return xplus<SDO>([&](int k) { return txz.at(o, k); }, g.cx); };
+        if constexpr (LT)
+            return xplus<SDO>(LaneRow<SDO>(M(o) * atxz(o), M(o + hoff) * atxz(o + hoff)), g.cx);
+        else
+            return xplus<SDO>([&](int k) { return M.at(o, k) * atxz.at(o, k); }, g.cx);
+    };
+    auto x_txz = [&](uint32_t o) {
+        if constexpr (LT)
+            return xplus<SDO>(LaneRow<SDO>(txz(o), txz(o + hoff)), g.cx);
+        else
+            return xplus<SDO>([&](int k) { return txz.at(o, k); }, g.cx);
+    };
     uint32_t cb = ((uint32_t)i0 * (uint32_t)g.pitch + (uint32_t)j) * 4u;
 #pragma unroll
     for (int k = 0; k < 2 * SDO; ++k) {
@@ -1224,13 +1334,13 @@ __global__ __launch_bounds__(MARCH_X) ADJ_VEL_ATTR void k_adj_vel_grad(
             fa2.v[q] = x_a2(o);
             fb3.v[q] = x_b3(o);
         }
-        if (img) {
+        if (imgl) {
             ftzz.v[k + 1] = tzz(o);
             ftxz.v[k + 1] = txz(o);
             if (k >= SDO) fdtxz.v[k - SDO] = x_txz(o);
         }
     }
-    const float ex = dxp[j];
+    const float ex = lane_ok ? dxp[j] : 0.0f;
     const float dt = g.dt;
     // ALONE: the shot adds its terms itself (see k_adj_str)
     int buf = 0;
@@ -1252,18 +1362,25 @@ __global__ __launch_bounds__(MARCH_X) ADJ_VEL_ATTR void k_adj_vel_grad(
             float *px = &cell_ref(A, cb), *pz = &cell_ref(A + fs, cb);
             float ux = *px, uz = *pz;
             const bool row_img = img && i >= lo && i <= hi;
-            float tzz_n = 0.0f, txz_n = 0.0f, bb = 0.0f;
+            const bool row_imgl = imgl && i >= lo && i <= hi;
+            float tzz_n = 0.0f, txz_n = 0.0f, txz_h = 0.0f, txx_c = 0.0f, txx_h = 0.0f, bb = 0.0f;
             float wx = 0.0f, wz = 0.0f;
             Taps<SDO, 0> ttxz;  // forward txz on the entering row (d/dx+)
             Taps<SDO, 1> ttxx;  // forward txx on the centre row (d/dx-)
-            if (img) {
+            if (imgl) {
                 tzz_n = tzz(n);
                 txz_n = txz(n);
-                ttxz.load(txz, n);
+                if constexpr (LT) txz_h = txz(n + hoff);
+                else ttxz.load(txz, n);
             }
             float gc[NV] = {};  // the row's accumulators (ALONE)
-            if (row_img) {
-                ttxx.load(txx, cb);
+            if (row_imgl) {
+                if constexpr (LT) {
+                    txx_c = txx(cb);
+                    txx_h = txx(cb + hoff);
+                } else {
+                    ttxx.load(txx, cb);
+                }
                 bb = B(cb);
                 if constexpr (ALONE) {
 #pragma unroll
@@ -1277,16 +1394,25 @@ __global__ __launch_bounds__(MARCH_X) ADJ_VEL_ATTR void k_adj_vel_grad(
             // (lanes outside the imaging region push values they never use)
             ftzz.push(tzz_n);
             ftxz.push(txz_n);
-            const float dxp_txz = fdtxz.pop(xplus<SDO>(ttxz, g.cx));
-            const float dxm_txx = xminus<SDO>(ttxx, g.cx);
+            float dxp_txz_n, dxm_txx;
+            if constexpr (LT) {
+                dxp_txz_n = xplus<SDO>(LaneRow<SDO>(txz_n, txz_h), g.cx);
+                dxm_txx = xminus<SDO>(LaneRow<SDO>(txx_c, txx_h), g.cx);
+            } else {
+                dxp_txz_n = xplus<SDO>(ttxz, g.cx);
+                dxm_txx = xminus<SDO>(ttxx, g.cx);
+            }
+            const float dxp_txz = fdtxz.pop(dxp_txz_n);
             const float a3 = wa3.dminus(g.cz);
             const float b1 = wb1.dplus(g.cz);
             const float b2 = wb2.dplus(g.cz);
             ux = fmaf(-dt, (2.0f * ex + ez) * ux + (a1 + a2 + a3), ux);
             uz = fmaf(-dt, (ex + 2.0f * ez) * uz + (b2 + b1 + b3), uz);
             const bool at_rec = i == rrow;
-            *px = at_rec ? ux + r_vx : ux;
-            *pz = at_rec ? uz + r_vz : uz;
+            if (lane_ok) {
+                *px = at_rec ? ux + r_vx : ux;
+                *pz = at_rec ? uz + r_vz : uz;
+            }
             if (!row_img) continue;
             const float dzp_tzz = ftzz.dplus(g.cz);
             const float dzm_txz = ftxz.dminus(g.cz);
