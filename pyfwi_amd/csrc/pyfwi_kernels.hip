@@ -1236,11 +1236,8 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot
 }
 
 // Adjoint velocities (src/PyFWI/elastic.cl:419-468) and the rho imaging term.
-#ifndef ADJ_VEL_ATTR
-#define ADJ_VEL_ATTR
-#endif
 template <int SDO, bool PRECOND, bool ALONE>
-__global__ __launch_bounds__(MARCH_X) ADJ_VEL_ATTR void k_adj_vel_grad(
+__global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
     DevGrid g, const pyfwi_shot *shots, int t, const float *fields, int64_t fields_stride,
     float *adj, const float *__restrict__ lam, const float *__restrict__ mu,
     const float *__restrict__ bu, const float *__restrict__ dxp,
