@@ -64,13 +64,13 @@ def test_forward_wells(gpu, acq_type, order):
 
 
 @pytest.mark.parametrize("acq_type", [0, 2])
-@pytest.mark.parametrize("chpr", [100, 0])
-def test_gradient_wells(gpu, acq_type, chpr):
+@pytest.mark.parametrize("chpr,order", [(100, 4), (0, 4), (100, 8)])
+def test_gradient_wells(gpu, acq_type, chpr, order):
     """Reference gradient for the residual of CostFunction.l2; for acq_type 2
     the residual's left-well columns are injected in the records' reversed
     order, as the reference does."""
-    print(f"wells gradient acq_type={acq_type} chpr={chpr}", flush=True)
-    W, st, model, model_obs = wells(acq_type, chpr=chpr)
+    print(f"wells gradient acq_type={acq_type} chpr={chpr} order={order}", flush=True)
+    W, st, model, model_obs = wells(acq_type, order=order, chpr=chpr)
     d_est = W.forward_modeling(model)
     d_obs, _ = oracle_section(st, model_obs)
     _, res = CostFunction("l2")(d_est, d_obs)
