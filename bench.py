@@ -39,7 +39,7 @@ HBM_PEAK_GBS = 8000.0
 # (bytes per cell of the whole grid and shot, extra bytes per cell of the
 # interior where the imaging condition runs and shot, bytes per interior cell
 # and shot group).  The reference-mode gradient accumulators are shared by
-# the SHOTS_PER_BLOCK shots of a block, which add their terms up first.
+# the shots of a block, which add their terms up first (shots_per_block).
 KERNEL_BYTES = {
     "fwd_vel": (32, 0, 0),       # R txx tzz txz vx vz b, W vx vz
     "fwd_str": (40, 0, 0),       # R vx vz txx tzz txz lam mu, W txx tzz txz
@@ -52,12 +52,16 @@ KERNEL_BYTES = {
     "fwd_step": (52, 0, 0),       # R 5 fields lam mu b, W 5 fields
     "rev_step": (52, 0, 0),
 }
-SHOTS_PER_BLOCK = 4
+def shots_per_block(order, shots):
+    """Shots per block of the stencil kernels, as the library picks them
+    (march_spb in pyfwi_amd/csrc/pyfwi_kernels.hip)."""
+    n = int(round(shots))
+    return 5 if order == 4 and n == shots and n % 5 == 0 and n % 4 != 0 else 4
 
 
-def launch_bytes(kernel, cells, interior, shots):
+def launch_bytes(kernel, cells, interior, shots, order=4):
     a, b, c = KERNEL_BYTES[kernel]
-    groups = math.ceil(shots / SHOTS_PER_BLOCK)
+    groups = math.ceil(shots / shots_per_block(order, shots))
     return (a * cells + b * interior) * shots + c * interior * groups
 
 
@@ -429,10 +433,11 @@ def main():
     # shots per launch: passes of `tuning[0]` shots (the last one may be smaller)
     npass = -(-int(W.ns) // tuning[0]) if tuning[0] else 1
     shots_per_launch = int(W.ns) / npass
-    per_launch_bytes = launch_bytes(dom, cells, interior, shots_per_launch)
+    order = int(W.sdo) * 2
+    per_launch_bytes = launch_bytes(dom, cells, interior, shots_per_launch, order)
     achieved = per_launch_bytes / avg_s / 1e9
     kernels = {k: {"avg_us": round(v[0] / v[1] * 1e3, 2), "launches": v[1],
-                   "GBps": round(launch_bytes(k, cells, interior, shots_per_launch)
+                   "GBps": round(launch_bytes(k, cells, interior, shots_per_launch, order)
                                  / (v[0] / v[1] / 1e3) / 1e9, 1)}
                for k, v in prof.items()}
     log(f"bench: kernels {kernels}")

@@ -11,10 +11,6 @@ namespace pyfwi {
 constexpr int HALO = 8;      // zero rows above and below each padded field (>= 2*SDO)
 constexpr int TILE_X = 64;   // one wavefront along x
 constexpr int TILE_Y = 4;    // 4 wavefronts per block along z
-#ifndef MARCH_THREADS
-#define MARCH_THREADS 256           // column-marching kernels: threads per block (4 waves)
-#endif
-constexpr int MARCH_X = MARCH_THREADS;
 constexpr int MARCH_ROWS_MIN = 8;   // strip height bounds (rows walked by a thread)
 constexpr int MARCH_ROWS_MAX = 64;
 // Wavefronts with work per CU that one launch aims for; the strip height is
@@ -28,9 +24,6 @@ constexpr int FUSED_WAVES = 4;      // fused two-stage kernels: wavefronts per b
 #ifndef MARCH_UNROLL
 #define MARCH_UNROLL 2              // rows per unrolled iteration: two rows of loads in flight per wave
 #endif
-#ifndef MARCH_SHOTS
-#define MARCH_SHOTS 4               // column-marching blocks: shots per block (one wave each at 256 threads)
-#endif
 #ifndef ADJ_VEL_UNROLL
 #define ADJ_VEL_UNROLL 2            // rows per unrolled iteration of the adjoint velocity + imaging kernel
 #endif
@@ -38,10 +31,18 @@ constexpr int FUSED_WAVES = 4;      // fused two-stage kernels: wavefronts per b
 #define ADJ_GROUP_ROWS 2            // adjoint kernels: rows per block-wide sum of the imaging terms
 #endif
 
+// Shots per column-marching block (one wave each, over the same 64 columns)
+// for a launch of `nshots` shots at half stencil width `sdo`: 4 or 5.
+int march_spb(int sdo, int nshots);
+
 // Gradient accumulator sets that the reference-mode adjoint kernels fill for
-// `nshots` shots of one pass: one per group of MARCH_SHOTS shots (the shots
-// of a block sum their imaging terms before they reach memory).
-inline int adj_acc_sets(int nshots) { return (nshots + MARCH_SHOTS - 1) / MARCH_SHOTS; }
+// `nshots` shots of one pass: one per block's group of shots (the shots of a
+// block sum their imaging terms before they reach memory).
+inline int adj_acc_sets(int sdo, int nshots)
+{
+    const int spb = march_spb(sdo, nshots);
+    return (nshots + spb - 1) / spb;
+}
 
 // Constants every kernel takes by value.
 struct DevGrid {

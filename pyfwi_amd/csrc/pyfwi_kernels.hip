@@ -330,10 +330,9 @@ struct LaneRow {
 
 // Tile of this block and shot / column / row strip of this thread.
 //
-// A block is MARCH_X threads: MARCH_SHOTS shots side by side, each over
-// MARCH_X / MARCH_SHOTS columns, on one row strip (one wave per shot by
-// default, so a block column is 64 cells wide and a narrow grid leaves few
-// lanes idle in its last column block).  Launches are 1-D.  The tiles (shot group, column block, row
+// A block is 64*SPB threads: SPB shots side by side, one wave each, over the
+// same 64 columns of one row strip (a narrow grid leaves few lanes idle in
+// its last column block).  Launches are 1-D.  The tiles (shot group, column block, row
 // strip) are numbered shot group fastest; workgroup w runs tile
 // (w % 8) * ceil(T/8) + w / 8.  Workgroups are handed to the 8 XCDs in turn,
 // so each XCD receives one contiguous run of tiles: the same cells of every
@@ -348,11 +347,10 @@ __device__ __forceinline__ int wave_id()
     return __builtin_amdgcn_readfirstlane((int)threadIdx.x / 64);
 }
 
-template <int SDO>
+template <int SDO, int SPB>
 __device__ __forceinline__ bool tile_lane(const DevGrid &g, int &s, int &j, int &i0, int &i1,
                                           bool &lane_ok)
 {
-    constexpr int WX = MARCH_X / 64 / MARCH_SHOTS;  // waves of one shot along x
     const int T = g.ncb * g.nstrips * g.nbatch;
     const int per = (T + NUM_XCD - 1) / NUM_XCD;
     const int w = (int)blockIdx.x;
@@ -363,8 +361,8 @@ __device__ __forceinline__ bool tile_lane(const DevGrid &g, int &s, int &j, int 
     const int cb = rest % g.ncb;
     const int st = rest / g.ncb;
     const int wave = wave_id(), lane = (int)threadIdx.x % 64;
-    s = grp * MARCH_SHOTS + wave / WX;
-    j = SDO + (cb * WX + wave % WX) * 64 + lane;
+    s = grp * SPB + wave;
+    j = SDO + cb * 64 + lane;
     i0 = SDO + st * g.mrows;
     i1 = min(i0 + g.mrows - 1, g.nz - SDO);
     lane_ok = j <= g.nx - SDO;
@@ -372,11 +370,11 @@ __device__ __forceinline__ bool tile_lane(const DevGrid &g, int &s, int &j, int 
 }
 
 // As tile_lane, false also for a lane whose column is off the grid.
-template <int SDO>
+template <int SDO, int SPB>
 __device__ __forceinline__ bool tile(const DevGrid &g, int &s, int &j, int &i0, int &i1)
 {
     bool lane_ok = false;
-    return tile_lane<SDO>(g, s, j, i0, i1, lane_ok) && lane_ok;
+    return tile_lane<SDO, SPB>(g, s, j, i0, i1, lane_ok) && lane_ok;
 }
 
 // Prime a window for the first row i0 of a strip: rows i0-SDO .. i0+SDO-1.
@@ -492,8 +490,8 @@ __device__ __forceinline__ RecLine residual_line(const pyfwi_shot &sh, int t, co
 // (per-shot strides in_stride / out_stride, in floats).  in == out updates in
 // place; the forward of a kept-state run reads the state of step t-1 from its
 // slot and writes step t into the next slot.
-template <int SDO, int DIR>
-__global__ __launch_bounds__(MARCH_X) void k_vel(DevGrid g, const pyfwi_shot *shots, int t,
+template <int SDO, int DIR, int SPB>
+__global__ __launch_bounds__(64 * SPB) void k_vel(DevGrid g, const pyfwi_shot *shots, int t,
                                                  const float *in, int64_t in_stride, float *out,
                                                  int64_t out_stride, const float *__restrict__ bu,
                                                  const float *__restrict__ dxp,
@@ -501,7 +499,7 @@ __global__ __launch_bounds__(MARCH_X) void k_vel(DevGrid g, const pyfwi_shot *sh
                                                  const float *__restrict__ src)
 {
     int s, j, i0, i1;
-    if (!tile<SDO>(g, s, j, i0, i1)) return;
+    if (!tile<SDO, SPB>(g, s, j, i0, i1)) return;
     if (DIR > 0 ? !fwd_active(shots, s, t) : !bwd_active(shots, s, t)) return;
     const int64_t fs = g.fstride;
     const uint32_t RB = (uint32_t)g.pitch * 4u;
@@ -577,8 +575,8 @@ __global__ __launch_bounds__(MARCH_X) void k_vel(DevGrid g, const pyfwi_shot *sh
 // of this step are read from `out` (written there by k_vel), the stresses of
 // the previous step from `in`, and the new stresses written to `out`.  Time
 // reversed: in == out.
-template <int SDO, int DIR>
-__global__ __launch_bounds__(MARCH_X) void k_str(DevGrid g, const pyfwi_shot *shots, int t,
+template <int SDO, int DIR, int SPB>
+__global__ __launch_bounds__(64 * SPB) void k_str(DevGrid g, const pyfwi_shot *shots, int t,
                                                  const float *in, int64_t in_stride, float *out,
                                                  int64_t out_stride, const float *__restrict__ lam,
                                                  const float *__restrict__ mu,
@@ -587,7 +585,7 @@ __global__ __launch_bounds__(MARCH_X) void k_str(DevGrid g, const pyfwi_shot *sh
                                                  const float *__restrict__ src, float *seis)
 {
     int s, j, i0, i1;
-    if (!tile<SDO>(g, s, j, i0, i1)) return;
+    if (!tile<SDO, SPB>(g, s, j, i0, i1)) return;
     if (DIR > 0 ? !fwd_active(shots, s, t) : !bwd_active(shots, s, t)) return;
     const int64_t fs = g.fstride;
     const uint32_t RB = (uint32_t)g.pitch * 4u;
@@ -1029,7 +1027,7 @@ __global__ void k_adj_inject(DevGrid g, const pyfwi_shot *shots, int t,
 // in LDS, and after a barrier the live waves share those rows, each adding
 // the shots' terms in shot order onto the group's accumulators.  The
 // accumulators then move once per group instead of once per shot.  gacc:
-// [adj_acc_sets(nshots)][6][fstride] = gmu, gmu_pre, glam, glam_pre, grho,
+// [adj_acc_sets(sdo, nshots)][6][fstride] = gmu, gmu_pre, glam, glam_pre, grho,
 // grho_pre; interior cells only.
 
 // The shots of the calling wave's group that take part in step t (waves of
@@ -1039,14 +1037,14 @@ struct GroupLive {
     int count = 0, rank = 0;
 };
 
+template <int SPB>
 __device__ __forceinline__ GroupLive group_live(const DevGrid &g, const pyfwi_shot *shots, int s,
                                                 int t)
 {
-    static_assert(MARCH_X == 64 * MARCH_SHOTS, "the shot-group sum takes one wave per shot");
     const int wave = wave_id(), s0 = s - wave;
     GroupLive r;
 #pragma unroll
-    for (int w = 0; w < MARCH_SHOTS; ++w)
+    for (int w = 0; w < SPB; ++w)
         if (s0 + w < g.nshot && bwd_active(shots, s0 + w, t)) r.mask |= 1u << w;
     r.count = __popc(r.mask);
     r.rank = __popc(r.mask & ((1u << wave) - 1u));
@@ -1058,7 +1056,7 @@ __device__ __forceinline__ GroupLive group_live(const DevGrid &g, const pyfwi_sh
 // out to the live waves by rank.  `c0`: byte offset of (row ic, the lane's
 // column).  Called by every live wave after the barrier that follows the
 // staging.
-template <int NV>
+template <int NV, int SPB>
 __device__ __forceinline__ void add_group_terms(const float *part, const GroupLive &gl, int ic,
                                                 int i1, bool img, int lo, int hi, float *G,
                                                 const int *slot, int64_t fs, uint32_t c0,
@@ -1077,10 +1075,10 @@ __device__ __forceinline__ void add_group_terms(const float *part, const GroupLi
             acc[v] = 0.0f;
         }
 #pragma unroll
-        for (int w = 0; w < MARCH_SHOTS; ++w) {
+        for (int w = 0; w < SPB; ++w) {
             if (!(gl.mask >> w & 1u)) continue;
 #pragma unroll
-            for (int v = 0; v < NV; ++v) acc[v] += part[((r * MARCH_SHOTS + w) * NV + v) * 64 + lane];
+            for (int v = 0; v < NV; ++v) acc[v] += part[((r * SPB + w) * NV + v) * 64 + lane];
         }
 #pragma unroll
         for (int v = 0; v < NV; ++v) cell_ref(G + slot[v] * fs, o) = cur[v] + acc[v];
@@ -1089,8 +1087,8 @@ __device__ __forceinline__ void add_group_terms(const float *part, const GroupLi
 
 // Adjoint stresses from adjoint velocities (src/PyFWI/elastic.cl:372-411) and
 // the lambda / mu imaging terms.
-template <int SDO, bool PRECOND, bool ALONE>
-__global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot *shots, int t,
+template <int SDO, bool PRECOND, bool ALONE, int SPB>
+__global__ __launch_bounds__(64 * SPB) void k_adj_str(DevGrid g, const pyfwi_shot *shots, int t,
                                                      const float *fields, int64_t fields_stride,
                                                      float *adj, const float *__restrict__ bu,
                                                      const float *__restrict__ dxp,
@@ -1098,11 +1096,11 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot
                                                      const float *__restrict__ res)
 {
     constexpr int NV = PRECOND ? 4 : 2;  // staged terms: mu, lam (, mu_pre, lam_pre)
-    __shared__ float part[2][ADJ_GROUP_ROWS * MARCH_SHOTS * NV * 64];
+    __shared__ float part[2][ADJ_GROUP_ROWS * SPB * NV * 64];
     int s, j, i0, i1;
-    if (!tile<SDO>(g, s, j, i0, i1)) return;
+    if (!tile<SDO, SPB>(g, s, j, i0, i1)) return;
     if (!bwd_active(shots, s, t)) return;
-    const GroupLive gl = group_live(g, shots, s, t);
+    const GroupLive gl = group_live<SPB>(g, shots, s, t);
     const int wave = wave_id(), lane = (int)threadIdx.x % 64;
     const int64_t fs = g.fstride;
     const uint32_t RB = (uint32_t)g.pitch * 4u;
@@ -1120,7 +1118,7 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot
     const bool img = j >= lo && j <= hix && i1 >= lo && i0 <= hi;
     const float *F = fields + (int64_t)s * fields_stride + g.origin;
     const CellRef vx{F}, vz{F + fs}, txx{F + 2 * fs}, tzz{F + 3 * fs}, txz{F + 4 * fs};
-    float *G = gacc + (int64_t)(s / MARCH_SHOTS) * 6 * fs + g.origin;
+    float *G = gacc + (int64_t)(s / SPB) * 6 * fs + g.origin;
     constexpr int slots[4] = {0, 2, 1, 3};  // accumulator of each staged term
     ZWin<SDO> wbz, wbx;  // b*avz for d/dz-, b*avx for d/dz+
     Fifo<SDO> fxp, fxm;  // d/dx+ (b*avx), d/dx- (b*avz)
@@ -1221,7 +1219,7 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot
                           + dt2 * sxz * sxz * (dzp_vz * dzp_vz + dxm_vz * dxm_vz);
                 term[3] = dt2 * (sxx * sxx + szz * szz) * (dxp_vx * dxp_vx + dzm_vz * dzm_vz);
             }
-            float *q = mine + r * MARCH_SHOTS * NV * 64;
+            float *q = mine + r * SPB * NV * 64;
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 if constexpr (ALONE) cell_ref(G + slots[v] * fs, cb) = gc[v] + term[v];
@@ -1230,21 +1228,21 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_str(DevGrid g, const pyfwi_shot
         }
         if constexpr (!ALONE) {
             __syncthreads();
-            add_group_terms<NV>(part[buf], gl, ic, i1, img, lo, hi, G, slots, fs, c0, RB);
+            add_group_terms<NV, SPB>(part[buf], gl, ic, i1, img, lo, hi, G, slots, fs, c0, RB);
         }
     }
 }
 
 // Adjoint velocities (src/PyFWI/elastic.cl:419-468) and the rho imaging term.
-template <int SDO, bool PRECOND, bool ALONE>
-__global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
+template <int SDO, bool PRECOND, bool ALONE, int SPB>
+__global__ __launch_bounds__(64 * SPB) void k_adj_vel_grad(
     DevGrid g, const pyfwi_shot *shots, int t, const float *fields, int64_t fields_stride,
     float *adj, const float *__restrict__ lam, const float *__restrict__ mu,
     const float *__restrict__ bu, const float *__restrict__ dxp,
     const float *__restrict__ dzp, float *gacc, const float *__restrict__ res)
 {
     constexpr int NV = PRECOND ? 2 : 1;  // staged terms: rho (, rho_pre)
-    __shared__ float part[2][ADJ_GROUP_ROWS * MARCH_SHOTS * NV * 64];
+    __shared__ float part[2][ADJ_GROUP_ROWS * SPB * NV * 64];
     // order 8 takes its x-taps from lane shifts: every lane of a wave stays,
     // and only those on the grid store.  (In k_adj_str, with three such
     // derivatives a row, the same change was 18 % slower at order 8; it
@@ -1253,12 +1251,12 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
     int s, j, i0, i1;
     bool lane_ok = true;
     if constexpr (LT) {
-        if (!tile_lane<SDO>(g, s, j, i0, i1, lane_ok)) return;
-    } else if (!tile<SDO>(g, s, j, i0, i1)) {
+        if (!tile_lane<SDO, SPB>(g, s, j, i0, i1, lane_ok)) return;
+    } else if (!tile<SDO, SPB>(g, s, j, i0, i1)) {
         return;
     }
     if (!bwd_active(shots, s, t)) return;
-    const GroupLive gl = group_live(g, shots, s, t);
+    const GroupLive gl = group_live<SPB>(g, shots, s, t);
     const int wave = wave_id(), lane = (int)threadIdx.x % 64;
     const int64_t fs = g.fstride;
     const uint32_t RB = (uint32_t)g.pitch * 4u;
@@ -1281,7 +1279,7 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
     const uint32_t hoff = halo_offset<SDO>(lane);
     const float *F = fields + (int64_t)s * fields_stride + g.origin;
     const CellRef vx{F}, vz{F + fs}, txx{F + 2 * fs}, tzz{F + 3 * fs}, txz{F + 4 * fs};
-    float *G = gacc + (int64_t)(s / MARCH_SHOTS) * 6 * fs + g.origin;
+    float *G = gacc + (int64_t)(s / SPB) * 6 * fs + g.origin;
     constexpr int slots[2] = {4, 5};  // accumulator of each staged term
 
     ZWin<SDO> wa3, wb1, wb2;  // mu*atxz (d/dz-), (lam+2mu)*atzz, lam*atxx (d/dz+)
@@ -1420,7 +1418,7 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
                 term[1] = dt2 * (wx * wx * (dxm_txx * dxm_txx + dzm_txz * dzm_txz)
                                  + wz * wz * (dzp_tzz * dzp_tzz + dxp_txz * dxp_txz));
             }
-            float *q = mine + r * MARCH_SHOTS * NV * 64;
+            float *q = mine + r * SPB * NV * 64;
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 if constexpr (ALONE) cell_ref(G + slots[v] * fs, cb) = gc[v] + term[v];
@@ -1429,7 +1427,7 @@ __global__ __launch_bounds__(MARCH_X) void k_adj_vel_grad(
         }
         if constexpr (!ALONE) {
             __syncthreads();
-            add_group_terms<NV>(part[buf], gl, ic, i1, img, lo, hi, G, slots, fs, c0, RB);
+            add_group_terms<NV, SPB>(part[buf], gl, ic, i1, img, lo, hi, G, slots, fs, c0, RB);
         }
     }
 }
@@ -1769,16 +1767,26 @@ static int strip_rows(const DevGrid &g, int nz_act, int64_t per_strip)
     return std::max(MARCH_ROWS_MIN, std::min(MARCH_ROWS_MAX, rows));
 }
 
-// 1-D grid of (column block, row strip, shot group) tiles, XCD-interleaved
-// (see tile()).
-static dim3 march_grid(DevGrid &g, int sdo, int nshots)
+// Shots per column-marching block.  Five when a launch's shots fill blocks
+// of five but not of four (the 10-shot passes of a 540 x 1040 grid): no idle
+// waves, fewer accumulator sets, the model's rows read once for more shots;
+// 2-3 % faster there (in-process A/B).  Four otherwise: order 8 in blocks of
+// five was 25 % slower (its adjoint kernels' registers leave room for few
+// resident blocks), and passes of 1-3 shots 6 % slower.
+int march_spb(int sdo, int nshots)
+{
+    return sdo == 2 && nshots % 5 == 0 && nshots % 4 != 0 ? 5 : 4;
+}
+
+// 1-D grid of (column block, row strip, shot group) tiles for blocks of spb
+// shots, XCD-interleaved (see tile()).
+static dim3 march_grid(DevGrid &g, int sdo, int nshots, int spb)
 {
     const int nx_act = g.nx - 2 * sdo + 1, nz_act = g.nz - 2 * sdo + 1;
-    constexpr int cols = MARCH_X / MARCH_SHOTS;  // columns of a block
-    g.ncb = (nx_act + cols - 1) / cols;
-    g.nbatch = (nshots + MARCH_SHOTS - 1) / MARCH_SHOTS;
+    g.ncb = (nx_act + 63) / 64;
+    g.nbatch = (nshots + spb - 1) / spb;
     g.nshot = nshots;
-    const int rows = strip_rows(g, nz_act, (int64_t)g.ncb * (MARCH_X / 64 / MARCH_SHOTS) * nshots);
+    const int rows = strip_rows(g, nz_act, (int64_t)g.ncb * nshots);
     g.mrows = rows;
     g.nstrips = (nz_act + rows - 1) / rows;
     const int T = g.ncb * g.nstrips * g.nbatch;
@@ -1792,16 +1800,26 @@ static dim3 march_grid(DevGrid &g, int sdo, int nshots)
         else KERNEL<4 __VA_ARGS__><<<grid, block, 0, st>>>(ARGS);          \
     } while (0)
 
+// Column-marching kernels: half stencil width and shots per block (order 8
+// always takes blocks of four, see march_spb).
+#define DISPATCH_MARCH(sdo, spb, KERNEL, ...)                                          \
+    do {                                                                               \
+        if ((sdo) == 2 && (spb) == 5) KERNEL<2 __VA_ARGS__, 5><<<grid, block, 0, st>>>(ARGS); \
+        else if ((sdo) == 2) KERNEL<2 __VA_ARGS__, 4><<<grid, block, 0, st>>>(ARGS);        \
+        else KERNEL<4 __VA_ARGS__, 4><<<grid, block, 0, st>>>(ARGS);                        \
+    } while (0)
+
 void launch_vel(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
                 const float *in, int64_t in_stride, float *out, int64_t out_stride,
                 const float *bu, const float *dxp, const float *dzp, hipStream_t st,
                 const float *src)
 {
     DevGrid gm = g;
-    dim3 grid = march_grid(gm, g.sdo, nshots), block(MARCH_X);
+    const int spb = march_spb(g.sdo, nshots);
+    dim3 grid = march_grid(gm, g.sdo, nshots, spb), block(64 * spb);
 #define ARGS gm, shots, t, in, in_stride, out, out_stride, bu, dxp, dzp, src
-    if (dir > 0) DISPATCH_SDO(g.sdo, k_vel, , 1);
-    else DISPATCH_SDO(g.sdo, k_vel, , -1);
+    if (dir > 0) DISPATCH_MARCH(g.sdo, spb, k_vel, , 1);
+    else DISPATCH_MARCH(g.sdo, spb, k_vel, , -1);
 #undef ARGS
 }
 
@@ -1811,10 +1829,11 @@ void launch_str(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, 
                 hipStream_t st, const float *src, float *seis)
 {
     DevGrid gm = g;
-    dim3 grid = march_grid(gm, g.sdo, nshots), block(MARCH_X);
+    const int spb = march_spb(g.sdo, nshots);
+    dim3 grid = march_grid(gm, g.sdo, nshots, spb), block(64 * spb);
 #define ARGS gm, shots, t, in, in_stride, out, out_stride, lam, mu, dxp, dzp, src, seis
-    if (dir > 0) DISPATCH_SDO(g.sdo, k_str, , 1);
-    else DISPATCH_SDO(g.sdo, k_str, , -1);
+    if (dir > 0) DISPATCH_MARCH(g.sdo, spb, k_str, , 1);
+    else DISPATCH_MARCH(g.sdo, spb, k_str, , -1);
 #undef ARGS
 }
 
@@ -1869,13 +1888,14 @@ void launch_adj_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t
                     hipStream_t st, const float *res)
 {
     DevGrid gm = g;
-    dim3 grid = march_grid(gm, g.sdo, nshots), block(MARCH_X);
+    const int spb = march_spb(g.sdo, nshots);
+    dim3 grid = march_grid(gm, g.sdo, nshots, spb), block(64 * spb);
     const bool alone = nshots == 1;  // no other shot to share the accumulators with
 #define ARGS gm, shots, t, fields, fields_stride, adj, bu, dxp, dzp, gacc, res
-    if (precond && alone) DISPATCH_SDO(g.sdo, k_adj_str, , true, true);
-    else if (precond) DISPATCH_SDO(g.sdo, k_adj_str, , true, false);
-    else if (alone) DISPATCH_SDO(g.sdo, k_adj_str, , false, true);
-    else DISPATCH_SDO(g.sdo, k_adj_str, , false, false);
+    if (precond && alone) DISPATCH_MARCH(g.sdo, spb, k_adj_str, , true, true);
+    else if (precond) DISPATCH_MARCH(g.sdo, spb, k_adj_str, , true, false);
+    else if (alone) DISPATCH_MARCH(g.sdo, spb, k_adj_str, , false, true);
+    else DISPATCH_MARCH(g.sdo, spb, k_adj_str, , false, false);
 #undef ARGS
 }
 
@@ -1886,13 +1906,14 @@ void launch_adj_vel_grad(const DevGrid &g, int nshots, const pyfwi_shot *shots, 
                          hipStream_t st, const float *res)
 {
     DevGrid gm = g;
-    dim3 grid = march_grid(gm, g.sdo, nshots), block(MARCH_X);
+    const int spb = march_spb(g.sdo, nshots);
+    dim3 grid = march_grid(gm, g.sdo, nshots, spb), block(64 * spb);
     const bool alone = nshots == 1;
 #define ARGS gm, shots, t, fields, fields_stride, adj, lam, mu, bu, dxp, dzp, gacc, res
-    if (precond && alone) DISPATCH_SDO(g.sdo, k_adj_vel_grad, , true, true);
-    else if (precond) DISPATCH_SDO(g.sdo, k_adj_vel_grad, , true, false);
-    else if (alone) DISPATCH_SDO(g.sdo, k_adj_vel_grad, , false, true);
-    else DISPATCH_SDO(g.sdo, k_adj_vel_grad, , false, false);
+    if (precond && alone) DISPATCH_MARCH(g.sdo, spb, k_adj_vel_grad, , true, true);
+    else if (precond) DISPATCH_MARCH(g.sdo, spb, k_adj_vel_grad, , true, false);
+    else if (alone) DISPATCH_MARCH(g.sdo, spb, k_adj_vel_grad, , false, true);
+    else DISPATCH_MARCH(g.sdo, spb, k_adj_vel_grad, , false, false);
 #undef ARGS
 }
 

@@ -6,7 +6,7 @@
 //   adj     [nshots][5][fstride]  adjoint state
 //   av      2 x [nshots][2][fstride]  exact mode velocity adjoint ping-pong
 //   gacc    [nsets][6][fstride]  gradient accumulators: one set per shot (exact
-//           mode) or per group of MARCH_SHOTS shots (reference mode)
+//           mode) or per block's group of shots (reference mode)
 //   snaps   [nshots][nchp][5][fstride]  reference checkpoints (pyfwi mode)
 //   ckpts   [nshots][ncp][5][fstride]   exact-mode checkpoints every K steps
 //   seg     [nshots][K][5][fstride]     exact-mode recomputed segment
@@ -729,7 +729,7 @@ int32_t pyfwi_forward(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, co
 
 namespace {
 
-// Reference adjoint of one pass (shots g0 .. g0+ng-1) into gacc[adj_acc_sets(ng)].
+// Reference adjoint of one pass (shots g0 .. g0+ng-1) into gacc[adj_acc_sets(g.sdo, ng)].
 // `snaps` holds the pass's kept states, per-shot stride cl.size() states, at
 // the steps of `cl`.
 int gradient_pyfwi(pyfwi_plan *p, int g0, int ng, const float *d_res, bool precond,
@@ -902,7 +902,7 @@ int32_t pyfwi_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, c
     const bool reuse = mode == PYFWI_MODE_EXACT && same && p->ckpts_all && p->ckpt_every == K;
     for (int g0 = 0; g0 < nshots; g0 += G) {
         const int ng = std::min(G, nshots - g0);
-        const int nsets = mode == PYFWI_MODE_PYFWI ? adj_acc_sets(ng) : ng;
+        const int nsets = mode == PYFWI_MODE_PYFWI ? adj_acc_sets(g.sdo, ng) : ng;
         if (int e = ensure(p->gacc, (size_t)nsets * 6 * g.fstride, p->stream, "gradient"))
             return e;
         launch_zero(p->gacc.p, (int64_t)nsets * 6 * g.fstride, p->stream);
@@ -1026,7 +1026,7 @@ int32_t pyfwi_misfit_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t ns
             return e;
         launch_l2_residual(ng, p->d_shots + g0, d_seis, d_obs, p->resid.p, comp_mask, p->d_misfit,
                            p->stream);
-        const int nsets = adj_acc_sets(ng);
+        const int nsets = adj_acc_sets(g.sdo, ng);
         if (int e = ensure(p->gacc, (size_t)nsets * 6 * g.fstride, p->stream, "gradient"))
             return e;
         launch_zero(p->gacc.p, (int64_t)nsets * 6 * g.fstride, p->stream);
