@@ -19,6 +19,11 @@ constexpr int MARCH_ROWS_MAX = 64;
 // than 22-30 per CU (12-17-row strips), although the forward kernels'
 // registers allow 32; 13-17 per CU loses 2-6 % again.
 constexpr int MARCH_WAVES_PER_CU = 20;
+// One-shot launches (one working wave per block) want many more waves: on
+// MI355X, 4-row strips ran a 540 x 1040 one-shot gradient 21 % faster than
+// 8-row strips, and 12-row strips a 2040 x 6040 one 12 % faster than 39-row.
+constexpr int MARCH_WAVES_PER_CU_ONE_SHOT = 48;
+constexpr int MARCH_ROWS_MIN_ONE_SHOT = 4;
 constexpr int NUM_XCD = 8;          // MI355X: 8 XCDs, each with its own L2
 constexpr int FUSED_WAVES = 4;      // fused two-stage kernels: wavefronts per block
 #ifndef MARCH_UNROLL

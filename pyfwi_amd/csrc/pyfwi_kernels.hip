@@ -1754,17 +1754,18 @@ static dim3 stencil_grid(const DevGrid &g, int sdo, int nshots)
 }
 
 // Strip height for a launch of `per_strip` working wavefronts per row strip:
-// as many strips as keep the launch within MARCH_WAVES_PER_CU such waves per
-// CU (tall strips, fewer halo rows, when many shots are batched; short strips
-// when a single shot must still fill the GPU), within the row bounds.  Waves
-// of a block without a shot leave at once and are not counted.
-static int strip_rows(const DevGrid &g, int nz_act, int64_t per_strip)
+// as many strips as keep the launch within `per_cu` such waves per CU (tall
+// strips, fewer halo rows, when many shots are batched; short strips when a
+// single shot must still fill the GPU), within [min_rows, MARCH_ROWS_MAX].
+// Waves of a block without a shot leave at once and are not counted.
+static int strip_rows(const DevGrid &g, int nz_act, int64_t per_strip,
+                      int per_cu = MARCH_WAVES_PER_CU, int min_rows = MARCH_ROWS_MIN)
 {
     if (g.rows_override > 0) return g.rows_override;
-    const int64_t room = (int64_t)MARCH_WAVES_PER_CU * g.ncu;
+    const int64_t room = (int64_t)per_cu * g.ncu;
     const int nstrips = (int)std::max<int64_t>(1, std::min<int64_t>(nz_act, room / per_strip));
     const int rows = (nz_act + nstrips - 1) / nstrips;
-    return std::max(MARCH_ROWS_MIN, std::min(MARCH_ROWS_MAX, rows));
+    return std::max(min_rows, std::min(MARCH_ROWS_MAX, rows));
 }
 
 // Shots per column-marching block.  Five when a launch's shots fill blocks
@@ -1786,7 +1787,10 @@ static dim3 march_grid(DevGrid &g, int sdo, int nshots, int spb)
     g.ncb = (nx_act + 63) / 64;
     g.nbatch = (nshots + spb - 1) / spb;
     g.nshot = nshots;
-    const int rows = strip_rows(g, nz_act, (int64_t)g.ncb * nshots);
+    const int rows = nshots == 1
+                         ? strip_rows(g, nz_act, g.ncb, MARCH_WAVES_PER_CU_ONE_SHOT,
+                                      MARCH_ROWS_MIN_ONE_SHOT)
+                         : strip_rows(g, nz_act, (int64_t)g.ncb * nshots);
     g.mrows = rows;
     g.nstrips = (nz_act + rows - 1) / rows;
     const int T = g.ncb * g.nstrips * g.nbatch;
