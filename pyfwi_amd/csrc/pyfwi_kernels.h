@@ -9,8 +9,6 @@
 namespace pyfwi {
 
 constexpr int HALO = 8;      // zero rows above and below each padded field (>= 2*SDO)
-constexpr int TILE_X = 64;   // one wavefront along x
-constexpr int TILE_Y = 4;    // 4 wavefronts per block along z
 constexpr int MARCH_ROWS_MIN = 8;   // strip height bounds (rows walked by a thread)
 constexpr int MARCH_ROWS_MAX = 64;
 // Wavefronts with work per CU that one launch aims for; the strip height is

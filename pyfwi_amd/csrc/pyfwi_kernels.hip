@@ -51,80 +51,9 @@ __device__ __forceinline__ float dplus(const float *__restrict__ p, int s,
     return r;
 }
 
-// Same with each tap the product of two arrays, a[k]*f[k].
-template <int SDO>
-__device__ __forceinline__ float dminus2(const float *__restrict__ f,
-                                         const float *__restrict__ a, int s,
-                                         const float c[4])
-{
-    auto v = [&](int k) { return f[k] * a[k]; };
-    float r = c[0] * (v(0) - v(-s));
-    r = fmaf(c[1], v(s) - v(-2 * s), r);
-    if constexpr (SDO == 4) {
-        r = fmaf(c[2], v(2 * s) - v(-3 * s), r);
-        r = fmaf(c[3], v(3 * s) - v(-4 * s), r);
-    }
-    return r;
-}
-
-template <int SDO>
-__device__ __forceinline__ float dplus2(const float *__restrict__ f,
-                                        const float *__restrict__ a, int s,
-                                        const float c[4])
-{
-    auto v = [&](int k) { return f[k] * a[k]; };
-    float r = c[0] * (v(s) - v(0));
-    r = fmaf(c[1], v(2 * s) - v(-s), r);
-    if constexpr (SDO == 4) {
-        r = fmaf(c[2], v(3 * s) - v(-2 * s), r);
-        r = fmaf(c[3], v(4 * s) - v(-3 * s), r);
-    }
-    return r;
-}
-
-// Taps of (lam + 2 mu) * f.
-template <int SDO>
-__device__ __forceinline__ float dminus_m(const float *__restrict__ f,
-                                          const float *__restrict__ l,
-                                          const float *__restrict__ m, int s,
-                                          const float c[4])
-{
-    auto v = [&](int k) { return (l[k] + 2.0f * m[k]) * f[k]; };
-    float r = c[0] * (v(0) - v(-s));
-    r = fmaf(c[1], v(s) - v(-2 * s), r);
-    if constexpr (SDO == 4) {
-        r = fmaf(c[2], v(2 * s) - v(-3 * s), r);
-        r = fmaf(c[3], v(3 * s) - v(-4 * s), r);
-    }
-    return r;
-}
-
-template <int SDO>
-__device__ __forceinline__ float dplus_m(const float *__restrict__ f,
-                                         const float *__restrict__ l,
-                                         const float *__restrict__ m, int s,
-                                         const float c[4])
-{
-    auto v = [&](int k) { return (l[k] + 2.0f * m[k]) * f[k]; };
-    float r = c[0] * (v(s) - v(0));
-    r = fmaf(c[1], v(2 * s) - v(-s), r);
-    if constexpr (SDO == 4) {
-        r = fmaf(c[2], v(3 * s) - v(-2 * s), r);
-        r = fmaf(c[3], v(4 * s) - v(-3 * s), r);
-    }
-    return r;
-}
-
-// Active shot test and cell coordinates shared by all stencil kernels.
-// Forward steps run for t < nt; backward steps (reconstruction and adjoint)
-// for 1 <= t < nt, the reference's loop `for t in range(nt-1, 0, -1)`.
-template <int SDO>
-__device__ __forceinline__ bool cell(const DevGrid &g, int &i, int &j)
-{
-    j = SDO + (int)(blockIdx.x * TILE_X + threadIdx.x);
-    i = SDO + (int)(blockIdx.y * TILE_Y + threadIdx.y);
-    return i <= g.nz - SDO && j <= g.nx - SDO;
-}
+// Active shot tests shared by the stencil kernels.  Forward steps run for
+// t < nt; backward steps (reconstruction and adjoint) for 1 <= t < nt, the
+// reference's loop `for t in range(nt-1, 0, -1)`.
 
 __device__ __forceinline__ bool fwd_active(const pyfwi_shot *shots, int s, int t)
 {
@@ -1443,124 +1372,175 @@ __global__ __launch_bounds__(64 * SPB) void k_adj_vel_grad(
 // The velocity adjoint ping-pongs between two buffers because its taps are
 // read while its centre is written.
 
-__device__ __forceinline__ float keepf(const DevGrid &g, const float *dxp,
-                                       const float *dzp, int i, int j)
-{
-    return 1.0f - g.dt * (dxp[j] + dzp[i]);
-}
-
-// Derivative taps of q = w(k) * keep(k) for a field w along x (s=1) or z.
-template <int SDO, int PLUS, bool ALONG_X>
-__device__ __forceinline__ float dq(const DevGrid &g, const float *w, const float *coef_a,
-                                    const float *coef_b, int mode, const float *dxp,
-                                    const float *dzp, int i, int j, const float c[4])
-{
-    // mode 0: w*keep; 1: a*w*keep; 2: (a + 2 b)*w*keep
-    auto val = [&](int k) {
-        const int ii = ALONG_X ? i : i + k;
-        const int jj = ALONG_X ? j + k : j;
-        const int64_t off = ALONG_X ? k : (int64_t)k * g.pitch;
-        float v = w[off] * keepf(g, dxp, dzp, ii, jj);
-        if (mode == 1) v = coef_a[off] * v;
-        if (mode == 2) v = (coef_a[off] + 2.0f * coef_b[off]) * v;
-        return v;
-    };
-    float r;
-    if (PLUS) {
-        r = c[0] * (val(1) - val(0));
-        r = fmaf(c[1], val(2) - val(-1), r);
-        if constexpr (SDO == 4) {
-            r = fmaf(c[2], val(3) - val(-2), r);
-            r = fmaf(c[3], val(4) - val(-3), r);
-        }
-    } else {
-        r = c[0] * (val(0) - val(-1));
-        r = fmaf(c[1], val(1) - val(-2), r);
-        if constexpr (SDO == 4) {
-            r = fmaf(c[2], val(2) - val(-3), r);
-            r = fmaf(c[3], val(3) - val(-4), r);
-        }
-    }
-    return r;
-}
-
 // vel: [nshots][2][fs] velocities after the velocity half step of step t
 // (V_t); gacc: [nshots][6][fs] with slots 0 (mu) and 2 (lam) updated.
-template <int SDO>
-__global__ __launch_bounds__(TILE_X *TILE_Y) void k_exact_str(
+// Column marching as the other stencil kernels: z-windows of the products
+// differentiated along z, x-derivatives on the entering row delayed SDO rows.
+template <int SDO, int SPB>
+__global__ __launch_bounds__(64 * SPB) void k_exact_str(
     DevGrid g, const pyfwi_shot *shots, int t, const float *vel, int64_t vel_shot_stride,
     float *adj, float *av, const float *__restrict__ lam,
     const float *__restrict__ mu, const float *__restrict__ dxp,
     const float *__restrict__ dzp, float *gacc)
 {
-    const int s = blockIdx.z;
+    int s, j, i0, i1;
+    if (!tile<SDO, SPB>(g, s, j, i0, i1)) return;
     if (t >= shots[s].nt) return;
-    int i, j;
-    if (!cell<SDO>(g, i, j)) return;
     const int64_t fs = g.fstride;
-    const int64_t c = (int64_t)i * g.pitch + j;
+    const uint32_t RB = (uint32_t)g.pitch * 4u;
+    const float dt = g.dt;
     float *A = adj + (int64_t)s * 5 * fs + g.origin;
     float *Av = av + (int64_t)s * 2 * fs + g.origin;
-    const float *qxx = A + 2 * fs + c, *qzz = A + 3 * fs + c, *qxz = A + 4 * fs + c;
-    const float *l = lam + g.origin + c, *m = mu + g.origin + c;
-    const float sx = dq<SDO, 0, true>(g, qxx, l, m, 2, dxp, dzp, i, j, g.cx)
-                   + dq<SDO, 0, true>(g, qzz, l, m, 1, dxp, dzp, i, j, g.cx)
-                   + dq<SDO, 0, false>(g, qxz, m, m, 1, dxp, dzp, i, j, g.cz);
-    const float sz = dq<SDO, 1, false>(g, qxx, l, m, 1, dxp, dzp, i, j, g.cz)
-                   + dq<SDO, 1, false>(g, qzz, l, m, 2, dxp, dzp, i, j, g.cz)
-                   + dq<SDO, 1, true>(g, qxz, m, m, 1, dxp, dzp, i, j, g.cx);
-    // the velocity adjoint lives in the current ping-pong buffer
-    Av[c] = fmaf(-g.dt, sx, Av[c]);
-    Av[fs + c] = fmaf(-g.dt, sz, Av[fs + c]);
-    const float keep = keepf(g, dxp, dzp, i, j);
-    const float q1 = qxx[0] * keep, q2 = qzz[0] * keep, q3 = qxz[0] * keep;
-    const float *V = vel + (int64_t)s * vel_shot_stride + g.origin + c;
-    const float exx = dplus<SDO>(V, 1, g.cx);
-    const float ezz = dminus<SDO>(V + fs, g.pitch, g.cz);
-    const float gxz = dminus<SDO>(V + fs, 1, g.cx) + dplus<SDO>(V, g.pitch, g.cz);
-    float *G = gacc + (int64_t)s * 6 * fs + g.origin + c;
-    G[2 * fs] += g.dt * (q1 + q2) * (exx + ezz);
-    G[0] += g.dt * (2.0f * q1 * exx + 2.0f * q2 * ezz + q3 * gxz);
+    const CellRef qxx{A + 2 * fs}, qzz{A + 3 * fs}, qxz{A + 4 * fs};
+    const CellRef L{lam + g.origin}, M{mu + g.origin};
+    const float *V = vel + (int64_t)s * vel_shot_stride + g.origin;
+    const CellRef vx{V}, vz{V + fs};
+    float *G = gacc + (int64_t)s * 6 * fs + g.origin;
+    // keep = 1 - dt*(dx + dz) of row r at column offset k
+    auto kp = [&](int r, int k) { return 1.0f - dt * (dxp[j + k] + dzp[r]); };
+    ZWin<SDO> wa3, wb1, wb2;  // mu*qxz*keep (d/dz-), lam*qxx*keep, (lam+2mu)*qzz*keep (d/dz+)
+    ZWin<SDO> wvz, wvx;       // forward vz (d/dz-), vx (d/dz+)
+    Fifo<SDO> fa1, fa2, fa3, fvx, fvz;  // x-derivatives of the entering row
+    auto x_a1 = [&](uint32_t o, int r) {
+        return xminus<SDO>([&](int k) {
+            return (L.at(o, k) + 2.0f * M.at(o, k)) * (qxx.at(o, k) * kp(r, k)); }, g.cx); };
+    auto x_a2 = [&](uint32_t o, int r) {
+        return xminus<SDO>([&](int k) { return L.at(o, k) * (qzz.at(o, k) * kp(r, k)); }, g.cx); };
+    auto x_a3 = [&](uint32_t o, int r) {
+        return xplus<SDO>([&](int k) { return M.at(o, k) * (qxz.at(o, k) * kp(r, k)); }, g.cx); };
+    auto x_vx = [&](uint32_t o) { return xplus<SDO>([&](int k) { return vx.at(o, k); }, g.cx); };
+    auto x_vz = [&](uint32_t o) { return xminus<SDO>([&](int k) { return vz.at(o, k); }, g.cx); };
+    auto z_a3 = [&](uint32_t o, int r) { return M(o) * (qxz(o) * kp(r, 0)); };
+    auto z_b1 = [&](uint32_t o, int r) { return L(o) * (qxx(o) * kp(r, 0)); };
+    auto z_b2 = [&](uint32_t o, int r) { return (L(o) + 2.0f * M(o)) * (qzz(o) * kp(r, 0)); };
+    uint32_t cb = ((uint32_t)i0 * (uint32_t)g.pitch + (uint32_t)j) * 4u;
+#pragma unroll
+    for (int k = 0; k < 2 * SDO; ++k) {
+        const uint32_t o = cb + (uint32_t)(k - SDO) * RB;
+        const int r = i0 + k - SDO;
+        wa3.v[k + 1] = z_a3(o, r);
+        wb1.v[k + 1] = z_b1(o, r);
+        wb2.v[k + 1] = z_b2(o, r);
+        wvz.v[k + 1] = vz(o);
+        wvx.v[k + 1] = vx(o);
+        if (k >= SDO) {
+            const int q = k - SDO;
+            fa1.v[q] = x_a1(o, r);
+            fa2.v[q] = x_a2(o, r);
+            fa3.v[q] = x_a3(o, r);
+            fvx.v[q] = x_vx(o);
+            fvz.v[q] = x_vz(o);
+        }
+    }
+#pragma unroll MARCH_UNROLL
+    for (int i = i0; i <= i1; ++i, cb += RB) {
+        const uint32_t n = cb + (uint32_t)SDO * RB;
+        const int rn = i + SDO;
+        wa3.push(z_a3(n, rn));
+        wb1.push(z_b1(n, rn));
+        wb2.push(z_b2(n, rn));
+        wvz.push(vz(n));
+        wvx.push(vx(n));
+        const float a1 = fa1.pop(x_a1(n, rn)), a2 = fa2.pop(x_a2(n, rn));
+        const float a3x = fa3.pop(x_a3(n, rn));
+        const float exx = fvx.pop(x_vx(n)), gzx = fvz.pop(x_vz(n));
+        float *pax = &cell_ref(Av, cb), *paz = &cell_ref(Av + fs, cb);
+        const float ax0 = *pax, az0 = *paz;
+        const float keep = kp(i, 0);
+        const float q1 = qxx(cb) * keep, q2 = qzz(cb) * keep, q3 = qxz(cb) * keep;
+        float *g0 = &cell_ref(G, cb), *g2 = &cell_ref(G + 2 * fs, cb);
+        const float gmu = *g0, glam = *g2;
+        const float sx = a1 + a2 + wa3.dminus(g.cz);
+        const float sz = wb1.dplus(g.cz) + wb2.dplus(g.cz) + a3x;
+        *pax = fmaf(-dt, sx, ax0);
+        *paz = fmaf(-dt, sz, az0);
+        const float ezz = wvz.dminus(g.cz);
+        const float gxz = gzx + wvx.dplus(g.cz);
+        *g2 = glam + dt * (q1 + q2) * (exx + ezz);
+        *g0 = gmu + dt * (2.0f * q1 * exx + 2.0f * q2 * ezz + q3 * gxz);
+    }
 }
 
 // tau: [nshots][3][fs] stresses after the injection of step t (input of the
 // update); gacc slot 4 accumulates dJ/db.  Writes the new stress adjoint in
-// place and p = Av*keep to av_out.  Then adds residual row t/dt_scale at the
-// receivers when step t is sampled.
-template <int SDO>
-__global__ __launch_bounds__(TILE_X *TILE_Y) void k_exact_vel(
+// place and p = Av*keep to av_out.  (The receiver sampling of S_t, which adds
+// the residual row, is k_exact_inject's.)
+template <int SDO, int SPB>
+__global__ __launch_bounds__(64 * SPB) void k_exact_vel(
     DevGrid g, const pyfwi_shot *shots, int t, const float *tau, int64_t tau_shot_stride,
     float *adj, const float *av_in, float *av_out, const float *__restrict__ bu,
     const float *__restrict__ dxp, const float *__restrict__ dzp, float *gacc)
 {
-    const int s = blockIdx.z;
+    int s, j, i0, i1;
+    if (!tile<SDO, SPB>(g, s, j, i0, i1)) return;
     if (t >= shots[s].nt) return;
-    int i, j;
-    if (!cell<SDO>(g, i, j)) return;
     const int64_t fs = g.fstride;
-    const int64_t c = (int64_t)i * g.pitch + j;
+    const uint32_t RB = (uint32_t)g.pitch * 4u;
+    const float dt = g.dt;
     float *A = adj + (int64_t)s * 5 * fs + g.origin;
     const float *Ain = av_in + (int64_t)s * 2 * fs + g.origin;
     float *Aout = av_out + (int64_t)s * 2 * fs + g.origin;
-    const float *pvx = Ain + c, *pvz = Ain + fs + c;
-    const float *b = bu + g.origin + c;
-    const float rxx = dq<SDO, 1, true>(g, pvx, b, b, 1, dxp, dzp, i, j, g.cx);
-    const float rzz = dq<SDO, 0, false>(g, pvz, b, b, 1, dxp, dzp, i, j, g.cz);
-    const float rxz = dq<SDO, 1, false>(g, pvx, b, b, 1, dxp, dzp, i, j, g.cz)
-                    + dq<SDO, 0, true>(g, pvz, b, b, 1, dxp, dzp, i, j, g.cx);
-    const float keep = keepf(g, dxp, dzp, i, j);
-    const float px = pvx[0] * keep, pz = pvz[0] * keep;
-    float *atxx = A + 2 * fs, *atzz = A + 3 * fs, *atxz = A + 4 * fs;
-    atxx[c] = fmaf(-g.dt, rxx, atxx[c] * keep);
-    atzz[c] = fmaf(-g.dt, rzz, atzz[c] * keep);
-    atxz[c] = fmaf(-g.dt, rxz, atxz[c] * keep);
-    Aout[c] = px;
-    Aout[fs + c] = pz;
-    const float *T = tau + (int64_t)s * tau_shot_stride + g.origin + c;
-    const float dtau_x = dminus<SDO>(T, 1, g.cx) + dminus<SDO>(T + 2 * fs, g.pitch, g.cz);
-    const float dtau_z = dplus<SDO>(T + fs, g.pitch, g.cz) + dplus<SDO>(T + 2 * fs, 1, g.cx);
-    float *G = gacc + (int64_t)s * 6 * fs + g.origin + c;
-    G[4 * fs] += g.dt * (px * dtau_x + pz * dtau_z);
+    const CellRef pvx{Ain}, pvz{Ain + fs}, B{bu + g.origin};
+    const float *T = tau + (int64_t)s * tau_shot_stride + g.origin;
+    const CellRef txx{T}, tzz{T + fs}, txz{T + 2 * fs};
+    float *G = gacc + (int64_t)s * 6 * fs + g.origin;
+    auto kp = [&](int r, int k) { return 1.0f - dt * (dxp[j + k] + dzp[r]); };
+    ZWin<SDO> wbz, wbx;    // b*pvz*keep (d/dz-), b*pvx*keep (d/dz+)
+    ZWin<SDO> wtxz, wtzz;  // forward txz (d/dz-), tzz (d/dz+)
+    Fifo<SDO> frxx, frxz, ftxx, ftxz;  // x-derivatives of the entering row
+    auto x_bx = [&](uint32_t o, int r) {
+        return xplus<SDO>([&](int k) { return B.at(o, k) * (pvx.at(o, k) * kp(r, k)); }, g.cx); };
+    auto x_bz = [&](uint32_t o, int r) {
+        return xminus<SDO>([&](int k) { return B.at(o, k) * (pvz.at(o, k) * kp(r, k)); }, g.cx); };
+    auto x_txx = [&](uint32_t o) { return xminus<SDO>([&](int k) { return txx.at(o, k); }, g.cx); };
+    auto x_txz = [&](uint32_t o) { return xplus<SDO>([&](int k) { return txz.at(o, k); }, g.cx); };
+    auto z_bz = [&](uint32_t o, int r) { return B(o) * (pvz(o) * kp(r, 0)); };
+    auto z_bx = [&](uint32_t o, int r) { return B(o) * (pvx(o) * kp(r, 0)); };
+    uint32_t cb = ((uint32_t)i0 * (uint32_t)g.pitch + (uint32_t)j) * 4u;
+#pragma unroll
+    for (int k = 0; k < 2 * SDO; ++k) {
+        const uint32_t o = cb + (uint32_t)(k - SDO) * RB;
+        const int r = i0 + k - SDO;
+        wbz.v[k + 1] = z_bz(o, r);
+        wbx.v[k + 1] = z_bx(o, r);
+        wtxz.v[k + 1] = txz(o);
+        wtzz.v[k + 1] = tzz(o);
+        if (k >= SDO) {
+            const int q = k - SDO;
+            frxx.v[q] = x_bx(o, r);
+            frxz.v[q] = x_bz(o, r);
+            ftxx.v[q] = x_txx(o);
+            ftxz.v[q] = x_txz(o);
+        }
+    }
+#pragma unroll MARCH_UNROLL
+    for (int i = i0; i <= i1; ++i, cb += RB) {
+        const uint32_t n = cb + (uint32_t)SDO * RB;
+        const int rn = i + SDO;
+        wbz.push(z_bz(n, rn));
+        wbx.push(z_bx(n, rn));
+        wtxz.push(txz(n));
+        wtzz.push(tzz(n));
+        const float rxx = frxx.pop(x_bx(n, rn)), rxz_x = frxz.pop(x_bz(n, rn));
+        const float dtx = ftxx.pop(x_txx(n)), dtz = ftxz.pop(x_txz(n));
+        const float keep = kp(i, 0);
+        const float px = pvx(cb) * keep, pz = pvz(cb) * keep;
+        float *pxx = &cell_ref(A + 2 * fs, cb), *pzz = &cell_ref(A + 3 * fs, cb),
+              *pxz = &cell_ref(A + 4 * fs, cb);
+        const float axx = *pxx, azz = *pzz, axz = *pxz;
+        float *g4 = &cell_ref(G + 4 * fs, cb);
+        const float gb = *g4;
+        const float rzz = wbz.dminus(g.cz);
+        const float rxz = wbx.dplus(g.cz) + rxz_x;
+        *pxx = fmaf(-dt, rxx, axx * keep);
+        *pzz = fmaf(-dt, rzz, azz * keep);
+        *pxz = fmaf(-dt, rxz, axz * keep);
+        cell_ref(Aout, cb) = px;
+        cell_ref(Aout + fs, cb) = pz;
+        const float dtau_x = dtx + wtxz.dminus(g.cz);
+        const float dtau_z = wtzz.dplus(g.cz) + dtz;
+        *g4 = gb + dt * (px * dtau_x + pz * dtau_z);
+    }
 }
 
 // Residual of sample t/dt_scale added at the receivers after backward step t
@@ -1747,11 +1727,6 @@ __global__ void k_unpad(DevGrid g, const float *src, float *dst, int nf)
 
 // ------------------------------------------------------- launch wrappers
 
-static dim3 stencil_grid(const DevGrid &g, int sdo, int nshots)
-{
-    const int nx_act = g.nx - 2 * sdo + 1, nz_act = g.nz - 2 * sdo + 1;
-    return dim3((nx_act + TILE_X - 1) / TILE_X, (nz_act + TILE_Y - 1) / TILE_Y, nshots);
-}
 
 // Strip height for a launch of `per_strip` working wavefronts per row strip:
 // as many strips as keep the launch within `per_cu` such waves per CU (tall
@@ -1926,9 +1901,11 @@ void launch_exact_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int
                       const float *lam, const float *mu, const float *dxp,
                       const float *dzp, float *gacc, hipStream_t st)
 {
-    dim3 grid = stencil_grid(g, g.sdo, nshots), block(TILE_X, TILE_Y);
-#define ARGS g, shots, t, vel, vel_stride, adj, av, lam, mu, dxp, dzp, gacc
-    DISPATCH_SDO(g.sdo, k_exact_str, );
+    DevGrid gm = g;
+    const int spb = march_spb(g.sdo, nshots);
+    dim3 grid = march_grid(gm, g.sdo, nshots, spb), block(64 * spb);
+#define ARGS gm, shots, t, vel, vel_stride, adj, av, lam, mu, dxp, dzp, gacc
+    DISPATCH_MARCH(g.sdo, spb, k_exact_str, );
 #undef ARGS
 }
 
@@ -1937,9 +1914,11 @@ void launch_exact_vel(const DevGrid &g, int nshots, const pyfwi_shot *shots, int
                       float *av_out, const float *bu, const float *dxp, const float *dzp,
                       float *gacc, hipStream_t st)
 {
-    dim3 grid = stencil_grid(g, g.sdo, nshots), block(TILE_X, TILE_Y);
-#define ARGS g, shots, t, tau, tau_stride, adj, av_in, av_out, bu, dxp, dzp, gacc
-    DISPATCH_SDO(g.sdo, k_exact_vel, );
+    DevGrid gm = g;
+    const int spb = march_spb(g.sdo, nshots);
+    dim3 grid = march_grid(gm, g.sdo, nshots, spb), block(64 * spb);
+#define ARGS gm, shots, t, tau, tau_stride, adj, av_in, av_out, bu, dxp, dzp, gacc
+    DISPATCH_MARCH(g.sdo, spb, k_exact_vel, );
 #undef ARGS
 }
 
