@@ -68,7 +68,8 @@ __device__ __forceinline__ bool bwd_active(const pyfwi_shot *shots, int s, int t
 // ------------------------------------------------------ column marching
 //
 // The stencil kernels below give each thread one column j and a strip of
-// rows (height chosen per launch, see march_grid).  The thread walks down the strip keeping, for every field
+// rows (height chosen per launch, see strip_rows).  The thread walks down the
+// strip keeping, for every field
 // differentiated along z, a register window of the 2*SDO+1 rows around the
 // current row, so each row of such a field is fetched from memory once.
 // Derivatives along x read the neighbouring columns of the same row; those
@@ -257,16 +258,6 @@ struct LaneRow {
     __device__ __forceinline__ float operator()(int k) const { return s[SDO + k]; }
 };
 
-// Tile of this block and shot / column / row strip of this thread.
-//
-// A block is 64*SPB threads: SPB shots side by side, one wave each, over the
-// same 64 columns of one row strip (a narrow grid leaves few lanes idle in
-// its last column block).  Launches are 1-D.  The tiles (shot group, column block, row
-// strip) are numbered shot group fastest; workgroup w runs tile
-// (w % 8) * ceil(T/8) + w / 8.  Workgroups are handed to the 8 XCDs in turn,
-// so each XCD receives one contiguous run of tiles: the same cells of every
-// shot of the launch meet in one L2, and neighbouring column blocks, which
-// share halo columns, run side by side.
 // Index of the calling wave in its block, in a scalar register: the compiler
 // cannot tell on its own that threadIdx.x / 64 is the same for a whole wave,
 // and would otherwise keep the shot index and every pointer derived from it
@@ -276,6 +267,17 @@ __device__ __forceinline__ int wave_id()
     return __builtin_amdgcn_readfirstlane((int)threadIdx.x / 64);
 }
 
+// Tile of this block and shot / column / row strip of this thread.
+//
+// A block is 64*SPB threads: SPB shots side by side, one wave each, over the
+// same 64 columns of one row strip (a narrow grid leaves few lanes idle in
+// its last column block).  Launches are 1-D.  The tiles (shot group, column
+// block, row strip) are numbered shot group fastest; workgroup w runs tile
+// (w % 8) * ceil(T/8) + w / 8.  Workgroups are handed to the 8 XCDs in turn,
+// so each XCD receives one contiguous run of tiles: the same cells of every
+// shot of the launch meet in one L2, and neighbouring column blocks, which
+// share halo columns, run side by side.  lane_ok: the lane's column is on
+// the grid; the result: the wave has a shot.
 template <int SDO, int SPB>
 __device__ __forceinline__ bool tile_lane(const DevGrid &g, int &s, int &j, int &i0, int &i1,
                                           bool &lane_ok)
