@@ -69,9 +69,9 @@ __device__ __forceinline__ bool bwd_active(const pyfwi_shot *shots, int s, int t
 //
 // The stencil kernels below give each thread one column j and a strip of
 // rows (height chosen per launch, see strip_rows).  The thread walks down the
-// strip keeping, for every field
-// differentiated along z, a register window of the 2*SDO+1 rows around the
-// current row, so each row of such a field is fetched from memory once.
+// strip keeping, for every field differentiated along z, a register window of
+// the 2*SDO+1 rows around the current row, so each row of such a field is
+// fetched from memory once.
 // Derivatives along x read the neighbouring columns of the same row; those
 // loads are coalesced across the wavefront and served by L1 / L2.
 
