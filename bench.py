@@ -47,8 +47,8 @@ KERNEL_BYTES = {
     "rev_vel": (32, 0, 0),
     "adj_str": (36, 8, 16),      # all: R avx avz b 3 atau, W 3 atau; interior: R fwd vx vz; group: RW g_mu g_lam
     "adj_vel_grad": (36, 16, 8),  # all: R 3 atau lam mu 2 av, W 2 av; interior: R fwd 3 tau, b; group: RW g_rho
-    "exact_str": (48, 0, 0),
-    "exact_vel": (52, 0, 0),
+    "exact_str": (60, 0, 0),      # R 3 atau lam mu 2 av 2 fwd v, RW g_mu g_lam (per shot), W 2 av
+    "exact_vel": (64, 0, 0),      # R 2 av b 3 fwd tau 3 atau, RW g_b (per shot), W 3 atau 2 av
     "fwd_step": (52, 0, 0),       # R 5 fields lam mu b, W 5 fields
     "rev_step": (52, 0, 0),
 }
@@ -161,6 +161,22 @@ def gradient_step(W, b, shots=None, seis=None, obs=None, res=None):
     plan.gradient(shots, b["src"], res, b["grad"], _native.MODE_PYFWI, 0, 0, _native.MEM_DEVICE)
 
 
+EXACT_CKPT = 50  # checkpoint spacing of the exact-adjoint bench step
+
+
+def exact_step(W, b, shots=None):
+    """One forward (checkpoints every EXACT_CKPT steps) and exact-adjoint
+    gradient of the given shots (all by default); segments are recomputed."""
+    from pyfwi_amd import _native
+    plan = b["plan"]
+    shots = b["shots"] if shots is None else shots
+    plan.forward(shots, b["src"], b["seis"], None, EXACT_CKPT, _native.MEM_DEVICE)
+    b["res"].copy_(b["seis"])
+    b["res"].sub_(b["obs"])
+    plan.gradient(shots, b["src"], b["res"], b["grad"], _native.MODE_EXACT, EXACT_CKPT, 0,
+                  _native.MEM_DEVICE)
+
+
 def per_shot_rate(W, b, torch, nsub, step):
     """The plain per-shot path: the same kernels on one shot at a time (one
     shot per pass, the reference's schedule), on the first `nsub` shots."""
@@ -220,6 +236,10 @@ def kernel_profile(W, b, torch, step):
     plan.set_timing(0)
     groups = {"fwd": ("fwd_vel", "fwd_str", "fwd_step", "fwd_sparse"),
               "bwd": ("rev_str", "rev_vel", "rev_step", "adj_str", "adj_vel_grad", "bwd_sparse")}
+    if "exact_str" in per:
+        # the exact adjoint's backward sweep interleaves the segment
+        # recomputation and brackets no loop: bracketed launch times as they are
+        return per
     out = {}
     for loop, kinds in groups.items():
         ks = [k for k in kinds if k in per]
@@ -354,7 +374,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the execution-shape autotuner")
     ap.add_argument("--tune", default="", help="fixed tuning 'shots_per_pass,fused,rows'")
-    ap.add_argument("--mode", default="kept", choices=["kept", "reverse"],
+    ap.add_argument("--mode", default="kept", choices=["kept", "reverse", "exact"],
                     help="kept: every forward state kept in HBM per pass (chpr=100), misfit and "
                          "gradient in one call; reverse: chpr=0, states rebuilt by time reversal")
     args = ap.parse_args()
@@ -380,7 +400,7 @@ def main():
     units = int(W.ns) * cells * W.nt
     log(f"bench: {w.name}: {int(W.ns)} shots x {W.nt} steps x {cells} cells")
 
-    step = kept_step if args.mode == "kept" else gradient_step
+    step = {"kept": kept_step, "reverse": gradient_step, "exact": exact_step}[args.mode]
     state_bytes = 5 * 4 * (int(W.tnz) + 16) * (((int(W.tnx) + 8 + 63) // 64) * 64)
     free, _total = torch.cuda.mem_get_info()
     max_pass = int(0.8 * free // (state_bytes * W.nt)) if args.mode == "kept" else None
@@ -392,7 +412,7 @@ def main():
         g, f, r = (int(x) for x in args.tune.split(","))
         tuning = (g, bool(f), r)
         b["plan"].set_tuning(*tuning)
-    elif not args.no_tune:
+    elif not args.no_tune and args.mode != "exact":
         from pyfwi_amd import tuning as tn
         if args.mode == "kept":
             trial = tn.truncate(b["shots"], 120)
@@ -478,14 +498,19 @@ def main():
         "config": {"workload": w.name, "nz": int(W.nz), "nx": int(W.nx), "npml": int(W.npml),
                    "cells": cells, "shots": int(W.ns), "nt": int(W.nt), "receivers": int(W.nr),
                    "order": int(W.sdo) * 2,
-                   "gradient": ("reference adjoint, chpr=100: every forward state kept in HBM "
-                                "per pass; L2 misfit on the GPU" if args.mode == "kept" else
-                                "reference adjoint, chpr=0: states rebuilt by time reversal"),
+                   "gradient": {
+                       "kept": "reference adjoint, chpr=100: every forward state kept in HBM "
+                               "per pass; L2 misfit on the GPU",
+                       "reverse": "reference adjoint, chpr=0: states rebuilt by time reversal",
+                       "exact": f"exact adjoint, checkpoints every {EXACT_CKPT} steps, segments "
+                                "recomputed"}[args.mode],
                    "parallelism": "single GPU, shots batched per launch",
                    "tuning": {"shots_per_pass": tuning[0], "fused_steps": bool(tuning[1]),
                               "strip_rows": tuning[2]}},
-        "kernel_timing": "per-step time of each time loop (HIP events around the loops) split "
-                         "between its kernels by event-bracketed launches",
+        "kernel_timing": ("event-bracketed launches (each holds a few microseconds of the "
+                          "event's stall)" if args.mode == "exact" else
+                          "per-step time of each time loop (HIP events around the loops) split "
+                          "between its kernels by event-bracketed launches"),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
