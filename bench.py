@@ -37,9 +37,10 @@ from pyfwi_amd.wave_propagation import checkpoint_steps  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes of one launch (FP32 words read + written once per cell):
 # (bytes per cell of the whole grid and shot, extra bytes per cell of the
-# interior where the imaging condition runs and shot, bytes per interior cell
-# and shot group).  The reference-mode gradient accumulators are shared by
-# the shots of a block, which add their terms up first (shots_per_block).
+# interior where the imaging condition runs and shot, bytes per imaging cell
+# and shot group).  The gradient accumulators are shared by the shots of a
+# block, which add their terms up first (shots_per_block); the exact adjoint
+# images every cell, the reference adjoint the interior.
 KERNEL_BYTES = {
     "fwd_vel": (32, 0, 0),       # R txx tzz txz vx vz b, W vx vz
     "fwd_str": (40, 0, 0),       # R vx vz txx tzz txz lam mu, W txx tzz txz
@@ -47,8 +48,8 @@ KERNEL_BYTES = {
     "rev_vel": (32, 0, 0),
     "adj_str": (36, 8, 16),      # all: R avx avz b 3 atau, W 3 atau; interior: R fwd vx vz; group: RW g_mu g_lam
     "adj_vel_grad": (36, 16, 8),  # all: R 3 atau lam mu 2 av, W 2 av; interior: R fwd 3 tau, b; group: RW g_rho
-    "exact_str": (60, 0, 0),      # R 3 atau lam mu 2 av 2 fwd v, RW g_mu g_lam (per shot), W 2 av
-    "exact_vel": (64, 0, 0),      # R 2 av b 3 fwd tau 3 atau, RW g_b (per shot), W 3 atau 2 av
+    "exact_str": (44, 0, 16),     # R 3 atau lam mu 2 av 2 fwd v, W 2 av; group: RW g_mu g_lam
+    "exact_vel": (56, 0, 8),      # R 2 av b 3 fwd tau 3 atau, W 3 atau 2 av; group: RW g_b
     "fwd_step": (52, 0, 0),       # R 5 fields lam mu b, W 5 fields
     "rev_step": (52, 0, 0),
 }
@@ -62,7 +63,8 @@ def shots_per_block(order, shots):
 def launch_bytes(kernel, cells, interior, shots, order=4):
     a, b, c = KERNEL_BYTES[kernel]
     groups = math.ceil(shots / shots_per_block(order, shots))
-    return (a * cells + b * interior) * shots + c * interior * groups
+    imaged = cells if kernel.startswith("exact") else interior
+    return (a * cells + b * interior) * shots + c * imaged * groups
 
 
 def log(msg):

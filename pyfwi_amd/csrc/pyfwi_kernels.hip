@@ -968,15 +968,19 @@ struct GroupLive {
     int count = 0, rank = 0;
 };
 
-template <int SPB>
+// FROM0: the exact adjoint's range of steps, 0 <= t < nt.
+template <int SPB, bool FROM0 = false>
 __device__ __forceinline__ GroupLive group_live(const DevGrid &g, const pyfwi_shot *shots, int s,
                                                 int t)
 {
     const int wave = wave_id(), s0 = s - wave;
     GroupLive r;
 #pragma unroll
-    for (int w = 0; w < SPB; ++w)
-        if (s0 + w < g.nshot && bwd_active(shots, s0 + w, t)) r.mask |= 1u << w;
+    for (int w = 0; w < SPB; ++w) {
+        const bool on = FROM0 ? t < shots[s0 + w < g.nshot ? s0 + w : s].nt
+                              : s0 + w < g.nshot && bwd_active(shots, s0 + w, t);
+        if (s0 + w < g.nshot && on) r.mask |= 1u << w;
+    }
     r.count = __popc(r.mask);
     r.rank = __popc(r.mask & ((1u << wave) - 1u));
     return r;
@@ -1375,19 +1379,25 @@ __global__ __launch_bounds__(64 * SPB) void k_adj_vel_grad(
 // read while its centre is written.
 
 // vel: [nshots][2][fs] velocities after the velocity half step of step t
-// (V_t); gacc: [nshots][6][fs] with slots 0 (mu) and 2 (lam) updated.
-// Column marching as the other stencil kernels: z-windows of the products
-// differentiated along z, x-derivatives on the entering row delayed SDO rows.
-template <int SDO, int SPB>
+// (V_t); gacc: [adj_acc_sets(sdo, nshots)][6][fs] with slots 0 (mu) and 2
+// (lam) updated, the shots of a block summing their terms first as in the
+// reference mode (every active cell here).  Column marching as the other
+// stencil kernels: z-windows of the products differentiated along z,
+// x-derivatives on the entering row delayed SDO rows.
+template <int SDO, bool ALONE, int SPB>
 __global__ __launch_bounds__(64 * SPB) void k_exact_str(
     DevGrid g, const pyfwi_shot *shots, int t, const float *vel, int64_t vel_shot_stride,
     float *adj, float *av, const float *__restrict__ lam,
     const float *__restrict__ mu, const float *__restrict__ dxp,
     const float *__restrict__ dzp, float *gacc)
 {
+    constexpr int NV = 2;  // staged terms: mu, lam
+    __shared__ float part[2][ADJ_GROUP_ROWS * SPB * NV * 64];
     int s, j, i0, i1;
     if (!tile<SDO, SPB>(g, s, j, i0, i1)) return;
     if (t >= shots[s].nt) return;
+    const GroupLive gl = group_live<SPB, true>(g, shots, s, t);
+    const int wave = wave_id(), lane = (int)threadIdx.x % 64;
     const int64_t fs = g.fstride;
     const uint32_t RB = (uint32_t)g.pitch * 4u;
     const float dt = g.dt;
@@ -1397,7 +1407,8 @@ __global__ __launch_bounds__(64 * SPB) void k_exact_str(
     const CellRef L{lam + g.origin}, M{mu + g.origin};
     const float *V = vel + (int64_t)s * vel_shot_stride + g.origin;
     const CellRef vx{V}, vz{V + fs};
-    float *G = gacc + (int64_t)s * 6 * fs + g.origin;
+    float *G = gacc + (int64_t)(s / SPB) * 6 * fs + g.origin;
+    constexpr int slots[2] = {0, 2};  // accumulator of each staged term
     // keep = 1 - dt*(dx + dz) of row r at column offset k
     auto kp = [&](int r, int k) { return 1.0f - dt * (dxp[j + k] + dzp[r]); };
     ZWin<SDO> wa3, wb1, wb2;  // mu*qxz*keep (d/dz-), lam*qxx*keep, (lam+2mu)*qzz*keep (d/dz+)
@@ -1434,48 +1445,75 @@ __global__ __launch_bounds__(64 * SPB) void k_exact_str(
             fvz.v[q] = x_vz(o);
         }
     }
+    // ALONE (launches of one shot): the shot adds its terms itself
+    int buf = 0;
+    for (int ic = i0; ic <= i1; ic += ADJ_GROUP_ROWS, buf ^= 1) {
+        const uint32_t c0 = cb;
+        float *mine = &part[buf][wave * NV * 64 + lane];
 #pragma unroll MARCH_UNROLL
-    for (int i = i0; i <= i1; ++i, cb += RB) {
-        const uint32_t n = cb + (uint32_t)SDO * RB;
-        const int rn = i + SDO;
-        wa3.push(z_a3(n, rn));
-        wb1.push(z_b1(n, rn));
-        wb2.push(z_b2(n, rn));
-        wvz.push(vz(n));
-        wvx.push(vx(n));
-        const float a1 = fa1.pop(x_a1(n, rn)), a2 = fa2.pop(x_a2(n, rn));
-        const float a3x = fa3.pop(x_a3(n, rn));
-        const float exx = fvx.pop(x_vx(n)), gzx = fvz.pop(x_vz(n));
-        float *pax = &cell_ref(Av, cb), *paz = &cell_ref(Av + fs, cb);
-        const float ax0 = *pax, az0 = *paz;
-        const float keep = kp(i, 0);
-        const float q1 = qxx(cb) * keep, q2 = qzz(cb) * keep, q3 = qxz(cb) * keep;
-        float *g0 = &cell_ref(G, cb), *g2 = &cell_ref(G + 2 * fs, cb);
-        const float gmu = *g0, glam = *g2;
-        const float sx = a1 + a2 + wa3.dminus(g.cz);
-        const float sz = wb1.dplus(g.cz) + wb2.dplus(g.cz) + a3x;
-        *pax = fmaf(-dt, sx, ax0);
-        *paz = fmaf(-dt, sz, az0);
-        const float ezz = wvz.dminus(g.cz);
-        const float gxz = gzx + wvx.dplus(g.cz);
-        *g2 = glam + dt * (q1 + q2) * (exx + ezz);
-        *g0 = gmu + dt * (2.0f * q1 * exx + 2.0f * q2 * ezz + q3 * gxz);
+        for (int r = 0; r < ADJ_GROUP_ROWS; ++r, cb += RB) {
+            const int i = ic + r;
+            if (i > i1) break;
+            const uint32_t n = cb + (uint32_t)SDO * RB;
+            const int rn = i + SDO;
+            wa3.push(z_a3(n, rn));
+            wb1.push(z_b1(n, rn));
+            wb2.push(z_b2(n, rn));
+            wvz.push(vz(n));
+            wvx.push(vx(n));
+            const float a1 = fa1.pop(x_a1(n, rn)), a2 = fa2.pop(x_a2(n, rn));
+            const float a3x = fa3.pop(x_a3(n, rn));
+            const float exx = fvx.pop(x_vx(n)), gzx = fvz.pop(x_vz(n));
+            float *pax = &cell_ref(Av, cb), *paz = &cell_ref(Av + fs, cb);
+            const float ax0 = *pax, az0 = *paz;
+            const float keep = kp(i, 0);
+            const float q1 = qxx(cb) * keep, q2 = qzz(cb) * keep, q3 = qxz(cb) * keep;
+            float gc[NV] = {};  // the row's accumulators (ALONE)
+            if constexpr (ALONE) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) gc[v] = cell_ref(G + slots[v] * fs, cb);
+            }
+            const float sx = a1 + a2 + wa3.dminus(g.cz);
+            const float sz = wb1.dplus(g.cz) + wb2.dplus(g.cz) + a3x;
+            *pax = fmaf(-dt, sx, ax0);
+            *paz = fmaf(-dt, sz, az0);
+            const float ezz = wvz.dminus(g.cz);
+            const float gxz = gzx + wvx.dplus(g.cz);
+            float term[NV];
+            term[0] = dt * (2.0f * q1 * exx + 2.0f * q2 * ezz + q3 * gxz);
+            term[1] = dt * (q1 + q2) * (exx + ezz);
+            float *q = mine + r * SPB * NV * 64;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                if constexpr (ALONE) cell_ref(G + slots[v] * fs, cb) = gc[v] + term[v];
+                else q[v * 64] = term[v];
+            }
+        }
+        if constexpr (!ALONE) {
+            __syncthreads();
+            add_group_terms<NV, SPB>(part[buf], gl, ic, i1, true, i0, i1, G, slots, fs, c0, RB);
+        }
     }
 }
 
 // tau: [nshots][3][fs] stresses after the injection of step t (input of the
-// update); gacc slot 4 accumulates dJ/db.  Writes the new stress adjoint in
-// place and p = Av*keep to av_out.  (The receiver sampling of S_t, which adds
-// the residual row, is k_exact_inject's.)
-template <int SDO, int SPB>
+// update); gacc slot 4 accumulates dJ/db (per block's group of shots, as in
+// k_exact_str).  Writes the new stress adjoint in place and p = Av*keep to
+// av_out.  (The receiver sampling of S_t, which adds the residual row, is
+// k_exact_inject's.)
+template <int SDO, bool ALONE, int SPB>
 __global__ __launch_bounds__(64 * SPB) void k_exact_vel(
     DevGrid g, const pyfwi_shot *shots, int t, const float *tau, int64_t tau_shot_stride,
     float *adj, const float *av_in, float *av_out, const float *__restrict__ bu,
     const float *__restrict__ dxp, const float *__restrict__ dzp, float *gacc)
 {
+    constexpr int NV = 1;  // staged term: b
+    __shared__ float part[2][ADJ_GROUP_ROWS * SPB * NV * 64];
     int s, j, i0, i1;
     if (!tile<SDO, SPB>(g, s, j, i0, i1)) return;
     if (t >= shots[s].nt) return;
+    const GroupLive gl = group_live<SPB, true>(g, shots, s, t);
+    const int wave = wave_id(), lane = (int)threadIdx.x % 64;
     const int64_t fs = g.fstride;
     const uint32_t RB = (uint32_t)g.pitch * 4u;
     const float dt = g.dt;
@@ -1485,7 +1523,8 @@ __global__ __launch_bounds__(64 * SPB) void k_exact_vel(
     const CellRef pvx{Ain}, pvz{Ain + fs}, B{bu + g.origin};
     const float *T = tau + (int64_t)s * tau_shot_stride + g.origin;
     const CellRef txx{T}, tzz{T + fs}, txz{T + 2 * fs};
-    float *G = gacc + (int64_t)s * 6 * fs + g.origin;
+    float *G = gacc + (int64_t)(s / SPB) * 6 * fs + g.origin;
+    constexpr int slots[1] = {4};  // accumulator of the staged term
     auto kp = [&](int r, int k) { return 1.0f - dt * (dxp[j + k] + dzp[r]); };
     ZWin<SDO> wbz, wbx;    // b*pvz*keep (d/dz-), b*pvx*keep (d/dz+)
     ZWin<SDO> wtxz, wtzz;  // forward txz (d/dz-), tzz (d/dz+)
@@ -1515,33 +1554,46 @@ __global__ __launch_bounds__(64 * SPB) void k_exact_vel(
             ftxz.v[q] = x_txz(o);
         }
     }
+    int buf = 0;
+    for (int ic = i0; ic <= i1; ic += ADJ_GROUP_ROWS, buf ^= 1) {
+        const uint32_t c0 = cb;
+        float *mine = &part[buf][wave * NV * 64 + lane];
 #pragma unroll MARCH_UNROLL
-    for (int i = i0; i <= i1; ++i, cb += RB) {
-        const uint32_t n = cb + (uint32_t)SDO * RB;
-        const int rn = i + SDO;
-        wbz.push(z_bz(n, rn));
-        wbx.push(z_bx(n, rn));
-        wtxz.push(txz(n));
-        wtzz.push(tzz(n));
-        const float rxx = frxx.pop(x_bx(n, rn)), rxz_x = frxz.pop(x_bz(n, rn));
-        const float dtx = ftxx.pop(x_txx(n)), dtz = ftxz.pop(x_txz(n));
-        const float keep = kp(i, 0);
-        const float px = pvx(cb) * keep, pz = pvz(cb) * keep;
-        float *pxx = &cell_ref(A + 2 * fs, cb), *pzz = &cell_ref(A + 3 * fs, cb),
-              *pxz = &cell_ref(A + 4 * fs, cb);
-        const float axx = *pxx, azz = *pzz, axz = *pxz;
-        float *g4 = &cell_ref(G + 4 * fs, cb);
-        const float gb = *g4;
-        const float rzz = wbz.dminus(g.cz);
-        const float rxz = wbx.dplus(g.cz) + rxz_x;
-        *pxx = fmaf(-dt, rxx, axx * keep);
-        *pzz = fmaf(-dt, rzz, azz * keep);
-        *pxz = fmaf(-dt, rxz, axz * keep);
-        cell_ref(Aout, cb) = px;
-        cell_ref(Aout + fs, cb) = pz;
-        const float dtau_x = dtx + wtxz.dminus(g.cz);
-        const float dtau_z = wtzz.dplus(g.cz) + dtz;
-        *g4 = gb + dt * (px * dtau_x + pz * dtau_z);
+        for (int r = 0; r < ADJ_GROUP_ROWS; ++r, cb += RB) {
+            const int i = ic + r;
+            if (i > i1) break;
+            const uint32_t n = cb + (uint32_t)SDO * RB;
+            const int rn = i + SDO;
+            wbz.push(z_bz(n, rn));
+            wbx.push(z_bx(n, rn));
+            wtxz.push(txz(n));
+            wtzz.push(tzz(n));
+            const float rxx = frxx.pop(x_bx(n, rn)), rxz_x = frxz.pop(x_bz(n, rn));
+            const float dtx = ftxx.pop(x_txx(n)), dtz = ftxz.pop(x_txz(n));
+            const float keep = kp(i, 0);
+            const float px = pvx(cb) * keep, pz = pvz(cb) * keep;
+            float *pxx = &cell_ref(A + 2 * fs, cb), *pzz = &cell_ref(A + 3 * fs, cb),
+                  *pxz = &cell_ref(A + 4 * fs, cb);
+            const float axx = *pxx, azz = *pzz, axz = *pxz;
+            float gc = 0.0f;  // the row's accumulator (ALONE)
+            if constexpr (ALONE) gc = cell_ref(G + 4 * fs, cb);
+            const float rzz = wbz.dminus(g.cz);
+            const float rxz = wbx.dplus(g.cz) + rxz_x;
+            *pxx = fmaf(-dt, rxx, axx * keep);
+            *pzz = fmaf(-dt, rzz, azz * keep);
+            *pxz = fmaf(-dt, rxz, axz * keep);
+            cell_ref(Aout, cb) = px;
+            cell_ref(Aout + fs, cb) = pz;
+            const float dtau_x = dtx + wtxz.dminus(g.cz);
+            const float dtau_z = wtzz.dplus(g.cz) + dtz;
+            const float term = dt * (px * dtau_x + pz * dtau_z);
+            if constexpr (ALONE) cell_ref(G + 4 * fs, cb) = gc + term;
+            else mine[r * SPB * NV * 64] = term;
+        }
+        if constexpr (!ALONE) {
+            __syncthreads();
+            add_group_terms<NV, SPB>(part[buf], gl, ic, i1, true, i0, i1, G, slots, fs, c0, RB);
+        }
     }
 }
 
@@ -1906,8 +1958,10 @@ void launch_exact_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int
     DevGrid gm = g;
     const int spb = march_spb(g.sdo, nshots);
     dim3 grid = march_grid(gm, g.sdo, nshots, spb), block(64 * spb);
+    const bool alone = nshots == 1;
 #define ARGS gm, shots, t, vel, vel_stride, adj, av, lam, mu, dxp, dzp, gacc
-    DISPATCH_MARCH(g.sdo, spb, k_exact_str, );
+    if (alone) DISPATCH_MARCH(g.sdo, spb, k_exact_str, , true);
+    else DISPATCH_MARCH(g.sdo, spb, k_exact_str, , false);
 #undef ARGS
 }
 
@@ -1919,8 +1973,10 @@ void launch_exact_vel(const DevGrid &g, int nshots, const pyfwi_shot *shots, int
     DevGrid gm = g;
     const int spb = march_spb(g.sdo, nshots);
     dim3 grid = march_grid(gm, g.sdo, nshots, spb), block(64 * spb);
+    const bool alone = nshots == 1;
 #define ARGS gm, shots, t, tau, tau_stride, adj, av_in, av_out, bu, dxp, dzp, gacc
-    DISPATCH_MARCH(g.sdo, spb, k_exact_vel, );
+    if (alone) DISPATCH_MARCH(g.sdo, spb, k_exact_vel, , true);
+    else DISPATCH_MARCH(g.sdo, spb, k_exact_vel, , false);
 #undef ARGS
 }
 

@@ -5,8 +5,8 @@
 //   fields  [nshots][5][fstride]  forward state of every shot of the batch
 //   adj     [nshots][5][fstride]  adjoint state
 //   av      2 x [nshots][2][fstride]  exact mode velocity adjoint ping-pong
-//   gacc    [nsets][6][fstride]  gradient accumulators: one set per shot (exact
-//           mode) or per block's group of shots (reference mode)
+//   gacc    [nsets][6][fstride]  gradient accumulators: one set per block's group
+//           of shots
 //   snaps   [nshots][nchp][5][fstride]  reference checkpoints (pyfwi mode)
 //   ckpts   [nshots][ncp][5][fstride]   exact-mode checkpoints every K steps
 //   seg     [nshots][K][5][fstride]     exact-mode recomputed segment
@@ -902,7 +902,7 @@ int32_t pyfwi_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, c
     const bool reuse = mode == PYFWI_MODE_EXACT && same && p->ckpts_all && p->ckpt_every == K;
     for (int g0 = 0; g0 < nshots; g0 += G) {
         const int ng = std::min(G, nshots - g0);
-        const int nsets = mode == PYFWI_MODE_PYFWI ? adj_acc_sets(g.sdo, ng) : ng;
+        const int nsets = adj_acc_sets(g.sdo, ng);
         if (int e = ensure(p->gacc, (size_t)nsets * 6 * g.fstride, p->stream, "gradient"))
             return e;
         launch_zero(p->gacc.p, (int64_t)nsets * 6 * g.fstride, p->stream);
