@@ -378,7 +378,8 @@ def main():
     ap.add_argument("--tune", default="", help="fixed tuning 'shots_per_pass,fused,rows'")
     ap.add_argument("--mode", default="kept", choices=["kept", "reverse", "exact"],
                     help="kept: every forward state kept in HBM per pass (chpr=100), misfit and "
-                         "gradient in one call; reverse: chpr=0, states rebuilt by time reversal")
+                         "gradient in one call; reverse: chpr=0, states rebuilt by time reversal; "
+                         "exact: the exact adjoint from checkpoints")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))

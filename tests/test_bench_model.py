@@ -1,0 +1,57 @@
+"""The measurement side of bench.py on the CPU: the byte model behind
+roofline.achieved, the block shape it assumes, the measured-traffic lookup
+and the one-GPU guard."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402
+
+CELLS, INTERIOR = 540 * 1040, 497 * 997  # C2: tnz x tnx, imaging region at order 4
+
+
+@pytest.mark.parametrize("order,shots,spb", [(4, 10, 5), (4, 5, 5), (4, 20, 4), (4, 8, 4),
+                                             (4, 1, 4), (8, 10, 4), (4, 6.5, 4)])
+def test_shots_per_block(order, shots, spb):
+    """Five-shot blocks only at order 4 when the shots fill fives but not fours."""
+    assert bench.shots_per_block(order, shots) == spb
+
+
+def test_byte_model_c2_launches():
+    """Per-launch algorithmic bytes of the 10-shot C2 launches, written out
+    from the per-cell reads and writes each kernel makes."""
+    groups = 2  # ten shots in blocks of five
+    assert bench.launch_bytes("fwd_vel", CELLS, INTERIOR, 10) == 32 * CELLS * 10
+    assert bench.launch_bytes("fwd_str", CELLS, INTERIOR, 10) == 40 * CELLS * 10
+    assert bench.launch_bytes("adj_str", CELLS, INTERIOR, 10) == \
+        (36 * CELLS + 8 * INTERIOR) * 10 + 16 * INTERIOR * groups
+    assert bench.launch_bytes("adj_vel_grad", CELLS, INTERIOR, 10) == \
+        (36 * CELLS + 16 * INTERIOR) * 10 + 8 * INTERIOR * groups
+    # the exact adjoint images every active cell
+    assert bench.launch_bytes("exact_vel", CELLS, INTERIOR, 20) == 56 * CELLS * 20 + 8 * CELLS * 5
+    # order 8 keeps blocks of four: three groups for ten shots
+    assert bench.launch_bytes("adj_str", CELLS, INTERIOR, 10, order=8) == \
+        (36 * CELLS + 8 * INTERIOR) * 10 + 16 * INTERIOR * 3
+
+
+def test_measured_traffic_lookup():
+    assert bench.measured_traffic("no_such_workload", "adj_vel_grad") is None
+    with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
+        d = json.load(f)
+    got = bench.measured_traffic(d["workload"], "adj_vel_grad")
+    if d["kernel_source_sha1"] == bench.kernel_source_sha1():
+        assert got == d["bytes_per_dispatch"]["adj_vel_grad"]
+    else:
+        assert got is None
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
