@@ -80,14 +80,17 @@ def kernel_source_sha1():
     return hashlib.sha1(b"".join(open(f, "rb").read() for f in files)).hexdigest()
 
 
-def measured_traffic(workload_name, kernel):
+def measured_traffic(workload_name, kernel, shots=None):
     """HBM bytes per dispatch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary, when it was taken with the current kernel sources."""
+    WRITE_SIZE summary, when it was taken with the current kernel sources (and,
+    given `shots`, with dispatches of that many shots)."""
     try:
         d = json.load(open(os.path.join(ROOT, "profiles", "hbm_traffic.json")))
     except (OSError, ValueError):
         return None
     if d.get("workload") != workload_name or d.get("kernel_source_sha1") != kernel_source_sha1():
+        return None
+    if shots is not None and d.get("shots_per_dispatch") != shots:
         return None
     return d.get("bytes_per_dispatch", {}).get(kernel)
 
@@ -453,15 +456,20 @@ def main():
     dom = max(prof, key=lambda k: prof[k][0])
     ms, n = prof[dom]
     avg_s = ms / n / 1e3
-    # shots per launch: passes of `tuning[0]` shots (the last one may be smaller)
-    npass = -(-int(W.ns) // tuning[0]) if tuning[0] else 1
-    shots_per_launch = int(W.ns) / npass
+    # passes of `tuning[0]` shots, the last one possibly smaller; every pass
+    # makes the same number of launches, so a launch's mean bytes is the mean
+    # over the passes
+    per = tuning[0] or int(W.ns)
+    passes = [min(per, int(W.ns) - g0) for g0 in range(0, int(W.ns), per)]
+    shots_per_launch = int(W.ns) / len(passes)
     order = int(W.sdo) * 2
-    per_launch_bytes = launch_bytes(dom, cells, interior, shots_per_launch, order)
+
+    def mean_bytes(kind):
+        return sum(launch_bytes(kind, cells, interior, n, order) for n in passes) / len(passes)
+    per_launch_bytes = mean_bytes(dom)
     achieved = per_launch_bytes / avg_s / 1e9
     kernels = {k: {"avg_us": round(v[0] / v[1] * 1e3, 2), "launches": v[1],
-                   "GBps": round(launch_bytes(k, cells, interior, shots_per_launch, order)
-                                 / (v[0] / v[1] / 1e3) / 1e9, 1)}
+                   "GBps": round(mean_bytes(k) / (v[0] / v[1] / 1e3) / 1e9, 1)}
                for k, v in prof.items()}
     log(f"bench: kernels {kernels}")
 
@@ -517,7 +525,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(w.name, dom),
+                     "traffic": (measured_traffic(w.name, dom, passes[0])
+                                 if len(set(passes)) == 1 else None),
                      "bytes_per_launch": per_launch_bytes, "avg_launch_us": round(avg_s * 1e6, 2),
                      "shots_per_launch": round(shots_per_launch, 3)},
         "gradient_roofline_frac": round(value * grad_bytes / HBM_PEAK_GBS, 4),

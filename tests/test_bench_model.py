@@ -46,6 +46,10 @@ def test_measured_traffic_lookup():
     got = bench.measured_traffic(d["workload"], "adj_vel_grad")
     if d["kernel_source_sha1"] == bench.kernel_source_sha1():
         assert got == d["bytes_per_dispatch"]["adj_vel_grad"]
+        n = d["shots_per_dispatch"]
+        assert bench.measured_traffic(d["workload"], "adj_vel_grad", n) == got
+        # dispatches of another size move other bytes: no figure
+        assert bench.measured_traffic(d["workload"], "adj_vel_grad", n - 2) is None
     else:
         assert got is None
 
