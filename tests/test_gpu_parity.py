@@ -139,6 +139,30 @@ def test_gradient_order8(gpu, chpr, tuning, balance):
         assert e < GRAD_TOL, (k, e)
 
 
+@pytest.mark.parametrize("chpr,ns", [(100, 10), (0, 10), (100, 13)])
+def test_gradient_many_shot_groups(gpu, chpr, ns):
+    """Passes of 10 shots run in two blocks of five that each sum into their
+    own accumulator set, 13 shots in three of four plus one; both against the
+    oracle."""
+    print(f"gradient chpr={chpr} shots={ns}", flush=True)
+    from pyfwi_amd.acquisition import Source
+    w = _small(nt=300)
+    src_loc, _ = workloads._surface_geometry(ns, 100, 1.0, 50, 2, 4, 2, 1)
+    w.src = Source(src_loc, 1.0, w.inpa["dt"])
+    w.src.Ricker(30)
+    W = WavePropagator(w.inpa, w.src, w.rec_loc, w.model_shape, components=2, chpr=chpr)
+    d = W.forward_modeling(w.model)
+    res = {k: np.float32(0.2 * v) for k, v in d.items()}
+    g = W.gradient(res, parameterization="lmd")
+    st = oracle_setup(w, chpr=chpr, components=2)
+    seis, states = O.forward(st, w.model, np.float32)
+    gr = O.gradient(st, w.model, res, states, parameterization="lmd")
+    for k in gr:
+        e = rel(g[k], gr[k])
+        print(f"  {k}: {e:.2e}")
+        assert e < GRAD_TOL, (k, e)
+
+
 def test_gradient_energy_balancing(gpu):
     w = _small(nt=500, energy_balancing=True)
     W = WavePropagator(w.inpa, w.src, w.rec_loc, w.model_shape, components=2, chpr=100)
