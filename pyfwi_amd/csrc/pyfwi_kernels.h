@@ -94,6 +94,12 @@ void launch_str(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, 
 void launch_inject_record(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
                           float *fields, int64_t fields_stride, const float *src, float *seis,
                           const int32_t *rtab, hipStream_t st);
+// The records of steps t0..t1-1 from a forward that kept every state (shot
+// s's state after step u at kept + s*shot_stride + u*slot_stride): the same
+// rows launch_inject_record(..., src = nullptr, ...) writes step by step.
+void launch_record_kept(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t0, int t1,
+                        const float *kept, int64_t shot_stride, int64_t slot_stride, float *seis,
+                        const int32_t *rtab, hipStream_t st);
 // Residual injection in memory at backward step t, components f0..f1-1 (all
 // five by default); with first_only, only for shots whose first backward
 // step t is.

@@ -859,6 +859,32 @@ __device__ __forceinline__ int recorded_field(const pyfwi_shot &sh, int f)
     return (f == 1 && (sh.rec_flags & PYFWI_REC_VZ_FROM_VX)) ? 0 : f;
 }
 
+// Record row k of one shot from the state F (cell (0, 0) of its first field;
+// nullptr: the zero state before the first step), the block's threads
+// striding over the receivers.
+__device__ __forceinline__ void record_row(const DevGrid &g, const pyfwi_shot &sh, int k,
+                                           const float *F, float *seis,
+                                           const int32_t *__restrict__ rtab)
+{
+    const int64_t fs = g.fstride;
+    float *rec = seis + sh.rec_offset;
+    const int32_t *cells = sh.rec_layout == PYFWI_REC_TABLE ? rtab + sh.rec_table : nullptr;
+    const int64_t c0 = (int64_t)sh.rec_z * g.pitch + sh.rec_x0;
+    for (int r = threadIdx.x; r < sh.n_rec; r += blockDim.x) {
+        int64_t c = c0 + (int64_t)r * sh.rec_dx;
+        bool live = F != nullptr;
+        if (cells) {
+            const int32_t e = cells[r];
+            live = live && e >= 0;
+            c = e >= 0 ? table_cell(g, e) : 0;
+        }
+#pragma unroll
+        for (int f = 0; f < 5; ++f)
+            rec[((int64_t)f * sh.n_samples + k) * sh.rec_stride + r] =
+                live ? F[recorded_field(sh, f) * fs + c] : 0.0f;
+    }
+}
+
 // Receiver gather then source injection of step t for every shot, one block
 // per shot (src/PyFWI/elastic_surface.cl:7-41; the well gathers of
 // elastic_crosswell.cl:1-33 and elastic.cl:43-98 through the receiver
@@ -877,24 +903,7 @@ __global__ void k_inject_record(DevGrid g, const pyfwi_shot *shots, int t,
     const int k = t / sh.dt_scale;
     const int64_t fs = g.fstride;
     float *F = fields + (int64_t)s * fields_stride + g.origin;
-    if (k < sh.n_samples) {
-        float *rec = seis + sh.rec_offset;
-        const int32_t *cells = sh.rec_layout == PYFWI_REC_TABLE ? rtab + sh.rec_table : nullptr;
-        const int64_t c0 = (int64_t)sh.rec_z * g.pitch + sh.rec_x0;
-        for (int r = threadIdx.x; r < sh.n_rec; r += blockDim.x) {
-            int64_t c = c0 + (int64_t)r * sh.rec_dx;
-            bool live = true;
-            if (cells) {
-                const int32_t e = cells[r];
-                live = e >= 0;
-                c = live ? table_cell(g, e) : 0;
-            }
-#pragma unroll
-            for (int f = 0; f < 5; ++f)
-                rec[((int64_t)f * sh.n_samples + k) * sh.rec_stride + r] =
-                    live ? F[recorded_field(sh, f) * fs + c] : 0.0f;
-        }
-    }
+    if (k < sh.n_samples) record_row(g, sh, k, F, seis, rtab);
     if (!src) return;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -905,6 +914,27 @@ __global__ void k_inject_record(DevGrid g, const pyfwi_shot *shots, int t,
         F[2 * fs + c] += sv[2];
         F[3 * fs + c] += sv[3];
     }
+}
+
+// Records of steps t0 .. t0+gridDim.x-1 gathered after a forward that kept
+// every state, one block per (step, shot): the gather of step t reads the
+// state step t-1 left in its slot (slot t-1 of shot s at
+// kept + s*shot_stride + (t-1)*slot_stride), the state k_inject_record reads
+// before step t; kept slots never hold a source term.
+__global__ void k_record_kept(DevGrid g, const pyfwi_shot *shots, int t0, const float *kept,
+                              int64_t shot_stride, int64_t slot_stride, float *seis,
+                              const int32_t *__restrict__ rtab)
+{
+    const int t = t0 + (int)blockIdx.x;
+    const int s = blockIdx.y;
+    const pyfwi_shot sh = shots[s];
+    if (t >= sh.nt || (t % sh.dt_scale) != 0) return;
+    const int k = t / sh.dt_scale;
+    if (k >= sh.n_samples) return;
+    const float *F = t > 0 ? kept + (int64_t)s * shot_stride + (int64_t)(t - 1) * slot_stride +
+                                 g.origin
+                           : nullptr;
+    record_row(g, sh, k, F, seis, rtab);
 }
 
 // ---------------------------------------------------- adjoint, pyfwi mode
@@ -1905,6 +1935,15 @@ void launch_inject_record(const DevGrid &g, int nshots, const pyfwi_shot *shots,
 {
     hipLaunchKernelGGL(k_inject_record, dim3(nshots), dim3(256), 0, st, g, shots, t, fields,
                        fields_stride, src, seis, rtab);
+}
+
+void launch_record_kept(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t0, int t1,
+                        const float *kept, int64_t shot_stride, int64_t slot_stride, float *seis,
+                        const int32_t *rtab, hipStream_t st)
+{
+    if (t1 <= t0 || nshots <= 0) return;
+    hipLaunchKernelGGL(k_record_kept, dim3(t1 - t0, nshots), dim3(256), 0, st, g, shots, t0, kept,
+                       shot_stride, slot_stride, seis, rtab);
 }
 
 void launch_adj_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,

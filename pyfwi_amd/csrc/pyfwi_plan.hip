@@ -527,18 +527,15 @@ int forward_step(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t, const floa
 
 // Forward step of a kept-state run: the state of step t-1 is read from `in`
 // (per-shot stride in_stride) and step t written to `out`; the source term
-// never enters memory, so every slot holds the reference's snapshot.
+// never enters memory, so every slot holds the reference's snapshot.  The
+// stress update writes the record sample when the receivers lie on one row
+// of the active cells; otherwise the caller gathers the records from the
+// slots after the loop (seis == nullptr here).
 int forward_step_kept(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t, const float *src,
                       float *seis, const float *in, int64_t in_stride, float *out,
                       int64_t out_stride)
 {
     const DevGrid &g = p->dg;
-    if (!p->rec_inline) {
-        Timed ts(p, PYFWI_K_FWD_SPARSE);
-        launch_inject_record(g, ng, dsh, t, const_cast<float *>(in), in_stride, nullptr, seis,
-                             p->d_rtab, p->stream);
-        seis = nullptr;
-    }
     {
         Timed tm(p, PYFWI_K_FWD_VEL);
         launch_vel(g, +1, ng, dsh, t, in, in_stride, out, out_stride, p->bu.p, p->dxp, p->dzp,
@@ -645,10 +642,17 @@ int forward_pass(pyfwi_plan *p, int g0, int ng, const float *d_src, float *d_sei
                 launch_copy_shots(ckpts + (int64_t)(t / K) * fps, (int64_t)ncp * fps, in,
                                   in_stride, fps, ng, p->stream);
             float *out = snaps + (int64_t)t * fps;
-            forward_step_kept(p, dsh, ng, t, d_src, d_seis, in, in_stride, out, ss);
+            forward_step_kept(p, dsh, ng, t, d_src, p->rec_inline ? d_seis : nullptr, in,
+                              in_stride, out, ss);
             in = out;
             in_stride = ss;
             if (int e = maybe_harvest(p)) return e;
+        }
+        // receiver tables: one gather launch over all steps instead of one per step
+        if (!p->rec_inline) {
+            Timed ts(p, PYFWI_K_FWD_SPARSE);
+            launch_record_kept(p->dg, ng, dsh, 0, nt, snaps, ss, fps, d_seis, p->d_rtab,
+                               p->stream);
         }
         return 0;
     }
