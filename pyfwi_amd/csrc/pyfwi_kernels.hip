@@ -1942,8 +1942,11 @@ void launch_record_kept(const DevGrid &g, int nshots, const pyfwi_shot *shots, i
                         const int32_t *rtab, hipStream_t st)
 {
     if (t1 <= t0 || nshots <= 0) return;
-    hipLaunchKernelGGL(k_record_kept, dim3(t1 - t0, nshots), dim3(256), 0, st, g, shots, t0, kept,
-                       shot_stride, slot_stride, seis, rtab);
+    // grid.y holds at most 65535 shots a launch
+    for (int s0 = 0; s0 < nshots; s0 += 65535)
+        hipLaunchKernelGGL(k_record_kept, dim3(t1 - t0, std::min(65535, nshots - s0)), dim3(256),
+                           0, st, g, shots + s0, t0, kept + (int64_t)s0 * shot_stride, shot_stride,
+                           slot_stride, seis, rtab);
 }
 
 void launch_adj_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
