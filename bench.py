@@ -289,8 +289,9 @@ def cpu_baseline(w, steps_cap, threads):
     O.raw_gradient(st, w.model, res, states, np.float32)
     dt = time.perf_counter() - t0
     cells = st.tnz * st.tnx
+    name, ncpu = cpu_model()
     return {"value": round(cells * nt / dt / 1e9, 5), "unit": "Gcell-updates/s",
-            "cores": threads, "kind": "port",
+            "cores": threads, "kind": "port", "cpu": name, "host_cpus": ncpu,
             "sample": f"1 shot x {nt} steps of the same grid ({st.tnz}x{st.tnx}), forward + "
                       f"reference adjoint, C oracle, {threads} OpenMP thread(s), {dt:.1f} s"}
 
@@ -305,6 +306,70 @@ def copy_calibration(torch, nbytes=1 << 30):
         b.copy_(a)
     torch.cuda.synchronize()
     log(f"bench: copy calibration {nbytes} bytes read + {nbytes} bytes written, 3 launches")
+
+
+def pass_kernel_bytes(kind, pass_nts, cells, interior, order=4):
+    """Algorithmic bytes and launch count of kernel `kind` over passes of
+    uneven shots.  pass_nts: per pass, the shots' nt in the order the plan
+    runs them (decreasing).  A forward launch of step t advances the shots
+    with t < nt_s, an adjoint launch of step t (nt-1 .. 1) those with
+    1 <= t < nt_s; the accumulators move once per block's group with a live
+    shot.  Shots per block follow the pass size, as the library picks them."""
+    a, b, c = KERNEL_BYTES[kind]
+    adjoint = kind.startswith("adj") or kind.startswith("exact")
+    total = 0.0
+    launches = 0
+    for nts in pass_nts:
+        nts = np.sort(np.asarray(nts, np.int64))[::-1]
+        ng = len(nts)
+        spb = shots_per_block(order, ng)
+        tmax = int(nts[0])
+        t = np.arange(1 if adjoint else 0, tmax)
+        live = (nts[None, :] > t[:, None]).sum(1)  # shots live at each launch
+        groups = -(-live // spb)
+        imaged = cells if kind.startswith("exact") else interior
+        total += float(((a * cells + b * interior) * live).sum() + (c * imaged * groups).sum())
+        launches += len(t)
+    return total, launches
+
+
+def cpu_model():
+    """CPU model name and logical CPU count of this host."""
+    name = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                name = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return name, os.cpu_count() or 1
+
+
+def cpu_baseline_shot(inpa, src_loc, rec_loc, model, wavelet, component, components, steps,
+                      threads):
+    """The CPU oracle (C, OpenMP over rows) on one shot -- source at
+    ``src_loc`` (x, z in metres), receivers ``rec_loc`` -- forward plus the
+    reference adjoint (chpr = 0) for ``steps`` time steps."""
+    from oracle import oracle as O
+    inpa = dict(inpa)
+    inpa["t"] = (steps - 1) * inpa["dt"] + inpa["dt"] / 2
+    st = O.Setup(inpa, np.asarray([src_loc], np.float64), rec_loc, model["vp"].shape, wavelet,
+                 component, chpr=0, components=components)
+    O.lib()
+    _set_omp_threads(threads)
+    t0 = time.perf_counter()
+    seis, states = O.forward(st, model, np.float32)
+    res = O.section(seis, 4)
+    O.raw_gradient(st, model, res, states, np.float32)
+    dt = time.perf_counter() - t0
+    cells = st.tnz * st.tnx
+    name, ncpu = cpu_model()
+    return {"value": round(cells * steps / dt / 1e9, 5), "unit": "Gcell-updates/s",
+            "cores": threads, "kind": "port", "cpu": name, "host_cpus": ncpu,
+            "sample": f"1 shot x {steps} steps of the same grid ({st.tnz}x{st.tnx}), forward + "
+                      f"reference adjoint (chpr=0), C oracle, {threads} OpenMP thread(s), "
+                      f"{dt:.1f} s"}
 
 
 def bench_uneven(args, torch):
@@ -322,59 +387,164 @@ def bench_uneven(args, torch):
     obs = torch.zeros(B.n_records, dtype=torch.float32, device=dev)
     seis = torch.zeros_like(obs)
     grad = torch.zeros(6 * B.tnz * B.tnx, dtype=torch.float32, device=dev)
+    units = B.cell_updates()
     log(f"bench: {wl.name}: {B.ns} shots, nt {min(B.nt)}..{max(B.nt)}, receivers "
-        f"{min(B.nr)}..{max(B.nr)}, {B.cell_updates() / 1e9:.1f} G cell-updates per gradient")
+        f"{min(B.nr)}..{max(B.nr)}, {units / 1e9:.1f} G cell-updates per gradient")
     B.set_model(wl.model_obs)
     B.forward(seis=obs, mem=_native.MEM_DEVICE, src_table=src)
     B.set_model(wl.model)
     torch.cuda.synchronize()
+    chp = checkpoint_steps(B.nt_max, 100)
+    mask = 3  # vx, vz records (components = 2)
 
-    def step():
-        return B.misfit_gradient(obs, grad=grad, seis=seis, mem=_native.MEM_DEVICE,
-                                 src_table=src)[0]
+    def run(shots):
+        return plan.misfit_gradient(shots, src, obs, seis, grad, chp, mask, 0,
+                                    _native.MEM_DEVICE)
+
+    def step(_W=None, _b=None):
+        return run(B.shots())
+
     for i in range(args.warmup):
         step()
         torch.cuda.synchronize()
         log(f"bench: warmup {i + 1}/{args.warmup} done")
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
+    misfit = None
     for i in range(args.steps):
         ts = time.perf_counter()
         misfit = step()
         torch.cuda.synchronize()
         log(f"bench: step {i + 1}/{args.steps}: {time.perf_counter() - ts:.2f} s")
     elapsed = (time.perf_counter() - t0) / args.steps
-    value = B.cell_updates() / elapsed / 1e9
+    value = units / elapsed / 1e9
     log(f"bench: {elapsed * 1e3:.0f} ms per gradient, {value:.2f} Gcell/s")
+    passes = plan.last_passes()
+    order_nt = sorted(B.nt, reverse=True)  # the plan runs the shots longest first
+    pass_nts, k = [], 0
+    for n in passes:
+        pass_nts.append(order_nt[k:k + n])
+        k += n
+    log(f"bench: passes {passes}")
+
+    # dominant kernel: its launches' algorithmic bytes over their durations
+    cells = B.tnz * B.tnx
+    sdo, npml = int(B.base.sdo), B.npml
+    interior = (B.tnz - 2 * (npml + sdo) + 1) * (B.tnx - 2 * (npml + sdo) + 1)
+    order = 2 * sdo
+    prof = kernel_profile(None, {"plan": plan}, torch, step)
+    kernels = {}
+    for kname, (ms, n) in prof.items():
+        nbytes, nl = pass_kernel_bytes(kname, pass_nts, cells, interior, order)
+        if nl != n:
+            log(f"bench: {kname}: {n} launches timed, {nl} in the byte model")
+        kernels[kname] = {"avg_us": round(ms / n * 1e3, 2), "launches": n,
+                          "bytes_per_launch": round(nbytes / nl),
+                          "GBps": round(nbytes / (ms / 1e3) / 1e9, 1)}
+    dom = max(prof, key=lambda k: prof[k][0])
+    log(f"bench: kernels {kernels}")
+    achieved = kernels[dom]["GBps"]
+
+    # forward modelling alone: all shots, records only
+    no_chp = np.zeros(0, np.int32)
+    plan.forward(B.shots(), src, seis, no_chp, 0, _native.MEM_DEVICE)
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    plan.forward(B.shots(), src, seis, no_chp, 0, _native.MEM_DEVICE)
+    torch.cuda.synchronize()
+    fwd_rate = units / (time.perf_counter() - ts) / 1e9
+    log(f"bench: forward-only {fwd_rate:.2f} Gcell/s")
+
+    ps = None
+    if args.per_shot_sample > 0:
+        # the same kernels on one shot at a time (the reference's schedule),
+        # on shots spread over the range of nt
+        order_ids = np.argsort(B.nt)
+        pick = order_ids[np.linspace(0, B.ns - 1, args.per_shot_sample).round().astype(int)]
+        sub = B.shots([int(k) for k in pick])
+        sub_units = cells * sum(int(B.nt[int(k)]) for k in pick)
+        plan.set_tuning(1, False, 0)
+        run(sub[:1])  # warm
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        run(sub)
+        torch.cuda.synchronize()
+        ps_time = time.perf_counter() - ts
+        plan.set_tuning(0, False, 0)
+        ps_rate = sub_units / ps_time / 1e9
+        log(f"bench: per-shot path {ps_rate:.2f} Gcell/s on shots {list(map(int, pick))}")
+        ps = {"value": round(ps_rate, 3), "unit": "Gcell-updates/s",
+              "shots_sampled": [int(k) for k in pick],
+              "nt_sampled": [int(B.nt[int(k)]) for k in pick],
+              "batched_over_per_shot": round(value / ps_rate, 3)}
+
+    cpu = cpu1 = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", 0) or os.cpu_count() or 1)
+        log(f"bench: cpu baseline, {threads} threads, {args.cpu_steps} steps")
+        sp, P0 = wl.specs[0], B.props[0]
+        inpa = dict(wl.inpa, rec_dis=float(sp.rec_loc[1, 0] - sp.rec_loc[0, 0]))
+        args_cpu = (inpa, sp.src_loc, sp.rec_loc, wl.model, P0.src.w, P0.src.component,
+                    wl.components)
+        cpu = cpu_baseline_shot(*args_cpu, args.cpu_steps, threads)
+        log(f"bench: cpu baseline, 1 thread, {args.cpu_steps_1t} steps")
+        cpu1 = cpu_baseline_shot(*args_cpu, args.cpu_steps_1t, 1)
+
     print(json.dumps({
         "metric": "fwd+adj gradient Gcell-updates/s, batched vs per-shot, % HBM roofline",
         "value": round(value, 3), "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(elapsed * 1e3, 1), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded random-layered model, per-shot Ricker sources)",
-        "config": {"workload": wl.name, "nz": 1000, "nx": 2000, "cells": B.tnz * B.tnx,
-                   "shots": B.ns, "nt_range": [min(B.nt), max(B.nt)],
+        "config": {"workload": wl.name, "nz": int(wl.model_shape[0]),
+                   "nx": int(wl.model_shape[1]), "npml": npml, "cells": cells,
+                   "order": order, "shots": B.ns, "nt_range": [min(B.nt), max(B.nt)],
                    "receivers_range": [min(B.nr), max(B.nr)],
-                   "gradient": "reference adjoint, every state kept (chpr=100), passes sized "
-                               "to free HBM",
-                   "cell_updates": B.cell_updates()},
+                   "fdom_range": [round(min(s.fdom for s in wl.specs), 2),
+                                  round(max(s.fdom for s in wl.specs), 2)],
+                   "gradient": "reference adjoint, every forward state kept in HBM (chpr=100), "
+                               "passes of shots sized to the free HBM, longest shots first; "
+                               "L2 misfit of vx, vz on the GPU",
+                   "passes": passes, "cell_updates": units,
+                   "parallelism": "single GPU, shots batched per launch"},
+        "kernel_timing": "per-step time of each time loop (HIP events around the loops) split "
+                         "between its kernels by event-bracketed launches",
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(wl.name, dom),
+                     "bytes_per_launch": kernels[dom]["bytes_per_launch"],
+                     "avg_launch_us": kernels[dom]["avg_us"]},
         "gradient_roofline_frac": round(value * 148.0 / HBM_PEAK_GBS, 4),
-        "misfit": misfit}), flush=True)
+        "forward_only": {"value": round(fwd_rate, 3), "unit": "Gcell-updates/s",
+                         "roofline_frac": round(fwd_rate * 52.0 / HBM_PEAK_GBS, 4)},
+        "misfit": misfit,
+        "per_shot": ps,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+        "cpu_baseline_1thread": cpu1}), flush=True)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="plain_medium")
+    ap.add_argument("--workload", default="uneven_batch",
+                    help="uneven_batch (1000x2000, 64 uneven shots; the default), plain_medium, "
+                         "large_step, differentiation, plain_small, crosswell_medium, "
+                         "wells_surface_medium")
     ap.add_argument("--nt", type=int, default=0, help="override time steps (profiling runs)")
     ap.add_argument("--order", type=int, default=0, choices=[0, 4, 8],
                     help="stencil order (0: the workload's own, 4)")
     ap.add_argument("--shots", type=int, default=0,
                     help="override the number of shots (e.g. one GPU's share of large_step)")
-    ap.add_argument("--per-shot-sample", type=int, default=2, help="0 skips the per-shot pass")
-    ap.add_argument("--cpu-steps", type=int, default=2000)
-    ap.add_argument("--cpu-steps-1t", type=int, default=150)
+    ap.add_argument("--per-shot-sample", type=int, default=0,
+                    help="shots run one per pass for the per-shot rate (0: the workload's "
+                         "default, 4 for uneven_batch, 2 otherwise; -1 skips)")
+    ap.add_argument("--cpu-steps", type=int, default=0,
+                    help="time steps of the CPU baseline (0: 600 for uneven_batch, 2000 else)")
+    ap.add_argument("--cpu-steps-1t", type=int, default=0,
+                    help="time steps of the 1-thread CPU baseline (0: 80 / 150)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the execution-shape autotuner")
@@ -396,7 +566,14 @@ def main():
     log("bench: building problem")
     if args.copy_calibration:
         copy_calibration(torch)
-    if args.workload == "uneven_batch":
+        if args.steps == 0:
+            return
+    big = args.workload == "uneven_batch"
+    if args.per_shot_sample == 0:
+        args.per_shot_sample = 4 if big else 2
+    args.cpu_steps = args.cpu_steps or (600 if big else 2000)
+    args.cpu_steps_1t = args.cpu_steps_1t or (80 if big else 150)
+    if big:
         return bench_uneven(args, torch)
     w, W = build_problem(args.workload, args.nt or None, args.shots or None, args.order)
     b = device_setup(w, W, torch)

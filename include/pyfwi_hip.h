@@ -35,7 +35,9 @@ enum {
     PYFWI_ENOMEM = 2,   /* device or host allocation failed         */
     PYFWI_EHIP = 3,     /* HIP runtime error                        */
     PYFWI_ESTATE = 4,   /* call out of order (e.g. no forward yet)  */
-    PYFWI_ENONFINITE = 5 /* model holds NaN or Inf                   */
+    PYFWI_ENONFINITE = 5, /* model holds NaN or Inf                  */
+    PYFWI_ECFL = 6       /* dt above the stability limit of the model
+                            (Fdm.dt_computation, fwi_tools.py:634-647)  */
 };
 
 enum { PYFWI_MEM_HOST = 0, PYFWI_MEM_DEVICE = 1 };
@@ -139,7 +141,13 @@ int32_t pyfwi_plan_set_stream(pyfwi_plan *plan, void *stream);
  * profiles dx_prof [tnx] (per column) and dz_prof [tnz] (per row), already
  * scaled by the model's maximum P velocity.  Replaces elastic_buffers and
  * pml_preparation (wave_propagation.py:570-591, :400-407); the reference's
- * full-grid damping arrays are constant along one axis. */
+ * full-grid damping arrays are constant along one axis.
+ * Returns PYFWI_ENONFINITE when the model holds NaN or Inf, and PYFWI_ECFL
+ * when the grid's dt exceeds the stability limit of Fdm.dt_computation
+ * (fwi_tools.py:634-647): 2 / (vp_max sqrt(5 sum|c_k| (1/dx^2 + 1/dz^2))),
+ * vp_max = max sqrt((lam + 2 mu) b); the reference runs such a model and
+ * diverges.  The model is kept (not used) after either error until a valid
+ * one is uploaded. */
 int32_t pyfwi_set_model(pyfwi_plan *plan, const float *lam, const float *mu,
                         const float *buoy, const float *dx_prof,
                         const float *dz_prof, int32_t mem);
@@ -228,6 +236,14 @@ int32_t pyfwi_plan_set_tuning(pyfwi_plan *plan, int32_t shots_per_pass, int32_t 
  * 710-727). */
 int32_t pyfwi_get_snapshot(pyfwi_plan *plan, int32_t shot, int32_t slot,
                            float *out, int32_t mem);
+
+/* Shots per pass of the last pyfwi_forward, pyfwi_gradient or
+ * pyfwi_misfit_gradient call, in the order they ran (no counterpart in the
+ * reference, which runs one shot at a time).  pyfwi_misfit_gradient runs the
+ * shots by decreasing nt (stable order) when the checkpoint list is every
+ * step; the other calls in the order given.  `sizes` receives up to `cap`
+ * entries, `count` the number of passes. */
+int32_t pyfwi_last_passes(pyfwi_plan *plan, int32_t *sizes, int32_t cap, int32_t *count);
 
 /* Wait for all work queued on the plan's stream. */
 int32_t pyfwi_synchronize(pyfwi_plan *plan);

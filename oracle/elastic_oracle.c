@@ -29,10 +29,25 @@
  *   pyfwi adjoint step    interior transposition: with zero damping it equals
  *                         the transpose of the stress-then-velocity step
  *   exact gradient        central finite differences in FP64
- *   pyfwi damping terms of the adjoint step and the imaging-condition formulas
- *                         (elastic.cl:395-408, 465-466, 485-524): pinned by
- *                         restatement and by agreement in direction with the
- *                         exact gradient only -- parity unpinned beyond that.
+ *   pyfwi adjoint damping one adjoint step on constant fields with constant
+ *                         damping scales each cell by exactly (1-dt dx),
+ *                         (1-dt dz), (1-dt(dx+dz)), (1-dt(2dx+dz)),
+ *                         (1-dt(dx+2dz)) (elastic.cl:395-408, 465-466)
+ *   reverse-step order    linear fields with damping: forward
+ *                         (v + dt b D)(1 - dt d), reverse v(1 - dt d) - dt b D
+ *                         in closed form (elastic.cl:175-182, 220-223,
+ *                         262-268, 303-306)
+ *   imaging condition     quadratic fields, on which the staggered stencils
+ *                         are exact at their half-cell points: Gmu, Glam, Grho
+ *                         and the three preconditioners equal the reference's
+ *                         formulas built from analytic derivatives, including
+ *                         the vz in "DzpVx" (elastic.cl:470-526)
+ *   backward loop         a 3-step case against the loop of
+ *                         wave_propagation.py:751-848 composed from the
+ *                         single-step primitives
+ *   Every check above is shown to fail on a build with the matching planted
+ *   mistake (ORC_MUTANT below).  The receiver-table rules and the crosswell
+ *   gathers are pinned in tests/test_oracle_wells.py.
  */
 #include <stdlib.h>
 #include <string.h>
@@ -127,6 +142,27 @@ int orc_receiver_cells(const orc_grid *g, const orc_acq *q, long *cell)
 }
 
 enum { TAP_F = 0, TAP_FA = 1, TAP_FM = 2 };
+
+/* Planted mistakes, one per build, for tests/test_oracle_pins.py to show that
+ * each of its checks catches the slip it is written for.  liborc.so is built
+ * with ORC_MUTANT 0 (none); `make -C oracle mutants` builds
+ * _mut/liborc_mut<k>.so with mistake k:
+ *   1  Adj_update_tau damps Atxx with dz instead of dx   (elastic.cl:395)
+ *   2  Adj_update_v damps Avx with dx+dz, not 2dx+dz     (elastic.cl:465)
+ *   3  Adj_update_v damps Avz with 2dx+dz, not dx+2dz    (elastic.cl:466)
+ *   4  the time-reversed vx step damps after the update  (elastic.cl:179-182)
+ *   5  Grad drops the 2 of the normal terms of Gmu       (elastic.cl:510)
+ *   6  Grad's "DzpVx" differentiates vx, not vz          (elastic.cl:485-486)
+ *   7  Grad scales Grho by b, not b^2                    (elastic.cl:520)
+ *   8  Gmu_precond with 2 dt^2, not 4 dt^2               (elastic.cl:513)
+ *   9  Glam_precond with txz^2 for tzz^2                 (elastic.cl:518)
+ *  10  Grho_precond with vx and vz swapped               (elastic.cl:523-524)
+ *  11  Adj_update_tau damps Atxz with dx only            (elastic.cl:408)
+ *  12  the backward loop never restores a checkpoint    (wave_propagation.py:760-776) */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+#define ORC_MUT(k, wrong, right) (ORC_MUTANT == (k) ? (wrong) : (right))
 
 #define R float
 #define SFX(x) x##_f

@@ -72,9 +72,9 @@ static void SFX(vel_x)(const orc_grid *g, int coeff, R *vx, const R *txx,
         long c = (long)i * g->nx + j;
         R dtxx = DXM(txx), dtxz = DZM(txz);
         R damp = dxp[c] + dzp[c];
-        if (coeff == 1) {
+        if (coeff == 1 || ORC_MUTANT == 4) {
             vx[c] += coeff * dt * (bu[c] * (dtxx + dtxz));
-            vx[c] += coeff * dt * (-damp * vx[c]);
+            vx[c] += coeff * dt * (-coeff * damp * vx[c]);
         } else {
             vx[c] += coeff * dt * (damp * vx[c]);
             vx[c] += coeff * dt * (bu[c] * (dtxx + dtxz));
@@ -157,11 +157,11 @@ static void SFX(adj_tau)(const orc_grid *g, const R *avx, const R *avz, R *atxx,
         long c = (long)i * g->nx + j;
         R dxp_bvx = SFX(der)(g, TAP_FA, avx, bu, 0, c, 1, 1, g->dx);
         R dzm_bvz = SFX(der)(g, TAP_FA, avz, bu, 0, c, g->nx, 0, g->dz);
-        atxx[c] += -dt * (dxp[c] * atxx[c] + dxp_bvx);
+        atxx[c] += -dt * (ORC_MUT(1, dzp[c], dxp[c]) * atxx[c] + dxp_bvx);
         atzz[c] += -dt * (dzp[c] * atzz[c] + dzm_bvz);
         R dzp_bvx = SFX(der)(g, TAP_FA, avx, bu, 0, c, g->nx, 1, g->dz);
         R dxm_bvz = SFX(der)(g, TAP_FA, avz, bu, 0, c, 1, 0, g->dx);
-        atxz[c] += -dt * ((dxp[c] + dzp[c]) * atxz[c] + (dzp_bvx + dxm_bvz));
+        atxz[c] += -dt * (ORC_MUT(11, dxp[c], dxp[c] + dzp[c]) * atxz[c] + (dzp_bvx + dxm_bvz));
     }
 }
 
@@ -179,8 +179,8 @@ static void SFX(adj_v)(const orc_grid *g, R *avx, R *avz, const R *atxx,
         R b1 = SFX(der)(g, TAP_FM, atzz, lam, mu, c, g->nx, 1, g->dz);
         R b2 = SFX(der)(g, TAP_FA, atxx, lam, 0, c, g->nx, 1, g->dz);
         R b3 = SFX(der)(g, TAP_FA, atxz, mu, 0, c, 1, 1, g->dx);
-        avx[c] += -dt * ((2 * dxp[c] + dzp[c]) * avx[c] + (a1 + a2 + a3));
-        avz[c] += -dt * ((dxp[c] + 2 * dzp[c]) * avz[c] + (b2 + b1 + b3));
+        avx[c] += -dt * (ORC_MUT(2, dxp[c] + dzp[c], 2 * dxp[c] + dzp[c]) * avx[c] + (a1 + a2 + a3));
+        avz[c] += -dt * (ORC_MUT(3, 2 * dxp[c] + dzp[c], dxp[c] + 2 * dzp[c]) * avz[c] + (b2 + b1 + b3));
     }
 }
 
@@ -200,16 +200,16 @@ static void SFX(grad)(const orc_grid *g, const R *F, const R *A, const R *bu,
     const int lo = g->npml + g->sdo;
     FOR_ACTIVE(lo, g->nz - g->sdo - g->npml, lo, g->nx - g->sdo - g->npml) {
         long c = (long)i * g->nx + j;
-        R dzp_vz = DZP(vz), dxm_vz = DXM(vz), dxp_vx = DXP(vx), dzm_vz = DZM(vz);
+        R dzp_vz = ORC_MUT(6, DZP(vx), DZP(vz)), dxm_vz = DXM(vz), dxp_vx = DXP(vx), dzm_vz = DZM(vz);
         R dzp_tzz = DZP(tzz), dxm_txx = DXM(txx), dxp_txz = DXP(txz), dzm_txz = DZM(txz);
-        gmu[c] += 2 * dt * (atxx[c] * dxp_vx + atzz[c] * dzm_vz) + dt * atxz[c] * (dzp_vz + dxm_vz);
-        gmu_p[c] += 4 * dt * dt * (txx[c] * txx[c] * dxp_vx * dxp_vx + tzz[c] * tzz[c] * dzm_vz * dzm_vz)
+        gmu[c] += ORC_MUT(5, 1, 2) * dt * (atxx[c] * dxp_vx + atzz[c] * dzm_vz) + dt * atxz[c] * (dzp_vz + dxm_vz);
+        gmu_p[c] += ORC_MUT(8, 2, 4) * dt * dt * (txx[c] * txx[c] * dxp_vx * dxp_vx + tzz[c] * tzz[c] * dzm_vz * dzm_vz)
                     + dt * dt * txz[c] * txz[c] * (dzp_vz * dzp_vz + dxm_vz * dxm_vz);
         glam[c] += dt * (atxx[c] + atzz[c]) * (dxp_vx + dzm_vz);
-        glam_p[c] += dt * dt * (txx[c] * txx[c] + tzz[c] * tzz[c]) * (dxp_vx * dxp_vx + dzm_vz * dzm_vz);
-        grho[c] += -(dt * (bu[c] * bu[c])) * (avx[c] * (dxm_txx + dzm_txz) + avz[c] * (dzp_tzz + dxp_txz));
-        grho_p[c] += dt * dt * (vx[c] * vx[c] * (dxm_txx * dxm_txx + dzm_txz * dzm_txz)
-                                + vz[c] * vz[c] * (dzp_tzz * dzp_tzz + dxp_txz * dxp_txz));
+        glam_p[c] += dt * dt * (txx[c] * txx[c] + ORC_MUT(9, txz[c] * txz[c], tzz[c] * tzz[c])) * (dxp_vx * dxp_vx + dzm_vz * dzm_vz);
+        grho[c] += -(dt * ORC_MUT(7, bu[c], bu[c] * bu[c])) * (avx[c] * (dxm_txx + dzm_txz) + avz[c] * (dzp_tzz + dxp_txz));
+        grho_p[c] += dt * dt * (ORC_MUT(10, vz[c] * vz[c], vx[c] * vx[c]) * (dxm_txx * dxm_txx + dzm_txz * dzm_txz)
+                                + ORC_MUT(10, vx[c] * vx[c], vz[c] * vz[c]) * (dzp_tzz * dzp_tzz + dxp_txz * dxp_txz));
     }
 }
 
@@ -395,7 +395,7 @@ int SFX(orc_gradient)(const orc_grid *g, const orc_acq *q, const R *lam,
         int chpc = nchp - 1;
         int t_src = q->nt_ext;
         for (int t = q->nt - 1; t > 0; --t) {
-            if (chpc >= 0 && t == chp[chpc]) {
+            if (ORC_MUTANT != 12 && chpc >= 0 && t == chp[chpc]) {
                 memcpy(F, W + ((long)s * nchp + chpc) * 5 * n, 5 * n * sizeof(R));
                 chpc--;
             } else {

@@ -57,6 +57,15 @@ def build():
     return LIB
 
 
+def mutant(k):
+    """The oracle built with planted mistake k (ORC_MUTANT, elastic_oracle.c),
+    for showing that a pinning check catches it."""
+    path = os.path.join(HERE, "_mut", f"liborc_mut{k}.so")
+    subprocess.run(["make", "-s", "-C", HERE, os.path.join("_mut", f"liborc_mut{k}.so")],
+                   check=True)
+    return ctypes.CDLL(path)
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -389,9 +398,10 @@ def exact_gradient(setup, model, res_records, dtype=np.float64, shots=None, medi
     return g3
 
 
-def step(name, grid, F, md, dtype=np.float64, A=None, G=None):
-    """Single-step primitives for unit tests: name in fwd, fwd_sv, rev, adj, grad."""
-    L = lib()
+def step(name, grid, F, md, dtype=np.float64, A=None, G=None, L=None):
+    """Single-step primitives for unit tests: name in fwd, fwd_sv, rev, adj, grad.
+    ``L``: another build of the library (a planted-mistake build), else liborc.so."""
+    L = lib() if L is None else L
     sfx = _suffix(dtype)
     args = (_p(md["lam"]), _p(md["mu"]), _p(md["b"]), _p(md["dpx"]), _p(md["dpz"]))
     if name in ("fwd", "fwd_sv", "rev"):

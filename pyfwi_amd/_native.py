@@ -32,9 +32,12 @@ EXPORTS = [
     "pyfwi_version", "pyfwi_last_error", "pyfwi_plan_create", "pyfwi_plan_destroy",
     "pyfwi_plan_set_stream", "pyfwi_set_receivers", "pyfwi_set_model", "pyfwi_forward",
     "pyfwi_gradient",
-    "pyfwi_misfit_gradient", "pyfwi_plan_set_tuning", "pyfwi_get_snapshot", "pyfwi_synchronize", "pyfwi_plan_set_timing",
-    "pyfwi_kernel_timing",
+    "pyfwi_misfit_gradient", "pyfwi_plan_set_tuning", "pyfwi_get_snapshot", "pyfwi_last_passes",
+    "pyfwi_synchronize", "pyfwi_plan_set_timing", "pyfwi_kernel_timing",
 ]
+
+# status codes (include/pyfwi_hip.h)
+OK, EINVAL, ENOMEM, EHIP, ESTATE, ENONFINITE, ECFL = range(7)
 
 
 class Grid(ctypes.Structure):
@@ -95,6 +98,8 @@ def load(path=None):
                                           ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_int32), i32, i32, i32, i32]
     lib.pyfwi_get_snapshot.argtypes = [P, i32, i32, fp, i32]
+    lib.pyfwi_last_passes.argtypes = [P, ctypes.POINTER(ctypes.c_int32), i32,
+                                      ctypes.POINTER(ctypes.c_int32)]
     lib.pyfwi_synchronize.argtypes = [P]
     lib.pyfwi_plan_set_timing.argtypes = [P, i32]
     lib.pyfwi_kernel_timing.argtypes = [P, i32, ctypes.POINTER(ctypes.c_double),
@@ -134,6 +139,31 @@ def ptr(a, mem=None):
     return a.data_ptr()
 
 
+def numel(a):
+    """Number of float elements of a numpy array or torch tensor."""
+    return int(a.size) if isinstance(a, np.ndarray) else int(a.numel())
+
+
+def extents(shots):
+    """Floats the source table and the record buffers must hold for a shot
+    table: max(src_offset + 5*ceil(nt/dt_scale)) and
+    max(rec_offset + 5*n_samples*rec_stride) (src_extent / rec_extent of the
+    library)."""
+    src = rec = 0
+    for sh in shots:
+        ninj = -(-int(sh.nt) // max(1, int(sh.dt_scale)))
+        src = max(src, int(sh.src_offset) + 5 * ninj)
+        rec = max(rec, int(sh.rec_offset) + 5 * int(sh.n_samples) * int(sh.rec_stride))
+    return src, rec
+
+
+def _need(a, n, what):
+    """The kernels read and write a caller buffer as far as the shot table
+    says; a shorter buffer would be read or written past its end."""
+    if a is not None and numel(a) < n:
+        raise ValueError(f"{what} holds {numel(a)} floats; the call needs {n}")
+
+
 class Plan:
     """Owner of one native plan (one grid on the current GPU)."""
 
@@ -166,11 +196,22 @@ class Plan:
             self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)) if t.size else None,
             int(t.size)))
 
+    @property
+    def cells(self):
+        return int(self.grid.tnz) * int(self.grid.tnx)
+
     def set_model(self, lam, mu, buoy, dxp, dzp, mem):
+        for a, n, what in ((lam, self.cells, "lam"), (mu, self.cells, "mu"),
+                           (buoy, self.cells, "buoyancy"), (dxp, int(self.grid.tnx), "dx profile"),
+                           (dzp, int(self.grid.tnz), "dz profile")):
+            _need(a, n, what)
         check(self.lib.pyfwi_set_model(self.h, ptr(lam, mem), ptr(mu, mem), ptr(buoy, mem),
                                        ptr(dxp, mem), ptr(dzp, mem), mem))
 
     def forward(self, shots, src, seis, chp, ckpt_every, mem):
+        ns, nr = extents(shots)
+        _need(src, ns, "source table")
+        _need(seis, nr, "records")
         arr = (Shot * len(shots))(*shots)
         chp = np.ascontiguousarray(chp, np.int32) if chp is not None else np.zeros(0, np.int32)
         cp = chp.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)) if chp.size else None
@@ -178,6 +219,10 @@ class Plan:
                                      int(chp.size), int(ckpt_every), mem))
 
     def gradient(self, shots, src, res, grad, mode, ckpt_every, flags, mem):
+        ns, nr = extents(shots)
+        _need(src, ns, "source table")
+        _need(res, nr, "residuals")
+        _need(grad, 6 * self.cells, "gradient")
         arr = (Shot * len(shots))(*shots)
         check(self.lib.pyfwi_gradient(self.h, arr, len(shots),
                                       ptr(src, mem) if src is not None else None,
@@ -186,6 +231,11 @@ class Plan:
 
     def misfit_gradient(self, shots, src, obs, seis, grad, chp, comp_mask, flags, mem):
         """Returns the L2 misfit; see pyfwi_misfit_gradient."""
+        ns, nr = extents(shots)
+        _need(src, ns, "source table")
+        _need(obs, nr, "observed records")
+        _need(seis, nr, "records")
+        _need(grad, 6 * self.cells, "gradient")
         arr = (Shot * len(shots))(*shots)
         chp = np.ascontiguousarray(chp, np.int32)
         out = ctypes.c_double()
@@ -200,7 +250,19 @@ class Plan:
                                              int(strip_rows)))
 
     def snapshot(self, shot, slot, out, mem):
+        _need(out, 5 * self.cells, "snapshot output")
         check(self.lib.pyfwi_get_snapshot(self.h, shot, slot, ptr(out, mem), mem))
+
+    def last_passes(self):
+        """Shots per pass of the last forward / gradient / misfit call."""
+        cap = 4096
+        buf = (ctypes.c_int32 * cap)()
+        n = ctypes.c_int32()
+        check(self.lib.pyfwi_last_passes(self.h, buf, cap, ctypes.byref(n)))
+        if n.value > cap:
+            buf = (ctypes.c_int32 * n.value)()
+            check(self.lib.pyfwi_last_passes(self.h, buf, n.value, ctypes.byref(n)))
+        return [int(buf[k]) for k in range(n.value)]
 
     def synchronize(self):
         check(self.lib.pyfwi_synchronize(self.h))

@@ -143,5 +143,10 @@ void launch_grad_sum(const DevGrid &g, int nsets, int nslot, const float *gacc, 
                      const float *bu, int b_to_rho, int accumulate, hipStream_t st);
 void launch_pad(const DevGrid &g, const float *src, float *dst, hipStream_t st);
 void launch_unpad(const DevGrid &g, const float *src, float *dst, int nf, hipStream_t st);
+// out[0]: bits of the largest (lam+2mu)*b of the padded model (the squared
+// largest P velocity); out[1]: 1 if lam, mu or b holds NaN or Inf.  The
+// caller zeroes the two ints first.
+void launch_model_check(const DevGrid &g, const float *lam, const float *mu, const float *bu,
+                        int *out, hipStream_t st);
 
 } // namespace pyfwi

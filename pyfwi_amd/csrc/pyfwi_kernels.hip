@@ -1746,6 +1746,15 @@ __global__ void k_zero(float *p, int64_t n)
         p[k] = 0.0f;
 }
 
+// Zero for buffers off the 16-byte grid (no memset node, so the clear can
+// sit in a captured graph).
+__global__ void k_zero1(float *p, int64_t n)
+{
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x)
+        p[k] = 0.0f;
+}
+
 __global__ void k_copy_shots(float *dst, int64_t dst_stride, const float *src,
                              int64_t src_stride, int64_t count)
 {
@@ -1793,6 +1802,36 @@ __global__ void k_pad(DevGrid g, const float *src, float *dst)
          k += (int64_t)gridDim.x * blockDim.x) {
         const int i = (int)(k / g.nx), j = (int)(k - (int64_t)i * g.nx);
         dst[g.origin + (int64_t)i * g.pitch + j] = src[k];
+    }
+}
+
+// Model check over the padded model: out[0] = bits of max((lam+2mu)*b), the
+// squared largest P velocity (non-negative floats order like their bit
+// patterns); out[1] = 1 when lam, mu or b holds NaN or Inf.  out: two ints
+// zeroed by the caller.
+__global__ void k_model_check(DevGrid g, const float *__restrict__ lam,
+                              const float *__restrict__ mu, const float *__restrict__ bu,
+                              int *out)
+{
+    const int64_t n = (int64_t)g.nz * g.nx;
+    float m = 0.0f;
+    int bad = 0;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(k / g.nx), j = (int)(k - (int64_t)i * g.nx);
+        const int64_t c = g.origin + (int64_t)i * g.pitch + j;
+        const float l = lam[c], u = mu[c], b = bu[c];
+        if (!isfinite(l) || !isfinite(u) || !isfinite(b)) bad = 1;
+        else m = fmaxf(m, (l + 2.0f * u) * b);
+    }
+    // wave-wide reduction, then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) {
+        m = fmaxf(m, __shfl_xor(m, off, 64));
+        bad |= __shfl_xor(bad, off, 64);
+    }
+    if (threadIdx.x % 64 == 0) {
+        atomicMax(&out[0], __float_as_int(fmaxf(m, 0.0f)));
+        if (bad) atomicOr(&out[1], 1);
     }
 }
 
@@ -2034,7 +2073,8 @@ void launch_zero(float *p, int64_t n, hipStream_t st)
 {
     if (n <= 0) return;
     if (reinterpret_cast<uintptr_t>(p) % 16) {  // a caller's buffer off the 16-byte grid
-        (void)hipMemsetAsync(p, 0, (size_t)n * sizeof(float), st);
+        const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_zero1, dim3((unsigned)blocks), dim3(256), 0, st, p, n);
         return;
     }
     const int64_t blocks = std::min<int64_t>((n / 4 + 255) / 256 + 1, 8192);
@@ -2062,6 +2102,12 @@ void launch_grad_sum(const DevGrid &g, int nsets, int nslot, const float *gacc, 
 void launch_pad(const DevGrid &g, const float *src, float *dst, hipStream_t st)
 {
     hipLaunchKernelGGL(k_pad, dim3(512), dim3(256), 0, st, g, src, dst);
+}
+
+void launch_model_check(const DevGrid &g, const float *lam, const float *mu, const float *bu,
+                        int *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_model_check, dim3(512), dim3(256), 0, st, g, lam, mu, bu, out);
 }
 
 void launch_unpad(const DevGrid &g, const float *src, float *dst, int nf, hipStream_t st)

@@ -92,6 +92,8 @@ struct pyfwi_plan {
     bool ckpts_all = false;  // ckpts holds every shot of the last forward
     DevBuf src_stage, seis_stage, res_stage, grad_stage, shot_tab_f, resid;
     double *d_misfit = nullptr;
+    int *d_check = nullptr;  // two ints for the model check
+    std::vector<int32_t> last_passes;  // shots per pass of the last call
     pyfwi_shot *d_shots = nullptr;
     int shots_cap = 0;
     // receiver table (pyfwi_set_receivers): host copy for the shot checks
@@ -197,6 +199,8 @@ int check_shots(const pyfwi_plan *p, const pyfwi_shot *shots, int nshots)
 {
     const pyfwi_grid &g = p->grid;
     if (!shots || nshots <= 0) return fail(PYFWI_EINVAL, "no shots");
+    // launches put the shots of a pass in grid.y, at most 65535
+    if (nshots > 65535) return fail(PYFWI_EINVAL, "at most 65535 shots a call");
     for (int s = 0; s < nshots; ++s) {
         const pyfwi_shot &h = shots[s];
         std::string w = "shot " + std::to_string(s) + ": ";
@@ -377,6 +381,7 @@ int32_t pyfwi_plan_destroy(pyfwi_plan *p)
                       &p->shot_tab_f, &p->resid})
         b->release();
     if (p->d_misfit) (void)hipFree(p->d_misfit);
+    if (p->d_check) (void)hipFree(p->d_check);
     if (p->d_shots) (void)hipFree(p->d_shots);
     if (p->d_rtab) (void)hipFree(p->d_rtab);
     for (auto e : p->ev_pool) (void)hipEventDestroy(e);
@@ -465,10 +470,33 @@ int32_t pyfwi_set_model(pyfwi_plan *p, const float *lam, const float *mu, const 
     }
     HIPCHK(hipMemcpyAsync(p->dxp, dx_prof, g.nx * sizeof(float), kind, p->stream));
     HIPCHK(hipMemcpyAsync(p->dzp, dz_prof, g.nz * sizeof(float), kind, p->stream));
-    HIPCHK(hipGetLastError());
-    if (mem == PYFWI_MEM_HOST) HIPCHK(hipStreamSynchronize(p->stream));  // host arrays may go away
-    p->model_set = true;
+    p->model_set = false;
     p->have_forward = false;
+    // largest P velocity (and NaN / Inf on device arrays) for the stability check
+    if (!p->d_check) HIPCHK(hipMalloc(&p->d_check, 2 * sizeof(int)));
+    launch_zero(reinterpret_cast<float *>(p->d_check), 2, p->stream);
+    launch_model_check(g, p->lam.p, p->mu.p, p->bu.p, p->d_check, p->stream);
+    HIPCHK(hipGetLastError());
+    int chk[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(chk, p->d_check, sizeof(chk), hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));  // also: host arrays may go away
+    if (chk[1]) return fail(PYFWI_ENONFINITE, "model holds NaN or Inf");
+    float vp2 = 0.0f;
+    std::memcpy(&vp2, &chk[0], sizeof(float));
+    const pyfwi_grid &gr = p->grid;
+    const double csum = std::fabs(gr.c[0]) + std::fabs(gr.c[1]) + std::fabs(gr.c[2]) +
+                        std::fabs(gr.c[3]);
+    const double a = csum / (gr.dx * gr.dx) + csum / (gr.dz * gr.dz);
+    const double vmax = std::sqrt((double)vp2);
+    if (vmax > 0) {
+        const double dt_max = 2.0 / vmax / std::sqrt(a * 5.0);
+        if (gr.dt > dt_max * (1.0 + 1e-6))
+            return fail(PYFWI_ECFL, "dt = " + std::to_string(gr.dt) +
+                                        " exceeds the stability limit " + std::to_string(dt_max) +
+                                        " of this model (largest P velocity " +
+                                        std::to_string(vmax) + ")");
+    }
+    p->model_set = true;
     return 0;
 }
 
@@ -710,8 +738,10 @@ int32_t pyfwi_forward(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, co
     }
     launch_zero(d_seis, (int64_t)nrec, p->stream);
     const int G = pass_size(p, nshots);
+    p->last_passes.clear();
     for (int g0 = 0; g0 < nshots; g0 += G) {
         const int ng = std::min(G, nshots - g0);
+        p->last_passes.push_back(ng);
         float *sn = nchp > 0 ? p->snaps.p + (int64_t)g0 * nchp * fps : nullptr;
         float *ck = ncp > 0 ? p->ckpts.p + (int64_t)g0 * ncp * fps : nullptr;
         if (int e = forward_pass(p, g0, ng, d_src, d_seis, cl, sn, ck, ncp, ckpt_every)) return e;
@@ -904,8 +934,10 @@ int32_t pyfwi_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, c
     const int G = pass_size(p, nshots);
     const int K = ckpt_every > 0 ? ckpt_every : (p->ckpt_every > 0 ? p->ckpt_every : 50);
     const bool reuse = mode == PYFWI_MODE_EXACT && same && p->ckpts_all && p->ckpt_every == K;
+    p->last_passes.clear();
     for (int g0 = 0; g0 < nshots; g0 += G) {
         const int ng = std::min(G, nshots - g0);
+        p->last_passes.push_back(ng);
         const int nsets = adj_acc_sets(g.sdo, ng);
         if (int e = ensure(p->gacc, (size_t)nsets * 6 * g.fstride, p->stream, "gradient"))
             return e;
@@ -1019,11 +1051,13 @@ int32_t pyfwi_misfit_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t ns
                                           " bytes) do not fit in device memory");
     }
     if (int e = ensure(p->snaps, slots * fps, p->stream, "checkpoints")) return e;
+    p->last_passes.clear();
     for (int g0 = 0; g0 < nshots;) {
         const int nt_pass = every ? shs[g0].nt : nt;
         const int stride = every ? nt_pass : nchp;
         int ng = p->tune_group > 0 ? p->tune_group : (int)(slots / (size_t)stride);
         ng = std::max(1, std::min(ng, nshots - g0));
+        p->last_passes.push_back(ng);
         std::vector<int32_t> cl_pass = cl;
         if (every) every_step(nt_pass, cl_pass);
         if (int e = forward_pass(p, g0, ng, d_src, d_seis, cl_pass, p->snaps.p, nullptr, 0, 1))
@@ -1090,6 +1124,14 @@ int32_t pyfwi_get_snapshot(pyfwi_plan *p, int32_t shot, int32_t slot, float *out
         HIPCHK(hipMemcpyAsync(out, d_out, 5 * n * sizeof(float), hipMemcpyDeviceToHost, p->stream));
         HIPCHK(hipStreamSynchronize(p->stream));
     }
+    return 0;
+}
+
+int32_t pyfwi_last_passes(pyfwi_plan *p, int32_t *sizes, int32_t cap, int32_t *count)
+{
+    if (!p || !count || cap < 0 || (cap > 0 && !sizes)) return fail(PYFWI_EINVAL, "null argument");
+    *count = (int32_t)p->last_passes.size();
+    for (int k = 0; k < cap && k < *count; ++k) sizes[k] = p->last_passes[k];
     return 0;
 }
 

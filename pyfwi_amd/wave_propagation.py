@@ -277,6 +277,18 @@ class WavePreparation:
                         out[name][:, :, s, c] = buf[f]
         return out
 
+    def gradient_window(self):
+        """Rows [z0, z1) and columns [x0, x1) of the cropped model where
+        gradient_reading keeps the gradient (src/PyFWI/wave_propagation.py:
+        303-311): below the source row for surface acquisitions, right of the
+        source well for crosswell (acq_type 0), sdo cells off the other
+        edges."""
+        sdo, nz, nx = int(self.sdo), int(self.nz), int(self.nx)
+        r0 = sdo + int(self.src_cts) + 2
+        if self.acq_type == 0:
+            return (sdo, nz - sdo), (r0, nx - sdo)
+        return (r0, nz - sdo), (sdo, nx - sdo)
+
     def gradient_reading(self, G):
         """Crop the absorbing layer, mask the source rows and edges and, with
         energy balancing, divide by sqrt of the preconditioners normalised to a
@@ -290,17 +302,13 @@ class WavePreparation:
             one = np.ones((tnz, tnx), np.float32)
             pre = {"mu": one, "lam": one, "rho": one}
 
+        (z0, z1), (x0, x1) = self.gradient_window()
+
         def factor_of(precond):
             factor = np.zeros((int(self.nz), int(self.nx)), np.float32)
-            denom = np.sqrt(precond)
-            r0 = sdo + int(self.src_cts) + 2
+            denom = np.sqrt(precond)[npml + z0:npml + z1, npml + x0:npml + x1]
             with np.errstate(divide="ignore", invalid="ignore"):
-                if self.acq_type == 0:  # columns right of the source well
-                    denom = denom[npml + sdo:tnz - npml - sdo, npml + r0:tnx - npml - sdo]
-                    factor[sdo:-sdo, r0:-sdo] = 1 / denom
-                else:
-                    denom = denom[npml + r0:tnz - npml - sdo, npml + sdo:tnx - npml - sdo]
-                    factor[r0:-sdo, sdo:-sdo] = 1 / denom
+                factor[z0:z1, x0:x1] = 1 / denom
                 factor = factor / np.abs(factor).max()
             return factor
 
@@ -623,17 +631,17 @@ def _fold_padding(g, npml):
 def _gradient_reading_tensor(W, G):
     """gradient_reading for device tensors (wave_propagation.py:279-331)."""
     import torch
-    npml, sdo = int(W.npml), int(W.sdo)
+    npml = int(W.npml)
     tnz, tnx, nz, nx = int(W.tnz), int(W.tnx), int(W.nz), int(W.nx)
-    r0 = sdo + int(W.src_cts) + 2
+    (z0, z1), (x0, x1) = W.gradient_window()
 
     def factor_of(pre):
         f = torch.zeros(nz, nx, dtype=torch.float32, device=G.device)
         if pre is None:
-            f[r0:nz - sdo, sdo:nx - sdo] = 1.0
+            f[z0:z1, x0:x1] = 1.0
             return f
-        d = torch.sqrt(pre)[npml + r0:tnz - npml - sdo, npml + sdo:tnx - npml - sdo]
-        f[r0:nz - sdo, sdo:nx - sdo] = 1 / d
+        d = torch.sqrt(pre)[npml + z0:npml + z1, npml + x0:npml + x1]
+        f[z0:z1, x0:x1] = 1 / d
         return f / f.abs().max()
 
     eb = W.energy_balancing

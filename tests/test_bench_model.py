@@ -59,3 +59,31 @@ def test_more_than_one_gpu_is_refused():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
+
+
+def test_pass_bytes_even_shots_match_launch_bytes():
+    """Passes of equal shots: the uneven byte model gives nt launches of the
+    per-launch bytes (forward) or nt - 1 (adjoint)."""
+    nt = 37
+    for kind in ("fwd_vel", "fwd_str", "adj_str", "adj_vel_grad"):
+        total, n = bench.pass_kernel_bytes(kind, [[nt] * 10, [nt] * 10], CELLS, INTERIOR)
+        steps = nt - 1 if kind.startswith("adj") else nt
+        assert n == 2 * steps
+        assert total == 2 * steps * bench.launch_bytes(kind, CELLS, INTERIOR, 10)
+
+
+def test_pass_bytes_uneven_shots():
+    """Uneven shots: a launch counts only its live shots, and a block's group
+    moves the accumulators while one of its shots is live.  Pass of 6 shots
+    (blocks of four: groups {0..3}, {4, 5}) with nt 10, 9, 8, 8, 5, 3."""
+    nts = [10, 9, 8, 8, 5, 3]
+    a, b, c = bench.KERNEL_BYTES["adj_vel_grad"]
+    want = 0
+    for t in range(1, 10):
+        live = [n > t for n in nts]
+        groups = int(any(live[:4])) + int(any(live[4:]))
+        want += (a * CELLS + b * INTERIOR) * sum(live) + c * INTERIOR * groups
+    total, n = bench.pass_kernel_bytes("adj_vel_grad", [nts], CELLS, INTERIOR)
+    assert n == 9 and total == want
+    total, n = bench.pass_kernel_bytes("fwd_vel", [nts], CELLS, INTERIOR)
+    assert n == 10 and total == 32 * CELLS * sum(nts)
