@@ -80,19 +80,47 @@ def kernel_source_sha1():
     return hashlib.sha1(b"".join(open(f, "rb").read() for f in files)).hexdigest()
 
 
+def _traffic_records(workload_name):
+    """The committed rocprofv3 FETCH_SIZE / WRITE_SIZE summaries
+    (profiles/hbm_traffic*.json) of this workload taken with the current
+    kernel sources."""
+    import glob
+    out = []
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "hbm_traffic*.json"))):
+        try:
+            d = json.load(open(path))
+        except (OSError, ValueError):
+            continue
+        if d.get("workload") == workload_name and \
+                d.get("kernel_source_sha1") == kernel_source_sha1():
+            out.append(d)
+    return out
+
+
 def measured_traffic(workload_name, kernel, shots=None):
-    """HBM bytes per dispatch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary, when it was taken with the current kernel sources (and,
-    given `shots`, with dispatches of that many shots)."""
-    try:
-        d = json.load(open(os.path.join(ROOT, "profiles", "hbm_traffic.json")))
-    except (OSError, ValueError):
-        return None
-    if d.get("workload") != workload_name or d.get("kernel_source_sha1") != kernel_source_sha1():
-        return None
-    if shots is not None and d.get("shots_per_dispatch") != shots:
-        return None
-    return d.get("bytes_per_dispatch", {}).get(kernel)
+    """HBM bytes per dispatch from the committed counter summary, when it was
+    taken with the current kernel sources (and, given `shots`, with
+    dispatches of that many shots)."""
+    for d in _traffic_records(workload_name):
+        if shots is not None and d.get("shots_per_dispatch") != shots:
+            continue
+        got = d.get("bytes_per_dispatch", {}).get(kernel)
+        if got is not None:
+            return got
+    return None
+
+
+def measured_traffic_mix(workload_name, kernel, launches_by_pass_size):
+    """Measured HBM bytes per dispatch of `kernel` averaged over a mix of
+    launches {pass size: launches}, from the counter summary's per-pass-size
+    figures; None unless every pass size of the mix was measured."""
+    for d in _traffic_records(workload_name):
+        per = d.get("bytes_per_dispatch_by_pass_size", {})
+        vals = {int(k): v.get(kernel) for k, v in per.items()}
+        if all(vals.get(n) is not None for n in launches_by_pass_size):
+            total = sum(launches_by_pass_size.values())
+            return sum(vals[n] * c for n, c in launches_by_pass_size.items()) / total
+    return None
 
 
 def build_problem(workload, nt=None, shots=None, order=0):
@@ -188,6 +216,7 @@ def per_shot_rate(W, b, torch, nsub, step):
     cells = int(W.tnz) * int(W.tnx)
     plan = b["plan"]
     plan.set_tuning(1, False, 0)
+    plan.set_graphs(False)  # the reference's schedule: every launch from the host
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     step(W, b, b["shots"][:nsub])
@@ -396,6 +425,8 @@ def bench_uneven(args, torch):
     torch.cuda.synchronize()
     chp = checkpoint_steps(B.nt_max, 100)
     mask = 3  # vx, vz records (components = 2)
+    graphs = args.graphs if args.graphs >= 0 else int(B.tnz * B.tnx < 1_000_000)
+    plan.set_graphs(bool(graphs))
 
     def run(shots):
         return plan.misfit_gradient(shots, src, obs, seis, grad, chp, mask, 0,
@@ -420,6 +451,14 @@ def bench_uneven(args, torch):
     value = units / elapsed / 1e9
     log(f"bench: {elapsed * 1e3:.0f} ms per gradient, {value:.2f} Gcell/s")
     passes = plan.last_passes()
+    if args.minimal:
+        print(json.dumps({"metric": "fwd+adj gradient Gcell-updates/s, batched vs per-shot, "
+                                    "% HBM roofline", "value": round(value, 3),
+                          "unit": "Gcell-updates/s", "n_gpus": 1, "steps": args.steps,
+                          "warmup": args.warmup, "ms_per_step": round(elapsed * 1e3, 1),
+                          "config": {"workload": wl.name, "passes": passes},
+                          "note": "--minimal: timed steps only"}), flush=True)
+        return
     order_nt = sorted(B.nt, reverse=True)  # the plan runs the shots longest first
     pass_nts, k = [], 0
     for n in passes:
@@ -444,6 +483,11 @@ def bench_uneven(args, torch):
     dom = max(prof, key=lambda k: prof[k][0])
     log(f"bench: kernels {kernels}")
     achieved = kernels[dom]["GBps"]
+    # launches of the dominant kernel by the size of their pass
+    launch_mix = {}
+    for nts in pass_nts:
+        n = max(nts) - (1 if dom.startswith("adj") else 0)
+        launch_mix[len(nts)] = launch_mix.get(len(nts), 0) + n
 
     # forward modelling alone: all shots, records only
     no_chp = np.zeros(0, np.int32)
@@ -505,13 +549,13 @@ def bench_uneven(args, torch):
                    "gradient": "reference adjoint, every forward state kept in HBM (chpr=100), "
                                "passes of shots sized to the free HBM, longest shots first; "
                                "L2 misfit of vx, vz on the GPU",
-                   "passes": passes, "cell_updates": units,
+                   "passes": passes, "cell_updates": units, "graphs": bool(graphs),
                    "parallelism": "single GPU, shots batched per launch"},
         "kernel_timing": "per-step time of each time loop (HIP events around the loops) split "
                          "between its kernels by event-bracketed launches",
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(wl.name, dom),
+                     "traffic": measured_traffic_mix(wl.name, dom, launch_mix),
                      "bytes_per_launch": kernels[dom]["bytes_per_launch"],
                      "avg_launch_us": kernels[dom]["avg_us"]},
         "gradient_roofline_frac": round(value * 148.0 / HBM_PEAK_GBS, 4),
@@ -546,6 +590,12 @@ def main():
     ap.add_argument("--cpu-steps-1t", type=int, default=0,
                     help="time steps of the 1-thread CPU baseline (0: 80 / 150)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graphs", type=int, default=-1, choices=[-1, 0, 1],
+                    help="replay the time loops as captured hipGraphs (1), launch eagerly (0); "
+                         "-1: graphs for grids under 1 M cells (launch-bound), eager otherwise")
+    ap.add_argument("--minimal", action="store_true",
+                    help="uneven_batch: the timed steps only (no kernel timing, forward-only, "
+                         "per-shot or CPU legs), for counter runs")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the execution-shape autotuner")
     ap.add_argument("--tune", default="", help="fixed tuning 'shots_per_pass,fused,rows'")
@@ -611,8 +661,10 @@ def main():
                                              b["grad"], torch.cuda.synchronize, log=log)
     else:
         b["plan"].set_tuning(*tuning)
+    graphs = args.graphs if args.graphs >= 0 else int(cells < 1_000_000)
+    b["plan"].set_graphs(bool(graphs))
     log(f"bench: mode={args.mode} tuning shots/pass={tuning[0]} fused={int(tuning[1])} "
-        f"rows={tuning[2]}")
+        f"rows={tuning[2]} graphs={graphs}")
 
     for i in range(args.warmup):
         step(W, b)
@@ -657,6 +709,7 @@ def main():
     if args.per_shot_sample > 0:
         ps_rate, ps_time = per_shot_rate(W, b, torch, args.per_shot_sample, step)
         b["plan"].set_tuning(*tuning)
+        b["plan"].set_graphs(bool(graphs))
         log(f"bench: per-shot path {ps_rate:.2f} Gcell/s")
         ps = {"value": round(ps_rate, 3), "shots_sampled": args.per_shot_sample,
               "batched_over_per_shot": round(value / ps_rate, 2)}
@@ -694,7 +747,7 @@ def main():
                                 "recomputed"}[args.mode],
                    "parallelism": "single GPU, shots batched per launch",
                    "tuning": {"shots_per_pass": tuning[0], "fused_steps": bool(tuning[1]),
-                              "strip_rows": tuning[2]}},
+                              "strip_rows": tuning[2], "graphs": bool(graphs)}},
         "kernel_timing": ("event-bracketed launches (each holds a few microseconds of the "
                           "event's stall)" if args.mode == "exact" else
                           "per-step time of each time loop (HIP events around the loops) split "

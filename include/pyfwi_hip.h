@@ -54,6 +54,10 @@ enum {
     PYFWI_MODE_EXACT = 1
 };
 
+/* pyfwi_forward `ckpt_every`: keep the exact adjoint's checkpoints at the
+ * interval ceil(sqrt(nt)) when they fit in device memory. */
+enum { PYFWI_CKPT_AUTO = -1 };
+
 /* Flags (pyfwi_gradient `flags`). */
 enum {
     PYFWI_F_PRECOND = 1 /* also accumulate the squared-energy preconditioners */
@@ -172,7 +176,11 @@ int32_t pyfwi_set_receivers(pyfwi_plan *plan, const int32_t *table, int64_t coun
  *         each shot's own steps 0..nt_s-1, so shots may differ in nt; any
  *         other list needs equal nt for all shots.
  *   ckpt_every  > 0: also keep the state at the start of every ckpt_every-th
- *         step for a later PYFWI_MODE_EXACT gradient (0: none) */
+ *         step for a later PYFWI_MODE_EXACT gradient (0: none);
+ *         PYFWI_CKPT_AUTO: every ceil(sqrt(nt)) steps, kept only when all
+ *         shots' checkpoints and one shot's adjoint working set fit in 90 %
+ *         of the free device memory (else none: the gradient then recomputes
+ *         them pass by pass) */
 int32_t pyfwi_forward(pyfwi_plan *plan, const pyfwi_shot *shots, int32_t nshots,
                       const float *src, float *seis, const int32_t *chp,
                       int32_t nchp, int32_t ckpt_every, int32_t mem);
@@ -190,7 +198,13 @@ int32_t pyfwi_forward(pyfwi_plan *plan, const pyfwi_shot *shots, int32_t nshots,
  *   mode  PYFWI_MODE_PYFWI or PYFWI_MODE_EXACT
  *   ckpt_every  exact mode: checkpoint interval in steps; when the last
  *         forward kept no checkpoints at this interval the forward is run
- *         again here (0: use the forward's interval, or 50)
+ *         again here, pass by pass (0: the forward's interval when it kept
+ *         checkpoints, else ceil(sqrt(nt))).  The shots run in passes sized
+ *         to the device memory (unless pyfwi_plan_set_tuning fixes them):
+ *         per shot, the checkpoints (ceil(nt/K) states, when recomputed),
+ *         the K-state segment recomputed from each checkpoint, and the
+ *         forward and adjoint states -- the two-level schedule, one extra
+ *         forward sweep.  PYFWI_ENOMEM when one shot does not fit.
  *   flags PYFWI_F_PRECOND (pyfwi mode) */
 int32_t pyfwi_gradient(pyfwi_plan *plan, const pyfwi_shot *shots,
                        int32_t nshots, const float *src, const float *res,
@@ -231,6 +245,19 @@ int32_t pyfwi_misfit_gradient(pyfwi_plan *plan, const pyfwi_shot *shots, int32_t
 int32_t pyfwi_plan_set_tuning(pyfwi_plan *plan, int32_t shots_per_pass, int32_t fused,
                               int32_t strip_rows);
 
+/* hipGraph replay (no counterpart in the reference, which launches every
+ * kernel from Python): enable = 1 captures each pass's forward and backward
+ * time loop into a graph the first time it runs with a given configuration
+ * (shot table, buffers, model, checkpoint list, flags) and replays it with
+ * one hipGraphLaunch on later calls with the same configuration, instead of
+ * submitting two to four launches per time step from the host.  Off by
+ * default; not used while timing is enabled or with fused steps.  Results do
+ * not depend on it.  enable = 0 releases the graphs. */
+int32_t pyfwi_plan_set_graphs(pyfwi_plan *plan, int32_t enable);
+
+/* Time loops captured into graphs and replayed since the plan was created. */
+int32_t pyfwi_graph_stats(pyfwi_plan *plan, int64_t *captures, int64_t *replays);
+
 /* Copy kept snapshot `slot` of shot `shot` ([5][tnz*tnx]: vx, vz, txx, tzz,
  * txz) to `out` (the reference's WavePropagator.W, wave_propagation.py:150-156,
  * 710-727). */
@@ -242,8 +269,11 @@ int32_t pyfwi_get_snapshot(pyfwi_plan *plan, int32_t shot, int32_t slot,
  * reference, which runs one shot at a time).  pyfwi_misfit_gradient runs the
  * shots by decreasing nt (stable order) when the checkpoint list is every
  * step; the other calls in the order given.  `sizes` receives up to `cap`
- * entries, `count` the number of passes. */
-int32_t pyfwi_last_passes(pyfwi_plan *plan, int32_t *sizes, int32_t cap, int32_t *count);
+ * entries, `count` the number of passes, `ckpt_every` (may be NULL) the
+ * checkpoint interval of an exact-mode gradient or of the forward's kept
+ * checkpoints (0: none). */
+int32_t pyfwi_last_passes(pyfwi_plan *plan, int32_t *sizes, int32_t cap, int32_t *count,
+                          int32_t *ckpt_every);
 
 /* Wait for all work queued on the plan's stream. */
 int32_t pyfwi_synchronize(pyfwi_plan *plan);

@@ -15,6 +15,7 @@ MEM_HOST = 0
 MEM_DEVICE = 1
 MODE_PYFWI = 0
 MODE_EXACT = 1
+CKPT_AUTO = -1
 F_PRECOND = 1
 REC_LINE = 0
 REC_TABLE = 1
@@ -32,7 +33,8 @@ EXPORTS = [
     "pyfwi_version", "pyfwi_last_error", "pyfwi_plan_create", "pyfwi_plan_destroy",
     "pyfwi_plan_set_stream", "pyfwi_set_receivers", "pyfwi_set_model", "pyfwi_forward",
     "pyfwi_gradient",
-    "pyfwi_misfit_gradient", "pyfwi_plan_set_tuning", "pyfwi_get_snapshot", "pyfwi_last_passes",
+    "pyfwi_misfit_gradient", "pyfwi_plan_set_tuning", "pyfwi_plan_set_graphs",
+    "pyfwi_graph_stats", "pyfwi_get_snapshot", "pyfwi_last_passes",
     "pyfwi_synchronize", "pyfwi_plan_set_timing", "pyfwi_kernel_timing",
 ]
 
@@ -98,7 +100,11 @@ def load(path=None):
                                           ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_int32), i32, i32, i32, i32]
     lib.pyfwi_get_snapshot.argtypes = [P, i32, i32, fp, i32]
+    lib.pyfwi_plan_set_graphs.argtypes = [P, i32]
+    lib.pyfwi_graph_stats.argtypes = [P, ctypes.POINTER(ctypes.c_int64),
+                                      ctypes.POINTER(ctypes.c_int64)]
     lib.pyfwi_last_passes.argtypes = [P, ctypes.POINTER(ctypes.c_int32), i32,
+                                      ctypes.POINTER(ctypes.c_int32),
                                       ctypes.POINTER(ctypes.c_int32)]
     lib.pyfwi_synchronize.argtypes = [P]
     lib.pyfwi_plan_set_timing.argtypes = [P, i32]
@@ -249,20 +255,33 @@ class Plan:
         check(self.lib.pyfwi_plan_set_tuning(self.h, int(shots_per_pass), 1 if fused else 0,
                                              int(strip_rows)))
 
+    def set_graphs(self, enable):
+        """Replay the time loops as captured hipGraphs (see pyfwi_plan_set_graphs)."""
+        check(self.lib.pyfwi_plan_set_graphs(self.h, 1 if enable else 0))
+
+    def graph_stats(self):
+        """(loops captured, loops replayed) since the plan was created."""
+        c, r = ctypes.c_int64(), ctypes.c_int64()
+        check(self.lib.pyfwi_graph_stats(self.h, ctypes.byref(c), ctypes.byref(r)))
+        return c.value, r.value
+
     def snapshot(self, shot, slot, out, mem):
         _need(out, 5 * self.cells, "snapshot output")
         check(self.lib.pyfwi_get_snapshot(self.h, shot, slot, ptr(out, mem), mem))
 
-    def last_passes(self):
-        """Shots per pass of the last forward / gradient / misfit call."""
+    def last_passes(self, with_interval=False):
+        """Shots per pass of the last forward / gradient / misfit call (and,
+        with_interval, the exact mode's checkpoint interval)."""
         cap = 4096
         buf = (ctypes.c_int32 * cap)()
-        n = ctypes.c_int32()
-        check(self.lib.pyfwi_last_passes(self.h, buf, cap, ctypes.byref(n)))
+        n, k = ctypes.c_int32(), ctypes.c_int32()
+        check(self.lib.pyfwi_last_passes(self.h, buf, cap, ctypes.byref(n), ctypes.byref(k)))
         if n.value > cap:
             buf = (ctypes.c_int32 * n.value)()
-            check(self.lib.pyfwi_last_passes(self.h, buf, n.value, ctypes.byref(n)))
-        return [int(buf[k]) for k in range(n.value)]
+            check(self.lib.pyfwi_last_passes(self.h, buf, n.value, ctypes.byref(n),
+                                             ctypes.byref(k)))
+        sizes = [int(buf[j]) for j in range(n.value)]
+        return (sizes, int(k.value)) if with_interval else sizes
 
     def synchronize(self):
         check(self.lib.pyfwi_synchronize(self.h))

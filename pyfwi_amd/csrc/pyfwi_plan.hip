@@ -93,7 +93,13 @@ struct pyfwi_plan {
     DevBuf src_stage, seis_stage, res_stage, grad_stage, shot_tab_f, resid;
     double *d_misfit = nullptr;
     int *d_check = nullptr;  // two ints for the model check
+    // hipGraph replay of the time loops (pyfwi_plan_set_graphs)
+    int graphs = 0;
+    hipStream_t cap_stream = nullptr;  // capture only: nothing runs on it
+    std::vector<std::pair<uint64_t, hipGraphExec_t>> graph_cache;
+    int64_t graph_captures = 0, graph_replays = 0;
     std::vector<int32_t> last_passes;  // shots per pass of the last call
+    int last_ckpt_every = 0;           // exact mode: checkpoint interval of the last call
     pyfwi_shot *d_shots = nullptr;
     int shots_cap = 0;
     // receiver table (pyfwi_set_receivers): host copy for the shot checks
@@ -248,9 +254,11 @@ size_t rec_extent(const pyfwi_shot *shots, int nshots)
     return m;
 }
 
-int upload_shots(pyfwi_plan *p, const pyfwi_shot *shots, int nshots)
+// Where the shots' sparse terms go: the stencil kernels apply sources on
+// active cells and record / inject receiver lines on active cells; other
+// shots take the in-memory gather and injection launches.
+void shot_flags(pyfwi_plan *p, const pyfwi_shot *shots, int nshots)
 {
-    p->h_shots.assign(shots, shots + nshots);
     const int sdo = p->grid.sdo, nz = p->grid.tnz, nx = p->grid.tnx;
     auto inside = [&](int z, int x) { return z >= sdo && z <= nz - sdo && x >= sdo && x <= nx - sdo; };
     p->src_inside = p->rec_inline = true;
@@ -265,6 +273,12 @@ int upload_shots(pyfwi_plan *p, const pyfwi_shot *shots, int nshots)
         if (sh.n_rec > 0 && (!inside(sh.rec_z, sh.rec_x0) || !inside(sh.rec_z, last)))
             p->rec_inline = false;
     }
+}
+
+int upload_shots(pyfwi_plan *p, const pyfwi_shot *shots, int nshots)
+{
+    p->h_shots.assign(shots, shots + nshots);
+    shot_flags(p, shots, nshots);
     if (nshots > p->shots_cap) {
         if (p->d_shots) (void)hipFree(p->d_shots);
         p->d_shots = nullptr;
@@ -309,6 +323,86 @@ int stage_in(pyfwi_plan *p, DevBuf &stage, const float *ptr, size_t n, int mem,
     if (int e = ensure(stage, n, p->stream, what)) return e;
     HIPCHK(hipMemcpyAsync(stage.p, ptr, n * sizeof(float), hipMemcpyHostToDevice, p->stream));
     *out = stage.p;
+    return 0;
+}
+
+// ---------------------------------------------------------- graph replay
+// With graphs on, a time loop is captured into a hipGraph the first time it
+// runs with a given configuration and replayed by one hipGraphLaunch
+// afterwards: the loop issues thousands of small launches, which the host
+// otherwise submits one by one.  The key hashes everything the loop's
+// launches take as arguments (buffers, shot table, model, steps, flags), so
+// a changed call captures a new graph.  The loops hold no allocation, memset
+// or synchronisation; capture happens on a private stream and the graph runs
+// on the plan's stream.
+
+struct KeyHash {
+    uint64_t h = 1469598103934665603ull;  // FNV-1a
+    KeyHash &add(const void *data, size_t n)
+    {
+        const unsigned char *c = static_cast<const unsigned char *>(data);
+        for (size_t k = 0; k < n; ++k) h = (h ^ c[k]) * 1099511628211ull;
+        return *this;
+    }
+    template <class T> KeyHash &val(const T &v) { return add(&v, sizeof(T)); }
+};
+
+// Hash of the plan state every captured loop depends on.
+KeyHash loop_key(const pyfwi_plan *p, int kind, int g0, int ng)
+{
+    KeyHash k;
+    k.val(kind).val(g0).val(ng).val(p->dg).val(p->src_inside).val(p->rec_inline)
+        .val(p->d_shots).val(p->d_rtab).val(p->lam.p).val(p->mu.p).val(p->bu.p)
+        .val(p->dxp).val(p->dzp).val(p->fields.p).val(p->adj.p).val(p->gacc.p);
+    k.add(p->h_shots.data() + g0, sizeof(pyfwi_shot) * (size_t)ng);
+    return k;
+}
+
+constexpr size_t GRAPH_CACHE_MAX = 256;
+
+// Run `body` (a loop of launches on p->stream) directly, or as a cached
+// graph when graphs are on.
+template <class F>
+int run_loop(pyfwi_plan *p, uint64_t key, F body)
+{
+    if (!p->graphs || p->timing || p->tune_fused) return body();
+    for (auto &ge : p->graph_cache)
+        if (ge.first == key) {
+            HIPCHK(hipGraphLaunch(ge.second, p->stream));
+            p->graph_replays++;
+            return 0;
+        }
+    if (!p->cap_stream) HIPCHK(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+    hipStream_t user = p->stream;
+    p->stream = p->cap_stream;
+    hipError_t be = hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal);
+    if (be != hipSuccess) {
+        p->stream = user;
+        return fail(PYFWI_EHIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(be));
+    }
+    const int e = body();
+    hipGraph_t graph = nullptr;
+    const hipError_t ce = hipStreamEndCapture(p->cap_stream, &graph);
+    p->stream = user;
+    if (e) {
+        if (ce == hipSuccess && graph) (void)hipGraphDestroy(graph);
+        return e;
+    }
+    if (ce != hipSuccess)
+        return fail(PYFWI_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess)
+        return fail(PYFWI_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    if (p->graph_cache.size() >= GRAPH_CACHE_MAX) {
+        HIPCHK(hipStreamSynchronize(p->stream));
+        for (auto &ge : p->graph_cache) (void)hipGraphExecDestroy(ge.second);
+        p->graph_cache.clear();
+    }
+    p->graph_cache.emplace_back(key, exec);
+    p->graph_captures++;
+    HIPCHK(hipGraphLaunch(exec, p->stream));
     return 0;
 }
 
@@ -385,6 +479,8 @@ int32_t pyfwi_plan_destroy(pyfwi_plan *p)
     if (p->d_shots) (void)hipFree(p->d_shots);
     if (p->d_rtab) (void)hipFree(p->d_rtab);
     for (auto e : p->ev_pool) (void)hipEventDestroy(e);
+    for (auto &ge : p->graph_cache) (void)hipGraphExecDestroy(ge.second);
+    if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
     if (p->own_stream) (void)hipStreamDestroy(p->own_stream);
     delete p;
     return 0;
@@ -659,46 +755,109 @@ int forward_pass(pyfwi_plan *p, int g0, int ng, const float *d_src, float *d_sei
     const int nchp = (int)cl.size();
     if (int e = prepare_fields(p, ng)) return e;
     Timed loop_tm(p, PYFWI_K_FWD_LOOP, 2, nt);
+    KeyHash key = loop_key(p, 1, g0, ng);
+    key.val(d_src).val(d_seis).val(snaps).val(ckpts).val(ncp).val(K);
+    key.add(cl.data(), cl.size() * sizeof(int32_t));
     // every step kept (chpr=100): the two-launch kernels step from one slot
     // into the next, so the kept states cost no extra traffic
     if (snaps && keeps_every_step(cl, nt) && !p->tune_fused && p->src_inside) {
-        const int64_t ss = (int64_t)nchp * fps;
-        const float *in = p->fields.p;
-        int64_t in_stride = fps;
+        return run_loop(p, key.h, [&]() -> int {
+            const int64_t ss = (int64_t)nchp * fps;
+            const float *in = p->fields.p;
+            int64_t in_stride = fps;
+            for (int t = 0; t < nt; ++t) {
+                if (ckpts && t % K == 0)
+                    launch_copy_shots(ckpts + (int64_t)(t / K) * fps, (int64_t)ncp * fps, in,
+                                      in_stride, fps, ng, p->stream);
+                float *out = snaps + (int64_t)t * fps;
+                forward_step_kept(p, dsh, ng, t, d_src, p->rec_inline ? d_seis : nullptr, in,
+                                  in_stride, out, ss);
+                in = out;
+                in_stride = ss;
+                if (int e = maybe_harvest(p)) return e;
+            }
+            // receiver tables: one gather launch over all steps instead of one per step
+            if (!p->rec_inline) {
+                Timed ts(p, PYFWI_K_FWD_SPARSE);
+                launch_record_kept(p->dg, ng, dsh, 0, nt, snaps, ss, fps, d_seis, p->d_rtab,
+                                   p->stream);
+            }
+            return 0;
+        });
+    }
+    return run_loop(p, key.h, [&]() -> int {
+        int chpc = 0;
         for (int t = 0; t < nt; ++t) {
             if (ckpts && t % K == 0)
-                launch_copy_shots(ckpts + (int64_t)(t / K) * fps, (int64_t)ncp * fps, in,
-                                  in_stride, fps, ng, p->stream);
-            float *out = snaps + (int64_t)t * fps;
-            forward_step_kept(p, dsh, ng, t, d_src, p->rec_inline ? d_seis : nullptr, in,
-                              in_stride, out, ss);
-            in = out;
-            in_stride = ss;
+                launch_copy_shots(ckpts + (int64_t)(t / K) * fps, (int64_t)ncp * fps, p->cur, fps,
+                                  fps, ng, p->stream);
+            forward_step(p, dsh, ng, t, d_src, d_seis);
+            if (snaps && std::find(cl.begin(), cl.end(), t) != cl.end()) {
+                if (chpc < nchp)
+                    launch_copy_shots(snaps + (int64_t)chpc * fps, (int64_t)nchp * fps, p->cur,
+                                      fps, fps, ng, p->stream);
+                chpc++;
+            }
             if (int e = maybe_harvest(p)) return e;
         }
-        // receiver tables: one gather launch over all steps instead of one per step
-        if (!p->rec_inline) {
-            Timed ts(p, PYFWI_K_FWD_SPARSE);
-            launch_record_kept(p->dg, ng, dsh, 0, nt, snaps, ss, fps, d_seis, p->d_rtab,
-                               p->stream);
-        }
         return 0;
+    });
+}
+
+} // namespace
+
+namespace {
+
+// Bytes of device memory the plan may use for a call: 90 % of what is free
+// plus the large buffers it holds and may give back (kept states and
+// checkpoints).
+double plan_budget(const pyfwi_plan *p)
+{
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0.0;
     }
-    int chpc = 0;
-    for (int t = 0; t < nt; ++t) {
-        if (ckpts && t % K == 0)
-            launch_copy_shots(ckpts + (int64_t)(t / K) * fps, (int64_t)ncp * fps, p->cur, fps, fps,
-                              ng, p->stream);
-        forward_step(p, dsh, ng, t, d_src, d_seis);
-        if (snaps && std::find(cl.begin(), cl.end(), t) != cl.end()) {
-            if (chpc < nchp)
-                launch_copy_shots(snaps + (int64_t)chpc * fps, (int64_t)nchp * fps, p->cur, fps, fps,
-                                  ng, p->stream);
-            chpc++;
-        }
-        if (int e = maybe_harvest(p)) return e;
-    }
-    return 0;
+    const double held = (double)(p->snaps.n + p->ckpts.n + p->seg.n) * sizeof(float);
+    return 0.9 * ((double)free_b + held);
+}
+
+// Checkpoint interval of the exact adjoint for shots of up to nt steps: the
+// two-level schedule keeps ceil(nt/K) checkpoints and recomputes one K-step
+// segment at a time into K slots, so K = ceil(sqrt(nt)) minimises the states
+// held per shot (ceil(nt/K) + K), at the cost of one extra forward sweep.
+int auto_interval(int nt) { return std::max(1, (int)std::ceil(std::sqrt((double)nt))); }
+
+// States (5 fields each) the exact adjoint holds per shot of a pass: the
+// checkpoints, the segment, the forward and adjoint states and the two
+// adjoint-velocity buffers (2 x 2 of 5 fields), rounded up.
+double exact_states_per_shot(int nt, int K, bool with_ckpts)
+{
+    const int ncp = (nt + K - 1) / K;
+    return (with_ckpts ? ncp : 0) + K + 2 + 0.8 + 1.2 / 4;  // gacc: 6 fields per group of >= 4
+}
+
+// Shots per pass for the exact adjoint of `nshots` shots of up to nt steps
+// within `budget` bytes: as many as fit, then balanced over the passes.
+int exact_pass_size(const pyfwi_plan *p, int nshots, int nt, int K, bool with_ckpts,
+                    double budget)
+{
+    const double per = exact_states_per_shot(nt, K, with_ckpts) * (double)fields_per_shot(p) *
+                       sizeof(float);
+    int g = (int)std::floor(budget / per);
+    if (g < 1) return 0;
+    g = std::min(g, nshots);
+    const int npass = (nshots + g - 1) / g;
+    return (nshots + npass - 1) / npass;
+}
+
+// Give back the kept-state buffer when a call needs the memory.
+void release_snaps(pyfwi_plan *p)
+{
+    if (!p->snaps.p) return;
+    (void)hipStreamSynchronize(p->stream);
+    p->snaps.release();
+    p->chp.clear();  // a reference-mode gradient now needs a new forward
 }
 
 } // namespace
@@ -712,7 +871,8 @@ int32_t pyfwi_forward(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, co
     if (!p->model_set) return fail(PYFWI_ESTATE, "pyfwi_set_model must come first");
     if (int e = check_shots(p, shots, nshots)) return e;
     if (nchp < 0 || (nchp > 0 && !chp)) return fail(PYFWI_EINVAL, "bad checkpoint list");
-    if (ckpt_every < 0) return fail(PYFWI_EINVAL, "ckpt_every must be >= 0");
+    if (ckpt_every < PYFWI_CKPT_AUTO)
+        return fail(PYFWI_EINVAL, "ckpt_every must be >= 0 or PYFWI_CKPT_AUTO");
     std::vector<pyfwi_shot> sh(shots, shots + nshots);
     const int nt = max_nt(sh);
     std::vector<int32_t> cl(chp, chp + std::max(nchp, 0));
@@ -723,6 +883,17 @@ int32_t pyfwi_forward(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, co
     if (nchp > 0)
         if (int e = ensure(p->snaps, (size_t)nshots * nchp * fps, p->stream, "checkpoints")) return e;
     int ncp = 0;
+    if (ckpt_every == PYFWI_CKPT_AUTO) {
+        // keep every shot's checkpoints only when they and one shot's exact
+        // adjoint working set fit; otherwise the gradient recomputes them
+        // pass by pass
+        ckpt_every = auto_interval(nt);
+        const int n_cp = (nt + ckpt_every - 1) / ckpt_every;
+        const double state = (double)fps * sizeof(float);
+        const double need = ((double)nshots * n_cp +
+                             exact_states_per_shot(nt, ckpt_every, false)) * state;
+        if (need > plan_budget(p) - (double)p->snaps.n * sizeof(float)) ckpt_every = 0;
+    }
     if (ckpt_every > 0) {
         ncp = (nt + ckpt_every - 1) / ckpt_every;
         if (int e = ensure(p->ckpts, (size_t)nshots * ncp * fps, p->stream, "checkpoints")) return e;
@@ -739,6 +910,7 @@ int32_t pyfwi_forward(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, co
     launch_zero(d_seis, (int64_t)nrec, p->stream);
     const int G = pass_size(p, nshots);
     p->last_passes.clear();
+    p->last_ckpt_every = ckpt_every;
     for (int g0 = 0; g0 < nshots; g0 += G) {
         const int ng = std::min(G, nshots - g0);
         p->last_passes.push_back(ng);
@@ -786,8 +958,11 @@ int gradient_pyfwi(pyfwi_plan *p, int g0, int ng, const float *d_res, bool preco
     }
     // every step kept: read the forward state of step t from its slot
     const bool direct = keeps_every_step(cl, nt);
-    int chpc = nchp - 1;
     Timed loop_tm(p, PYFWI_K_BWD_LOOP, 2, nt - 1);
+    KeyHash key = loop_key(p, 2, g0, ng);
+    key.val(d_res).val(precond).val(snaps).add(cl.data(), cl.size() * sizeof(int32_t));
+    return run_loop(p, key.h, [&]() -> int {
+    int chpc = nchp - 1;
     for (int t = nt - 1; t > 0; --t) {
         const float *fwd = p->cur;
         int64_t fwd_stride = fps;
@@ -828,6 +1003,7 @@ int gradient_pyfwi(pyfwi_plan *p, int g0, int ng, const float *d_res, bool preco
         if (int e = maybe_harvest(p)) return e;
     }
     return 0;
+    });
 }
 
 // Exact adjoint of one pass, forward states recomputed from checkpoints
@@ -905,7 +1081,7 @@ int32_t pyfwi_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, c
     if (!p->model_set) return fail(PYFWI_ESTATE, "pyfwi_set_model must come first");
     if (mode != PYFWI_MODE_PYFWI && mode != PYFWI_MODE_EXACT) return fail(PYFWI_EINVAL, "unknown mode");
     if (int e = check_shots(p, shots, nshots)) return e;
-    if (ckpt_every < 0) return fail(PYFWI_EINVAL, "ckpt_every must be >= 0");
+    if (ckpt_every < 0) return fail(PYFWI_EINVAL, "ckpt_every must be >= 0 (0: automatic)");
     const DevGrid &g = p->dg;
     const int64_t fps = fields_per_shot(p);
     const size_t nrec = rec_extent(shots, nshots);
@@ -931,9 +1107,36 @@ int32_t pyfwi_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, c
         if (int e = ensure(p->grad_stage, 6 * n, p->stream, "gradient output")) return e;
         d_grad = p->grad_stage.p;
     }
-    const int G = pass_size(p, nshots);
-    const int K = ckpt_every > 0 ? ckpt_every : (p->ckpt_every > 0 ? p->ckpt_every : 50);
-    const bool reuse = mode == PYFWI_MODE_EXACT && same && p->ckpts_all && p->ckpt_every == K;
+    int G = pass_size(p, nshots);
+    const int nt_all = max_nt(std::vector<pyfwi_shot>(shots, shots + nshots));
+    const bool reuse = mode == PYFWI_MODE_EXACT && same && p->ckpts_all &&
+                       (ckpt_every == 0 || ckpt_every == p->ckpt_every);
+    const int K = reuse ? p->ckpt_every : (ckpt_every > 0 ? ckpt_every : auto_interval(nt_all));
+    p->last_ckpt_every = mode == PYFWI_MODE_EXACT ? K : 0;
+    if (mode == PYFWI_MODE_EXACT && p->tune_group == 0) {
+        // checkpoint schedule sized to the device memory: shots per pass from
+        // what the pass's checkpoints (unless kept by the forward), segment
+        // and working states take
+        double budget = plan_budget(p) - (reuse ? (double)p->ckpts.n * sizeof(float) : 0.0);
+        G = exact_pass_size(p, nshots, nt_all, K, !reuse, budget - (double)p->snaps.n * 4.0);
+        if (G < 1) {
+            release_snaps(p);
+            G = exact_pass_size(p, nshots, nt_all, K, !reuse, budget);
+        }
+        if (G < 1)
+            return fail(PYFWI_ENOMEM, "the exact adjoint of one shot (checkpoints every " +
+                                          std::to_string(K) + " steps) does not fit in device "
+                                          "memory");
+        // buffers are only grown: give back those larger than this schedule's
+        // so that the others can grow
+        const size_t ncp_all = (size_t)((nt_all + K - 1) / K);
+        HIPCHK(hipStreamSynchronize(p->stream));
+        if (!reuse && p->ckpts.n > (size_t)G * ncp_all * fps) {
+            p->ckpts.release();
+            p->ckpts_all = false;
+        }
+        if (p->seg.n > (size_t)G * K * fps) p->seg.release();
+    }
     p->last_passes.clear();
     for (int g0 = 0; g0 < nshots; g0 += G) {
         const int ng = std::min(G, nshots - g0);
@@ -1034,8 +1237,9 @@ int32_t pyfwi_misfit_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t ns
     // than all shots need
     size_t slots;
     const size_t first = every ? (size_t)shs[0].nt : (size_t)nchp;
-    size_t need = 0;
-    for (auto &x : shs) need += every ? (size_t)x.nt : (size_t)nchp;
+    // a pass gives each of its shots the slots of its longest shot, so all
+    // shots in one pass take nshots * first slots
+    const size_t need = (size_t)nshots * first;
     if (p->tune_group > 0) {
         slots = (size_t)std::min(p->tune_group, nshots) * first;
     } else if (p->snaps.n / (size_t)fps >= first) {
@@ -1052,6 +1256,7 @@ int32_t pyfwi_misfit_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t ns
     }
     if (int e = ensure(p->snaps, slots * fps, p->stream, "checkpoints")) return e;
     p->last_passes.clear();
+    p->last_ckpt_every = 0;
     for (int g0 = 0; g0 < nshots;) {
         const int nt_pass = every ? shs[g0].nt : nt;
         const int stride = every ? nt_pass : nchp;
@@ -1084,6 +1289,27 @@ int32_t pyfwi_misfit_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t ns
     }
     HIPCHK(hipStreamSynchronize(p->stream));
     if (int e = harvest_timing(p)) return e;
+    return 0;
+}
+
+int32_t pyfwi_plan_set_graphs(pyfwi_plan *p, int32_t enable)
+{
+    if (!p) return fail(PYFWI_EINVAL, "null plan");
+    if (enable != 0 && enable != 1) return fail(PYFWI_EINVAL, "enable must be 0 or 1");
+    if (!enable && !p->graph_cache.empty()) {
+        HIPCHK(hipStreamSynchronize(p->stream));
+        for (auto &ge : p->graph_cache) (void)hipGraphExecDestroy(ge.second);
+        p->graph_cache.clear();
+    }
+    p->graphs = enable;
+    return 0;
+}
+
+int32_t pyfwi_graph_stats(pyfwi_plan *p, int64_t *captures, int64_t *replays)
+{
+    if (!p || !captures || !replays) return fail(PYFWI_EINVAL, "null argument");
+    *captures = p->graph_captures;
+    *replays = p->graph_replays;
     return 0;
 }
 
@@ -1127,9 +1353,11 @@ int32_t pyfwi_get_snapshot(pyfwi_plan *p, int32_t shot, int32_t slot, float *out
     return 0;
 }
 
-int32_t pyfwi_last_passes(pyfwi_plan *p, int32_t *sizes, int32_t cap, int32_t *count)
+int32_t pyfwi_last_passes(pyfwi_plan *p, int32_t *sizes, int32_t cap, int32_t *count,
+                          int32_t *ckpt_every)
 {
     if (!p || !count || cap < 0 || (cap > 0 && !sizes)) return fail(PYFWI_EINVAL, "null argument");
+    if (ckpt_every) *ckpt_every = p->last_ckpt_every;
     *count = (int32_t)p->last_passes.size();
     for (int k = 0; k < cap && k < *count; ++k) sizes[k] = p->last_passes[k];
     return 0;

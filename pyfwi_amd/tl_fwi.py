@@ -1,37 +1,175 @@
-"""Time-lapse full-waveform inversion (src/PyFWI/tl_fwi.py:13-584).
+"""Time-lapse full-waveform inversion with the interface of PyFWI's
+``TimeLapse`` (src/PyFWI/tl_fwi.py:13-584): same constructor, ``__call__``
+signature, strategy codes, result and history keys, and optimiser settings.
 
-``TimeLapse`` inverts a baseline and a monitor data set for the model change
-between them, with the strategies of the reference:
-- 'pa' parallel (independent baseline and monitor inversions),
-- 'cc' cascaded (the monitor from the inverted baseline),
-- 'dd' double difference (Asnaashari et al. 2011),
-- 'sim' simultaneous (both models in one L-BFGS problem, coupled by a small
-  model-difference penalty),
-- 'wa' weighted-averaged cascaded, 'cu' cross updating (Maharramov and Biondi
-  2014), 'cd' central difference and 'cj' cascaded joint.
-Every misfit evaluation runs on the GPU through the ``FWI`` driver (or, for
-the simultaneous strategy, ``tl_fprime``), with the one-call
-pyfwi_misfit_gradient path where it applies.
+Every strategy is a row of ``STRATEGIES``: a list of inversion legs and the
+outputs built from their models.  A leg inverts one data set (the baseline,
+the monitor, or the double-difference composite) from a start model (the
+initial model, an earlier leg's result, or the mean of two), or runs the
+joint problem over both vintages.  ``TimeLapse.run_plan`` executes a row:
+legs in order, the clock read where a row asks for it, then the outputs.
+
+  'pa' parallel          b1 <- m0 | base;  m1 <- m0 | monitor
+  'cc' cascaded          b1 <- m0 | base;  m1 <- b1 | monitor
+  'dd' double difference b1 <- m0 | base;  m1 <- m0 | monitor - base + F(b1)
+                         (Asnaashari et al. 2011)
+  'wa' weighted average  b1, m1 <- b1 | monitor, b2 <- m1 | base
+  'cu' cross updating    as 'wa', then m2 <- b2 | monitor (Maharramov and
+                         Biondi 2014)
+  'cd' central diff.     b1 <- m0 | base, m2 <- b1 | monitor,
+                         m1 <- m0 | monitor, b2 <- m1 | base
+  'sim' simultaneous     the joint problem from m0 (after an inversion of the
+                         driver's current data, as the reference does)
+  'cj' cascaded joint    joint from m0, then joint from the mean of its two
+                         models
+
+The joint problem (tl_fwi.py:114-222) minimises both vintages' data misfits
+plus gamma/2 ||m - b||^2 over one window of parameters of both models at once
+(L-BFGS-B, memory 30).  Every misfit evaluation runs on the GPU through
+``FWI.data_term`` (the one-call pyfwi_misfit_gradient path where it applies).
 """
 import copy
 import timeit
+from dataclasses import dataclass, field
 
 import numpy as np
 from scipy.optimize import fmin_l_bfgs_b
-from scipy.optimize._optimize import MemoizeJac
 
 from . import fwi_tools as tools
-from .fwi import FWI
+from .fwi import FWI, Window
 from .processing import prepare_residual
 from .wave_propagation import WavePropagator
 
-__all__ = ["TimeLapse"]
+__all__ = ["TimeLapse", "STRATEGIES", "Leg", "Plan"]
+
+GAMMA = 1e-10  # weight of the model-difference coupling of the joint problem
+
+
+@dataclass(frozen=True)
+class Leg:
+    """One inversion: `name` <- inversion of `data` ('base', 'monitor',
+    'composite' or 'current': whatever the driver holds) from `start` ('m0',
+    a leg name, or a pair of names for their mean).  kind 'joint' runs the
+    joint problem and yields `name`_b and `name`_m."""
+    name: str
+    start: object
+    data: str = "base"
+    kind: str = "fwi"
+
+
+@dataclass
+class Plan:
+    legs: list
+    # output key -> ('model', leg) | ('diff', a, b) | ('mean_diff', (a, b), (c, d))
+    outputs: dict
+    # history key -> leg name, or ('sim', leg) for {'sim': history of leg}
+    hist: dict
+    clocks: dict = field(default_factory=dict)  # output key -> leg after which to read
+    banner: str = ""
+
+
+STRATEGIES = {
+    "pa": Plan([Leg("b1", "m0", "base"), Leg("m1", "m0", "monitor")],
+               {"b1": ("model", "b1"), "m1": ("model", "m1"), "dm": ("diff", "m1", "b1")},
+               {"b1": "b1", "m1": "m1"}, {"pt": "m1"}, "parallel"),
+    "cc": Plan([Leg("b1", "m0", "base"), Leg("m1", "b1", "monitor")],
+               {"b1": ("model", "b1"), "m1": ("model", "m1"), "dm": ("diff", "m1", "b1")},
+               {"b1": "b1", "m1": "m1"}, {"pt": "m1"}, "cascaded"),
+    "dd": Plan([Leg("b1", "m0", "base"), Leg("m1", "m0", "composite")],
+               {"b1": ("model", "b1"), "m1": ("model", "m1"), "dm": ("diff", "m1", "b1")},
+               {"b1": "b1", "m1": "m1"}, {"pt": "m1"}, "double difference"),
+    "wa": Plan([Leg("b1", "m0", "base"), Leg("m1", "b1", "monitor"), Leg("b2", "m1", "base")],
+               {"dm11": ("diff", "m1", "b1"), "dm12": ("diff", "m1", "b2"),
+                "m1": ("model", "m1"), "b1": ("model", "b1"), "b2": ("model", "b2")},
+               {"b1": "b1", "m1": "m1", "b2": "b2"}, {}, "weighted average"),
+    "cu": Plan([Leg("b1", "m0", "base"), Leg("m1", "b1", "monitor"), Leg("b2", "m1", "base"),
+                Leg("m2", "b2", "monitor")],
+               {"dm11": ("diff", "m1", "b1"), "dm12": ("diff", "m1", "b2"),
+                "dm21": ("diff", "m2", "b1"), "dm22": ("diff", "m2", "b2"),
+                "b1": ("model", "b1"), "m1": ("model", "m1"), "b2": ("model", "b2"),
+                "m2": ("model", "m2")},
+               {"b1": "b1", "m1": "m1", "b2": "b2", "m2": "m2"},
+               {"pt_cc": "m1", "pt_wa": "b2", "pt_cu": "m2"}, "cross updating"),
+    "cd": Plan([Leg("b1", "m0", "base"), Leg("b1m", "b1", "monitor"),
+                Leg("m1", "m0", "monitor"), Leg("m1b", "m1", "base")],
+               {"dm0": ("diff", "m1", "b1"), "dm1": ("diff", "b1m", "m1b"),
+                "dm3": ("diff", "b1m", "b1"), "dm4": ("diff", "m1", "m1b"),
+                "dm34": ("mean_diff", ("b1m", "b1"), ("m1", "m1b")),
+                "b1": ("model", "b1"), "m1": ("model", "m1"), "m2": ("model", "b1m"),
+                "b2": ("model", "m1b")},
+               {"b1": "b1", "m1": "m1", "m2": "b1m", "b2": "m1b"}, {"pt": "m1b"},
+               "central difference"),
+    "sim": Plan([Leg("ref", "m0", "current"), Leg("j", "ref", kind="joint")],
+                {"dm": ("diff", "j_m", "j_b"), "b1": ("model", "j_b"), "m1": ("model", "j_m")},
+                {"sim": "j"}, {"pt": "j"}, "simultaneous"),
+    "cj": Plan([Leg("j1", "m0", kind="joint"), Leg("j2", ("j1_b", "j1_m"), kind="joint")],
+               {"dm0": ("diff", "j1_m", "j1_b"), "dm7": ("diff", "j2_m", "j2_b"),
+                "dm3": ("diff", "j2_m", "j1_b"), "dm6": ("diff", "j1_m", "j2_b"),
+                "b1": ("model", "j1_b"), "m1": ("model", "j1_m"), "b2": ("model", "j2_b"),
+                "m2": ("model", "j2_m")},
+               {"sim1": ("sim", "j1"), "sim2": ("sim", "j2")}, {"pt": "j2"},
+               "cascaded joint"),
+}
+
+
+class JointObjective:
+    """Both vintages' data misfits plus GAMMA/2 ||m_w - b_w||^2 over one
+    window w of the two stacked model vectors x = [b; m] (tl_fwi.py:150-222):
+    the unknowns are [b_w; m_w].  The last point's value and gradient are
+    kept for the optimiser's separate f and f' calls."""
+
+    def __init__(self, tl, x, window, freq):
+        self.tl, self.x, self.w, self.freq = tl, np.array(x), window, freq
+        self._p = None
+        self._fg = None
+
+    def _eval(self, p):
+        if self._p is None or not np.array_equal(p, self._p):
+            self._p = np.array(p, copy=True)
+            self._fg = self.evaluate(self._p)
+        return self._fg
+
+    def value(self, p):
+        return self._eval(p)[0]
+
+    def gradient(self, p):
+        return self._eval(p)[1]
+
+    def split(self, p):
+        """Full model vectors of both vintages with the window set from p."""
+        n3 = 3 * self.w.cells
+        k = self.w.count * self.w.cells
+        b = self.w.put(self.x[:n3], p[:k])
+        m = self.w.put(self.x[n3:], p[k:2 * k])
+        return b, m
+
+    def evaluate(self, p):
+        tl, w = self.tl, self.w
+        inv = tl.inv_obj
+        b, m = self.split(p)
+        grads, rms_data = [], 0.0
+        for vec, d_obs in ((b, tl.b_dobs), (m, tl.m_dobs)):
+            pcs = inv.vec2dict(vec, tl.nz, tl.nx)
+            rms, g_dv = tl.data_term(inv.model_to_dv(pcs, inv.param_functions_args), d_obs,
+                                     self.freq)
+            grads.append(w.take(inv.dict2vec(inv.grad_from_dv(g_dv, inv.param_functions_args,
+                                                              pcs))))
+            rms_data += rms
+        diff = w.take(m).astype(np.float64) - w.take(b)
+        total = rms_data + GAMMA * 0.5 * float(diff @ diff)
+        tl.xrest = np.concatenate((b, m))
+        if tl.verbose:
+            print(f"TL joint band {self.freq}, window {w.first + 1}: misfit {total:.6e} "
+                  f"(data {rms_data:.6e}); |m - b| in [{np.abs(diff).min():.4g}, "
+                  f"{np.abs(diff).max():.4g}]", flush=True)
+        grad = np.concatenate((grads[0] - GAMMA * diff, grads[1] + GAMMA * diff))
+        return total, np.float64(grad)
 
 
 class TimeLapse(WavePropagator):
-    """Parameters as the reference (tl_fwi.py:13-56): baseline and monitor
-    observed data, ``inpa``, source, receivers, model shape, components,
-    chpr, n_well_rec and optional ``param_functions`` (see ``FWI``)."""
+    """Baseline and monitor observed data, ``inpa``, source, receivers, model
+    shape, components, chpr, n_well_rec and optional ``param_functions``
+    (see ``FWI``)."""
 
     def __init__(self, b_dobs, m_dobs, inpa, src, rec_loc, model_shape, components, chpr,
                  n_well_rec=0, param_functions=None):
@@ -46,247 +184,148 @@ class TimeLapse(WavePropagator):
         self.CF = self.inv_obj.CF
         self.fn = self.inv_obj.fn
         self.device_misfit = True
+        self.verbose = True
+        self.xrest = None  # both vintages at the joint problem's last evaluation
 
     def __call__(self, b_m0, iter, freqs, tl_method, n_params, k_0, k_end):
-        """Run the strategy `tl_method` from the initial model b_m0; returns
-        (models and differences, misfit histories) (tl_fwi.py:58-112)."""
-        kw = dict(n_params=n_params, k_0=k_0, k_end=k_end)
-        runs = {
-            "pa": lambda: self.parallel_inversion(b_m0, freqs, iter, **kw),
-            "cc": lambda: self.cascaded_inversion(b_m0, freqs, iter, **kw),
-            "dd": lambda: self.double_difference_inversion(b_m0, freqs, iter, **kw),
-            "sim": lambda: self.sim_inversion(b_m0, freqs, iter, data_to_invert=self.b_dobs, **kw),
-            "wa": lambda: self.weighted_averaged_cascaded(b_m0, freqs, iter, **kw),
-            "cj": lambda: self.cascaded_joint(b_m0, freqs, iter, **kw),
-            "cd": lambda: self.central_difference(b_m0, freqs, iter, **kw),
-            "cu": lambda: self.cross_updating(b_m0, freqs, iter, **kw),
-        }
-        if tl_method not in runs:
+        """Run strategy `tl_method` (see STRATEGIES) from the initial model
+        b_m0; returns (models, differences and timings; misfit histories)."""
+        if tl_method not in STRATEGIES:
             raise ValueError(f"unknown time-lapse method {tl_method!r}")
-        return runs[tl_method]()
+        return self.run_plan(STRATEGIES[tl_method], b_m0, freqs, iter, n_params, k_0, k_end)
 
-    # ------------------------------------------------------------ helpers
+    # ------------------------------------------------------------ plans
+    def run_plan(self, plan, m0, freqs, iter, n_params, k_0, k_end):
+        models = {"m0": copy.deepcopy(m0)}
+        hist, clock = {}, {}
+        tic = timeit.default_timer()
+        for leg in plan.legs:
+            start = self._start_model(models, leg.start)
+            if self.verbose:
+                print(f"TL {plan.banner}: leg {leg.name} ({leg.kind}, data {leg.data})",
+                      flush=True)
+            if leg.kind == "joint":
+                b, m, h = self.joint_inversion(start, freqs, iter, n_params, k_0, k_end)
+                models[leg.name + "_b"], models[leg.name + "_m"] = b, m
+                hist[leg.name] = h
+            else:
+                models[leg.name], hist[leg.name] = self._invert(
+                    start, self._data(leg.data, models), freqs, iter, n_params, k_0, k_end)
+            clock[leg.name] = timeit.default_timer() - tic
+        out = {}
+        for key, spec in plan.outputs.items():
+            if spec[0] == "model":
+                out[key] = models[spec[1]]
+            elif spec[0] == "diff":
+                out[key] = tools.dict_diff(models[spec[1]], models[spec[2]])
+            else:  # mean of two differences
+                (a, b), (c, d) = spec[1], spec[2]
+                out[key] = tools.dict_summation(tools.dict_diff(models[a], models[b]),
+                                                tools.dict_diff(models[c], models[d]),
+                                                division=2)
+        for key, leg in plan.clocks.items():
+            out[key] = clock[leg]
+        hists = {}
+        for key, leg in plan.hist.items():
+            hists[key] = {leg[0]: hist[leg[1]]} if isinstance(leg, tuple) else hist[leg]
+        return out, hists
+
+    def _start_model(self, models, start):
+        if isinstance(start, tuple):
+            return tools.dict_summation(models[start[0]], models[start[1]], division=2)
+        return copy.deepcopy(models[start])
+
+    def _data(self, which, models):
+        if which == "base":
+            return self.b_dobs
+        if which == "monitor":
+            return self.m_dobs
+        if which == "current":
+            return self.inv_obj.d_obs
+        # double difference: the monitor - baseline difference added to the
+        # data of the inverted baseline (tl_fwi.py:429-433)
+        d_ref = prepare_residual(self.forward_modeling(models["b1"], show=False),
+                                 s=self.inv_obj.sd)
+        return tools.dict_summation(tools.dict_diff(self.m_dobs, self.b_dobs), d_ref)
+
     def _invert(self, m0, d_obs, freqs, iter, n_params, k_0, k_end):
-        """One FWI (L-BFGS) of `d_obs` from m0."""
+        """One L-BFGS inversion of `d_obs` from m0 with the FWI driver."""
         self.inv_obj.d_obs = d_obs
         return self.inv_obj(m0, method=2, freqs=freqs, iter=iter, n_params=n_params, k_0=k_0,
                             k_end=k_end)
 
-    def _vintage(self, model_pcs, d_obs, freq):
-        """Data misfit and gradient vector of one vintage (tl_fwi.py:170-205)."""
+    # ------------------------------------------------------------ joint problem
+    def data_term(self, model_dv, d_obs, freq):
+        """One vintage's data misfit and gradient dict, on this object's own
+        propagator (as the reference's tl_fprime)."""
         inv = self.inv_obj
-        model = inv.model_to_dv(model_pcs, inv.param_functions_args)
         if self.device_misfit and self.device_misfit_applies(d_obs, freq, inv.sd, self.CF):
-            rms, grad_dv = self.misfit_gradient_device(model, d_obs)
-        else:
-            dpre = prepare_residual(self.forward_modeling(model, show=False), inv.sd)
-            rms, adj = tools.cost_seismic(d_pre0=dpre, d_obs0=d_obs, fun=self.CF, fn=self.fn,
-                                          freq=freq, order=3, axis=1, sourc_weight=None,
-                                          ns=self.ns, nr=self.nr)
-            grad_dv = self.gradient(adj)
-        grad = inv.grad_from_dv(grad_dv, inv.param_functions_args, model_pcs)
-        return rms, inv.dict2vec(grad)
+            return self.misfit_gradient_device(model_dv, d_obs)
+        dpre = prepare_residual(self.forward_modeling(model_dv, show=False), inv.sd)
+        rms, adj = tools.cost_seismic(d_pre0=dpre, d_obs0=d_obs, fun=self.CF, fn=self.fn,
+                                      freq=freq, order=3, axis=1, sourc_weight=None, ns=self.ns,
+                                      nr=self.nr)
+        return rms, self.gradient(adj)
 
-    # ------------------------------------------------------------ simultaneous
+    def joint_inversion(self, m0, freqs, iter, n_params, k_0, k_end):
+        """Both vintages from m0 by the joint problem, band after band;
+        returns (baseline, monitor, misfit of the first window per band)."""
+        vec = np.asarray(self.inv_obj.dict2vec(m0))
+        x = np.concatenate((vec, vec))
+        per_band = []
+        for band, freq in enumerate(freqs):
+            x, misfits = self.tl_lbfgs(x, iter[band], freq, n_params, k_0, k_end)
+            per_band.append(misfits[0])
+        n3 = 3 * self.n_elements
+        v2d = self.inv_obj.vec2dict
+        return v2d(x[:n3], self.nz, self.nx), v2d(x[n3:], self.nz, self.nx), per_band
+
     def tl_lbfgs(self, x0, iter, freq, n_params, k0, k_end):
-        """L-BFGS-B over [baseline, monitor] parameter groups, memory 30
-        (tl_fwi.py:114-147).  k0, k_end are 1-based here, as in the
-        reference."""
-        n = self.nz * self.nx
-        self.xrest = np.copy(x0)
-        hist_all = []
-        fun = MemoizeJac(self.tl_fprime)
-        for k in np.arange(k0 - 1, k_end - 1, n_params):
-            print("Parameter number {} to {}".format(k + 1, k + n_params))
-            m_opt = np.hstack((x0[k * n:(k + n_params) * n],
-                               x0[(3 + k) * n:3 * n + (k + n_params) * n]))
-            m_opt, hist, _ = fmin_l_bfgs_b(fun, m_opt, fun.derivative, args=(freq, k, n_params),
-                                           m=30, factr=1e-10, pgtol=1e-12, bounds=None,
-                                           maxfun=15000, maxiter=iter, callback=None, maxls=20)
-            print(m_opt.max(), m_opt.min())
-            hist_all.append(hist)
-            x0[k * n:(k + n_params) * n] = np.float32(m_opt[:n_params * n])
-            x0[(3 + k) * n:(3 + k + n_params) * n] = np.float32(m_opt[n_params * n:])
-        return x0, hist_all
+        """One band of the joint problem: L-BFGS-B with memory 30 over the
+        windows k0 .. k_end - 1 (1-based) of both vintages (tl_fwi.py:114-148).
+        Returns (x, misfit per window)."""
+        x = np.array(x0, np.float64)
+        misfits = []
+        for w in Window.sweep(k0 - 1, k_end - 1, n_params, self.n_elements):
+            obj = JointObjective(self, x, w, freq)
+            n3 = 3 * self.n_elements
+            p0 = np.concatenate((w.take(x[:n3]), w.take(x[n3:])))
+            p, fmin, _ = fmin_l_bfgs_b(obj.value, p0, obj.gradient, m=30, factr=1e-10,
+                                       pgtol=1e-12, bounds=None, maxfun=15000, maxiter=iter,
+                                       maxls=20)
+            b, m = obj.split(np.float32(p))
+            x = np.concatenate((b, m)).astype(np.float64)
+            misfits.append(fmin)
+        return x, misfits
 
     def tl_fprime(self, x, freq, k, n_params):
-        """Misfit of both vintages plus gamma/2 ||m - b||^2 (gamma = 1e-10)
-        and its gradient for parameter group k (tl_fwi.py:149-222).  The
-        working vector is kept in ``xrest``."""
-        n = self.n_elements
-        x1 = self.xrest
-        x1[k * n:(k + n_params) * n] = x[:n_params * n]
-        x1[3 * n + k * n:3 * n + (k + n_params) * n] = x[n_params * n:2 * n_params * n]
-        b_pcs = self.inv_obj.vec2dict(x1[:3 * n], self.nz, self.nx)
-        m_pcs = self.inv_obj.vec2dict(x1[3 * n:], self.nz, self.nx)
-        b_rms, b_grad = self._vintage(b_pcs, self.b_dobs, freq)
-        b = x1[k * n:(k + n_params) * n]
-        m = x1[(3 + k) * n:(3 + k + n_params) * n]
-        diff = m - b
-        rms_diff = 0.5 * np.dot(diff, diff)
-        gamma = 1e-10
-        m_rms, m_grad = self._vintage(m_pcs, self.m_dobs, freq)
-        rms_data = b_rms + m_rms
-        rms = rms_data + gamma * rms_diff
-        print(k, np.abs(diff).min(), np.abs(diff).max())
-        print("TL fprime_single_parameter for f= {}: rms is: {} for data: {} and model: {}"
-              .format(freq, rms, rms_data, rms_diff))
-        grad = np.hstack((b_grad[k * n:(k + n_params) * n] + gamma * (b - m),
-                          m_grad[k * n:(k + n_params) * n] + gamma * (m - b)))
-        return rms, np.float64(grad)
+        """Joint misfit and gradient at the window values x = [b_w; m_w] of
+        window k (0-based) with the rest of both models from ``xrest``
+        (tl_fwi.py:150-222)."""
+        obj = JointObjective(self, self.xrest, Window(int(k), int(n_params), self.n_elements),
+                             freq)
+        return obj.evaluate(np.asarray(x))
 
     def sim_inversion(self, b_m0, freqs, iter, n_params, k_0, k_end, data_to_invert=None):
-        """Simultaneous inversion, optionally from a first inversion of the
-        driver's current data (tl_fwi.py:505-552).  As in the reference,
-        `data_to_invert` is stored but the first inversion uses the data the
-        FWI driver holds."""
-        tic = timeit.default_timer()
-        if data_to_invert is not None:
-            self.dobs = data_to_invert
-            ref, _ = self.inv_obj(b_m0, method=2, freqs=freqs, iter=iter, n_params=n_params,
-                                  k_0=k_0, k_end=k_end)
-            b0, m0 = copy.deepcopy(ref), copy.deepcopy(ref)
+        """The simultaneous strategy; with ``data_to_invert`` the joint problem
+        starts from an inversion of the driver's current data, which the
+        reference runs without installing ``data_to_invert`` (tl_fwi.py:
+        476-491)."""
+        sim = STRATEGIES["sim"]
+        if data_to_invert is None:
+            plan = Plan([Leg("j", "m0", kind="joint")], sim.outputs, sim.hist, sim.clocks,
+                        sim.banner)
         else:
-            b0, m0 = copy.deepcopy(b_m0), copy.deepcopy(b_m0)
-        x0 = np.hstack((self.inv_obj.dict2vec(b0), self.inv_obj.dict2vec(m0)))
-        rms2 = []
-        for c, freq in enumerate(freqs):
-            self.restForTest = x0
-            x0, hist1 = self.tl_lbfgs(x0, iter[c], freq, n_params, k_0, k_end)
-            rms2.append(hist1[0])
-        models = self.ins_TL_output(x0, False)
-        inverted = {"dm": self.ins_TL_output(x0, True), "b1": models["model_b"],
-                    "m1": models["model_m"], "pt": timeit.default_timer() - tic}
-        return inverted, {"sim": rms2}
+            self.dobs = data_to_invert
+            plan = sim
+        return self.run_plan(plan, b_m0, freqs, iter, n_params, k_0, k_end)
 
     def ins_TL_output(self, xk, diff=True):
-        """[baseline, monitor] vector -> their difference (diff) or both
-        models as dicts (tl_fwi.py:554-584)."""
+        """Model dicts of a stacked [b; m] vector: the difference m - b, or
+        {'model_b', 'model_m'} (tl_fwi.py:555-584)."""
         half = int(xk.shape[0] / 2)
+        v2d = self.inv_obj.vec2dict
         if diff:
-            return self.inv_obj.vec2dict(xk[half:] - xk[:half], self.nz, self.nx)
-        return {"model_b": self.inv_obj.vec2dict(xk[:half], self.nz, self.nx),
-                "model_m": self.inv_obj.vec2dict(xk[half:2 * half], self.nz, self.nx)}
-
-    # ------------------------------------------------------------ sequential strategies
-    def parallel_inversion(self, b_m0, freqs, iter, n_params, k_0, k_end):
-        """Baseline and monitor each from b_m0 (tl_fwi.py:402-428)."""
-        tic = timeit.default_timer()
-        kw = dict(n_params=n_params, k_0=k_0, k_end=k_end)
-        b1, rms1 = self._invert(b_m0, self.b_dobs, freqs, iter, **kw)
-        print("===== Inversion of baseline is done ======")
-        m1, rms2 = self._invert(b_m0, self.m_dobs, freqs, iter, **kw)
-        inverted = {"b1": b1, "m1": m1, "dm": tools.dict_diff(m1, b1, positivity=False),
-                    "pt": timeit.default_timer() - tic}
-        print("Parallel")
-        return inverted, {"b1": rms1, "m1": rms2}
-
-    def cascaded_inversion(self, b_m0, freqs, iter, n_params, k_0, k_end):
-        """The monitor from the inverted baseline (tl_fwi.py:471-503)."""
-        tic = timeit.default_timer()
-        kw = dict(n_params=n_params, k_0=k_0, k_end=k_end)
-        b1, rms1 = self._invert(b_m0, self.b_dobs, freqs, iter, **kw)
-        initial = b1.copy()
-        print("===== Inversion of baseline is done ======")
-        m1, rms2 = self._invert(b1, self.m_dobs, freqs, iter, **kw)
-        inverted = {"b1": b1, "m1": m1, "dm": tools.dict_diff(m1, initial, positivity=False),
-                    "pt": timeit.default_timer() - tic}
-        print("Cascaded")
-        return inverted, {"b1": rms1, "m1": rms2}
-
-    def double_difference_inversion(self, b_m0, freqs, iter, n_params, k_0, k_end):
-        """Invert the composite data d_ref + (d_monitor - d_baseline), with
-        d_ref modelled in the inverted baseline (Asnaashari et al. 2011;
-        tl_fwi.py:430-469)."""
-        tic = timeit.default_timer()
-        kw = dict(n_params=n_params, k_0=k_0, k_end=k_end)
-        b1, rms1 = self._invert(b_m0, self.b_dobs, freqs, iter, **kw)
-        print("===== Inversion of baseline is done ======")
-        d_ref = prepare_residual(self.forward_modeling(b1, show=False), s=self.inv_obj.sd)
-        d_comp = tools.dict_summation(tools.dict_diff(self.m_dobs, self.b_dobs), d_ref)
-        composite, rms2 = self._invert(b_m0.copy(), d_comp, freqs, iter, **kw)
-        inverted = {"b1": b1, "m1": composite,
-                    "dm": tools.dict_diff(composite, b1, positivity=False),
-                    "pt": timeit.default_timer() - tic}
-        print("double_difference_inversion")
-        return inverted, {"b1": rms1, "m1": rms2}
-
-    def weighted_averaged_cascaded(self, b_m0, freqs, iter, n_params, k_0, k_end):
-        """Baseline, monitor from it, baseline again from the monitor
-        (tl_fwi.py:224-255)."""
-        kw = dict(n_params=n_params, k_0=k_0, k_end=k_end)
-        print("*** First inversion ***")
-        b1, rms_b1 = self._invert(copy.deepcopy(b_m0), self.b_dobs, freqs, iter, **kw)
-        print("*** Second inversion ***")
-        m1, rms_m1 = self._invert(b1, self.m_dobs, freqs, iter, **kw)
-        print("*** Third inversion ***")
-        b2, rms_b2 = self._invert(m1, self.b_dobs, freqs, iter, **kw)
-        inverted = {"dm11": tools.dict_diff(m1, b1), "dm12": tools.dict_diff(m1, b2),
-                    "m1": m1, "b1": b1, "b2": b2}
-        print("Weighted average")
-        return inverted, {"b1": rms_b1, "m1": rms_m1, "b2": rms_b2}
-
-    def cross_updating(self, b_m0, freqs, iter, n_params, k_0, k_end):
-        """Four alternating inversions (Maharramov and Biondi 2014;
-        tl_fwi.py:257-305)."""
-        kw = dict(n_params=n_params, k_0=k_0, k_end=k_end)
-        tic = timeit.default_timer()
-        print("*** First inversion ***")
-        b1, rms_b1 = self._invert(copy.deepcopy(b_m0), self.b_dobs, freqs, iter, **kw)
-        print("*** Second inversion ***")
-        m1, rms_m1 = self._invert(b1, self.m_dobs, freqs, iter, **kw)
-        t_cc = timeit.default_timer() - tic
-        print("*** Third inversion ***")
-        b2, rms_b2 = self._invert(m1, self.b_dobs, freqs, iter, **kw)
-        t_wa = timeit.default_timer() - tic
-        print("*** Fourth inversion ***")
-        m2, rms_m2 = self._invert(b2, self.m_dobs, freqs, iter, **kw)
-        t_cu = timeit.default_timer() - tic
-        inverted = {"dm11": tools.dict_diff(m1, b1), "dm12": tools.dict_diff(m1, b2),
-                    "dm21": tools.dict_diff(m2, b1), "dm22": tools.dict_diff(m2, b2),
-                    "b1": b1, "m1": m1, "b2": b2, "m2": m2,
-                    "pt_cc": t_cc, "pt_wa": t_wa, "pt_cu": t_cu}
-        print("cross_updating")
-        return inverted, {"b1": rms_b1, "m1": rms_m1, "b2": rms_b2, "m2": rms_m2}
-
-    def central_difference(self, b_m0, freqs, iter, n_params, k_0, k_end):
-        """Baseline then monitor, and monitor then baseline
-        (tl_fwi.py:307-353)."""
-        kw = dict(n_params=n_params, k_0=k_0, k_end=k_end)
-        m0 = copy.deepcopy(b_m0)
-        tic = timeit.default_timer()
-        print("*** Central time lapse *** ")
-        b1, rms_b1 = self._invert(m0, self.b_dobs, freqs, iter, **kw)
-        print("*** Second INVERSION ***")
-        b1m, rms_b1m = self._invert(b1, self.m_dobs, freqs, iter, **kw)
-        print("*** Third INVERSION ***")
-        m1, rms_m1 = self._invert(m0, self.m_dobs, freqs, iter, **kw)
-        print("*** Fourth INVERSION ***")
-        m1b, rms_m1b = self._invert(m1, self.b_dobs, freqs, iter, **kw)
-        inverted = {"dm0": tools.dict_diff(m1, b1), "dm1": tools.dict_diff(b1m, m1b),
-                    "dm3": tools.dict_diff(b1m, b1), "dm4": tools.dict_diff(m1, m1b),
-                    "dm34": tools.dict_summation(tools.dict_diff(b1m, b1),
-                                                 tools.dict_diff(m1, m1b), division=2),
-                    "b1": b1, "m1": m1, "m2": b1m, "b2": m1b,
-                    "pt": timeit.default_timer() - tic}
-        print("central difference")
-        return inverted, {"b1": rms_b1, "m1": rms_m1, "m2": rms_b1m, "b2": rms_m1b}
-
-    def cascaded_joint(self, m0, freqs, iter, n_params, k_0, k_end):
-        """Two simultaneous inversions, the second from the average of the
-        first's models (tl_fwi.py:355-400)."""
-        kw = dict(n_params=n_params, k_0=k_0, k_end=k_end)
-        tic = timeit.default_timer()
-        print("*** FIRST INVERSION ***")
-        r1, rms1 = self.sim_inversion(m0, freqs, iter=iter, data_to_invert=None, **kw)
-        b1, m1 = r1["b1"], r1["m1"]
-        ave = tools.dict_summation(b1, m1, division=2)
-        print("*** SECOND INVERSION ***")
-        r2, rms2 = self.sim_inversion(ave, freqs, iter=iter, data_to_invert=None, **kw)
-        b1b, b1m = r2["b1"], r2["m1"]
-        inverted = {"dm0": tools.dict_diff(m1, b1), "dm7": tools.dict_diff(b1m, b1b),
-                    "dm3": tools.dict_diff(b1m, b1), "dm6": tools.dict_diff(m1, b1b),
-                    "b1": b1, "m1": m1, "b2": b1b, "m2": b1m,
-                    "pt": timeit.default_timer() - tic}
-        print("joint_cascaded")
-        return inverted, {"sim1": rms1, "sim2": rms2}
+            return v2d(xk[half:] - xk[:half], self.nz, self.nx)
+        return {"model_b": v2d(xk[:half], self.nz, self.nx),
+                "model_m": v2d(xk[half:2 * half], self.nz, self.nx)}

@@ -10,12 +10,15 @@ Mirror of PyFWI's ``PyFWI.torchfwi.Fwi`` (src/PyFWI/torchfwi.py:7-91).
 The backward kernel is selected by ``W.grad_mode``:
   'pyfwi' (default)  the reference's adjoint (drop-in parity)
   'exact'            the exact discrete adjoint, with forward states recomputed
-                     from checkpoints kept every ``W.ckpt_every`` steps
-                     (default 50) instead of storing every step
+                     from checkpoints instead of storing every step: every
+                     ``W.ckpt_every`` steps when set, else every ceil(sqrt(nt))
+                     steps with the shots in passes sized to the free device
+                     memory (the two-level schedule of pyfwi_gradient)
 """
 import numpy as np
 import torch
 
+from . import _native
 from .processing import prepare_residual
 
 __all__ = ["Fwi"]
@@ -34,9 +37,9 @@ class Fwi(torch.autograd.Function):
         ctx.W = W
         ctx.mode = mode
         if vp.is_cuda:
-            k = getattr(W, "ckpt_every", 50) if mode == "exact" else 0
-            d = W.forward_modeling_tensor(vp, vs, rho, ckpt_every=k)
-            ctx.ckpt_every = k
+            k = getattr(W, "ckpt_every", 0) or _native.CKPT_AUTO
+            d = W.forward_modeling_tensor(vp, vs, rho, ckpt_every=k if mode == "exact" else 0)
+            ctx.ckpt_every = max(k, 0)
             return d["taux"], d["tauz"]
         model = {"vp": vp.detach().cpu().numpy().astype(np.float32),
                  "vs": vs.detach().cpu().numpy().astype(np.float32),

@@ -17,7 +17,7 @@ from .acquisition import Source
 __all__ = ["Workload", "layered3", "random_layered", "gaussian_anomaly",
            "plain_small", "plain_medium", "crosswell_medium", "wells_surface_medium",
            "differentiation", "large_step",
-           "t_for_nt", "gardner", "UnevenWorkload", "uneven_batch", "uneven_small"]
+           "t_for_nt", "gardner", "UnevenWorkload", "uneven_batch", "uneven_small", "subset"]
 
 
 def gardner(vp):
@@ -204,6 +204,23 @@ def large_step(seed=5, ns=32, nt=6000):
     src.Ricker(15)
     return Workload("large_step_2000x6000", inpa, src, rec_loc, model, None, components=0,
                     chpr=0, meta={"nt": nt, "shots": ns, "seed": seed})
+
+
+def subset(w, ids, nt=None):
+    """The workload restricted to shots `ids` (same model, receivers and
+    wavelet), optionally with ``nt`` time steps."""
+    from copy import deepcopy
+    dh, dt = w.inpa["dh"], w.inpa["dt"]
+    loc = np.c_[w.src.i, w.src.j][list(ids)].astype(np.float64) * dh
+    src = Source(loc, dh, dt)
+    src.component = w.src.component.copy()
+    src.w = w.src.w.copy()
+    out = deepcopy(w)
+    out.src = src
+    if nt is not None:
+        out.inpa["t"] = t_for_nt(nt, dt)
+    out.meta = dict(w.meta, shots=len(ids), shot_ids=list(ids))
+    return out
 
 
 @dataclass

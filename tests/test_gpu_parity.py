@@ -99,10 +99,12 @@ def test_gradient_pyfwi_small(gpu, chpr):
     st = oracle_setup(w, chpr=chpr, components=2)
     seis, states = O.forward(st, w.model, np.float32)
     gr = O.gradient(st, w.model, res, states, parameterization="lmd")
+    seis_d, states_d = O.forward(st, w.model, np.float64)
+    gr_d = O.gradient(st, w.model, res, states_d, parameterization="lmd", dtype=np.float64)
     for k in gr:
-        e = rel(g[k], gr[k])
-        print(f"pyfwi gradient chpr={chpr} {k}: {e:.2e}")
-        assert e < GRAD_TOL, (k, e)
+        e, e64 = rel(g[k], gr[k]), rel(g[k], gr_d[k])
+        print(f"pyfwi gradient chpr={chpr} {k}: vs oracle {e:.2e}, vs FP64 {e64:.2e}")
+        assert e < GRAD_TOL and e64 < GRAD_TOL, (k, e, e64)
     # the 'dv' parameterisation applies the reference's chain rule
     gv = W.gradient(res, parameterization="dv")
     md = st.medium(w.model, np.float32)
