@@ -185,6 +185,23 @@ int32_t pyfwi_forward(pyfwi_plan *plan, const pyfwi_shot *shots, int32_t nshots,
                       const float *src, float *seis, const int32_t *chp,
                       int32_t nchp, int32_t ckpt_every, int32_t mem);
 
+/* Forward modelling that allocates nothing: pyfwi_forward's records for
+ * `nshots` shots in one pass, with no checkpoint list, on caller device
+ * memory only (`src`, `seis` as with PYFWI_MEM_DEVICE).  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * forward states and the shot table live in one fixed device array of
+ * PYFWI_SCRATCH_FLOATS floats (64 MiB), which each GPU holds from the first
+ * plan created on it until the process ends; a call whose shots need more
+ * (shot table + nshots * 5 padded fields) returns PYFWI_EINVAL.  Every call
+ * on one GPU shares that array: calls that overlap on one GPU (two threads,
+ * two streams) race.  The plan's model (pyfwi_set_model) and receiver table
+ * are used as they are; no states are kept, so no gradient may follow.
+ * Records equal pyfwi_forward's bit for bit. */
+enum { PYFWI_SCRATCH_FLOATS = 16777216 };
+int32_t pyfwi_forward_noalloc(pyfwi_plan *plan, const pyfwi_shot *shots, int32_t nshots,
+                              const float *src, float *seis);
+
 /* Adjoint-state gradient for the shots of the last pyfwi_forward call
  * (__adjoint_modelling_per_source + __kernel_gradient,
  * wave_propagation.py:735-856).

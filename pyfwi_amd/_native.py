@@ -32,7 +32,7 @@ TIMING_LAUNCHES, TIMING_LOOPS = 1, 2
 EXPORTS = [
     "pyfwi_version", "pyfwi_last_error", "pyfwi_plan_create", "pyfwi_plan_destroy",
     "pyfwi_plan_set_stream", "pyfwi_set_receivers", "pyfwi_set_model", "pyfwi_forward",
-    "pyfwi_gradient",
+    "pyfwi_forward_noalloc", "pyfwi_gradient",
     "pyfwi_misfit_gradient", "pyfwi_plan_set_tuning", "pyfwi_plan_set_graphs",
     "pyfwi_graph_stats", "pyfwi_get_snapshot", "pyfwi_last_passes",
     "pyfwi_synchronize", "pyfwi_plan_set_timing", "pyfwi_kernel_timing",
@@ -94,6 +94,7 @@ def load(path=None):
     lib.pyfwi_set_model.argtypes = [P, fp, fp, fp, fp, fp, i32]
     lib.pyfwi_forward.argtypes = [P, ctypes.POINTER(Shot), i32, fp, fp,
                                   ctypes.POINTER(ctypes.c_int32), i32, i32, i32]
+    lib.pyfwi_forward_noalloc.argtypes = [P, ctypes.POINTER(Shot), i32, fp, fp]
     lib.pyfwi_gradient.argtypes = [P, ctypes.POINTER(Shot), i32, fp, fp, fp, i32, i32, i32, i32]
     lib.pyfwi_plan_set_tuning.argtypes = [P, i32, i32, i32]
     lib.pyfwi_misfit_gradient.argtypes = [P, ctypes.POINTER(Shot), i32, fp, fp, fp, fp,
@@ -223,6 +224,16 @@ class Plan:
         cp = chp.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)) if chp.size else None
         check(self.lib.pyfwi_forward(self.h, arr, len(shots), ptr(src, mem), ptr(seis, mem), cp,
                                      int(chp.size), int(ckpt_every), mem))
+
+    def forward_noalloc(self, shots, src, seis):
+        """Records of all shots from device buffers without any allocation
+        (see pyfwi_forward_noalloc)."""
+        ns, nr = extents(shots)
+        _need(src, ns, "source table")
+        _need(seis, nr, "records")
+        arr = (Shot * len(shots))(*shots)
+        check(self.lib.pyfwi_forward_noalloc(self.h, arr, len(shots), ptr(src, MEM_DEVICE),
+                                             ptr(seis, MEM_DEVICE)))
 
     def gradient(self, shots, src, res, grad, mode, ckpt_every, flags, mem):
         ns, nr = extents(shots)

@@ -21,6 +21,7 @@
 #include <cstring>
 #include <string>
 #include <initializer_list>
+#include <mutex>
 #include <vector>
 
 #include "../../include/pyfwi_hip.h"
@@ -44,6 +45,37 @@ int fail(int code, const std::string &msg)
         if (e_ != hipSuccess)                                                        \
             return fail(PYFWI_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
+
+// ------------------------------------------------ allocation-free forward
+// pyfwi_forward_noalloc keeps the forward states and the shot table of its
+// call in this array instead of allocating.  Each GPU the library runs on
+// holds its own copy until the process ends; host code uses the per-device
+// address from hipGetSymbolAddress, never the array's name.
+static __device__ __align__(256) float scratch[PYFWI_SCRATCH_FLOATS];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_ptr[MAX_DEVICES] = {};
+bool g_scratch_done[MAX_DEVICES] = {};
+
+// Address of `scratch` on device `dev`, looked up once per device with that
+// device current (plan creation); null when the lookup failed.
+float *scratch_on(int dev)
+{
+    if (dev < 0 || dev >= MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_done[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            ptr = nullptr;
+        }
+        g_scratch_ptr[dev] = static_cast<float *>(ptr);
+        g_scratch_done[dev] = true;
+    }
+    return g_scratch_ptr[dev];
+}
 
 struct DevBuf {
     float *p = nullptr;
@@ -451,6 +483,7 @@ int32_t pyfwi_plan_create(const pyfwi_grid *grid, void *stream, pyfwi_plan **out
     }
     d.dt = (float)g.dt;
     if (ncu > 0) d.ncu = ncu;
+    (void)scratch_on(dev);  // set-up of pyfwi_forward_noalloc on this device
     if (stream) {
         p->stream = (hipStream_t)stream;
     } else {
@@ -1289,6 +1322,54 @@ int32_t pyfwi_misfit_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t ns
     }
     HIPCHK(hipStreamSynchronize(p->stream));
     if (int e = harvest_timing(p)) return e;
+    return 0;
+}
+
+int32_t pyfwi_forward_noalloc(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots,
+                              const float *src, float *seis)
+{
+    if (!p || !src || !seis) return fail(PYFWI_EINVAL, "null argument");
+    if (int e = check_shots(p, shots, nshots)) return e;
+    const int64_t fps = fields_per_shot(p);
+    // scratch layout: the shot table (256-byte aligned), then the states
+    const int64_t tab = ((int64_t)nshots * (int64_t)sizeof(pyfwi_shot) + 255) / 256 * 64;
+    const int64_t need = tab + (int64_t)nshots * fps;
+    if (need > (int64_t)PYFWI_SCRATCH_FLOATS)
+        return fail(PYFWI_EINVAL, "pyfwi_forward_noalloc: " + std::to_string(nshots) +
+                                      " shots of this grid need " + std::to_string(need * 4) +
+                                      " bytes of scratch; the maximum is " +
+                                      std::to_string((int64_t)PYFWI_SCRATCH_FLOATS * 4));
+    if (!p->model_set) return fail(PYFWI_ESTATE, "pyfwi_set_model must come first");
+    if (int e = check_device_ptrs(PYFWI_MEM_DEVICE, {{src, "source table"}, {seis, "records"}}))
+        return e;
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev != p->device) return fail(PYFWI_EINVAL, "the plan's device is not current");
+    float *base = scratch_on(dev);
+    if (!base) return fail(PYFWI_EHIP, "hipGetSymbolAddress failed for the scratch array");
+    pyfwi_shot *dsh = reinterpret_cast<pyfwi_shot *>(base);
+    float *F = base + tab;
+    shot_flags(p, shots, nshots);
+    p->have_forward = false;  // no states are kept for a gradient
+    HIPCHK(hipMemcpyAsync(dsh, shots, sizeof(pyfwi_shot) * nshots, hipMemcpyHostToDevice,
+                          p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));  // the caller's table may go away
+    const DevGrid &g = p->dg;
+    launch_zero(seis, (int64_t)rec_extent(shots, nshots), p->stream);
+    launch_zero(F, (int64_t)nshots * fps, p->stream);
+    const int nt = max_nt(std::vector<pyfwi_shot>(shots, shots + nshots));
+    for (int t = 0; t < nt; ++t) {
+        // the two-launch step of pyfwi_forward (forward_step), untimed
+        if (!p->src_inside || !p->rec_inline)
+            launch_inject_record(g, nshots, dsh, t, F, fps, p->src_inside ? nullptr : src, seis,
+                                 p->d_rtab, p->stream);
+        const float *s_in = p->src_inside ? src : nullptr;
+        launch_vel(g, +1, nshots, dsh, t, F, fps, F, fps, p->bu.p, p->dxp, p->dzp, p->stream,
+                   s_in);
+        launch_str(g, +1, nshots, dsh, t, F, fps, F, fps, p->lam.p, p->mu.p, p->dxp, p->dzp,
+                   p->stream, s_in, p->src_inside && p->rec_inline ? seis : nullptr);
+    }
+    HIPCHK(hipGetLastError());
     return 0;
 }
 
