@@ -194,19 +194,18 @@ def gradient_step(W, b, shots=None, seis=None, obs=None, res=None):
     plan.gradient(shots, b["src"], res, b["grad"], _native.MODE_PYFWI, 0, 0, _native.MEM_DEVICE)
 
 
-EXACT_CKPT = 50  # checkpoint spacing of the exact-adjoint bench step
-
-
 def exact_step(W, b, shots=None):
-    """One forward (checkpoints every EXACT_CKPT steps) and exact-adjoint
-    gradient of the given shots (all by default); segments are recomputed."""
+    """One forward and exact-adjoint gradient of the given shots (all by
+    default), as the autograd path runs it: the forward keeps checkpoints
+    every ceil(sqrt(nt)) steps when all shots' fit in HBM, else the gradient
+    recomputes them pass by pass; each segment is recomputed once."""
     from pyfwi_amd import _native
     plan = b["plan"]
     shots = b["shots"] if shots is None else shots
-    plan.forward(shots, b["src"], b["seis"], None, EXACT_CKPT, _native.MEM_DEVICE)
+    plan.forward(shots, b["src"], b["seis"], None, _native.CKPT_AUTO, _native.MEM_DEVICE)
     b["res"].copy_(b["seis"])
     b["res"].sub_(b["obs"])
-    plan.gradient(shots, b["src"], b["res"], b["grad"], _native.MODE_EXACT, EXACT_CKPT, 0,
+    plan.gradient(shots, b["src"], b["res"], b["grad"], _native.MODE_EXACT, 0, 0,
                   _native.MEM_DEVICE)
 
 
@@ -427,6 +426,8 @@ def bench_uneven(args, torch):
     mask = 3  # vx, vz records (components = 2)
     graphs = args.graphs if args.graphs >= 0 else int(B.tnz * B.tnx < 1_000_000)
     plan.set_graphs(bool(graphs))
+    if args.rows:
+        plan.set_tuning(0, False, args.rows)
 
     def run(shots):
         return plan.misfit_gradient(shots, src, obs, seis, grad, chp, mask, 0,
@@ -568,6 +569,130 @@ def bench_uneven(args, torch):
         "cpu_baseline_1thread": cpu1}), flush=True)
 
 
+def _inversion_setup(args, torch):
+    """The workload with observed data of its baseline (and monitor) model,
+    and the initial model; the wavelet and the data are scaled so that the
+    gradient at the initial model peaks near 1e2, above the L-BFGS stopping
+    thresholds of the reference's settings (the problem is linear in the
+    source amplitude: misfit and gradient scale with its square)."""
+    from pyfwi_amd import workloads
+    from pyfwi_amd.wave_propagation import WavePropagator
+    w = getattr(workloads, args.workload)(**({"ns": args.shots} if args.shots else {}))
+    if args.nt:
+        w.inpa["t"] = workloads.t_for_nt(args.nt, w.inpa["dt"])
+    W = WavePropagator(w.inpa, w.src, w.rec_loc, w.model_shape, components=w.components,
+                       chpr=w.chpr)
+    t0 = time.perf_counter()
+    obs = {"base": W.forward_modeling(w.model_obs)}
+    if "monitor" not in w.meta:
+        # a -8 % vp patch in the middle of the model
+        nz, nx = w.model_shape
+        mon = {k: np.array(v) for k, v in w.model_obs.items()}
+        mon["vp"][nz // 2 - nz // 10:nz // 2 + nz // 10, nx // 2 - nx // 10:nx // 2 + nx // 10] \
+            *= np.float32(0.92)
+        w.meta["monitor"] = mon
+    if args.mode == "timelapse":
+        obs["monitor"] = W.forward_modeling(w.meta["monitor"])
+    W._plan.close()
+    log(f"bench: observed data of {len(obs)} model(s), {time.perf_counter() - t0:.1f} s")
+    m0 = w.meta.get("initial", w.model)
+    return w, obs, m0
+
+
+def _scale_problem(fwi, obs, m0, w):
+    """Evaluate the misfit gradient at m0 once (timed), then scale the
+    wavelet and the observed data so that the gradient peaks near 1e2."""
+    from pyfwi_amd import fwi_tools as T
+    t0 = time.perf_counter()
+    rms, g = fwi.fprime(np.float32(T.vel_dict2vec(m0)), None)
+    dt = time.perf_counter() - t0
+    gmax = float(np.abs(g).max())
+    a = float(np.sqrt(1e2 / gmax)) if gmax > 0 else 1.0
+    fwi.src.w = np.float32(fwi.src.w * a)
+    for d in obs.values():
+        for k in d:
+            d[k] *= np.float32(a)
+    fwi.__dict__.pop("_obs_cache", None)
+    return rms, gmax, a, dt
+
+
+def bench_lbfgs(args, torch):
+    """L-BFGS-B inversion (the FWI driver, reference settings) of the
+    workload's observed data from its initial model, vp only, one band with
+    no lowpass: every misfit evaluation is one pyfwi_misfit_gradient call of
+    all shots (reference adjoint, the workload's checkpoint list)."""
+    from pyfwi_amd.fwi import FWI
+    w, obs, m0 = _inversion_setup(args, torch)
+    fwi = FWI(obs["base"], w.inpa, w.src, w.rec_loc, w.model_shape, components=w.components,
+              chpr=w.chpr)
+    fwi.verbose = False
+    rms0, gmax, a, t_eval = _scale_problem(fwi, {"base": fwi.d_obs}, m0, w)
+    log(f"bench: first gradient {t_eval:.1f} s (misfit {rms0:.4e}, |g| {gmax:.3e}); "
+        f"wavelet and data scaled by {a:.3e}")
+    evals = []
+    inner = fwi.fprime
+
+    def timed_fprime(m, freq):
+        ts = time.perf_counter()
+        out = inner(m, freq)
+        evals.append((time.perf_counter() - ts, float(out[0])))
+        log(f"bench: evaluation {len(evals)}: misfit {out[0]:.6e}, {evals[-1][0]:.1f} s")
+        return out
+    fwi.fprime = timed_fprime
+    t0 = time.perf_counter()
+    m_est, hist = fwi(m0, method="lbfgs", iter=[args.iters], freqs=[None], n_params=1, k_0=1,
+                      k_end=2)
+    total = time.perf_counter() - t0
+    cells = (w.model_shape[0] + 2 * w.inpa["npml"]) * (w.model_shape[1] + 2 * w.inpa["npml"])
+    units = cells * len(w.src.i) * int(fwi.nt)
+    t_mean = float(np.mean([e[0] for e in evals]))
+    err0 = float(np.linalg.norm(m0["vp"] - w.model_obs["vp"]))
+    err1 = float(np.linalg.norm(m_est["vp"] - w.model_obs["vp"]))
+    print(json.dumps({
+        "metric": "L-BFGS inversion: misfit evaluations (forward + L2 misfit + reference "
+                  "gradient) of the full workload", "workload": w.name,
+        "shots": int(len(w.src.i)), "nt": int(fwi.nt), "cells": cells,
+        "iterations": args.iters, "evaluations": len(evals), "seconds": round(total, 1),
+        "seconds_per_evaluation": round(t_mean, 2),
+        "gradient_gcell_updates_per_s": round(units / t_mean / 1e9, 3),
+        "misfit_first": evals[0][1] if evals else None, "misfit_last": evals[-1][1] if evals
+        else None, "misfit_history": [e[1] for e in evals],
+        "final_misfit_per_window": np.asarray(hist).ravel().tolist(),
+        "vp_error_l2": {"initial": err0, "final": err1},
+        "scale": a, "checkpoints": f"chpr={w.chpr}"}), flush=True)
+
+
+def bench_timelapse(args, torch):
+    """Time-lapse pair (baseline and monitor observed data, the monitor with
+    vp -8 % in a patch) inverted with the cascaded strategy: baseline from
+    the initial model, monitor from the baseline result; `--iters` L-BFGS
+    iterations each."""
+    from pyfwi_amd.tl_fwi import TimeLapse
+    w, obs, m0 = _inversion_setup(args, torch)
+    tl = TimeLapse(obs["base"], obs["monitor"], w.inpa, w.src, w.rec_loc, w.model_shape,
+                   components=w.components, chpr=w.chpr)
+    tl.verbose = tl.inv_obj.verbose = False
+    rms0, gmax, a, t_eval = _scale_problem(tl.inv_obj, {"b": tl.b_dobs, "m": tl.m_dobs}, m0, w)
+    tl.src.w = tl.inv_obj.src.w
+    tl.inv_obj.d_obs = tl.b_dobs
+    log(f"bench: first gradient {t_eval:.1f} s; scaled by {a:.3e}")
+    t0 = time.perf_counter()
+    out, hist = tl(m0, iter=[args.iters], freqs=[None], tl_method="cc", n_params=1, k_0=1,
+                   k_end=2)
+    total = time.perf_counter() - t0
+    dm = out["dm"]["vp"]
+    patch = (w.meta["monitor"]["vp"] != w.model_obs["vp"])
+    inside = float(dm[patch].mean())
+    outside = float(np.abs(dm[~patch]).mean())
+    print(json.dumps({
+        "metric": "time-lapse pair, cascaded strategy", "workload": w.name,
+        "shots": int(len(w.src.i)), "iterations_per_inversion": args.iters,
+        "seconds": round(total, 1), "misfit_history": {k: np.asarray(v).ravel().tolist()
+                                                       for k, v in hist.items()},
+        "dvp_patch_mean": inside, "dvp_outside_abs_mean": outside,
+        "pt": out.get("pt"), "scale": a}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -590,6 +715,8 @@ def main():
     ap.add_argument("--cpu-steps-1t", type=int, default=0,
                     help="time steps of the 1-thread CPU baseline (0: 80 / 150)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--rows", type=int, default=0,
+                    help="uneven_batch: fixed strip height of the stencil kernels (0: automatic)")
     ap.add_argument("--graphs", type=int, default=-1, choices=[-1, 0, 1],
                     help="replay the time loops as captured hipGraphs (1), launch eagerly (0); "
                          "-1: graphs for grids under 1 M cells (launch-bound), eager otherwise")
@@ -599,10 +726,14 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the execution-shape autotuner")
     ap.add_argument("--tune", default="", help="fixed tuning 'shots_per_pass,fused,rows'")
-    ap.add_argument("--mode", default="kept", choices=["kept", "reverse", "exact"],
+    ap.add_argument("--iters", type=int, default=5,
+                    help="L-BFGS iterations of --mode lbfgs / timelapse")
+    ap.add_argument("--mode", default="kept",
+                    choices=["kept", "reverse", "exact", "lbfgs", "timelapse"],
                     help="kept: every forward state kept in HBM per pass (chpr=100), misfit and "
                          "gradient in one call; reverse: chpr=0, states rebuilt by time reversal; "
-                         "exact: the exact adjoint from checkpoints")
+                         "exact: the exact adjoint from checkpoints sized to the free HBM; "
+                         "lbfgs / timelapse: the inversion drivers on the workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -625,6 +756,10 @@ def main():
     args.cpu_steps_1t = args.cpu_steps_1t or (80 if big else 150)
     if big:
         return bench_uneven(args, torch)
+    if args.mode == "lbfgs":
+        return bench_lbfgs(args, torch)
+    if args.mode == "timelapse":
+        return bench_timelapse(args, torch)
     w, W = build_problem(args.workload, args.nt or None, args.shots or None, args.order)
     b = device_setup(w, W, torch)
     cells = int(W.tnz) * int(W.tnx)
@@ -680,6 +815,15 @@ def main():
     ms_per_step = elapsed / args.steps * 1e3
     value = units / (elapsed / args.steps) / 1e9
     log(f"bench: timed {args.steps} steps, {ms_per_step:.1f} ms/step, {value:.2f} Gcell/s")
+    if args.minimal:
+        passes, ck = b["plan"].last_passes(with_interval=True)
+        print(json.dumps({"metric": "fwd+adj gradient Gcell-updates/s", "value": round(value, 3),
+                          "unit": "Gcell-updates/s", "steps": args.steps,
+                          "ms_per_step": round(ms_per_step, 1), "mode": args.mode,
+                          "config": {"workload": w.name, "shots": int(W.ns), "nt": int(W.nt),
+                                     "cells": cells, "passes": passes, "ckpt_every": ck},
+                          "note": "--minimal: timed steps only"}), flush=True)
+        return
 
     prof = kernel_profile(W, b, torch, step)
     dom = max(prof, key=lambda k: prof[k][0])
@@ -743,8 +887,8 @@ def main():
                        "kept": "reference adjoint, chpr=100: every forward state kept in HBM "
                                "per pass; L2 misfit on the GPU",
                        "reverse": "reference adjoint, chpr=0: states rebuilt by time reversal",
-                       "exact": f"exact adjoint, checkpoints every {EXACT_CKPT} steps, segments "
-                                "recomputed"}[args.mode],
+                       "exact": "exact adjoint, two-level checkpoints sized to the free HBM, "
+                                "segments recomputed"}[args.mode],
                    "parallelism": "single GPU, shots batched per launch",
                    "tuning": {"shots_per_pass": tuning[0], "fused_steps": bool(tuning[1]),
                               "strip_rows": tuning[2], "graphs": bool(graphs)}},

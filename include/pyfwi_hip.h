@@ -251,6 +251,23 @@ int32_t pyfwi_misfit_gradient(pyfwi_plan *plan, const pyfwi_shot *shots, int32_t
                               double *misfit, const int32_t *chp, int32_t nchp,
                               int32_t comp_mask, int32_t flags, int32_t mem);
 
+/* Record filter of pyfwi_misfit_gradient: the multiscale lowpass of
+ * cost_seismic (src/PyFWI/fwi_tools.py:1290-1345, lowpass / adj_lowpass
+ * :1189-1255).  The reference pads each trace of n samples with `pad` zeros
+ * (512), multiplies the N-point spectrum (N = n + pad) by the Butterworth
+ * response H and by conj(H), and keeps n samples; that is the circular
+ * convolution y[t] = sum_{|m| <= L} k[|m|] x[(t - m) mod N] with the
+ * symmetric kernel k = ifft(|H|^2), whose taps k[0..half_len] the caller
+ * computes (pyfwi_amd.fwi_tools.lowpass_taps).  With a filter set,
+ * pyfwi_misfit_gradient filters the modelled and the observed records,
+ * takes the misfit of the filtered records and injects the filtered
+ * residual (the filter is its own adjoint); the `seis` output stays
+ * unfiltered.  half_len < 0 removes the filter; half_len at most
+ * PYFWI_FIR_MAX_HALF. */
+enum { PYFWI_FIR_MAX_HALF = 6000 };
+int32_t pyfwi_set_record_filter(pyfwi_plan *plan, const double *taps, int32_t half_len,
+                                int32_t pad);
+
 /* Execution shape of later calls (no counterpart in the reference, which runs
  * one shot at a time): shots_per_pass > 0 runs a call's shots in passes of
  * that many shots (0: all in one pass; smaller passes need less memory and

@@ -33,7 +33,7 @@ EXPORTS = [
     "pyfwi_version", "pyfwi_last_error", "pyfwi_plan_create", "pyfwi_plan_destroy",
     "pyfwi_plan_set_stream", "pyfwi_set_receivers", "pyfwi_set_model", "pyfwi_forward",
     "pyfwi_forward_noalloc", "pyfwi_gradient",
-    "pyfwi_misfit_gradient", "pyfwi_plan_set_tuning", "pyfwi_plan_set_graphs",
+    "pyfwi_misfit_gradient", "pyfwi_set_record_filter", "pyfwi_plan_set_tuning", "pyfwi_plan_set_graphs",
     "pyfwi_graph_stats", "pyfwi_get_snapshot", "pyfwi_last_passes",
     "pyfwi_synchronize", "pyfwi_plan_set_timing", "pyfwi_kernel_timing",
 ]
@@ -102,6 +102,7 @@ def load(path=None):
                                           ctypes.POINTER(ctypes.c_int32), i32, i32, i32, i32]
     lib.pyfwi_get_snapshot.argtypes = [P, i32, i32, fp, i32]
     lib.pyfwi_plan_set_graphs.argtypes = [P, i32]
+    lib.pyfwi_set_record_filter.argtypes = [P, ctypes.POINTER(ctypes.c_double), i32, i32]
     lib.pyfwi_graph_stats.argtypes = [P, ctypes.POINTER(ctypes.c_int64),
                                       ctypes.POINTER(ctypes.c_int64)]
     lib.pyfwi_last_passes.argtypes = [P, ctypes.POINTER(ctypes.c_int32), i32,
@@ -265,6 +266,16 @@ class Plan:
     def set_tuning(self, shots_per_pass=0, fused=False, strip_rows=0):
         check(self.lib.pyfwi_plan_set_tuning(self.h, int(shots_per_pass), 1 if fused else 0,
                                              int(strip_rows)))
+
+    def set_record_filter(self, taps, pad=512):
+        """Lowpass of pyfwi_misfit_gradient's misfit (taps k[0..L], float64),
+        or none with taps=None; see pyfwi_set_record_filter."""
+        if taps is None:
+            check(self.lib.pyfwi_set_record_filter(self.h, None, -1, 0))
+            return
+        t = np.ascontiguousarray(taps, np.float64)
+        check(self.lib.pyfwi_set_record_filter(
+            self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), int(t.size) - 1, int(pad)))
 
     def set_graphs(self, enable):
         """Replay the time loops as captured hipGraphs (see pyfwi_plan_set_graphs)."""

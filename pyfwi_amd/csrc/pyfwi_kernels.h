@@ -124,6 +124,13 @@ void launch_adj_vel_grad(const DevGrid &g, int nshots, const pyfwi_shot *shots, 
                          hipStream_t st, const float *res = nullptr);
 void launch_l2_residual(int nshots, const pyfwi_shot *shots, const float *seis, const float *obs,
                         float *res, int comp_mask, double *misfit, hipStream_t st);
+// Records `in` -> `out` (same layout, distinct buffers) filtered along time
+// by the symmetric circular FIR k[0..L] on traces padded to n_samples + pad
+// (the multiscale lowpass); components of comp_mask (0: txx and tzz), the
+// n_rec columns of each shot; h_shots: host copy of the shot table.
+// Needs 2 * (L + 128) + 1 + 4096 floats of LDS: L <= 15000.
+void launch_fir(int nshots, const pyfwi_shot *shots, const pyfwi_shot *h_shots, const float *in,
+                float *out, int comp_mask, const float *taps, int L, int pad, hipStream_t st);
 void launch_exact_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
                       const float *vel, int64_t vel_stride, float *adj, float *av,
                       const float *lam, const float *mu, const float *dxp,

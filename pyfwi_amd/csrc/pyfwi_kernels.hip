@@ -1730,6 +1730,105 @@ void launch_l2_residual(int nshots, const pyfwi_shot *shots, const float *seis, 
                        comp_mask, misfit);
 }
 
+// ------------------------------------------------------ record filtering
+//
+// The multiscale lowpass of the misfit (lowpass / adj_lowpass,
+// src/PyFWI/fwi_tools.py:1189-1255) pads each trace of n samples with `pad`
+// zeros, multiplies its N-point spectrum (N = n + pad) by H and conj(H) (or
+// conj(H) and H) and keeps the first n samples.  |H|^2 is real and even on
+// the N-point circle, so both are the same circular convolution with the
+// symmetric kernel k = ifft(|H|^2):
+//   y[t] = sum_{m=-L..L} k[|m|] x[(t - m) mod N],  x = 0 past the trace,
+// with the taps k[0..L] computed on the host (L: where k has decayed).  A
+// block filters 64 time samples of 64 traces (one receiver each lane) of one
+// shot and component: 4 waves, 16 consecutive outputs per lane; the input
+// window [t0 - L, t0 + 64 + L) is staged through LDS 64 rows at a time, and
+// the taps of a lane's 16 outputs slide through registers one tap per row.
+
+constexpr int FIR_T = 64;      // time samples per block
+constexpr int FIR_C = 64;      // window rows per LDS chunk
+constexpr int FIR_J = 16;      // outputs per lane (FIR_T / 4 waves)
+
+__global__ __launch_bounds__(256) void k_fir(const pyfwi_shot *shots, const float *__restrict__ in,
+                                             float *__restrict__ out, int comp_mask,
+                                             const float *__restrict__ taps, int L, int pad)
+{
+    extern __shared__ float fir_lds[];
+    const int s = blockIdx.y / 5, f = blockIdx.y % 5;
+    const pyfwi_shot sh = shots[s];
+    const bool on = comp_mask == 0 ? (f == 2 || f == 3) : ((comp_mask >> f) & 1);
+    const int ns = sh.n_samples, nr = sh.n_rec;
+    const int t0 = blockIdx.x * FIR_T, r0 = blockIdx.z * 64;
+    if (!on || t0 >= ns || r0 >= nr) return;  // whole block: uniform
+    const int N = ns + pad;
+    const int lane = threadIdx.x % 64, tg = threadIdx.x / 64;
+    const int r = r0 + lane;
+    const int64_t plane = (int64_t)ns * sh.rec_stride;
+    const float *X = in + sh.rec_offset + f * plane;
+    float *Y = out + sh.rec_offset + f * plane;
+    // taps k[m], m = -(L + FIR_T + FIR_C) .. L + FIR_T + FIR_C, zero past +-L
+    const int M = L + FIR_T + FIR_C;
+    float *ks = fir_lds;                    // 2M + 1 values, ks[M + m] = k[|m|]
+    float *xs = fir_lds + ((2 * M + 1 + 63) / 64) * 64;  // FIR_C x 64 window rows
+    for (int k = threadIdx.x; k < 2 * M + 1; k += 256) {
+        const int m = k - M;
+        const int a = m < 0 ? -m : m;
+        ks[k] = a <= L ? taps[a] : 0.0f;
+    }
+    float acc[FIR_J];
+#pragma unroll
+    for (int j = 0; j < FIR_J; ++j) acc[j] = 0.0f;
+    const int tj0 = t0 + tg * FIR_J;  // this lane's first output
+    const int w0 = t0 - L, w1 = t0 + FIR_T + L;  // input window [w0, w1)
+    for (int c0 = w0; c0 < w1; c0 += FIR_C) {
+        __syncthreads();  // taps staged / previous chunk consumed
+        for (int k = threadIdx.x; k < FIR_C * 64; k += 256) {
+            const int row = k / 64, col = k % 64;
+            int idx = c0 + row;  // circular index on N, zero past the trace
+            if (idx < 0) idx += N;
+            if (idx >= N) idx -= N;
+            const int rr = r0 + col;
+            float v = 0.0f;
+            if (c0 + row < w1 && idx >= 0 && idx < ns && rr < nr)
+                v = X[(int64_t)idx * sh.rec_stride + rr];
+            xs[row * 64 + col] = v;
+        }
+        __syncthreads();
+        // kw[j] = k[tj0 + j - tau] for tau = c0 + c (sliding one tap per row)
+        float kw[FIR_J];
+        const int base = M + tj0 - c0;
+#pragma unroll
+        for (int j = 0; j < FIR_J; ++j) kw[j] = ks[base + j];
+        for (int c = 0; c < FIR_C; ++c) {
+            const float xv = xs[c * 64 + lane];
+#pragma unroll
+            for (int j = 0; j < FIR_J; ++j) acc[j] = fmaf(kw[j], xv, acc[j]);
+#pragma unroll
+            for (int j = FIR_J - 1; j > 0; --j) kw[j] = kw[j - 1];
+            kw[0] = ks[base - c - 1];
+        }
+    }
+    if (r >= nr) return;
+#pragma unroll
+    for (int j = 0; j < FIR_J; ++j)
+        if (tj0 + j < ns) Y[(int64_t)(tj0 + j) * sh.rec_stride + r] = acc[j];
+}
+
+void launch_fir(int nshots, const pyfwi_shot *shots, const pyfwi_shot *h_shots, const float *in,
+                float *out, int comp_mask, const float *taps, int L, int pad, hipStream_t st)
+{
+    int ns_max = 0, nr_max = 0;
+    for (int k = 0; k < nshots; ++k) {
+        ns_max = std::max(ns_max, h_shots[k].n_samples);
+        nr_max = std::max(nr_max, h_shots[k].n_rec);
+    }
+    if (ns_max == 0 || nr_max == 0) return;
+    const int M = L + FIR_T + FIR_C;
+    const size_t lds = (((size_t)(2 * M + 1) + 63) / 64 * 64 + (size_t)FIR_C * 64) * sizeof(float);
+    dim3 grid((ns_max + FIR_T - 1) / FIR_T, nshots * 5, (nr_max + 63) / 64);
+    hipLaunchKernelGGL(k_fir, grid, dim3(256), lds, st, shots, in, out, comp_mask, taps, L, pad);
+}
+
 // ------------------------------------------------------------- utilities
 
 // Copy `count` floats per shot (count % 4 == 0, 16-byte aligned bases).

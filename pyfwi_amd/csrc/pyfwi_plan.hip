@@ -123,6 +123,9 @@ struct pyfwi_plan {
     int tune_group = 0, tune_fused = 0, tune_rows = 0;
     bool ckpts_all = false;  // ckpts holds every shot of the last forward
     DevBuf src_stage, seis_stage, res_stage, grad_stage, shot_tab_f, resid;
+    // record filter of the misfit (pyfwi_set_record_filter): taps k[0..L]
+    DevBuf fir_taps, seis_f, obs_f, resid_raw;
+    int fir_L = -1, fir_pad = 0;
     double *d_misfit = nullptr;
     int *d_check = nullptr;  // two ints for the model check
     // hipGraph replay of the time loops (pyfwi_plan_set_graphs)
@@ -502,6 +505,8 @@ int32_t pyfwi_plan_destroy(pyfwi_plan *p)
 {
     if (!p) return 0;
     (void)hipStreamSynchronize(p->stream);
+    for (DevBuf *b : {&p->fir_taps, &p->seis_f, &p->obs_f, &p->resid_raw})
+        b->release();
     for (DevBuf *b : {&p->lam, &p->mu, &p->bu, &p->dxp_buf, &p->dzp_buf, &p->scratch_in, &p->fields, &p->fields2,
                       &p->adj, &p->av0, &p->av1, &p->gacc, &p->snaps, &p->ckpts, &p->seg,
                       &p->src_stage, &p->seis_stage, &p->res_stage, &p->grad_stage,
@@ -1257,6 +1262,11 @@ int32_t pyfwi_misfit_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t ns
         d_grad = p->grad_stage.p;
     }
     if (int e = ensure(p->resid, nrec, p->stream, "residuals")) return e;
+    const bool filtered = p->fir_L >= 0;
+    if (filtered) {
+        for (DevBuf *b : {&p->seis_f, &p->obs_f, &p->resid_raw})
+            if (int e = ensure(*b, nrec, p->stream, "filtered records")) return e;
+    }
     if (!p->d_misfit) HIPCHK(hipMalloc(&p->d_misfit, sizeof(double)));
     HIPCHK(hipMemsetAsync(p->d_misfit, 0, sizeof(double), p->stream));
     launch_zero(d_seis, (int64_t)nrec, p->stream);
@@ -1300,8 +1310,23 @@ int32_t pyfwi_misfit_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t ns
         if (every) every_step(nt_pass, cl_pass);
         if (int e = forward_pass(p, g0, ng, d_src, d_seis, cl_pass, p->snaps.p, nullptr, 0, 1))
             return e;
-        launch_l2_residual(ng, p->d_shots + g0, d_seis, d_obs, p->resid.p, comp_mask, p->d_misfit,
-                           p->stream);
+        if (filtered) {
+            // the misfit of the lowpassed records; its adjoint source is the
+            // lowpassed residual (the filter is symmetric)
+            const pyfwi_shot *dsp = p->d_shots + g0, *hsp = p->h_shots.data() + g0;
+            launch_fir(ng, dsp, hsp, d_seis, p->seis_f.p, comp_mask, p->fir_taps.p, p->fir_L,
+                       p->fir_pad, p->stream);
+            launch_fir(ng, dsp, hsp, d_obs, p->obs_f.p, comp_mask, p->fir_taps.p, p->fir_L,
+                       p->fir_pad, p->stream);
+            launch_l2_residual(ng, dsp, p->seis_f.p, p->obs_f.p, p->resid_raw.p, comp_mask,
+                               p->d_misfit, p->stream);
+            launch_zero(p->resid.p, (int64_t)nrec, p->stream);
+            launch_fir(ng, dsp, hsp, p->resid_raw.p, p->resid.p, comp_mask, p->fir_taps.p,
+                       p->fir_L, p->fir_pad, p->stream);
+        } else {
+            launch_l2_residual(ng, p->d_shots + g0, d_seis, d_obs, p->resid.p, comp_mask,
+                               p->d_misfit, p->stream);
+        }
         const int nsets = adj_acc_sets(g.sdo, ng);
         if (int e = ensure(p->gacc, (size_t)nsets * 6 * g.fstride, p->stream, "gradient"))
             return e;
@@ -1370,6 +1395,28 @@ int32_t pyfwi_forward_noalloc(pyfwi_plan *p, const pyfwi_shot *shots, int32_t ns
                    p->stream, s_in, p->src_inside && p->rec_inline ? seis : nullptr);
     }
     HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int32_t pyfwi_set_record_filter(pyfwi_plan *p, const double *taps, int32_t half_len,
+                                int32_t pad)
+{
+    if (!p) return fail(PYFWI_EINVAL, "null plan");
+    if (half_len < 0) {
+        p->fir_L = -1;
+        return 0;
+    }
+    if (!taps || pad < 0 || half_len > PYFWI_FIR_MAX_HALF)
+        return fail(PYFWI_EINVAL, "record filter: taps needed, pad >= 0, half length <= " +
+                                      std::to_string(PYFWI_FIR_MAX_HALF));
+    std::vector<float> t(taps, taps + half_len + 1);
+    for (float v : t)
+        if (!std::isfinite(v)) return fail(PYFWI_ENONFINITE, "record filter holds NaN or Inf");
+    HIPCHK(hipStreamSynchronize(p->stream));  // earlier calls may still read the old taps
+    if (int e = ensure(p->fir_taps, t.size(), p->stream, "record filter")) return e;
+    HIPCHK(hipMemcpy(p->fir_taps.p, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+    p->fir_L = half_len;
+    p->fir_pad = pad;
     return 0;
 }
 

@@ -11,11 +11,12 @@ Structure of this module:
   evaluated once per point for the optimiser's separate f and f' calls;
 - ``OPTIMIZERS``: the two scipy solvers and their settings;
 - ``FWI.data_term``: forward modelling, cost and adjoint gradient of the
-  whole model on the GPU.  When the band needs no lowpass, the cost is L2 with
-  unit data weight and the observed data are as the plan expects them, it is
-  one ``pyfwi_misfit_gradient`` call (records, residual, misfit and gradient
-  stay on the device); otherwise forward_modeling -> cost_seismic ->
-  gradient.  Both give the same misfit and gradient.
+  whole model on the GPU.  When the cost is L2 with unit data weight and the
+  observed data are as the plan expects them, it is one
+  ``pyfwi_misfit_gradient`` call (records, residual, misfit and gradient
+  stay on the device, a lowpass band included); otherwise
+  forward_modeling -> cost_seismic -> gradient.  Both give the same misfit
+  and gradient.
 
 The driver runs the frequency bands in order and, in each band, the windows
 k_0 .. k_end in steps of n_params (fwi.py:109-158, 174-227).
@@ -254,7 +255,7 @@ class FWI(WavePropagator):
         forward_modeling, cost_seismic (multiscale lowpass, cost function,
         data weight) and gradient."""
         if self.device_misfit and self.device_misfit_applies(d_obs, freq, self.sd, self.CF):
-            return self.misfit_gradient_device(model_dv, d_obs)
+            return self.misfit_gradient_device(model_dv, d_obs, freq=freq)
         d_est = prepare_residual(self.forward_modeling(model_dv, show=False), self.sd)
         rms, adj_src = tools.cost_seismic(d_est, d_obs, fun=self.CF, fn=self.fn, freq=freq,
                                           order=3, axis=1)

@@ -422,6 +422,26 @@ def _zero_phase(x1, highcut, fn, order, first_conj):
     return np.fft.ifft(spec, n=n, axis=1).real[:, :-_PAD, :]
 
 
+def lowpass_taps(highcut, fn, order, n, pad=_PAD, tol=1e-12):
+    """Taps k[0..L] of the symmetric kernel of lowpass / adj_lowpass on traces
+    of n samples: both multiply the (n + pad)-point spectrum of the padded
+    trace by |H|^2 (H and conj(H) in either order), a circular convolution
+    with k = ifft(|H|^2), real and even.  L is the last lag up to half the
+    circle where |k| exceeds tol * k[0].  For the device filter of
+    pyfwi_misfit_gradient (pyfwi_set_record_filter), which sums lags -L..L:
+    on an even circle lags +-N/2 are one residue, so that tap is halved."""
+    N = int(n) + int(pad)
+    h = _butter_response(highcut, fn, order, N)
+    k = np.fft.ifft(np.abs(h) ** 2).real
+    half = N // 2
+    big = np.nonzero(np.abs(k[:half + 1]) > tol * abs(k[0]))[0]
+    L = int(big[-1]) if big.size else 0
+    taps = np.array(k[:L + 1], np.float64)
+    if N % 2 == 0 and L == half:
+        taps[L] *= 0.5
+    return taps
+
+
 def lowpass(x1, highcut, fn, order=1, axis=1, show=False):
     """Zero-phase Butterworth lowpass along time of records [ncomp, nt, ntr]:
     pad 512 zero samples, multiply the spectrum by H and then by conj(H), and

@@ -260,7 +260,7 @@ class TimeLapse(WavePropagator):
         propagator (as the reference's tl_fprime)."""
         inv = self.inv_obj
         if self.device_misfit and self.device_misfit_applies(d_obs, freq, inv.sd, self.CF):
-            return self.misfit_gradient_device(model_dv, d_obs)
+            return self.misfit_gradient_device(model_dv, d_obs, freq=freq)
         dpre = prepare_residual(self.forward_modeling(model_dv, show=False), inv.sd)
         rms, adj = tools.cost_seismic(d_pre0=dpre, d_obs0=d_obs, fun=self.CF, fn=self.fn,
                                       freq=freq, order=3, axis=1, sourc_weight=None, ns=self.ns,

@@ -435,11 +435,11 @@ class WavePropagator(WavePreparation):
     # ------------------------------------------------ one-call misfit + gradient
     def device_misfit_applies(self, d_obs, freq=None, sd=1.0, cost=None):
         """True when ``cost_seismic`` followed by ``gradient`` reduces to
-        pyfwi_misfit_gradient: no lowpass band, L2 cost, unit data weight,
-        all shots in one batch, and for the pressure (components=0) equal
-        'taux' and 'tauz' observations (the plan compares the pressure with
-        one record)."""
-        if freq or self.per_shot:
+        pyfwi_misfit_gradient: L2 cost, unit data weight, all shots in one
+        batch, and for the pressure (components=0) equal 'taux' and 'tauz'
+        observations (the plan compares the pressure with one record).  A
+        lowpass band runs on the device too (pyfwi_set_record_filter)."""
+        if self.per_shot:
             return False
         if cost is not None and cost.cost_function_type != "l2":
             return False
@@ -459,11 +459,15 @@ class WavePropagator(WavePreparation):
             hit = cache[id(d_obs)] = (d_obs, np.ascontiguousarray(rec))
         return hit[1]
 
-    def misfit_gradient_device(self, model0, d_obs, parameterization="dv"):
+    def misfit_gradient_device(self, model0, d_obs, parameterization="dv", freq=None):
         """L2 misfit of ``model0`` against ``d_obs`` and its gradient in one
         pyfwi_misfit_gradient call; the same values as forward_modeling,
-        CostFunction.l2 and gradient (fwi.py:229-252).  Updates
+        cost_seismic (L2, lowpass band ``freq`` with the order-3 filter of
+        fwi.py:239-240) and gradient (fwi.py:229-252).  Updates
         ``seismogram``; no stand-alone ``gradient`` may follow."""
+        from .fwi_tools import lowpass_taps
+        self._plan.set_record_filter(
+            lowpass_taps(freq, self.fn, 3, int(self.nt_ext)) if freq else None)
         model = {k: expand_model(np.asarray(v, np.float32), int(self.tnz), int(self.tnx),
                                  self.npml) for k, v in model0.items()}
         self.pml_preparation(model["vp"].max())

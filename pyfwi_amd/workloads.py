@@ -187,23 +187,35 @@ def differentiation(seed=3, ns=10, nt=2500):
                     chpr=0, meta={"nt": nt, "shots": ns, "seed": seed})
 
 
+# the time-lapse reservoir patch of large_step: 200 x 600 cells
+LARGE_STEP_PATCH = (slice(1100, 1300), slice(2700, 3300))
+
+
 def large_step(seed=5, ns=32, nt=6000):
     """nz=2000, nx=6000, dh=2.5 m, random layers with 5 anomalies, 32 shots,
-    nt=6000, 15 Hz, 3000 receivers."""
+    nt=6000, 15 Hz, 3000 receivers.  ``model`` holds the layers with the
+    anomalies (the baseline); ``model_obs`` the same (observed data of the
+    baseline); ``meta['initial']`` the layers alone (the inversions' start)
+    and ``meta['monitor']`` the baseline with vp -8 % in the 200 x 600-cell
+    patch LARGE_STEP_PATCH."""
     dh = 2.5
     rng = np.random.default_rng(seed + 1000)
-    model = random_layered(2000, 6000, 20, seed)
+    layers = random_layered(2000, 6000, 20, seed)
+    model = layers
     for _ in range(5):
         model = gaussian_anomaly(model, rng.uniform(300, 1700), rng.uniform(500, 5500),
                                  rng.uniform(30, 80), rng.choice([-0.1, 0.1]))
+    monitor = {k: v.copy() for k, v in model.items()}
+    monitor["vp"][LARGE_STEP_PATCH] *= np.float32(0.92)
     dt = _cfl_dt(model, dh)
     inpa = _base_inpa(dh, dt, nt)
     inpa["rec_dis"] = 2 * dh
     src_loc, rec_loc = _surface_geometry(ns, 6000, dh, 3000, 2, 4, 2, 0)
     src = Source(src_loc, dh, dt)
     src.Ricker(15)
-    return Workload("large_step_2000x6000", inpa, src, rec_loc, model, None, components=0,
-                    chpr=0, meta={"nt": nt, "shots": ns, "seed": seed})
+    return Workload("large_step_2000x6000", inpa, src, rec_loc, model, model, components=0,
+                    chpr=0, meta={"nt": nt, "shots": ns, "seed": seed, "initial": layers,
+                                  "monitor": monitor})
 
 
 def subset(w, ids, nt=None):

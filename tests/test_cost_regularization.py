@@ -214,3 +214,27 @@ def test_source_weighting_and_model_vectors():
     back = T.vec2vel_dict(v, 5, 7)
     for k in m:
         np.testing.assert_array_equal(back[k], m[k])
+
+
+@pytest.mark.parametrize("freq,nt", [(40.0, 300), (15.0, 700), (5.0, 257), (5.0, 256)])
+def test_lowpass_is_a_symmetric_circular_fir(freq, nt):
+    """lowpass and adj_lowpass (zero-pad 512, multiply the spectrum by H and
+    conj(H)) equal the circular convolution with the taps of lowpass_taps,
+    y[t] = sum_{|m| <= L} k[|m|] x[(t - m) mod N] -- what the device record
+    filter computes -- to 1e-10 of the output."""
+    from pyfwi_amd import fwi_tools as T
+    fn = 1 / 2 / 0.0006
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((2, nt, 7))
+    k = T.lowpass_taps(freq, fn, 3, nt)
+    N, L = nt + 512, len(k) - 1
+    assert 2 * L <= N
+    m = np.arange(-L, L + 1)
+    taps = k[np.abs(m)]
+    y = np.zeros_like(x)
+    for t in range(nt):
+        idx = (t - m) % N
+        ok = idx < nt
+        y[:, t, :] = np.einsum("m,cmr->cr", taps[ok], x[:, idx[ok], :])
+    for ref in (T.lowpass(x, freq, fn, order=3), T.adj_lowpass(x, freq, fn, order=3)):
+        assert np.abs(y - ref).max() < 1e-10 * np.abs(ref).max()
