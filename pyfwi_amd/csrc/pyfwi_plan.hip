@@ -1176,7 +1176,11 @@ int32_t pyfwi_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, c
         if (p->seg.n > (size_t)G * K * fps) p->seg.release();
     }
     p->last_passes.clear();
-    for (int g0 = 0; g0 < nshots; g0 += G) {
+    // exact mode sized to memory: the passes' shots spread evenly (at most G)
+    const bool even = mode == PYFWI_MODE_EXACT && p->tune_group == 0;
+    const int npass = (nshots + G - 1) / G;
+    for (int g0 = 0, k = 0; g0 < nshots; g0 += G, ++k) {
+        if (even) G = nshots / npass + (k < nshots % npass ? 1 : 0);
         const int ng = std::min(G, nshots - g0);
         p->last_passes.push_back(ng);
         const int nsets = adj_acc_sets(g.sdo, ng);

@@ -314,3 +314,44 @@ def test_loop_timing(gpu):
     with pytest.raises(_native.NativeError):
         W._plan.set_timing(3)
     W._plan.set_timing(0)
+
+
+def test_exact_schedule_sized_to_memory(gpu):
+    """The exact adjoint with the automatic two-level schedule (forward with
+    PYFWI_CKPT_AUTO, gradient with interval 0: checkpoints every
+    ceil(sqrt(nt)) steps, kept by the forward, passes from the free memory)
+    equals the one with checkpoints every 50 steps recomputed by the gradient
+    -- the recomputed states are the forward's bit for bit, so the two differ
+    only by the order of the shot groups' sums -- and reports its schedule."""
+    import math
+    import torch
+    from pyfwi_amd import _native
+    from pyfwi_amd.fwi_tools import expand_model
+    w = _small(nt=400)
+    W = WavePropagator(w.inpa, w.src, w.rec_loc, w.model_shape, components=4, chpr=0)
+    m = {k: expand_model(v, int(W.tnz), int(W.tnx), W.npml) for k, v in w.model.items()}
+    W.pml_preparation(m["vp"].max())
+    W.elastic_buffers(m)
+    plan = W._plan
+    plan.set_stream(torch.cuda.current_stream().cuda_stream)
+    dev = torch.device("cuda:0")
+    shots = W._shots(list(range(int(W.ns))))
+    src = torch.from_numpy(W._source_table()).to(dev)
+    n = int(W.ns) * 5 * int(W.nt_ext) * int(W.nr)
+    seis = torch.zeros(n, device=dev)
+    grads = []
+    for fwd_ck, grad_ck in ((_native.CKPT_AUTO, 0), (0, 50)):
+        plan.forward(shots, src, seis, None, fwd_ck, _native.MEM_DEVICE)
+        res = seis * 0.3
+        g = torch.zeros(6 * int(W.tnz) * int(W.tnx), device=dev)
+        plan.gradient(shots, src, res, g, _native.MODE_EXACT, grad_ck, 0, _native.MEM_DEVICE)
+        torch.cuda.synchronize()
+        passes, k = plan.last_passes(with_interval=True)
+        print(f"exact schedule: forward ckpt {fwd_ck}, gradient ckpt {grad_ck}: passes {passes}, "
+              f"interval {k}", flush=True)
+        assert k == (math.ceil(math.sqrt(int(W.nt))) if grad_ck == 0 else grad_ck)
+        assert sum(passes) == int(W.ns)
+        grads.append(g.cpu().numpy())
+    e = rel(grads[0], grads[1])
+    print(f"  automatic vs every-50 schedule: {e:.2e}")
+    assert np.isfinite(grads[0]).all() and e < 1e-6
