@@ -110,19 +110,6 @@ def measured_traffic(workload_name, kernel, shots=None):
     return None
 
 
-def measured_traffic_mix(workload_name, kernel, launches_by_pass_size):
-    """Measured HBM bytes per dispatch of `kernel` averaged over a mix of
-    launches {pass size: launches}, from the counter summary's per-pass-size
-    figures; None unless every pass size of the mix was measured."""
-    for d in _traffic_records(workload_name):
-        per = d.get("bytes_per_dispatch_by_pass_size", {})
-        vals = {int(k): v.get(kernel) for k, v in per.items()}
-        if all(vals.get(n) is not None for n in launches_by_pass_size):
-            total = sum(launches_by_pass_size.values())
-            return sum(vals[n] * c for n, c in launches_by_pass_size.items()) / total
-    return None
-
-
 def build_problem(workload, nt=None, shots=None, order=0):
     from pyfwi_amd import workloads
     from pyfwi_amd.wave_propagation import WavePropagator
@@ -342,18 +329,19 @@ def pass_kernel_bytes(kind, pass_nts, cells, interior, order=4):
     runs them (decreasing).  A forward launch of step t advances the shots
     with t < nt_s, an adjoint launch of step t (nt-1 .. 1) those with
     1 <= t < nt_s; the accumulators move once per block's group with a live
-    shot.  Shots per block follow the pass size, as the library picks them."""
+    shot.  Each launch takes only the live shots (the plan's time-major kept
+    states), so shots per block follow the live count, as the library picks
+    them."""
     a, b, c = KERNEL_BYTES[kind]
     adjoint = kind.startswith("adj") or kind.startswith("exact")
     total = 0.0
     launches = 0
     for nts in pass_nts:
         nts = np.sort(np.asarray(nts, np.int64))[::-1]
-        ng = len(nts)
-        spb = shots_per_block(order, ng)
         tmax = int(nts[0])
         t = np.arange(1 if adjoint else 0, tmax)
         live = (nts[None, :] > t[:, None]).sum(1)  # shots live at each launch
+        spb = np.where((order == 4) & (live % 5 == 0) & (live % 4 != 0), 5, 4)
         groups = -(-live // spb)
         imaged = cells if kind.startswith("exact") else interior
         total += float(((a * cells + b * interior) * live).sum() + (c * imaged * groups).sum())
@@ -460,12 +448,13 @@ def bench_uneven(args, torch):
                           "config": {"workload": wl.name, "passes": passes},
                           "note": "--minimal: timed steps only"}), flush=True)
         return
-    order_nt = sorted(B.nt, reverse=True)  # the plan runs the shots longest first
+    # the passes take consecutive runs of the plan's shot order
+    order_ids = plan.last_order()
     pass_nts, k = [], 0
     for n in passes:
-        pass_nts.append(order_nt[k:k + n])
+        pass_nts.append([int(B.nt[j]) for j in order_ids[k:k + n]])
         k += n
-    log(f"bench: passes {passes}")
+    log(f"bench: passes {passes}, nt {pass_nts}")
 
     # dominant kernel: its launches' algorithmic bytes over their durations
     cells = B.tnz * B.tnx
@@ -484,11 +473,13 @@ def bench_uneven(args, torch):
     dom = max(prof, key=lambda k: prof[k][0])
     log(f"bench: kernels {kernels}")
     achieved = kernels[dom]["GBps"]
-    # launches of the dominant kernel by the size of their pass
+    # launches of the dominant kernel by the number of shots they advance
     launch_mix = {}
+    t0_dom = 1 if dom.startswith("adj") else 0
     for nts in pass_nts:
-        n = max(nts) - (1 if dom.startswith("adj") else 0)
-        launch_mix[len(nts)] = launch_mix.get(len(nts), 0) + n
+        for t in range(t0_dom, max(nts)):
+            live = sum(1 for x in nts if x > t)
+            launch_mix[live] = launch_mix.get(live, 0) + 1
 
     # forward modelling alone: all shots, records only
     no_chp = np.zeros(0, np.int32)
@@ -548,16 +539,17 @@ def bench_uneven(args, torch):
                    "fdom_range": [round(min(s.fdom for s in wl.specs), 2),
                                   round(max(s.fdom for s in wl.specs), 2)],
                    "gradient": "reference adjoint, every forward state kept in HBM (chpr=100), "
-                               "passes of shots sized to the free HBM, longest shots first; "
-                               "L2 misfit of vx, vz on the GPU",
+                               "shots packed first-fit by decreasing nt into passes whose "
+                               "kept states fill the free HBM; L2 misfit of vx, vz on the GPU",
                    "passes": passes, "cell_updates": units, "graphs": bool(graphs),
                    "parallelism": "single GPU, shots batched per launch"},
         "kernel_timing": "per-step time of each time loop (HIP events around the loops) split "
                          "between its kernels by event-bracketed launches",
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic_mix(wl.name, dom, launch_mix),
+                     "traffic": measured_traffic(wl.name, dom),
                      "bytes_per_launch": kernels[dom]["bytes_per_launch"],
+                     "launches_by_live_shots": {str(k): v for k, v in sorted(launch_mix.items())},
                      "avg_launch_us": kernels[dom]["avg_us"]},
         "gradient_roofline_frac": round(value * 148.0 / HBM_PEAK_GBS, 4),
         "forward_only": {"value": round(fwd_rate, 3), "unit": "Gcell-updates/s",
@@ -570,11 +562,8 @@ def bench_uneven(args, torch):
 
 
 def _inversion_setup(args, torch):
-    """The workload with observed data of its baseline (and monitor) model,
-    and the initial model; the wavelet and the data are scaled so that the
-    gradient at the initial model peaks near 1e2, above the L-BFGS stopping
-    thresholds of the reference's settings (the problem is linear in the
-    source amplitude: misfit and gradient scale with its square)."""
+    """The workload with observed data of its baseline (and, for the
+    time-lapse pair, monitor) model, and the initial model."""
     from pyfwi_amd import workloads
     from pyfwi_amd.wave_propagation import WavePropagator
     w = getattr(workloads, args.workload)(**({"ns": args.shots} if args.shots else {}))
@@ -599,98 +588,170 @@ def _inversion_setup(args, torch):
     return w, obs, m0
 
 
-def _scale_problem(fwi, obs, m0, w):
-    """Evaluate the misfit gradient at m0 once (timed), then scale the
-    wavelet and the observed data so that the gradient peaks near 1e2."""
+KMS = 1000.0  # the inversions' unknowns: vp, vs in km/s and rho in g/cm^3
+
+
+def km_s_parameters():
+    """param_functions (the reference's hook for other parameterisations,
+    fwi.py:52-66) whose unknowns are vp, vs and rho in units of 1000 (km/s,
+    g/cm^3): of order one, so that the L-BFGS first step (1/|g|) is of the
+    model's scale.  Models given to and returned by the drivers are in these
+    units; the physics and the misfit are unchanged and the gradient follows
+    the chain rule."""
     from pyfwi_amd import fwi_tools as T
+
+    def model_to_dv(m, args):
+        return {k: np.float32(v * KMS) for k, v in m.items()}
+
+    def grad_from_dv(g, args, m):
+        return {k: v * KMS for k, v in g.items()}
+    return {"dict2vec": T.vel_dict2vec, "vec2dict": T.vec2vel_dict,
+            "model_to_dv": model_to_dv, "grad_from_dv": grad_from_dv}
+
+
+def _in_km_s(m):
+    return {k: np.float32(v / KMS) for k, v in m.items()}
+
+
+class _Budget(Exception):
+    """The run's time budget is spent; the results so far are reported."""
+
+
+def _instrument(fwi, m0, obs, budget, t_start):
+    """Evaluate the misfit gradient at m0 once and scale the wavelet and the
+    observed data (the problem is linear in the source amplitude: misfit and
+    gradient scale with its square) so that |g| = 0.03 in the km/s unknowns,
+    which puts the L-BFGS first step at about a hundredth of the model's
+    norm; the scaled first evaluation is then served to the optimiser.  Every
+    later evaluation is timed and logged; past `budget` seconds (from
+    t_start) the run stops.  Returns (evaluations [(seconds, misfit)],
+    scale)."""
+    x0 = np.asarray(fwi.dict2vec(m0))
     t0 = time.perf_counter()
-    rms, g = fwi.fprime(np.float32(T.vel_dict2vec(m0)), None)
-    dt = time.perf_counter() - t0
-    gmax = float(np.abs(g).max())
-    a = float(np.sqrt(1e2 / gmax)) if gmax > 0 else 1.0
+    rms, g = fwi.fprime(np.float32(x0), None)
+    t_first = time.perf_counter() - t0
+    norm = float(np.linalg.norm(g))
+    a = float(np.sqrt(0.03 / norm)) if norm > 0 else 1.0
     fwi.src.w = np.float32(fwi.src.w * a)
-    for d in obs.values():
+    for d in obs:
         for k in d:
             d[k] *= np.float32(a)
     fwi.__dict__.pop("_obs_cache", None)
-    return rms, gmax, a, dt
-
-
-def bench_lbfgs(args, torch):
-    """L-BFGS-B inversion (the FWI driver, reference settings) of the
-    workload's observed data from its initial model, vp only, one band with
-    no lowpass: every misfit evaluation is one pyfwi_misfit_gradient call of
-    all shots (reference adjoint, the workload's checkpoint list)."""
-    from pyfwi_amd.fwi import FWI
-    w, obs, m0 = _inversion_setup(args, torch)
-    fwi = FWI(obs["base"], w.inpa, w.src, w.rec_loc, w.model_shape, components=w.components,
-              chpr=w.chpr)
-    fwi.verbose = False
-    rms0, gmax, a, t_eval = _scale_problem(fwi, {"base": fwi.d_obs}, m0, w)
-    log(f"bench: first gradient {t_eval:.1f} s (misfit {rms0:.4e}, |g| {gmax:.3e}); "
-        f"wavelet and data scaled by {a:.3e}")
-    evals = []
+    first = (np.float32(x0), rms * a * a, g * (a * a))
+    evals = [(t_first, float(first[1]))]
+    log(f"bench: first gradient {t_first:.1f} s (misfit {rms:.4e}, |g| {norm:.3e}); scaled "
+        f"by {a:.3e}")
     inner = fwi.fprime
 
-    def timed_fprime(m, freq):
+    base = obs[0]
+
+    def fprime(m, freq):
+        if freq is None and fwi.d_obs is base and np.array_equal(np.float32(m), first[0]):
+            return first[1], first[2]
+        if time.perf_counter() - t_start > budget:
+            raise _Budget()
         ts = time.perf_counter()
         out = inner(m, freq)
         evals.append((time.perf_counter() - ts, float(out[0])))
         log(f"bench: evaluation {len(evals)}: misfit {out[0]:.6e}, {evals[-1][0]:.1f} s")
         return out
-    fwi.fprime = timed_fprime
+    fwi.fprime = fprime
+    return evals, a
+
+
+def bench_lbfgs(args, torch):
+    """L-BFGS-B inversion (the FWI driver with the reference's settings) of
+    the workload's observed data from its initial model, vp only, one band
+    with no lowpass, in km/s unknowns: every misfit evaluation is one
+    pyfwi_misfit_gradient call of all shots (reference adjoint, the
+    workload's checkpoint list).  Stops after `--time-budget` seconds."""
+    from pyfwi_amd.fwi import FWI
+    t_start = time.perf_counter()
+    w, obs, m0 = _inversion_setup(args, torch)
+    fwi = FWI(obs["base"], w.inpa, w.src, w.rec_loc, w.model_shape, components=w.components,
+              chpr=w.chpr, param_functions=km_s_parameters())
+    fwi.verbose = False
+    m0 = _in_km_s(m0)
+    evals, a = _instrument(fwi, m0, [fwi.d_obs], args.time_budget, t_start)
+    iters = []
+    from pyfwi_amd import fwi as fwi_mod
+    run = fwi_mod._run_lbfgs
+
+    def counting(f, fp, x0, n):
+        from scipy.optimize import fmin_l_bfgs_b
+        x, fopt, _ = fmin_l_bfgs_b(f, x0, fp, m=10, factr=1e7, pgtol=1e-8, bounds=None,
+                                   maxfun=15000, maxiter=n, maxls=10,
+                                   callback=lambda xk: iters.append(len(evals)))
+        return x, fopt
+    fwi_mod.OPTIMIZERS["lbfgs"] = counting
+    stopped = None
     t0 = time.perf_counter()
-    m_est, hist = fwi(m0, method="lbfgs", iter=[args.iters], freqs=[None], n_params=1, k_0=1,
-                      k_end=2)
+    try:
+        m_est, hist = fwi(m0, method="lbfgs", iter=[args.iters], freqs=[None], n_params=1,
+                          k_0=1, k_end=2)
+        err1 = float(np.linalg.norm(m_est["vp"] * KMS - w.model_obs["vp"]))
+    except _Budget:
+        stopped, err1 = "time budget", None
+    finally:
+        fwi_mod.OPTIMIZERS["lbfgs"] = run
     total = time.perf_counter() - t0
     cells = (w.model_shape[0] + 2 * w.inpa["npml"]) * (w.model_shape[1] + 2 * w.inpa["npml"])
     units = cells * len(w.src.i) * int(fwi.nt)
     t_mean = float(np.mean([e[0] for e in evals]))
-    err0 = float(np.linalg.norm(m0["vp"] - w.model_obs["vp"]))
-    err1 = float(np.linalg.norm(m_est["vp"] - w.model_obs["vp"]))
     print(json.dumps({
         "metric": "L-BFGS inversion: misfit evaluations (forward + L2 misfit + reference "
                   "gradient) of the full workload", "workload": w.name,
         "shots": int(len(w.src.i)), "nt": int(fwi.nt), "cells": cells,
-        "iterations": args.iters, "evaluations": len(evals), "seconds": round(total, 1),
+        "iterations_asked": args.iters, "iterations_done": len(iters),
+        "evaluations_at_iteration_ends": iters, "evaluations": len(evals),
+        "stopped": stopped, "seconds": round(total, 1),
         "seconds_per_evaluation": round(t_mean, 2),
         "gradient_gcell_updates_per_s": round(units / t_mean / 1e9, 3),
-        "misfit_first": evals[0][1] if evals else None, "misfit_last": evals[-1][1] if evals
-        else None, "misfit_history": [e[1] for e in evals],
-        "final_misfit_per_window": np.asarray(hist).ravel().tolist(),
-        "vp_error_l2": {"initial": err0, "final": err1},
-        "scale": a, "checkpoints": f"chpr={w.chpr}"}), flush=True)
+        "misfit_history": [e[1] for e in evals],
+        "vp_error_l2": {"initial": float(np.linalg.norm(m0["vp"] * KMS - w.model_obs["vp"])),
+                        "final": err1},
+        "scale": a, "unknowns": "vp in km/s (param_functions)",
+        "checkpoints": f"chpr={w.chpr}"}), flush=True)
 
 
 def bench_timelapse(args, torch):
     """Time-lapse pair (baseline and monitor observed data, the monitor with
     vp -8 % in a patch) inverted with the cascaded strategy: baseline from
     the initial model, monitor from the baseline result; `--iters` L-BFGS
-    iterations each."""
+    iterations each, km/s unknowns."""
     from pyfwi_amd.tl_fwi import TimeLapse
+    t_start = time.perf_counter()
     w, obs, m0 = _inversion_setup(args, torch)
     tl = TimeLapse(obs["base"], obs["monitor"], w.inpa, w.src, w.rec_loc, w.model_shape,
-                   components=w.components, chpr=w.chpr)
+                   components=w.components, chpr=w.chpr, param_functions=km_s_parameters())
     tl.verbose = tl.inv_obj.verbose = False
-    rms0, gmax, a, t_eval = _scale_problem(tl.inv_obj, {"b": tl.b_dobs, "m": tl.m_dobs}, m0, w)
-    tl.src.w = tl.inv_obj.src.w
-    tl.inv_obj.d_obs = tl.b_dobs
-    log(f"bench: first gradient {t_eval:.1f} s; scaled by {a:.3e}")
+    inv = tl.inv_obj
+    m0 = _in_km_s(m0)
+    inv.d_obs = tl.b_dobs
+    evals, a = _instrument(inv, m0, [tl.b_dobs, tl.m_dobs], args.time_budget, t_start)
+    tl.src.w = inv.src.w
     t0 = time.perf_counter()
-    out, hist = tl(m0, iter=[args.iters], freqs=[None], tl_method="cc", n_params=1, k_0=1,
-                   k_end=2)
+    out, hist, stopped = {}, {}, None
+    try:
+        out, hist = tl(m0, iter=[args.iters], freqs=[None], tl_method="cc", n_params=1, k_0=1,
+                       k_end=2)
+    except _Budget:
+        stopped = "time budget"
     total = time.perf_counter() - t0
-    dm = out["dm"]["vp"]
-    patch = (w.meta["monitor"]["vp"] != w.model_obs["vp"])
-    inside = float(dm[patch].mean())
-    outside = float(np.abs(dm[~patch]).mean())
-    print(json.dumps({
-        "metric": "time-lapse pair, cascaded strategy", "workload": w.name,
-        "shots": int(len(w.src.i)), "iterations_per_inversion": args.iters,
-        "seconds": round(total, 1), "misfit_history": {k: np.asarray(v).ravel().tolist()
-                                                       for k, v in hist.items()},
-        "dvp_patch_mean": inside, "dvp_outside_abs_mean": outside,
-        "pt": out.get("pt"), "scale": a}), flush=True)
+    line = {"metric": "time-lapse pair, cascaded strategy", "workload": w.name,
+            "shots": int(len(w.src.i)), "iterations_per_inversion": args.iters,
+            "evaluations": len(evals), "stopped": stopped, "seconds": round(total, 1),
+            "seconds_per_evaluation": round(float(np.mean([e[0] for e in evals])), 2),
+            "misfit_history": [e[1] for e in evals], "scale": a,
+            "unknowns": "vp in km/s (param_functions)"}
+    if out:
+        dm = out["dm"]["vp"] * KMS
+        patch = w.meta["monitor"]["vp"] != w.model_obs["vp"]
+        line.update(dvp_patch_mean=float(dm[patch].mean()),
+                    dvp_outside_abs_mean=float(np.abs(dm[~patch]).mean()),
+                    hist={k: np.asarray(v).ravel().tolist() for k, v in hist.items()},
+                    pt=out.get("pt"))
+    print(json.dumps(line), flush=True)
 
 
 def main():
@@ -728,6 +789,8 @@ def main():
     ap.add_argument("--tune", default="", help="fixed tuning 'shots_per_pass,fused,rows'")
     ap.add_argument("--iters", type=int, default=5,
                     help="L-BFGS iterations of --mode lbfgs / timelapse")
+    ap.add_argument("--time-budget", type=float, default=1e9,
+                    help="--mode lbfgs / timelapse: stop (and report) after this many seconds")
     ap.add_argument("--mode", default="kept",
                     choices=["kept", "reverse", "exact", "lbfgs", "timelapse"],
                     help="kept: every forward state kept in HBM per pass (chpr=100), misfit and "

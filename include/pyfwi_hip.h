@@ -236,10 +236,13 @@ int32_t pyfwi_gradient(pyfwi_plan *plan, const pyfwi_shot *shots,
  * `chp` (as in pyfwi_forward: shots may differ in nt only when chp is every
  * step).  When chp is every step 0..nt-1
  * (chpr = 100) the forward states are kept in HBM for the pass and read
- * directly by the adjoint, so no time reversal is needed.  Shots per pass:
- * the plan's shots_per_pass when set, else as many as the kept states
- * (nchp * 5 fields per shot) of 90 % of the free device memory allow;
- * PYFWI_ENOMEM when not even one shot fits.
+ * directly by the adjoint, so no time reversal is needed; a shot then keeps
+ * nt_s states (5 fields each), and the shots, by decreasing nt, are packed
+ * first-fit into passes whose kept states fill 90 % of the free device
+ * memory (pyfwi_last_order gives the order).  With another checkpoint list
+ * each shot keeps nchp states and the passes take as many shots, in the
+ * order given, as that memory holds.  With the plan's shots_per_pass set,
+ * passes of that many shots.  PYFWI_ENOMEM when not even one shot fits.
  *   obs        observed records, layout of `seis` (rec_offset per shot)
  *   seis       optional output of the modelled records (may be NULL)
  *   grad       output [6][tnz*tnx] as pyfwi_gradient
@@ -259,10 +262,10 @@ int32_t pyfwi_misfit_gradient(pyfwi_plan *plan, const pyfwi_shot *shots, int32_t
  * convolution y[t] = sum_{|m| <= L} k[|m|] x[(t - m) mod N] with the
  * symmetric kernel k = ifft(|H|^2), whose taps k[0..half_len] the caller
  * computes (pyfwi_amd.fwi_tools.lowpass_taps).  With a filter set,
- * pyfwi_misfit_gradient filters the modelled and the observed records,
- * takes the misfit of the filtered records and injects the filtered
- * residual (the filter is its own adjoint); the `seis` output stays
- * unfiltered.  half_len < 0 removes the filter; half_len at most
+ * pyfwi_misfit_gradient takes the misfit of the filtered records,
+ * K seis - K obs, computed as K (seis - obs) since K is linear, and injects
+ * K applied to that filtered residual (the filter is its own adjoint); the
+ * `seis` output stays unfiltered.  half_len < 0 removes the filter; half_len at most
  * PYFWI_FIR_MAX_HALF. */
 enum { PYFWI_FIR_MAX_HALF = 6000 };
 int32_t pyfwi_set_record_filter(pyfwi_plan *plan, const double *taps, int32_t half_len,
@@ -300,14 +303,24 @@ int32_t pyfwi_get_snapshot(pyfwi_plan *plan, int32_t shot, int32_t slot,
 
 /* Shots per pass of the last pyfwi_forward, pyfwi_gradient or
  * pyfwi_misfit_gradient call, in the order they ran (no counterpart in the
- * reference, which runs one shot at a time).  pyfwi_misfit_gradient runs the
- * shots by decreasing nt (stable order) when the checkpoint list is every
- * step; the other calls in the order given.  `sizes` receives up to `cap`
+ * reference, which runs one shot at a time).  The passes take consecutive
+ * runs of the order pyfwi_last_order gives.  `sizes` receives up to `cap`
  * entries, `count` the number of passes, `ckpt_every` (may be NULL) the
  * checkpoint interval of an exact-mode gradient or of the forward's kept
  * checkpoints (0: none). */
 int32_t pyfwi_last_passes(pyfwi_plan *plan, int32_t *sizes, int32_t cap, int32_t *count,
                           int32_t *ckpt_every);
+
+/* The caller's shot indices (0-based positions in the `shots` argument) of
+ * the last call, in the order its passes ran them (no counterpart in the
+ * reference).  pyfwi_forward and pyfwi_gradient run the shots in the order
+ * given.  pyfwi_misfit_gradient with every step kept orders them by
+ * decreasing nt and packs them into passes first-fit (each shot joins the
+ * first pass whose kept states have room for its nt states), a pass's shots
+ * again by decreasing nt; with a tuned pass size (pyfwi_plan_set_tuning),
+ * consecutive runs of that order.  `order` receives up to `cap` entries,
+ * `count` the number of shots. */
+int32_t pyfwi_last_order(pyfwi_plan *plan, int32_t *order, int32_t cap, int32_t *count);
 
 /* Wait for all work queued on the plan's stream. */
 int32_t pyfwi_synchronize(pyfwi_plan *plan);

@@ -34,7 +34,7 @@ EXPORTS = [
     "pyfwi_plan_set_stream", "pyfwi_set_receivers", "pyfwi_set_model", "pyfwi_forward",
     "pyfwi_forward_noalloc", "pyfwi_gradient",
     "pyfwi_misfit_gradient", "pyfwi_set_record_filter", "pyfwi_plan_set_tuning", "pyfwi_plan_set_graphs",
-    "pyfwi_graph_stats", "pyfwi_get_snapshot", "pyfwi_last_passes",
+    "pyfwi_graph_stats", "pyfwi_get_snapshot", "pyfwi_last_passes", "pyfwi_last_order",
     "pyfwi_synchronize", "pyfwi_plan_set_timing", "pyfwi_kernel_timing",
 ]
 
@@ -76,7 +76,7 @@ def load(path=None):
     global _lib
     if _lib is not None:
         return _lib
-    path = path or LIB_PATH
+    path = path or os.environ.get("PYFWI_LIB") or LIB_PATH  # PYFWI_LIB: another build
     if not os.path.exists(path):
         raise RuntimeError(
             f"{path} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
@@ -108,6 +108,8 @@ def load(path=None):
     lib.pyfwi_last_passes.argtypes = [P, ctypes.POINTER(ctypes.c_int32), i32,
                                       ctypes.POINTER(ctypes.c_int32),
                                       ctypes.POINTER(ctypes.c_int32)]
+    lib.pyfwi_last_order.argtypes = [P, ctypes.POINTER(ctypes.c_int32), i32,
+                                     ctypes.POINTER(ctypes.c_int32)]
     lib.pyfwi_synchronize.argtypes = [P]
     lib.pyfwi_plan_set_timing.argtypes = [P, i32]
     lib.pyfwi_kernel_timing.argtypes = [P, i32, ctypes.POINTER(ctypes.c_double),
@@ -304,6 +306,15 @@ class Plan:
                                              ctypes.byref(k)))
         sizes = [int(buf[j]) for j in range(n.value)]
         return (sizes, int(k.value)) if with_interval else sizes
+
+    def last_order(self):
+        """The caller's shot indices of the last call in the order its passes
+        ran them (the passes take consecutive runs of it)."""
+        n = ctypes.c_int32()
+        check(self.lib.pyfwi_last_order(self.h, None, 0, ctypes.byref(n)))
+        buf = (ctypes.c_int32 * max(n.value, 1))()
+        check(self.lib.pyfwi_last_order(self.h, buf, n.value, ctypes.byref(n)))
+        return [int(buf[j]) for j in range(n.value)]
 
     def synchronize(self):
         check(self.lib.pyfwi_synchronize(self.h))

@@ -30,6 +30,9 @@ constexpr int FUSED_WAVES = 4;      // fused two-stage kernels: wavefronts per b
 #ifndef ADJ_VEL_UNROLL
 #define ADJ_VEL_UNROLL 2            // rows per unrolled iteration of the adjoint velocity + imaging kernel
 #endif
+#ifndef ONE_SHOT_PACK
+#define ONE_SHOT_PACK 0             // one-shot launches: waves of a block on adjacent column blocks
+#endif
 #ifndef ADJ_GROUP_ROWS
 #define ADJ_GROUP_ROWS 2            // adjoint kernels: rows per block-wide sum of the imaging terms
 #endif
@@ -45,6 +48,15 @@ inline int adj_acc_sets(int sdo, int nshots)
 {
     const int spb = march_spb(sdo, nshots);
     return (nshots + spb - 1) / spb;
+}
+
+// Accumulator sets for launches of any 1 .. nshots shots (a pass whose
+// launches advance only the shots live at each step).
+inline int adj_acc_sets_upto(int sdo, int nshots)
+{
+    int m = 1;
+    for (int n = 1; n <= nshots; ++n) m = adj_acc_sets(sdo, n) > m ? adj_acc_sets(sdo, n) : m;
+    return m;
 }
 
 // Constants every kernel takes by value.

@@ -282,6 +282,24 @@ template <int SDO, int SPB>
 __device__ __forceinline__ bool tile_lane(const DevGrid &g, int &s, int &j, int &i0, int &i1,
                                           bool &lane_ok)
 {
+    if (ONE_SHOT_PACK && SPB > 1 && g.nshot == 1) {
+        // one-shot launch: the block's waves take SPB adjacent column blocks of
+        // one strip of the shot, instead of one column block with SPB - 1 idle waves
+        const int ncq = (g.ncb + SPB - 1) / SPB;
+        const int T = ncq * g.nstrips;
+        const int per = (T + NUM_XCD - 1) / NUM_XCD;
+        const int w = (int)blockIdx.x;
+        const int logical = (w % NUM_XCD) * per + w / NUM_XCD;
+        if (logical >= T) return false;
+        const int cb = (logical % ncq) * SPB + wave_id();
+        const int st = logical / ncq;
+        s = 0;
+        j = SDO + cb * 64 + (int)threadIdx.x % 64;
+        i0 = SDO + st * g.mrows;
+        i1 = min(i0 + g.mrows - 1, g.nz - SDO);
+        lane_ok = j <= g.nx - SDO;
+        return cb < g.ncb;
+    }
     const int T = g.ncb * g.nstrips * g.nbatch;
     const int per = (T + NUM_XCD - 1) / NUM_XCD;
     const int w = (int)blockIdx.x;
@@ -1065,7 +1083,8 @@ __global__ __launch_bounds__(64 * SPB) void k_adj_str(DevGrid g, const pyfwi_sho
     int s, j, i0, i1;
     if (!tile<SDO, SPB>(g, s, j, i0, i1)) return;
     if (!bwd_active(shots, s, t)) return;
-    const GroupLive gl = group_live<SPB>(g, shots, s, t);
+    GroupLive gl;  // the shot group (one-shot launches have none)
+    if constexpr (!ALONE) gl = group_live<SPB>(g, shots, s, t);
     const int wave = wave_id(), lane = (int)threadIdx.x % 64;
     const int64_t fs = g.fstride;
     const uint32_t RB = (uint32_t)g.pitch * 4u;
@@ -1221,7 +1240,8 @@ __global__ __launch_bounds__(64 * SPB) void k_adj_vel_grad(
         return;
     }
     if (!bwd_active(shots, s, t)) return;
-    const GroupLive gl = group_live<SPB>(g, shots, s, t);
+    GroupLive gl;  // the shot group (one-shot launches have none)
+    if constexpr (!ALONE) gl = group_live<SPB>(g, shots, s, t);
     const int wave = wave_id(), lane = (int)threadIdx.x % 64;
     const int64_t fs = g.fstride;
     const uint32_t RB = (uint32_t)g.pitch * 4u;
@@ -1426,7 +1446,8 @@ __global__ __launch_bounds__(64 * SPB) void k_exact_str(
     int s, j, i0, i1;
     if (!tile<SDO, SPB>(g, s, j, i0, i1)) return;
     if (t >= shots[s].nt) return;
-    const GroupLive gl = group_live<SPB, true>(g, shots, s, t);
+    GroupLive gl;  // the shot group (one-shot launches have none)
+    if constexpr (!ALONE) gl = group_live<SPB, true>(g, shots, s, t);
     const int wave = wave_id(), lane = (int)threadIdx.x % 64;
     const int64_t fs = g.fstride;
     const uint32_t RB = (uint32_t)g.pitch * 4u;
@@ -1542,7 +1563,8 @@ __global__ __launch_bounds__(64 * SPB) void k_exact_vel(
     int s, j, i0, i1;
     if (!tile<SDO, SPB>(g, s, j, i0, i1)) return;
     if (t >= shots[s].nt) return;
-    const GroupLive gl = group_live<SPB, true>(g, shots, s, t);
+    GroupLive gl;  // the shot group (one-shot launches have none)
+    if constexpr (!ALONE) gl = group_live<SPB, true>(g, shots, s, t);
     const int wave = wave_id(), lane = (int)threadIdx.x % 64;
     const int64_t fs = g.fstride;
     const uint32_t RB = (uint32_t)g.pitch * 4u;
@@ -1675,7 +1697,8 @@ __global__ void k_exact_inject(DevGrid g, const pyfwi_shot *shots, int t, float 
 // FP64.  comp_mask bit f selects component f (vx, vz, txx, tzz, txz); mask 0
 // means the pressure output (txx + tzz)/2 compared with obs[txx], whose
 // residual goes to both normal stresses and counts twice, as the reference's
-// 'taux'/'tauz' pair does.
+// 'taux'/'tauz' pair does.  obs == nullptr: observed records of zero;
+// misfit == nullptr: the residual only.
 __global__ void k_l2_residual(const pyfwi_shot *shots, const float *__restrict__ seis,
                               const float *__restrict__ obs, float *res, int comp_mask,
                               double *misfit)
@@ -1683,7 +1706,7 @@ __global__ void k_l2_residual(const pyfwi_shot *shots, const float *__restrict__
     const int s = blockIdx.y;
     const pyfwi_shot sh = shots[s];
     const int64_t plane = (int64_t)sh.n_samples * sh.rec_stride;
-    const float *S = seis + sh.rec_offset, *D = obs + sh.rec_offset;
+    const float *S = seis + sh.rec_offset, *D = obs ? obs + sh.rec_offset : nullptr;
     float *R = res + sh.rec_offset;
     double acc = 0.0;
     for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < plane;
@@ -1695,7 +1718,7 @@ __global__ void k_l2_residual(const pyfwi_shot *shots, const float *__restrict__
         }
         if (comp_mask == 0) {
             const float p = (S[2 * plane + k] + S[3 * plane + k]) / 2;
-            const float r = p - D[2 * plane + k];
+            const float r = D ? p - D[2 * plane + k] : p;
             R[k] = R[plane + k] = R[4 * plane + k] = 0.0f;
             R[2 * plane + k] = r;
             R[3 * plane + k] = r;
@@ -1705,13 +1728,14 @@ __global__ void k_l2_residual(const pyfwi_shot *shots, const float *__restrict__
             for (int f = 0; f < 5; ++f) {
                 float r = 0.0f;
                 if (comp_mask & (1 << f)) {
-                    r = S[f * plane + k] - D[f * plane + k];
+                    r = D ? S[f * plane + k] - D[f * plane + k] : S[f * plane + k];
                     acc += 0.5 * (double)r * r;
                 }
                 R[f * plane + k] = r;
             }
         }
     }
+    if (!misfit) return;
     // block reduction, one atomic per block
     __shared__ double part[256];
     part[threadIdx.x] = acc;
@@ -1990,7 +2014,8 @@ static dim3 march_grid(DevGrid &g, int sdo, int nshots, int spb)
                          : strip_rows(g, nz_act, (int64_t)g.ncb * nshots);
     g.mrows = rows;
     g.nstrips = (nz_act + rows - 1) / rows;
-    const int T = g.ncb * g.nstrips * g.nbatch;
+    const int T = ONE_SHOT_PACK && spb > 1 && nshots == 1 ? (g.ncb + spb - 1) / spb * g.nstrips
+                                                          : g.ncb * g.nstrips * g.nbatch;
     const int per = (T + NUM_XCD - 1) / NUM_XCD;
     return dim3(per * NUM_XCD);
 }

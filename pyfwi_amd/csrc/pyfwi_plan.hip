@@ -124,7 +124,7 @@ struct pyfwi_plan {
     bool ckpts_all = false;  // ckpts holds every shot of the last forward
     DevBuf src_stage, seis_stage, res_stage, grad_stage, shot_tab_f, resid;
     // record filter of the misfit (pyfwi_set_record_filter): taps k[0..L]
-    DevBuf fir_taps, seis_f, obs_f, resid_raw;
+    DevBuf fir_taps, filt_a, filt_b;
     int fir_L = -1, fir_pad = 0;
     double *d_misfit = nullptr;
     int *d_check = nullptr;  // two ints for the model check
@@ -134,6 +134,7 @@ struct pyfwi_plan {
     std::vector<std::pair<uint64_t, hipGraphExec_t>> graph_cache;
     int64_t graph_captures = 0, graph_replays = 0;
     std::vector<int32_t> last_passes;  // shots per pass of the last call
+    std::vector<int32_t> last_order;   // the caller's shot indices in the order they ran
     int last_ckpt_every = 0;           // exact mode: checkpoint interval of the last call
     pyfwi_shot *d_shots = nullptr;
     int shots_cap = 0;
@@ -505,7 +506,7 @@ int32_t pyfwi_plan_destroy(pyfwi_plan *p)
 {
     if (!p) return 0;
     (void)hipStreamSynchronize(p->stream);
-    for (DevBuf *b : {&p->fir_taps, &p->seis_f, &p->obs_f, &p->resid_raw})
+    for (DevBuf *b : {&p->fir_taps, &p->filt_a, &p->filt_b})
         b->release();
     for (DevBuf *b : {&p->lam, &p->mu, &p->bu, &p->dxp_buf, &p->dzp_buf, &p->scratch_in, &p->fields, &p->fields2,
                       &p->adj, &p->av0, &p->av1, &p->gacc, &p->snaps, &p->ckpts, &p->seg,
@@ -762,6 +763,13 @@ int check_chp_nt(const std::vector<pyfwi_shot> &sh, const std::vector<int32_t> &
     return 0;
 }
 
+// The shots of a call run in the order given.
+void identity_order(pyfwi_plan *p, int nshots)
+{
+    p->last_order.resize((size_t)nshots);
+    for (int k = 0; k < nshots; ++k) p->last_order[(size_t)k] = k;
+}
+
 int pass_size(const pyfwi_plan *p, int nshots)
 {
     return p->tune_group > 0 ? std::min(p->tune_group, nshots) : nshots;
@@ -781,10 +789,45 @@ int prepare_fields(pyfwi_plan *p, int ng)
     return 0;
 }
 
+// Time-major layout of the kept states of a pass that keeps every step:
+// state t of the shots live at step t (t < nt_s) side by side, the pass's
+// shots ordered by decreasing nt so that they are a prefix.  State t of shot
+// s lies off[t] + s states from the buffer start, so a pass holds the sum of
+// its shots' nt states, where one slot per step of the longest shot for
+// every shot would hold ng * max nt.  The launches of step t then advance
+// only its live[t] shots (the forward's step t and the backward's step t
+// both take the shots with t < nt_s), so a step where one shot is left runs
+// the one-shot kernels.
+struct TimeMajor {
+    std::vector<int64_t> off;
+    std::vector<int> live;
+};
+
+int time_major(const pyfwi_shot *sh, int ng, TimeMajor &tm)
+{
+    const int nt = sh[0].nt;
+    for (int k = 1; k < ng; ++k)
+        if (sh[k].nt > sh[k - 1].nt)
+            return fail(PYFWI_EINVAL, "time-major kept states need shots by decreasing nt");
+    tm.off.assign((size_t)nt, 0);
+    tm.live.assign((size_t)nt, 0);
+    int64_t cum = 0;
+    int live = ng;
+    for (int t = 0; t < nt; ++t) {
+        while (live > 0 && sh[live - 1].nt <= t) --live;
+        tm.off[(size_t)t] = cum;
+        tm.live[(size_t)t] = live;
+        cum += live;
+    }
+    return 0;
+}
+
 // Forward time loop of one pass of ng shots starting at shot g0.  `snaps`
-// and `ckpts` (or null) have per-shot strides nchp*fps and ncp*fps.
+// and `ckpts` (or null) have per-shot strides nchp*fps and ncp*fps, or, with
+// `tm` (every step kept), the kept states are laid out time-major.
 int forward_pass(pyfwi_plan *p, int g0, int ng, const float *d_src, float *d_seis,
-                 const std::vector<int32_t> &cl, float *snaps, float *ckpts, int ncp, int K)
+                 const std::vector<int32_t> &cl, float *snaps, float *ckpts, int ncp, int K,
+                 const TimeMajor *tm = nullptr)
 {
     const int64_t fps = fields_per_shot(p);
     const pyfwi_shot *dsh = p->d_shots + g0;
@@ -794,31 +837,40 @@ int forward_pass(pyfwi_plan *p, int g0, int ng, const float *d_src, float *d_sei
     if (int e = prepare_fields(p, ng)) return e;
     Timed loop_tm(p, PYFWI_K_FWD_LOOP, 2, nt);
     KeyHash key = loop_key(p, 1, g0, ng);
-    key.val(d_src).val(d_seis).val(snaps).val(ckpts).val(ncp).val(K);
+    key.val(d_src).val(d_seis).val(snaps).val(ckpts).val(ncp).val(K).val(tm != nullptr);
     key.add(cl.data(), cl.size() * sizeof(int32_t));
     // every step kept (chpr=100): the two-launch kernels step from one slot
     // into the next, so the kept states cost no extra traffic
     if (snaps && keeps_every_step(cl, nt) && !p->tune_fused && p->src_inside) {
         return run_loop(p, key.h, [&]() -> int {
-            const int64_t ss = (int64_t)nchp * fps;
+            const int64_t ss = tm ? fps : (int64_t)nchp * fps;  // between shots
             const float *in = p->fields.p;
             int64_t in_stride = fps;
             for (int t = 0; t < nt; ++t) {
                 if (ckpts && t % K == 0)
                     launch_copy_shots(ckpts + (int64_t)(t / K) * fps, (int64_t)ncp * fps, in,
                                       in_stride, fps, ng, p->stream);
-                float *out = snaps + (int64_t)t * fps;
-                forward_step_kept(p, dsh, ng, t, d_src, p->rec_inline ? d_seis : nullptr, in,
-                                  in_stride, out, ss);
+                float *out = snaps + (tm ? tm->off[(size_t)t] : (int64_t)t) * fps;
+                forward_step_kept(p, dsh, tm ? tm->live[(size_t)t] : ng, t, d_src,
+                                  p->rec_inline ? d_seis : nullptr, in, in_stride, out, ss);
                 in = out;
                 in_stride = ss;
                 if (int e = maybe_harvest(p)) return e;
             }
-            // receiver tables: one gather launch over all steps instead of one per step
+            // receiver tables: one gather launch over all steps instead of one
+            // per step (time-major: one per step, the step's states side by side)
             if (!p->rec_inline) {
                 Timed ts(p, PYFWI_K_FWD_SPARSE);
-                launch_record_kept(p->dg, ng, dsh, 0, nt, snaps, ss, fps, d_seis, p->d_rtab,
-                                   p->stream);
+                if (!tm) {
+                    launch_record_kept(p->dg, ng, dsh, 0, nt, snaps, ss, fps, d_seis, p->d_rtab,
+                                       p->stream);
+                } else {
+                    // the gather of step t reads state t-1 (none for t = 0)
+                    for (int t = 0; t < nt; ++t)
+                        launch_record_kept(p->dg, ng, dsh, t, t + 1,
+                                           snaps + (t > 0 ? tm->off[(size_t)t - 1] - (t - 1) : 0) * fps,
+                                           fps, fps, d_seis, p->d_rtab, p->stream);
+                }
             }
             return 0;
         });
@@ -830,7 +882,11 @@ int forward_pass(pyfwi_plan *p, int g0, int ng, const float *d_src, float *d_sei
                 launch_copy_shots(ckpts + (int64_t)(t / K) * fps, (int64_t)ncp * fps, p->cur, fps,
                                   fps, ng, p->stream);
             forward_step(p, dsh, ng, t, d_src, d_seis);
-            if (snaps && std::find(cl.begin(), cl.end(), t) != cl.end()) {
+            if (snaps && tm) {
+                // every step kept, time-major: the live shots' states of step t
+                launch_copy_shots(snaps + tm->off[(size_t)t] * fps, fps, p->cur, fps, fps,
+                                  tm->live[(size_t)t], p->stream);
+            } else if (snaps && std::find(cl.begin(), cl.end(), t) != cl.end()) {
                 if (chpc < nchp)
                     launch_copy_shots(snaps + (int64_t)chpc * fps, (int64_t)nchp * fps, p->cur,
                                       fps, fps, ng, p->stream);
@@ -948,6 +1004,7 @@ int32_t pyfwi_forward(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, co
     launch_zero(d_seis, (int64_t)nrec, p->stream);
     const int G = pass_size(p, nshots);
     p->last_passes.clear();
+    identity_order(p, nshots);
     p->last_ckpt_every = ckpt_every;
     for (int g0 = 0; g0 < nshots; g0 += G) {
         const int ng = std::min(G, nshots - g0);
@@ -975,9 +1032,10 @@ namespace {
 
 // Reference adjoint of one pass (shots g0 .. g0+ng-1) into gacc[adj_acc_sets(g.sdo, ng)].
 // `snaps` holds the pass's kept states, per-shot stride cl.size() states, at
-// the steps of `cl`.
+// the steps of `cl`, or laid out time-major (`tm`, every step kept).
 int gradient_pyfwi(pyfwi_plan *p, int g0, int ng, const float *d_res, bool precond,
-                   const float *snaps, const std::vector<int32_t> &cl)
+                   const float *snaps, const std::vector<int32_t> &cl,
+                   const TimeMajor *tm = nullptr)
 {
     const DevGrid &g = p->dg;
     const int64_t fps = fields_per_shot(p);
@@ -998,15 +1056,16 @@ int gradient_pyfwi(pyfwi_plan *p, int g0, int ng, const float *d_res, bool preco
     const bool direct = keeps_every_step(cl, nt);
     Timed loop_tm(p, PYFWI_K_BWD_LOOP, 2, nt - 1);
     KeyHash key = loop_key(p, 2, g0, ng);
-    key.val(d_res).val(precond).val(snaps).add(cl.data(), cl.size() * sizeof(int32_t));
+    key.val(d_res).val(precond).val(snaps).val(tm != nullptr);
+    key.add(cl.data(), cl.size() * sizeof(int32_t));
     return run_loop(p, key.h, [&]() -> int {
     int chpc = nchp - 1;
     for (int t = nt - 1; t > 0; --t) {
         const float *fwd = p->cur;
         int64_t fwd_stride = fps;
         if (direct) {
-            fwd = snaps + (int64_t)t * fps;
-            fwd_stride = (int64_t)nchp * fps;
+            fwd = snaps + (tm ? tm->off[(size_t)t] : (int64_t)t) * fps;
+            fwd_stride = tm ? fps : (int64_t)nchp * fps;
         } else if (chpc >= 0 && t == cl[chpc]) {
             launch_copy_shots(p->cur, fps, snaps + (int64_t)chpc * fps, (int64_t)nchp * fps, fps,
                               ng, p->stream);
@@ -1028,14 +1087,17 @@ int gradient_pyfwi(pyfwi_plan *p, int g0, int ng, const float *d_res, bool preco
             Timed ts(p, PYFWI_K_BWD_SPARSE);
             launch_adj_inject(g, ng, dsh, t, p->adj.p, d_res, p->d_rtab, p->stream, 0, 2, true);
         }
+        // time-major: the live shots of step t only (gacc holds the sets of
+        // every launch size up to ng, adj_acc_sets_upto)
+        const int nl = tm && direct ? tm->live[(size_t)t] : ng;
         {
-            Timed tm(p, PYFWI_K_ADJ_STR);
-            launch_adj_str(g, ng, dsh, t, fwd, fwd_stride, p->adj.p, p->bu.p, p->dxp, p->dzp,
+            Timed tmr(p, PYFWI_K_ADJ_STR);
+            launch_adj_str(g, nl, dsh, t, fwd, fwd_stride, p->adj.p, p->bu.p, p->dxp, p->dzp,
                            p->gacc.p, precond, p->stream, r_on);
         }
         {
-            Timed tm(p, PYFWI_K_ADJ_VEL);
-            launch_adj_vel_grad(g, ng, dsh, t, fwd, fwd_stride, p->adj.p, p->lam.p, p->mu.p,
+            Timed tmr(p, PYFWI_K_ADJ_VEL);
+            launch_adj_vel_grad(g, nl, dsh, t, fwd, fwd_stride, p->adj.p, p->lam.p, p->mu.p,
                                 p->bu.p, p->dxp, p->dzp, p->gacc.p, precond, p->stream, r_on);
         }
         if (int e = maybe_harvest(p)) return e;
@@ -1176,6 +1238,7 @@ int32_t pyfwi_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, c
         if (p->seg.n > (size_t)G * K * fps) p->seg.release();
     }
     p->last_passes.clear();
+    identity_order(p, nshots);
     // exact mode sized to memory: the passes' shots spread evenly (at most G)
     const bool even = mode == PYFWI_MODE_EXACT && p->tune_group == 0;
     const int npass = (nshots + G - 1) / G;
@@ -1245,13 +1308,6 @@ int32_t pyfwi_misfit_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t ns
     const int64_t fps = fields_per_shot(p);
     const size_t nrec = rec_extent(shots, nshots), nsrc = src_extent(shots, nshots);
     const size_t n = (size_t)g.nz * g.nx;
-    // passes run the shots longest first, so that each pass's kept states
-    // are sized by its own longest shot; outputs follow the shots' offsets
-    std::vector<pyfwi_shot> shs = sh;
-    if (every)
-        std::stable_sort(shs.begin(), shs.end(),
-                         [](const pyfwi_shot &a, const pyfwi_shot &b) { return a.nt > b.nt; });
-    if (int e = upload_shots(p, shs.data(), nshots)) return e;
     const float *d_src = nullptr, *d_obs = nullptr;
     if (int e = stage_in(p, p->src_stage, src, nsrc, mem, &d_src, "source table")) return e;
     if (int e = stage_in(p, p->res_stage, obs, nrec, mem, &d_obs, "observed records")) return e;
@@ -1268,7 +1324,7 @@ int32_t pyfwi_misfit_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t ns
     if (int e = ensure(p->resid, nrec, p->stream, "residuals")) return e;
     const bool filtered = p->fir_L >= 0;
     if (filtered) {
-        for (DevBuf *b : {&p->seis_f, &p->obs_f, &p->resid_raw})
+        for (DevBuf *b : {&p->filt_a, &p->filt_b})
             if (int e = ensure(*b, nrec, p->stream, "filtered records")) return e;
     }
     if (!p->d_misfit) HIPCHK(hipMalloc(&p->d_misfit, sizeof(double)));
@@ -1277,66 +1333,116 @@ int32_t pyfwi_misfit_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t ns
     p->chp = cl;
     p->last_shots = sh;
     p->have_forward = false;  // kept states belong to the last pass only
-    // kept-state slots: the tuned pass size times the longest shot; else the
-    // buffer held from an earlier call when it takes the longest shot (a
-    // reallocation of this size costs seconds), or as many states as 90 % of
-    // the free memory holds (the buffer held counts as free), never more
-    // than all shots need
-    size_t slots;
-    const size_t first = every ? (size_t)shs[0].nt : (size_t)nchp;
-    // a pass gives each of its shots the slots of its longest shot, so all
-    // shots in one pass take nshots * first slots
-    const size_t need = (size_t)nshots * first;
+    // the passes: runs of `order` (the caller's shot indices) of the sizes in
+    // `sizes`, each run by decreasing nt
+    std::vector<int32_t> order(nshots), sizes;
+    for (int k = 0; k < nshots; ++k) order[k] = k;
+    // every step kept: states are laid out time-major in each pass (a pass
+    // holds the sum of its shots' nt), the shots longest first; else each
+    // shot of a pass holds the nchp states of the checkpoint list
+    if (every)
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int32_t a, int32_t b) { return sh[a].nt > sh[b].nt; });
+    auto states_of = [&](int32_t k) -> size_t { return every ? (size_t)sh[k].nt : (size_t)nchp; };
+    size_t need = 0;
+    for (int k = 0; k < nshots; ++k) need += states_of(k);
+    const size_t first = states_of(order[0]);
+    // kept-state buffer, in states: with a tuned pass size, the largest of
+    // those passes; else the buffer held from an earlier call when it takes
+    // the longest shot (a reallocation of this size costs seconds), or as
+    // many states as 90 % of the free memory holds (the buffer held counts as
+    // free), never more than all shots need
+    size_t slots = 0;
     if (p->tune_group > 0) {
-        slots = (size_t)std::min(p->tune_group, nshots) * first;
-    } else if (p->snaps.n / (size_t)fps >= first) {
-        slots = std::min(need, p->snaps.n / (size_t)fps);
+        for (int g0 = 0; g0 < nshots; g0 += p->tune_group) {
+            size_t sum = 0;
+            for (int k = g0; k < std::min(nshots, g0 + p->tune_group); ++k)
+                sum += states_of(order[k]);
+            slots = std::max(slots, sum);
+            sizes.push_back(std::min(p->tune_group, nshots - g0));
+        }
     } else {
-        size_t free_b = 0, total_b = 0;
-        HIPCHK(hipMemGetInfo(&free_b, &total_b));
-        const double avail = 0.9 * ((double)free_b + (double)p->snaps.n * sizeof(float));
-        slots = std::min(need, (size_t)(avail / ((double)fps * sizeof(float))));
-        if (slots < first)
-            return fail(PYFWI_ENOMEM, "the kept states of one shot (" +
-                                          std::to_string((long long)(first * fps * sizeof(float))) +
-                                          " bytes) do not fit in device memory");
+        if (p->snaps.n / (size_t)fps >= first) {
+            slots = std::min(need, p->snaps.n / (size_t)fps);
+        } else {
+            size_t free_b = 0, total_b = 0;
+            HIPCHK(hipMemGetInfo(&free_b, &total_b));
+            const double avail = 0.9 * ((double)free_b + (double)p->snaps.n * sizeof(float));
+            slots = std::min(need, (size_t)(avail / ((double)fps * sizeof(float))));
+            if (slots < first)
+                return fail(PYFWI_ENOMEM, "the kept states of one shot (" +
+                                              std::to_string((long long)(first * fps * sizeof(float))) +
+                                              " bytes) do not fit in device memory");
+        }
+        // first fit by decreasing nt: each shot joins the first pass with
+        // room for its states.  A launch costs a fixed part plus a part per
+        // live shot, so the step count summed over the passes (one launch
+        // pair per step of a pass's longest shot) is what the packing saves:
+        // long shots share their passes with short ones instead of running
+        // alone.  Each pass keeps its shots by decreasing nt.
+        std::vector<std::vector<int32_t>> bins;
+        std::vector<size_t> used;
+        for (int32_t k : order) {
+            size_t b = 0;
+            while (b < bins.size() && used[b] + states_of(k) > slots) ++b;
+            if (b == bins.size()) {
+                bins.emplace_back();
+                used.push_back(0);
+            }
+            bins[b].push_back(k);
+            used[b] += states_of(k);
+        }
+        order.clear();
+        for (auto &b : bins) {
+            order.insert(order.end(), b.begin(), b.end());
+            sizes.push_back((int32_t)b.size());
+        }
     }
+    std::vector<pyfwi_shot> shs(nshots);
+    for (int k = 0; k < nshots; ++k) shs[k] = sh[order[k]];
+    if (int e = upload_shots(p, shs.data(), nshots)) return e;
     if (int e = ensure(p->snaps, slots * fps, p->stream, "checkpoints")) return e;
-    p->last_passes.clear();
+    p->last_passes = sizes;
+    p->last_order = order;
     p->last_ckpt_every = 0;
-    for (int g0 = 0; g0 < nshots;) {
+    for (int g0 = 0, b = 0; g0 < nshots; ++b) {
+        const int ng = sizes[b];
         const int nt_pass = every ? shs[g0].nt : nt;
-        const int stride = every ? nt_pass : nchp;
-        int ng = p->tune_group > 0 ? p->tune_group : (int)(slots / (size_t)stride);
-        ng = std::max(1, std::min(ng, nshots - g0));
-        p->last_passes.push_back(ng);
         std::vector<int32_t> cl_pass = cl;
-        if (every) every_step(nt_pass, cl_pass);
-        if (int e = forward_pass(p, g0, ng, d_src, d_seis, cl_pass, p->snaps.p, nullptr, 0, 1))
+        TimeMajor tm;
+        if (every) {
+            every_step(nt_pass, cl_pass);
+            if (int e = time_major(shs.data() + g0, ng, tm)) return e;
+        }
+        const TimeMajor *tmp = every ? &tm : nullptr;
+        if (int e = forward_pass(p, g0, ng, d_src, d_seis, cl_pass, p->snaps.p, nullptr, 0, 1,
+                                 tmp))
             return e;
         if (filtered) {
-            // the misfit of the lowpassed records; its adjoint source is the
-            // lowpassed residual (the filter is symmetric)
+            // the misfit of the lowpassed records, K d - K d_obs = K (d - d_obs)
+            // (the filter is linear): the residual is filtered once, so the
+            // float32 sums see the residual, not the records it is the small
+            // difference of.  Its adjoint source is K K r (K is symmetric).
             const pyfwi_shot *dsp = p->d_shots + g0, *hsp = p->h_shots.data() + g0;
-            launch_fir(ng, dsp, hsp, d_seis, p->seis_f.p, comp_mask, p->fir_taps.p, p->fir_L,
-                       p->fir_pad, p->stream);
-            launch_fir(ng, dsp, hsp, d_obs, p->obs_f.p, comp_mask, p->fir_taps.p, p->fir_L,
-                       p->fir_pad, p->stream);
-            launch_l2_residual(ng, dsp, p->seis_f.p, p->obs_f.p, p->resid_raw.p, comp_mask,
+            launch_l2_residual(ng, dsp, d_seis, d_obs, p->filt_a.p, comp_mask, nullptr,
+                               p->stream);
+            launch_fir(ng, dsp, hsp, p->filt_a.p, p->filt_b.p, comp_mask, p->fir_taps.p,
+                       p->fir_L, p->fir_pad, p->stream);
+            launch_l2_residual(ng, dsp, p->filt_b.p, nullptr, p->filt_a.p, comp_mask,
                                p->d_misfit, p->stream);
             launch_zero(p->resid.p, (int64_t)nrec, p->stream);
-            launch_fir(ng, dsp, hsp, p->resid_raw.p, p->resid.p, comp_mask, p->fir_taps.p,
+            launch_fir(ng, dsp, hsp, p->filt_a.p, p->resid.p, comp_mask, p->fir_taps.p,
                        p->fir_L, p->fir_pad, p->stream);
         } else {
             launch_l2_residual(ng, p->d_shots + g0, d_seis, d_obs, p->resid.p, comp_mask,
                                p->d_misfit, p->stream);
         }
-        const int nsets = adj_acc_sets(g.sdo, ng);
+        const int nsets = every ? adj_acc_sets_upto(g.sdo, ng) : adj_acc_sets(g.sdo, ng);
         if (int e = ensure(p->gacc, (size_t)nsets * 6 * g.fstride, p->stream, "gradient"))
             return e;
         launch_zero(p->gacc.p, (int64_t)nsets * 6 * g.fstride, p->stream);
         if (int e = gradient_pyfwi(p, g0, ng, p->resid.p, (flags & PYFWI_F_PRECOND) != 0,
-                                   p->snaps.p, cl_pass))
+                                   p->snaps.p, cl_pass, tmp))
             return e;
         launch_grad_sum(g, nsets, 6, p->gacc.p, d_grad, p->bu.p, 0, g0 > 0, p->stream);
         g0 += ng;
@@ -1492,6 +1598,14 @@ int32_t pyfwi_last_passes(pyfwi_plan *p, int32_t *sizes, int32_t cap, int32_t *c
     if (ckpt_every) *ckpt_every = p->last_ckpt_every;
     *count = (int32_t)p->last_passes.size();
     for (int k = 0; k < cap && k < *count; ++k) sizes[k] = p->last_passes[k];
+    return 0;
+}
+
+int32_t pyfwi_last_order(pyfwi_plan *p, int32_t *order, int32_t cap, int32_t *count)
+{
+    if (!p || !count || cap < 0 || (cap > 0 && !order)) return fail(PYFWI_EINVAL, "null argument");
+    *count = (int32_t)p->last_order.size();
+    for (int k = 0; k < cap && k < *count; ++k) order[k] = p->last_order[(size_t)k];
     return 0;
 }
 

@@ -73,16 +73,22 @@ def test_pass_bytes_even_shots_match_launch_bytes():
 
 
 def test_pass_bytes_uneven_shots():
-    """Uneven shots: a launch counts only its live shots, and a block's group
-    moves the accumulators while one of its shots is live.  Pass of 6 shots
-    (blocks of four: groups {0..3}, {4, 5}) with nt 10, 9, 8, 8, 5, 3."""
+    """Uneven shots: a launch takes only its live shots (a prefix of the pass,
+    longest first), in blocks of five when that count is a multiple of five
+    but not of four, else four; each block's group moves the accumulators
+    once.  Pass of 6 shots with nt 10, 9, 8, 8, 5, 3: live 6, 6, 5, 5, 4, 4,
+    4, 2, 1 at t = 1..9, so 2, 2, 1, 1, 1, 1, 1, 1, 1 groups."""
     nts = [10, 9, 8, 8, 5, 3]
     a, b, c = bench.KERNEL_BYTES["adj_vel_grad"]
     want = 0
+    groups_seen = []
     for t in range(1, 10):
-        live = [n > t for n in nts]
-        groups = int(any(live[:4])) + int(any(live[4:]))
-        want += (a * CELLS + b * INTERIOR) * sum(live) + c * INTERIOR * groups
+        live = sum(n > t for n in nts)
+        spb = 5 if live % 5 == 0 and live % 4 != 0 else 4
+        groups = -(-live // spb)
+        groups_seen.append(groups)
+        want += (a * CELLS + b * INTERIOR) * live + c * INTERIOR * groups
+    assert groups_seen == [2, 2, 1, 1, 1, 1, 1, 1, 1]
     total, n = bench.pass_kernel_bytes("adj_vel_grad", [nts], CELLS, INTERIOR)
     assert n == 9 and total == want
     total, n = bench.pass_kernel_bytes("fwd_vel", [nts], CELLS, INTERIOR)

@@ -197,8 +197,9 @@ def test_c4_uneven_pair_kept_states(gpu):
 
 
 def test_c4_all_shots_batched_equals_per_shot(gpu):
-    """All 64 uneven shots at full nt: batched (passes sized to the free HBM)
-    against one shot a pass -- records bit for bit, gradient to the float
+    """All 64 uneven shots at full nt: batched (shots packed first-fit into
+    passes whose time-major kept states fill the free HBM) against one shot a
+    pass -- records bit for bit, gradient to the float
     rounding of the shot groups' sums, misfit to 1e-9."""
     import torch
     from pyfwi_amd.batch import UnevenBatch
@@ -220,7 +221,19 @@ def test_c4_all_shots_batched_equals_per_shot(gpu):
                                       src_table=src)
         torch.cuda.synchronize()
         passes = B.plan.last_passes()
+        order = B.plan.last_order()
         print(f"C4 64 shots, passes {passes}: misfit {misfit:.8e}", flush=True)
+        # the passes: consecutive runs of a permutation of the shots, each by
+        # decreasing nt; packed first-fit, the longest shot shares its pass
+        assert sorted(order) == list(range(B.ns)) and sum(passes) == B.ns
+        k = 0
+        for n in passes:
+            nts = [B.nt[j] for j in order[k:k + n]]
+            assert nts == sorted(nts, reverse=True), nts
+            k += n
+        if per_pass == 0:
+            assert B.nt[order[0]] == max(B.nt) and passes[0] >= 2
+            print(f"  first pass nt {[B.nt[j] for j in order[:passes[0]]]}", flush=True)
         outs.append((seis, grad, misfit))
     B.plan.set_tuning(0, False, 0)
     _release(B.plan)

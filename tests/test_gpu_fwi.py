@@ -70,8 +70,10 @@ def test_fprime_matches_oracle(gpu, freq):
 def test_device_path_equals_reference_sequence(gpu, freq):
     """The one-call device path (with the lowpass band as the device record
     filter) against forward_modeling -> cost_seismic (host FFT lowpass) ->
-    gradient: misfit and gradient to 1e-6 without a band; with one, to 1e-5
-    (float32 filter sums against the host's float64 FFT)."""
+    gradient: misfit and gradient to 1e-6 without a band; with one, to 1e-5:
+    the device filters the residual with float32 sums, the host filters the
+    two record sets apart with float64 FFTs (filtering the records apart in
+    float32 would miss by their rounding times |d|/|r|, about 900 here)."""
     w, st, d_obs = _setup()
     fwi = FWI(d_obs, w.inpa, w.src, w.rec_loc, w.model_shape, components=2, chpr=100)
     m = np.float32(T.vel_dict2vec(w.model))
