@@ -30,9 +30,6 @@ constexpr int FUSED_WAVES = 4;      // fused two-stage kernels: wavefronts per b
 #ifndef ADJ_VEL_UNROLL
 #define ADJ_VEL_UNROLL 2            // rows per unrolled iteration of the adjoint velocity + imaging kernel
 #endif
-#ifndef ONE_SHOT_PACK
-#define ONE_SHOT_PACK 0             // one-shot launches: waves of a block on adjacent column blocks
-#endif
 #ifndef ADJ_GROUP_ROWS
 #define ADJ_GROUP_ROWS 2            // adjoint kernels: rows per block-wide sum of the imaging terms
 #endif

@@ -276,15 +276,16 @@ __device__ __forceinline__ int wave_id()
 // (w % 8) * ceil(T/8) + w / 8.  Workgroups are handed to the 8 XCDs in turn,
 // so each XCD receives one contiguous run of tiles: the same cells of every
 // shot of the launch meet in one L2, and neighbouring column blocks, which
-// share halo columns, run side by side.  lane_ok: the lane's column is on
-// the grid; the result: the wave has a shot.
+// share halo columns, run side by side.  A launch of one shot (which shares
+// no accumulators) gives the block's SPB waves SPB adjacent column blocks of
+// one strip instead, so no wave idles.  lane_ok: the lane's column is on the
+// grid; the result: the wave has a tile.
 template <int SDO, int SPB>
 __device__ __forceinline__ bool tile_lane(const DevGrid &g, int &s, int &j, int &i0, int &i1,
                                           bool &lane_ok)
 {
-    if (ONE_SHOT_PACK && SPB > 1 && g.nshot == 1) {
-        // one-shot launch: the block's waves take SPB adjacent column blocks of
-        // one strip of the shot, instead of one column block with SPB - 1 idle waves
+    if (SPB > 1 && g.nshot == 1) {
+        // one-shot launch: tiles of SPB column blocks, one per wave
         const int ncq = (g.ncb + SPB - 1) / SPB;
         const int T = ncq * g.nstrips;
         const int per = (T + NUM_XCD - 1) / NUM_XCD;
@@ -2014,8 +2015,9 @@ static dim3 march_grid(DevGrid &g, int sdo, int nshots, int spb)
                          : strip_rows(g, nz_act, (int64_t)g.ncb * nshots);
     g.mrows = rows;
     g.nstrips = (nz_act + rows - 1) / rows;
-    const int T = ONE_SHOT_PACK && spb > 1 && nshots == 1 ? (g.ncb + spb - 1) / spb * g.nstrips
-                                                          : g.ncb * g.nstrips * g.nbatch;
+    // one shot: tiles of spb column blocks (see tile_lane)
+    const int T = spb > 1 && nshots == 1 ? (g.ncb + spb - 1) / spb * g.nstrips
+                                         : g.ncb * g.nstrips * g.nbatch;
     const int per = (T + NUM_XCD - 1) / NUM_XCD;
     return dim3(per * NUM_XCD);
 }
