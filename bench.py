@@ -323,6 +323,26 @@ def copy_calibration(torch, nbytes=1 << 30):
     log(f"bench: copy calibration {nbytes} bytes read + {nbytes} bytes written, 3 launches")
 
 
+def copy_rate(torch, nbytes=1 << 30, reps=5):
+    """Bytes moved per second (read + write) by a plain device-to-device copy
+    of `nbytes`, timed with events on torch's stream: what a streaming kernel
+    reaches on this HBM, to set the stencil kernels' rates against."""
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    rate = 2.0 * nbytes * reps / (e0.elapsed_time(e1) / 1e3)
+    del a, b
+    torch.cuda.empty_cache()
+    return rate
+
+
 def pass_kernel_bytes(kind, pass_nts, cells, interior, order=4):
     """Algorithmic bytes and launch count of kernel `kind` over passes of
     uneven shots.  pass_nts: per pass, the shots' nt in the order the plan
@@ -490,6 +510,8 @@ def bench_uneven(args, torch):
     torch.cuda.synchronize()
     fwd_rate = units / (time.perf_counter() - ts) / 1e9
     log(f"bench: forward-only {fwd_rate:.2f} Gcell/s")
+    copy_gbs = copy_rate(torch) / 1e9
+    log(f"bench: device copy {copy_gbs:.0f} GB/s")
 
     ps = None
     if args.per_shot_sample > 0:
@@ -550,7 +572,9 @@ def bench_uneven(args, torch):
                      "traffic": measured_traffic(wl.name, dom),
                      "bytes_per_launch": kernels[dom]["bytes_per_launch"],
                      "launches_by_live_shots": {str(k): v for k, v in sorted(launch_mix.items())},
-                     "avg_launch_us": kernels[dom]["avg_us"]},
+                     "avg_launch_us": kernels[dom]["avg_us"],
+                     "device_copy_GBps": round(copy_gbs, 1),
+                     "achieved_over_copy": round(achieved / copy_gbs, 4)},
         "gradient_roofline_frac": round(value * 148.0 / HBM_PEAK_GBS, 4),
         "forward_only": {"value": round(fwd_rate, 3), "unit": "Gcell-updates/s",
                          "roofline_frac": round(fwd_rate * 52.0 / HBM_PEAK_GBS, 4)},
