@@ -50,6 +50,7 @@ KERNEL_BYTES = {
     "adj_vel_grad": (36, 16, 8),  # all: R 3 atau lam mu 2 av, W 2 av; interior: R fwd 3 tau, b; group: RW g_rho
     "exact_str": (44, 0, 16),     # R 3 atau lam mu 2 av 2 fwd v, W 2 av; group: RW g_mu g_lam
     "exact_vel": (56, 0, 8),      # R 2 av b 3 fwd tau 3 atau, W 3 atau 2 av; group: RW g_b
+    "adj_step": (52, 20, 24),     # all: R 5 adjoint lam mu b, W 5 adjoint; interior: R 5 fwd; group: RW 6 acc
     "fwd_step": (52, 0, 0),       # R 5 fields lam mu b, W 5 fields
     "rev_step": (52, 0, 0),
 }
@@ -255,7 +256,8 @@ def kernel_profile(W, b, torch, step):
              "bwd": plan.kernel_timing(_native.K_BWD_LOOP)[0]}
     plan.set_timing(0)
     groups = {"fwd": ("fwd_vel", "fwd_str", "fwd_step", "fwd_sparse"),
-              "bwd": ("rev_str", "rev_vel", "rev_step", "adj_str", "adj_vel_grad", "bwd_sparse")}
+              "bwd": ("rev_str", "rev_vel", "rev_step", "adj_str", "adj_vel_grad", "adj_step",
+                      "bwd_sparse")}
     if "exact_str" in per:
         # the exact adjoint's backward sweep interleaves the segment
         # recomputation and brackets no loop: bracketed launch times as they are
@@ -434,8 +436,8 @@ def bench_uneven(args, torch):
     mask = 3  # vx, vz records (components = 2)
     graphs = args.graphs if args.graphs >= 0 else int(B.tnz * B.tnx < 1_000_000)
     plan.set_graphs(bool(graphs))
-    if args.rows:
-        plan.set_tuning(0, False, args.rows)
+    if args.rows or args.adj_fused:
+        plan.set_tuning(0, _native.TUNE_ADJ_FUSED if args.adj_fused else 0, args.rows)
 
     def run(shots):
         return plan.misfit_gradient(shots, src, obs, seis, grad, chp, mask, 0,
@@ -809,6 +811,9 @@ def main():
                     help="uneven_batch: the timed steps only (no kernel timing, forward-only, "
                          "per-shot or CPU legs), for counter runs")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--adj-fused", action="store_true",
+                    help="reference adjoint in one launch a step instead of two "
+                         "(stresses, then velocities)")
     ap.add_argument("--no-tune", action="store_true", help="skip the execution-shape autotuner")
     ap.add_argument("--tune", default="", help="fixed tuning 'shots_per_pass,fused,rows'")
     ap.add_argument("--iters", type=int, default=5,
@@ -882,6 +887,9 @@ def main():
             tuning, _ = tn.autotune_gradient(b["plan"], b["shots"], b["src"], b["seis"], b["res"],
                                              b["grad"], torch.cuda.synchronize, log=log)
     else:
+        b["plan"].set_tuning(*tuning)
+    if args.adj_fused:
+        tuning = (tuning[0], int(tuning[1]) | _native.TUNE_ADJ_FUSED, tuning[2])
         b["plan"].set_tuning(*tuning)
     graphs = args.graphs if args.graphs >= 0 else int(cells < 1_000_000)
     b["plan"].set_graphs(bool(graphs))

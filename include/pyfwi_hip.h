@@ -274,11 +274,15 @@ int32_t pyfwi_set_record_filter(pyfwi_plan *plan, const double *taps, int32_t ha
 /* Execution shape of later calls (no counterpart in the reference, which runs
  * one shot at a time): shots_per_pass > 0 runs a call's shots in passes of
  * that many shots (0: all in one pass; smaller passes need less memory and
- * can keep the working set in the 256 MB on-package cache); fused != 0 does
- * each time step as one fused launch (ping-pong buffers) instead of two
- * half-step launches (the default, 0); strip_rows > 0 fixes the rows each
- * thread walks (0: chosen per launch).  Results do not depend on these
- * choices. */
+ * can keep the working set in the 256 MB on-package cache); `fused` is a
+ * set of bits: PYFWI_TUNE_FUSED_STEPS does each forward and time-reversed
+ * step as one launch (ping-pong buffers) instead of two half-step launches;
+ * PYFWI_TUNE_ADJ_FUSED runs the reference adjoint of each backward step as
+ * one launch instead of two (stresses, then velocities; order 4 with
+ * receivers on the active cells, else ignored); strip_rows > 0 fixes the
+ * rows each thread walks (0: chosen per launch).  Results do not depend on
+ * these choices beyond rounding. */
+enum { PYFWI_TUNE_FUSED_STEPS = 1, PYFWI_TUNE_ADJ_FUSED = 2 };
 int32_t pyfwi_plan_set_tuning(pyfwi_plan *plan, int32_t shots_per_pass, int32_t fused,
                               int32_t strip_rows);
 
@@ -349,7 +353,9 @@ enum {
     PYFWI_K_BWD_LOOP = 11, /* backward time loop of a pass (enable = 2)  */
     PYFWI_K_FWD_SPARSE = 12, /* record gather / source injection launches  */
     PYFWI_K_BWD_SPARSE = 13, /* residual injection launches                */
-    PYFWI_K_COUNT = 14
+    PYFWI_K_ADJ_STEP = 14,   /* adjoint step in one launch (stresses, velocities,
+                                imaging condition)                         */
+    PYFWI_K_COUNT = 15
 };
 int32_t pyfwi_plan_set_timing(pyfwi_plan *plan, int32_t enable);
 int32_t pyfwi_kernel_timing(pyfwi_plan *plan, int32_t kind, double *total_ms,

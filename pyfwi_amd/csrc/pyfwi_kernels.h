@@ -17,11 +17,22 @@ constexpr int MARCH_ROWS_MAX = 64;
 // than 22-30 per CU (12-17-row strips), although the forward kernels'
 // registers allow 32; 13-17 per CU loses 2-6 % again.
 constexpr int MARCH_WAVES_PER_CU = 20;
-// One-shot launches (one working wave per block) want many more waves: on
-// MI355X, 4-row strips ran a 540 x 1040 one-shot gradient 21 % faster than
-// 8-row strips, and 12-row strips a 2040 x 6040 one 12 % faster than 39-row.
-constexpr int MARCH_WAVES_PER_CU_ONE_SHOT = 48;
+// When one round of MARCH_WAVES_PER_CU waves would need strips taller than
+// MARCH_ROWS_ONE_ROUND rows, the launch takes MARCH_ROWS_MANY-row strips
+// instead (several rounds of waves): on MI355X at 1040 x 2040 cells with 8
+// shots a launch, 12-row strips ran all four stencil kernels 3-13 % faster
+// than the 55 rows one round needs (8- and 12-row strips within 1 %, 24-row
+// 2-4 % and 48-row 6-13 % slower than 12).
+constexpr int MARCH_ROWS_ONE_ROUND = 18;
+constexpr int MARCH_ROWS_MANY = 12;
+// One-shot launches (one working wave per block): 10 working waves per CU,
+// strips of 4 to 12 rows.  On MI355X, 4-row strips ran a 540 x 1040 one-shot
+// gradient 21 % faster than 8-row strips; 12-row strips ran a 1040 x 2040
+// one 6 % faster than 4-row strips, and a 2040 x 6040 one 12 % faster than
+// 39-row strips.
+constexpr int MARCH_WAVES_PER_CU_ONE_SHOT = 10;
 constexpr int MARCH_ROWS_MIN_ONE_SHOT = 4;
+constexpr int MARCH_ROWS_MAX_ONE_SHOT = 12;
 constexpr int NUM_XCD = 8;          // MI355X: 8 XCDs, each with its own L2
 constexpr int FUSED_WAVES = 4;      // fused two-stage kernels: wavefronts per block
 #ifndef MARCH_UNROLL
@@ -131,6 +142,15 @@ void launch_adj_vel_grad(const DevGrid &g, int nshots, const pyfwi_shot *shots, 
                          const float *lam, const float *mu, const float *bu,
                          const float *dxp, const float *dzp, float *gacc, bool precond,
                          hipStream_t st, const float *res = nullptr);
+// Both of the above in one launch (order 4 only): the adjoint state is read
+// from `ain` and written to `aout` (distinct buffers, [nshots][5][fstride]
+// each, swapped by the caller every step); the same values, residual terms
+// and accumulators as launch_adj_str followed by launch_adj_vel_grad.
+void launch_adj_fused(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
+                      const float *fields, int64_t fields_stride, const float *ain, float *aout,
+                      const float *lam, const float *mu, const float *bu, const float *dxp,
+                      const float *dzp, float *gacc, bool precond, hipStream_t st,
+                      const float *res = nullptr);
 void launch_l2_residual(int nshots, const pyfwi_shot *shots, const float *seis, const float *obs,
                         float *res, int comp_mask, double *misfit, hipStream_t st);
 // Records `in` -> `out` (same layout, distinct buffers) filtered along time

@@ -1418,6 +1418,369 @@ __global__ __launch_bounds__(64 * SPB) void k_adj_vel_grad(
     }
 }
 
+// ------------------------------------------- adjoint, pyfwi mode, one launch
+//
+// Both adjoint kernels above in one launch per backward step.  Stage 1 is
+// k_adj_str at row i (adjoint stresses, the lambda / mu imaging terms);
+// stage 2 is k_adj_vel_grad at row q = i - SDO (adjoint velocities, the rho
+// term), fed by stage 1's new stresses through register windows along z and
+// lane shifts along x, so the new stresses never go through memory before
+// stage 2 reads them.  The arithmetic of every value is that of the two
+// kernels.
+//
+// The adjoint state is read from `ain` and written to `aout` (the caller
+// swaps them every step): the tiles around this one read its cells for
+// their halo work while it writes.  A wave covers 64 columns and owns the
+// 64 - 2*SDO in the middle; its SDO lanes on each side run stage 1 only, for
+// the x-derivatives of their neighbours, and each strip runs stage 1 on SDO
+// more rows above and below.  Stage 1 is forced to zero off the active cells,
+// which the reference never updates.  Per shot and cell the adjoint state is
+// read and written once (40 B), the forward state read once (20 B) and the
+// model once (12 B): 72 B against 92 B for the two launches, plus the
+// accumulators per block's group of shots.  Order 4 only.
+
+// Tile of a fused adjoint launch: as tile_lane, with column blocks of
+// 64 - 2*SDO owned columns; lane l of block cb sits on column cb*SPAN + l.
+template <int SDO, int SPB>
+__device__ __forceinline__ bool tile_adj_fused(const DevGrid &g, int &s, int &j, int &i0, int &i1,
+                                               bool &own, bool &colact)
+{
+    constexpr int SPAN = 64 - 2 * SDO;
+    const int lane = (int)threadIdx.x % 64;
+    int cb, st;
+    if (SPB > 1 && g.nshot == 1) {
+        const int ncq = (g.ncb + SPB - 1) / SPB;
+        const int T = ncq * g.nstrips;
+        const int per = (T + NUM_XCD - 1) / NUM_XCD;
+        const int w = (int)blockIdx.x;
+        const int logical = (w % NUM_XCD) * per + w / NUM_XCD;
+        if (logical >= T) return false;
+        cb = (logical % ncq) * SPB + wave_id();
+        st = logical / ncq;
+        s = 0;
+        if (cb >= g.ncb) return false;
+    } else {
+        const int T = g.ncb * g.nstrips * g.nbatch;
+        const int per = (T + NUM_XCD - 1) / NUM_XCD;
+        const int w = (int)blockIdx.x;
+        const int logical = (w % NUM_XCD) * per + w / NUM_XCD;
+        if (logical >= T) return false;
+        const int rest = logical / g.nbatch;
+        cb = rest % g.ncb;
+        st = rest / g.ncb;
+        s = (logical % g.nbatch) * SPB + wave_id();
+        if (s >= g.nshot) return false;
+    }
+    j = cb * SPAN + lane;
+    colact = j >= SDO && j <= g.nx - SDO;
+    own = lane >= SDO && lane < 64 - SDO && colact;
+    i0 = SDO + st * g.mrows;
+    i1 = min(i0 + g.mrows - 1, g.nz - SDO);
+    return true;
+}
+
+// A lane's value at column offsets -SDO..SDO from the neighbouring lanes
+// (wave shifts by one lane); the SDO edge lanes on each side get zeros from
+// outside the wave, so only the owned lanes of a fused tile may use them.
+template <int SDO>
+struct LaneNbr {
+    float s[2 * SDO + 1];
+    __device__ __forceinline__ explicit LaneNbr(float c)
+    {
+        s[SDO] = c;
+        float l = c, r = c;
+#pragma unroll
+        for (int k = 1; k <= SDO; ++k) {
+            l = dpp_mov<DPP_WAVE_SHR1>(0.0f, l);  // lane i <- lane i-1
+            r = dpp_mov<DPP_WAVE_SHL1>(0.0f, r);  // lane i <- lane i+1
+            s[SDO - k] = l;
+            s[SDO + k] = r;
+        }
+    }
+    __device__ __forceinline__ float operator()(int k) const { return s[SDO + k]; }
+};
+
+template <int SDO, bool PRECOND, bool ALONE, int SPB>
+__global__ __launch_bounds__(64 * SPB) void k_adj_fused(
+    DevGrid g, const pyfwi_shot *shots, int t, const float *__restrict__ fields,
+    int64_t fields_stride, const float *__restrict__ ain, float *__restrict__ aout,
+    const float *__restrict__ lam, const float *__restrict__ mu, const float *__restrict__ bu,
+    const float *__restrict__ dxp, const float *__restrict__ dzp, float *__restrict__ gacc,
+    const float *__restrict__ res)
+{
+    constexpr int NV1 = PRECOND ? 4 : 2;  // stage 1 terms: mu, lam (, mu_pre, lam_pre)
+    constexpr int NV2 = PRECOND ? 2 : 1;  // stage 2 terms: rho (, rho_pre)
+    constexpr int NV = NV1 + NV2;
+    constexpr int PART = ALONE ? 1 : ADJ_GROUP_ROWS * SPB * NV * 64;
+    __shared__ float part[2][PART];
+    int s, j, i0, i1;
+    bool own, colact;
+    if (!tile_adj_fused<SDO, SPB>(g, s, j, i0, i1, own, colact)) return;
+    if (!bwd_active(shots, s, t)) return;
+    GroupLive gl;  // the shot group (one-shot launches have none)
+    if constexpr (!ALONE) gl = group_live<SPB>(g, shots, s, t);
+    const int wave = wave_id(), lane = (int)threadIdx.x % 64;
+    const int64_t fs = g.fstride;
+    const uint32_t RB = (uint32_t)g.pitch * 4u;
+    // bases 2*SDO rows above cell (0, 0), so that the byte offsets of every
+    // row the kernel touches (from -SDO) are non-negative
+    const int64_t org = g.origin - (int64_t)2 * SDO * g.pitch;
+    const float *AI = ain + (int64_t)s * 5 * fs + org;
+    float *AO = aout + (int64_t)s * 5 * fs + org;
+    const CellRef avx{AI}, avz{AI + fs}, atxx{AI + 2 * fs}, atzz{AI + 3 * fs}, atxz{AI + 4 * fs};
+    const CellRef B{bu + org}, L{lam + org}, M{mu + org};
+    const float *F = fields + (int64_t)s * fields_stride + org;
+    const CellRef vx{F}, vz{F + fs}, txx{F + 2 * fs}, tzz{F + 3 * fs}, txz{F + 4 * fs};
+    float *G = gacc + (int64_t)(s / SPB) * 6 * fs + org;
+    constexpr int slots1[4] = {0, 2, 1, 3};  // accumulators of the stage 1 terms
+    constexpr int slots2[2] = {4, 5};        // and of the stage 2 terms
+    // step t's residual on the adjoint stresses at this lane's receiver
+    // (stage 1, halo rows and lanes included); step t-1's on the adjoint
+    // velocities it writes (stage 2)
+    const RecLine rq = residual_line(shots[s], t, res);
+    const int rr = colact && rq.z >= i0 - SDO && rq.z <= i1 + SDO ? rq.index(j) : -1;
+    const int rrow = rr >= 0 ? rq.z : -1;
+    const float r_txx = rr >= 0 ? rq.base[2 * rq.fs + rr] : 0.0f;
+    const float r_tzz = rr >= 0 ? rq.base[3 * rq.fs + rr] : 0.0f;
+    const float r_txz = rr >= 0 ? rq.base[4 * rq.fs + rr] : 0.0f;
+    const RecLine rq2 = residual_line(shots[s], t - 1, res);
+    const int rr2 = own && rq2.z >= i0 && rq2.z <= i1 ? rq2.index(j) : -1;
+    const int rrow2 = rr2 >= 0 ? rq2.z : -1;
+    const float r_vx = rr2 >= 0 ? rq2.base[rr2] : 0.0f;
+    const float r_vz = rr2 >= 0 ? rq2.base[rq2.fs + rr2] : 0.0f;
+    const int lo = g.npml + SDO, hi = g.nz - SDO - g.npml, hix = g.nx - SDO - g.npml;
+    const bool img = own && j >= lo && j <= hix && i1 >= lo && i0 <= hi;
+    const int lo1 = max(i0, lo), hi1 = min(i1, hi);  // imaged rows of the strip
+    // a wave with an imaging lane loads the forward state (wave-uniform)
+    const bool imgl = __builtin_amdgcn_readfirstlane((int)(__ballot(img) != 0)) != 0;
+    const float ex = colact ? dxp[j] : 0.0f;
+    const float dt = g.dt;
+    auto dx_bvx = [&](uint32_t o) {
+        return xplus<SDO>([&](int k) { return avx.at(o, k) * B.at(o, k); }, g.cx); };
+    auto dx_bvz = [&](uint32_t o) {
+        return xminus<SDO>([&](int k) { return avz.at(o, k) * B.at(o, k); }, g.cx); };
+
+    ZWin<SDO> wbz, wbx;      // b*avz (d/dz-), b*avx (d/dz+) around stage 1's row
+    Fifo<SDO> fxp, fxm;      // d/dx+ (b*avx), d/dx- (b*avz), taken on the entering row
+    Fifo<2 * SDO> dvx, dvz;  // old adjoint velocities, delayed to stage 2's row
+    ZWin<SDO> fvz{};         // forward vz around stage 1's row
+    Fifo<SDO> fdvz{};        // its d/dx-, taken on the entering row
+    uint32_t cb = ((uint32_t)(i0 - SDO + 2 * SDO) * (uint32_t)g.pitch + (uint32_t)j) * 4u;
+#pragma unroll
+    for (int k = 0; k < 2 * SDO; ++k) {
+        const uint32_t o = cb + (uint32_t)(k - SDO) * RB;
+        const float bo = B(o), uo = avx(o), wo = avz(o);
+        wbz.v[k + 1] = wo * bo;
+        wbx.v[k + 1] = uo * bo;
+        dvx.v[k] = uo;
+        dvz.v[k] = wo;
+        if (k >= SDO) {
+            fxp.v[k - SDO] = dx_bvx(o);
+            fxm.v[k - SDO] = dx_bvz(o);
+        }
+        if (imgl) {
+            Taps<SDO, 1> tvz;
+            tvz.load(vz, o);
+            fvz.v[k + 1] = tvz(0);
+            if (k >= SDO) fdvz.v[k - SDO] = xminus<SDO>(tvz, g.cx);
+        }
+    }
+    ZWin<SDO> wa3{}, wb1{}, wb2{};  // new mu*Atxz (d/dz-), (lam+2mu)*Atzz, lam*Atxx (d/dz+)
+    Fifo<SDO> fa1{}, fa2{}, fb3{};  // d/dx- (lam+2mu)*Atxx, d/dx- lam*Atzz, d/dx+ mu*Atxz
+    ZWin<SDO> ftzz{}, ftxz{};       // forward tzz (d/dz+), txz (d/dz-) around stage 2's row
+    Fifo<SDO> fdtxz{};              // d/dx+ forward txz, delayed to stage 2's row
+
+    int buf = 0;
+    for (int ic = i0 - SDO; ic <= i1 + SDO; ic += ADJ_GROUP_ROWS, buf ^= 1) {
+        const uint32_t c0 = cb;
+        float *mine = &part[buf][(wave * NV) * 64 + lane];
+#pragma unroll
+        for (int r = 0; r < ADJ_GROUP_ROWS; ++r, cb += RB) {
+            const int i = ic + r, q = i - SDO;
+            if (i > i1 + SDO) break;
+            const uint32_t n = cb + (uint32_t)SDO * RB, cq = cb - (uint32_t)SDO * RB;
+            const bool row_img1 = img && i >= lo1 && i <= hi1;
+            const bool row_img2 = img && q >= lo1 && q <= hi1;
+            // loads: adjoint state, model, forward state, accumulators
+            const float bn = B(n), uxn = avx(n), uzn = avz(n);
+            const float dxpb_n = dx_bvx(n), dxmb_n = dx_bvz(n);
+            const float a0 = atxx(cb), b0 = atzz(cb), d0 = atxz(cb);
+            const float l = L(cb), m = M(cb);
+            Taps<SDO, 1> tvz;   // forward vz on the entering row (d/dx-)
+            Taps<SDO, 0> tvx;   // forward vx on stage 1's row (d/dx+)
+            Taps<SDO, 0> ttxz;  // forward txz on stage 1's row (d/dx+)
+            Taps<SDO, 1> ttxx;  // forward txx on stage 2's row (d/dx-)
+            float f_tzz = 0.0f, s_txx = 0.0f, w_vx = 0.0f, bq = 0.0f;
+            if (imgl) {
+                tvz.load(vz, n);
+                tvx.load(vx, cb);
+                ttxz.load(txz, cb);
+                ttxx.load(txx, cq);
+                f_tzz = tzz(cb);
+                if constexpr (PRECOND) {
+                    s_txx = txx(cb);
+                    w_vx = vx(cq);
+                }
+                bq = B(cq);
+            }
+            float gc1[NV1] = {}, gc2[NV2] = {};  // the rows' accumulators (ALONE)
+            if constexpr (ALONE) {
+                if (row_img2) {
+#pragma unroll
+                    for (int v = 0; v < NV2; ++v) gc2[v] = cell_ref(G + slots2[v] * fs, cq);
+                }
+                if (row_img1) {
+#pragma unroll
+                    for (int v = 0; v < NV1; ++v) gc1[v] = cell_ref(G + slots1[v] * fs, cb);
+                }
+            }
+            // stage 1: adjoint stresses at row i, on every lane
+            wbz.push(uzn * bn);
+            wbx.push(uxn * bn);
+            const float dxpb = fxp.pop(dxpb_n);
+            const float dxmb = fxm.pop(dxmb_n);
+            const float uxq = dvx.pop(uxn), uzq = dvz.pop(uzn);
+            const float ez = dzp[i];
+            const float dzmb = wbz.dminus(g.cz);
+            const float dzpb = wbx.dplus(g.cz);
+            const bool at_rec = i == rrow;
+            const float a = at_rec ? a0 + r_txx : a0;
+            const float bb = at_rec ? b0 + r_tzz : b0;
+            const float d = at_rec ? d0 + r_txz : d0;
+            const bool act1 = colact && i >= SDO && i <= g.nz - SDO;
+            const float aa = act1 ? fmaf(-dt, ex * a + dxpb, a) : 0.0f;
+            const float ab = act1 ? fmaf(-dt, ez * bb + dzmb, bb) : 0.0f;
+            const float ad = act1 ? fmaf(-dt, (ex + ez) * d + (dzpb + dxmb), d) : 0.0f;
+            // stage 1's products for stage 2: z-windows, and x-derivatives
+            // from the neighbouring lanes delayed to stage 2's row
+            const float lm = l + 2.0f * m;
+            wa3.push(m * ad);
+            wb1.push(lm * ab);
+            wb2.push(l * aa);
+            const float a1 = fa1.pop(xminus<SDO>(LaneNbr<SDO>(lm * aa), g.cx));
+            const float a2 = fa2.pop(xminus<SDO>(LaneNbr<SDO>(l * ab), g.cx));
+            const float b3 = fb3.pop(xplus<SDO>(LaneNbr<SDO>(m * ad), g.cx));
+            // forward state around both rows
+            fvz.push(tvz(0));
+            const float dxm_vz = fdvz.pop(xminus<SDO>(tvz, g.cx));
+            ftzz.push(f_tzz);
+            ftxz.push(ttxz(0));
+            const float dxp_txz = fdtxz.pop(xplus<SDO>(ttxz, g.cx));
+            float term1[NV1], term2[NV2];
+            if (row_img1) {
+                const float dxp_vx = xplus<SDO>(tvx, g.cx);
+                const float dzp_vz = fvz.dplus(g.cz);
+                const float dzm_vz = fvz.dminus(g.cz);
+                term1[0] = 2.0f * dt * (aa * dxp_vx + ab * dzm_vz) + dt * ad * (dzp_vz + dxm_vz);
+                term1[1] = dt * (aa + ab) * (dxp_vx + dzm_vz);
+                if constexpr (PRECOND) {
+                    const float dt2 = dt * dt;
+                    const float s_tzz = ftzz.v[2 * SDO], s_txz = ftxz.v[2 * SDO];  // row i
+                    term1[2] = 4.0f * dt2 * (s_txx * s_txx * dxp_vx * dxp_vx +
+                                             s_tzz * s_tzz * dzm_vz * dzm_vz) +
+                               dt2 * s_txz * s_txz * (dzp_vz * dzp_vz + dxm_vz * dxm_vz);
+                    term1[3] = dt2 * (s_txx * s_txx + s_tzz * s_tzz) *
+                               (dxp_vx * dxp_vx + dzm_vz * dzm_vz);
+                }
+            }
+            // stage 2: adjoint velocities at row q
+            const float a3 = wa3.dminus(g.cz);
+            const float b1 = wb1.dplus(g.cz);
+            const float b2 = wb2.dplus(g.cz);
+            const float ezq = dzp[q];
+            const float ux = fmaf(-dt, (2.0f * ex + ezq) * uxq + (a1 + a2 + a3), uxq);
+            const float uz = fmaf(-dt, (ex + 2.0f * ezq) * uzq + (b2 + b1 + b3), uzq);
+            if (row_img2) {
+                const float dxm_txx = xminus<SDO>(ttxx, g.cx);
+                const float dzp_tzz = ftzz.dplus(g.cz);
+                const float dzm_txz = ftxz.dminus(g.cz);
+                term2[0] = -(dt * (bq * bq)) * (ux * (dxm_txx + dzm_txz) + uz * (dzp_tzz + dxp_txz));
+                if constexpr (PRECOND) {
+                    const float dt2 = dt * dt;
+                    const float wz = fvz.v[0];  // forward vz at row q
+                    term2[1] = dt2 * (w_vx * w_vx * (dxm_txx * dxm_txx + dzm_txz * dzm_txz) +
+                                      wz * wz * (dzp_tzz * dzp_tzz + dxp_txz * dxp_txz));
+                }
+            }
+            // stores
+            if (own && i >= i0 && i <= i1) {
+                cell_ref(AO + 2 * fs, cb) = aa;
+                cell_ref(AO + 3 * fs, cb) = ab;
+                cell_ref(AO + 4 * fs, cb) = ad;
+            }
+            if (own && q >= i0) {
+                const bool at_rec2 = q == rrow2;
+                cell_ref(AO, cq) = at_rec2 ? ux + r_vx : ux;
+                cell_ref(AO + fs, cq) = at_rec2 ? uz + r_vz : uz;
+            }
+            if constexpr (ALONE) {
+                if (row_img1) {
+#pragma unroll
+                    for (int v = 0; v < NV1; ++v) cell_ref(G + slots1[v] * fs, cb) = gc1[v] + term1[v];
+                }
+                if (row_img2) {
+#pragma unroll
+                    for (int v = 0; v < NV2; ++v) cell_ref(G + slots2[v] * fs, cq) = gc2[v] + term2[v];
+                }
+            } else {
+                float *pq = mine + r * SPB * NV * 64;
+                if (row_img1) {
+#pragma unroll
+                    for (int v = 0; v < NV1; ++v) pq[v * 64] = term1[v];
+                }
+                if (row_img2) {
+#pragma unroll
+                    for (int v = 0; v < NV2; ++v) pq[(NV1 + v) * 64] = term2[v];
+                }
+            }
+        }
+        if constexpr (!ALONE) {
+            __syncthreads();
+            // the staged rows dealt out to the live waves, each adding the
+            // shots' terms in shot order onto the group's accumulators
+            for (int r = gl.rank; r < ADJ_GROUP_ROWS; r += gl.count) {
+                const int i = ic + r, q = i - SDO;
+                if (i > i1 + SDO) break;
+                if (!img) continue;
+                const uint32_t o1 = c0 + (uint32_t)r * RB, o2 = o1 - (uint32_t)SDO * RB;
+                const float *pr = &part[buf][r * SPB * NV * 64 + lane];
+                if (i >= lo1 && i <= hi1) {
+                    float cur[NV1], acc[NV1];
+#pragma unroll
+                    for (int v = 0; v < NV1; ++v) {
+                        cur[v] = cell_ref(G + slots1[v] * fs, o1);
+                        acc[v] = 0.0f;
+                    }
+#pragma unroll
+                    for (int w = 0; w < SPB; ++w) {
+                        if (!(gl.mask >> w & 1u)) continue;
+#pragma unroll
+                        for (int v = 0; v < NV1; ++v) acc[v] += pr[(w * NV + v) * 64];
+                    }
+#pragma unroll
+                    for (int v = 0; v < NV1; ++v) cell_ref(G + slots1[v] * fs, o1) = cur[v] + acc[v];
+                }
+                if (q >= lo1 && q <= hi1) {
+                    float cur[NV2], acc[NV2];
+#pragma unroll
+                    for (int v = 0; v < NV2; ++v) {
+                        cur[v] = cell_ref(G + slots2[v] * fs, o2);
+                        acc[v] = 0.0f;
+                    }
+#pragma unroll
+                    for (int w = 0; w < SPB; ++w) {
+                        if (!(gl.mask >> w & 1u)) continue;
+#pragma unroll
+                        for (int v = 0; v < NV2; ++v) acc[v] += pr[(w * NV + NV1 + v) * 64];
+                    }
+#pragma unroll
+                    for (int v = 0; v < NV2; ++v) cell_ref(G + slots2[v] * fs, o2) = cur[v] + acc[v];
+                }
+            }
+        }
+    }
+}
+
 // ------------------------------------------------- adjoint, exact mode
 //
 // Exact transpose of the forward step, with D+^T = -D- and D-^T = -D+ on
@@ -1978,16 +2341,35 @@ __global__ void k_unpad(DevGrid g, const float *src, float *dst, int nf)
 // Strip height for a launch of `per_strip` working wavefronts per row strip:
 // as many strips as keep the launch within `per_cu` such waves per CU (tall
 // strips, fewer halo rows, when many shots are batched; short strips when a
-// single shot must still fill the GPU), within [min_rows, MARCH_ROWS_MAX].
-// Waves of a block without a shot leave at once and are not counted.
-static int strip_rows(const DevGrid &g, int nz_act, int64_t per_strip,
-                      int per_cu = MARCH_WAVES_PER_CU, int min_rows = MARCH_ROWS_MIN)
+// single shot must still fill the GPU), within [min_rows, max_rows].  Waves
+// of a block without a shot leave at once and are not counted.  The height is
+// then evened out over the strips, so the last strip is not a sliver that
+// costs a round of waves of its own.
+static int strip_rows(const DevGrid &g, int nz_act, int64_t per_strip, int per_cu, int min_rows,
+                      int max_rows)
 {
     if (g.rows_override > 0) return g.rows_override;
     const int64_t room = (int64_t)per_cu * g.ncu;
     const int nstrips = (int)std::max<int64_t>(1, std::min<int64_t>(nz_act, room / per_strip));
-    const int rows = (nz_act + nstrips - 1) / nstrips;
-    return std::max(min_rows, std::min(MARCH_ROWS_MAX, rows));
+    int rows = (nz_act + nstrips - 1) / nstrips;
+    rows = std::max(min_rows, std::min(max_rows, rows));
+    const int n = (nz_act + rows - 1) / rows;
+    return std::max(min_rows, (nz_act + n - 1) / n);
+}
+
+// Strip height of a column-marching launch of `nshots` shots, `ncb` column
+// blocks each (tall_ok: the exact adjoint's kernels, whose launches of 20
+// shots at 540 x 1040 cells ran 4-7 % faster with the one-round 36-row strips
+// than with 12, 18 or 24 rows).
+static int march_rows(const DevGrid &g, int nz_act, int ncb, int nshots, bool tall_ok = false)
+{
+    if (nshots == 1)
+        return strip_rows(g, nz_act, ncb, MARCH_WAVES_PER_CU_ONE_SHOT, MARCH_ROWS_MIN_ONE_SHOT,
+                          MARCH_ROWS_MAX_ONE_SHOT);
+    const int one_round = strip_rows(g, nz_act, (int64_t)ncb * nshots, MARCH_WAVES_PER_CU,
+                                     MARCH_ROWS_MIN, MARCH_ROWS_MAX);
+    if (tall_ok || one_round <= MARCH_ROWS_ONE_ROUND || g.rows_override > 0) return one_round;
+    return strip_rows(g, nz_act, 1, 1, MARCH_ROWS_MANY, MARCH_ROWS_MANY);
 }
 
 // Shots per column-marching block.  Five when a launch's shots fill blocks
@@ -2003,16 +2385,14 @@ int march_spb(int sdo, int nshots)
 
 // 1-D grid of (column block, row strip, shot group) tiles for blocks of spb
 // shots, XCD-interleaved (see tile()).
-static dim3 march_grid(DevGrid &g, int sdo, int nshots, int spb)
+static dim3 march_grid(DevGrid &g, int sdo, int nshots, int spb, int span = 64,
+                       bool tall_ok = false)
 {
     const int nx_act = g.nx - 2 * sdo + 1, nz_act = g.nz - 2 * sdo + 1;
-    g.ncb = (nx_act + 63) / 64;
+    g.ncb = (nx_act + span - 1) / span;
     g.nbatch = (nshots + spb - 1) / spb;
     g.nshot = nshots;
-    const int rows = nshots == 1
-                         ? strip_rows(g, nz_act, g.ncb, MARCH_WAVES_PER_CU_ONE_SHOT,
-                                      MARCH_ROWS_MIN_ONE_SHOT)
-                         : strip_rows(g, nz_act, (int64_t)g.ncb * nshots);
+    const int rows = march_rows(g, nz_act, g.ncb, nshots, tall_ok);
     g.mrows = rows;
     g.nstrips = (nz_act + rows - 1) / rows;
     // one shot: tiles of spb column blocks (see tile_lane)
@@ -2074,7 +2454,8 @@ static dim3 fused_grid(DevGrid &g, int sdo, int nshots)
     g.ncb = (nx_act + span - 1) / span;
     g.nbatch = nshots;  // fused tiles: one shot per block
     g.nshot = nshots;
-    const int rows = strip_rows(g, nz_act, (int64_t)g.ncb * FUSED_WAVES * nshots);
+    const int rows = strip_rows(g, nz_act, (int64_t)g.ncb * FUSED_WAVES * nshots,
+                                MARCH_WAVES_PER_CU, MARCH_ROWS_MIN, MARCH_ROWS_MAX);
     g.mrows = rows;
     g.nstrips = (nz_act + rows - 1) / rows;
     const int T = g.ncb * g.nstrips * g.nbatch;
@@ -2157,6 +2538,31 @@ void launch_adj_vel_grad(const DevGrid &g, int nshots, const pyfwi_shot *shots, 
 #undef ARGS
 }
 
+void launch_adj_fused(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
+                      const float *fields, int64_t fields_stride, const float *ain, float *aout,
+                      const float *lam, const float *mu, const float *bu, const float *dxp,
+                      const float *dzp, float *gacc, bool precond, hipStream_t st,
+                      const float *res)
+{
+    if (g.sdo != 2) return;  // order 4 only; the plan takes the two launches otherwise
+    DevGrid gm = g;
+    const int spb = march_spb(g.sdo, nshots);
+    dim3 grid = march_grid(gm, g.sdo, nshots, spb, 64 - 2 * g.sdo), block(64 * spb);
+    const bool alone = nshots == 1;
+#define ARGS gm, shots, t, fields, fields_stride, ain, aout, lam, mu, bu, dxp, dzp, gacc, res
+#define LAUNCH(P, A)                                                                   \
+    do {                                                                               \
+        if (spb == 5) k_adj_fused<2, P, A, 5><<<grid, block, 0, st>>>(ARGS);            \
+        else k_adj_fused<2, P, A, 4><<<grid, block, 0, st>>>(ARGS);                     \
+    } while (0)
+    if (precond && alone) LAUNCH(true, true);
+    else if (precond) LAUNCH(true, false);
+    else if (alone) LAUNCH(false, true);
+    else LAUNCH(false, false);
+#undef LAUNCH
+#undef ARGS
+}
+
 void launch_exact_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
                       const float *vel, int64_t vel_stride, float *adj, float *av,
                       const float *lam, const float *mu, const float *dxp,
@@ -2164,7 +2570,7 @@ void launch_exact_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int
 {
     DevGrid gm = g;
     const int spb = march_spb(g.sdo, nshots);
-    dim3 grid = march_grid(gm, g.sdo, nshots, spb), block(64 * spb);
+    dim3 grid = march_grid(gm, g.sdo, nshots, spb, 64, true), block(64 * spb);
     const bool alone = nshots == 1;
 #define ARGS gm, shots, t, vel, vel_stride, adj, av, lam, mu, dxp, dzp, gacc
     if (alone) DISPATCH_MARCH(g.sdo, spb, k_exact_str, , true);
@@ -2179,7 +2585,7 @@ void launch_exact_vel(const DevGrid &g, int nshots, const pyfwi_shot *shots, int
 {
     DevGrid gm = g;
     const int spb = march_spb(g.sdo, nshots);
-    dim3 grid = march_grid(gm, g.sdo, nshots, spb), block(64 * spb);
+    dim3 grid = march_grid(gm, g.sdo, nshots, spb, 64, true), block(64 * spb);
     const bool alone = nshots == 1;
 #define ARGS gm, shots, t, tau, tau_stride, adj, av_in, av_out, bu, dxp, dzp, gacc
     if (alone) DISPATCH_MARCH(g.sdo, spb, k_exact_vel, , true);

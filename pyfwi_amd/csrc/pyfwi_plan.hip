@@ -117,10 +117,11 @@ struct pyfwi_plan {
     DevBuf lam, mu, bu, dxp_buf, dzp_buf, scratch_in;
     // damping profiles with HALO zero entries on both sides: dxp[-HALO..nx+HALO)
     float *dxp = nullptr, *dzp = nullptr;
-    DevBuf fields, fields2, adj, av0, av1, gacc, snaps, ckpts, seg;
+    DevBuf fields, fields2, adj, adj2, av0, av1, gacc, snaps, ckpts, seg;
     float *cur = nullptr;  // buffer (fields or fields2) holding the current forward state
     // tuning: shots per pass (0: all), fused steps, strip height (0: automatic)
     int tune_group = 0, tune_fused = 0, tune_rows = 0;
+    int adj_fused = 0;  // reference adjoint in one launch a step (PYFWI_TUNE_ADJ_FUSED)
     bool ckpts_all = false;  // ckpts holds every shot of the last forward
     DevBuf src_stage, seis_stage, res_stage, grad_stage, shot_tab_f, resid;
     // record filter of the misfit (pyfwi_set_record_filter): taps k[0..L]
@@ -389,7 +390,8 @@ KeyHash loop_key(const pyfwi_plan *p, int kind, int g0, int ng)
     KeyHash k;
     k.val(kind).val(g0).val(ng).val(p->dg).val(p->src_inside).val(p->rec_inline)
         .val(p->d_shots).val(p->d_rtab).val(p->lam.p).val(p->mu.p).val(p->bu.p)
-        .val(p->dxp).val(p->dzp).val(p->fields.p).val(p->adj.p).val(p->gacc.p);
+        .val(p->dxp).val(p->dzp).val(p->fields.p).val(p->adj.p).val(p->adj2.p)
+        .val(p->adj_fused).val(p->gacc.p);
     k.add(p->h_shots.data() + g0, sizeof(pyfwi_shot) * (size_t)ng);
     return k;
 }
@@ -509,7 +511,7 @@ int32_t pyfwi_plan_destroy(pyfwi_plan *p)
     for (DevBuf *b : {&p->fir_taps, &p->filt_a, &p->filt_b})
         b->release();
     for (DevBuf *b : {&p->lam, &p->mu, &p->bu, &p->dxp_buf, &p->dzp_buf, &p->scratch_in, &p->fields, &p->fields2,
-                      &p->adj, &p->av0, &p->av1, &p->gacc, &p->snaps, &p->ckpts, &p->seg,
+                      &p->adj, &p->adj2, &p->av0, &p->av1, &p->gacc, &p->snaps, &p->ckpts, &p->seg,
                       &p->src_stage, &p->seis_stage, &p->res_stage, &p->grad_stage,
                       &p->shot_tab_f, &p->resid})
         b->release();
@@ -1045,6 +1047,14 @@ int gradient_pyfwi(pyfwi_plan *p, int g0, int ng, const float *d_res, bool preco
     const int nchp = (int)cl.size();
     if (int e = ensure(p->adj, (size_t)ng * fps, p->stream, "adjoint fields")) return e;
     launch_zero(p->adj.p, (int64_t)ng * fps, p->stream);
+    // one launch a step (order 4, receivers on the active cells): the
+    // adjoint state moves between two buffers, read from one and written to
+    // the other, so both start at zero and stay zero off the active cells
+    const bool one_launch = p->adj_fused && p->rec_inline && g.sdo == 2;
+    if (one_launch) {
+        if (int e = ensure(p->adj2, (size_t)ng * fps, p->stream, "adjoint fields")) return e;
+        launch_zero(p->adj2.p, (int64_t)ng * fps, p->stream);
+    }
     if (int e = prepare_fields(p, ng)) return e;
     // steps at which a shot of the pass starts its backward sweep
     std::vector<char> first_step(nt + 1, 0);
@@ -1060,6 +1070,7 @@ int gradient_pyfwi(pyfwi_plan *p, int g0, int ng, const float *d_res, bool preco
     key.add(cl.data(), cl.size() * sizeof(int32_t));
     return run_loop(p, key.h, [&]() -> int {
     int chpc = nchp - 1;
+    float *ain = p->adj.p, *aout = one_launch ? p->adj2.p : p->adj.p;
     for (int t = nt - 1; t > 0; --t) {
         const float *fwd = p->cur;
         int64_t fwd_stride = fps;
@@ -1082,14 +1093,24 @@ int gradient_pyfwi(pyfwi_plan *p, int g0, int ng, const float *d_res, bool preco
         const float *r_on = p->rec_inline ? d_res : nullptr;
         if (!p->rec_inline) {
             Timed ts(p, PYFWI_K_BWD_SPARSE);
-            launch_adj_inject(g, ng, dsh, t, p->adj.p, d_res, p->d_rtab, p->stream);
+            launch_adj_inject(g, ng, dsh, t, ain, d_res, p->d_rtab, p->stream);
         } else if (first_step[t]) {
             Timed ts(p, PYFWI_K_BWD_SPARSE);
-            launch_adj_inject(g, ng, dsh, t, p->adj.p, d_res, p->d_rtab, p->stream, 0, 2, true);
+            launch_adj_inject(g, ng, dsh, t, ain, d_res, p->d_rtab, p->stream, 0, 2, true);
         }
         // time-major: the live shots of step t only (gacc holds the sets of
         // every launch size up to ng, adj_acc_sets_upto)
         const int nl = tm && direct ? tm->live[(size_t)t] : ng;
+        if (one_launch) {
+            {
+                Timed tmr(p, PYFWI_K_ADJ_STEP);
+                launch_adj_fused(g, nl, dsh, t, fwd, fwd_stride, ain, aout, p->lam.p, p->mu.p,
+                                 p->bu.p, p->dxp, p->dzp, p->gacc.p, precond, p->stream, r_on);
+            }
+            std::swap(ain, aout);
+            if (int e = maybe_harvest(p)) return e;
+            continue;
+        }
         {
             Timed tmr(p, PYFWI_K_ADJ_STR);
             launch_adj_str(g, nl, dsh, t, fwd, fwd_stride, p->adj.p, p->bu.p, p->dxp, p->dzp,
@@ -1559,7 +1580,8 @@ int32_t pyfwi_plan_set_tuning(pyfwi_plan *p, int32_t shots_per_pass, int32_t fus
         return fail(PYFWI_EINVAL, "shots_per_pass and strip_rows must be >= 0 "
                                   "(strip_rows 0 or >= 2*sdo)");
     p->tune_group = shots_per_pass;
-    p->tune_fused = fused ? 1 : 0;
+    p->tune_fused = (fused & PYFWI_TUNE_FUSED_STEPS) ? 1 : 0;
+    p->adj_fused = (fused & PYFWI_TUNE_ADJ_FUSED) ? 1 : 0;
     p->tune_rows = strip_rows;
     p->dg.rows_override = strip_rows;
     p->have_forward = false;
