@@ -881,7 +881,8 @@ def main():
             def run():
                 b["plan"].misfit_gradient(trial, b["src"], b["obs"], b["seis"], b["grad"],
                                           chp_t, 0, 0, _native.MEM_DEVICE)
-            cands = tn.default_candidates(int(W.ns), max_pass, fused=(False,))
+            cands = tn.default_candidates(int(W.ns), max_pass,
+                                          fused=(0, _native.TUNE_ADJ_FUSED))
             tuning, _ = tn.autotune(b["plan"], run, torch.cuda.synchronize, cands, log=log)
         else:
             tuning, _ = tn.autotune_gradient(b["plan"], b["shots"], b["src"], b["seis"], b["res"],
@@ -985,7 +986,9 @@ def main():
                        "exact": "exact adjoint, two-level checkpoints sized to the free HBM, "
                                 "segments recomputed"}[args.mode],
                    "parallelism": "single GPU, shots batched per launch",
-                   "tuning": {"shots_per_pass": tuning[0], "fused_steps": bool(tuning[1]),
+                   "tuning": {"shots_per_pass": tuning[0],
+                              "fused_steps": bool(int(tuning[1]) & _native.TUNE_FUSED_STEPS),
+                              "adjoint_one_launch": bool(int(tuning[1]) & _native.TUNE_ADJ_FUSED),
                               "strip_rows": tuning[2], "graphs": bool(graphs)}},
         "kernel_timing": ("event-bracketed launches (each holds a few microseconds of the "
                           "event's stall)" if args.mode == "exact" else

@@ -1,7 +1,8 @@
 """Execution-shape autotuner for the batched gradient.
 
-A plan can run a call's shots in passes of G shots, do each time step as one
-fused launch or two in-place launches, and fix the strip height of the
+A plan can run a call's shots in passes of G shots, do each forward or
+time-reversed step as one fused launch or two in-place launches, do each
+reference-adjoint step as one launch or two, and fix the strip height of the
 column-marching kernels (``pyfwi_plan_set_tuning``).  The best choice depends
 on the grid, the number of shots and how much of the working set the 256 MB
 on-package cache holds, so it is measured: each candidate runs a forward and
@@ -18,15 +19,17 @@ from .wave_propagation import checkpoint_steps
 __all__ = ["default_candidates", "truncate", "autotune", "autotune_gradient"]
 
 
-def default_candidates(nshots, max_pass=None, fused=(False, True)):
-    """(shots per pass, fused, strip rows) triples; 0 rows = automatic."""
+def default_candidates(nshots, max_pass=None, fused=(0, _native.TUNE_FUSED_STEPS),
+                       rows=(12, 24)):
+    """(shots per pass, fused bits, strip rows) triples; 0 rows = automatic.
+    `fused`: the bit sets of ``Plan.set_tuning`` to try (TUNE_FUSED_STEPS,
+    TUNE_ADJ_FUSED; True stands for TUNE_FUSED_STEPS); the fixed strip
+    heights `rows` are tried with the largest pass and each bit set."""
     top = nshots if max_pass is None else max(1, min(nshots, max_pass))
     passes = sorted({g for g in (top, 10, 8, 5, 4, 2) if 1 <= g <= top}, reverse=True)
-    out = []
-    for g in passes:
-        for f in fused:
-            out.append((g, f, 0))
-    out += [(passes[0], False, 16), (passes[0], False, 48)]
+    bits = [int(f) for f in fused]
+    out = [(g, f, 0) for g in passes for f in bits]
+    out += [(passes[0], f, r) for f in bits for r in rows]
     return out
 
 

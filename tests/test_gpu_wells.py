@@ -173,3 +173,40 @@ def test_receiver_table_checks(gpu):
     with pytest.raises(_native.NativeError):
         plan.forward(sh, W._source_table(), np.zeros(5 * int(W.nt_ext) * int(W.nr), np.float32),
                      np.zeros(0, np.int32), 0, _native.MEM_HOST)
+
+
+@pytest.mark.parametrize("acq_type", [0, 2])
+def test_fwi_wells_gpu_tensors(gpu, acq_type):
+    """Fwi.apply with well acquisitions: the GPU-tensor path (device
+    gradient read-out, src/PyFWI/wave_propagation.py:279-331, whose mask for
+    crosswell keeps the columns right of the source well) equals the
+    CPU-tensor path (the host read-out) and the oracle's 'dv' gradient."""
+    import torch
+    from pyfwi_amd.torchfwi import Fwi
+    print(f"Fwi.apply wells acq_type={acq_type}", flush=True)
+    W0, st, model, model_obs = wells(acq_type, nt=300, components=0)
+    d_obs, _ = oracle_section(st, model_obs, components=0)
+    grads = {}
+    outs = {}
+    for dev in (gpu, torch.device("cpu")):
+        W, _, _, _ = wells(acq_type, nt=300, components=0)
+        m = {k: torch.tensor(v, device=dev, requires_grad=True) for k, v in model.items()}
+        tx, tz = Fwi.apply(W, m["vp"], m["vs"], m["rho"])
+        ox = torch.tensor(d_obs["taux"], device=dev)
+        oz = torch.tensor(d_obs["tauz"], device=dev)
+        (0.5 * ((tx - ox) ** 2).sum() + 0.5 * ((tz - oz) ** 2).sum()).backward()
+        grads[dev.type] = {k: m[k].grad.cpu().numpy() for k in m}
+        outs[dev.type] = (tx.detach().cpu().numpy(), tz.detach().cpu().numpy())
+    res = {"taux": outs["cuda"][0] - d_obs["taux"], "tauz": outs["cuda"][1] - d_obs["tauz"]}
+    _, states = O.forward(st, model, np.float32)
+    gr = O.gradient(st, model, res, states, parameterization="dv")
+    for k in ("vp", "vs", "rho"):
+        e_cpu = rel(grads["cuda"][k], grads["cpu"][k])
+        e_orc = rel(grads["cuda"][k], gr[k])
+        print(f"  {k}: vs CPU tensors {e_cpu:.2e}, vs oracle {e_orc:.2e}")
+        assert e_cpu < 1e-5, (k, e_cpu)
+        assert e_orc < GRAD_TOL, (k, e_orc)
+    if acq_type == 0:  # the crosswell mask: nothing left of the source well
+        sdo, src_col = int(W0.sdo), int(W0.src_cts)
+        for k in ("vp", "vs", "rho"):
+            assert not grads["cuda"][k][:, :sdo + src_col + 2].any(), k
