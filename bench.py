@@ -1,16 +1,17 @@
 """Benchmark: fwd+adj gradient Gcell-updates/s on one MI355X, batched vs per-shot.
 
-Workload (default): plain_medium_500x1000 -- nz=500, nx=1000, dh=5 m,
-seeded 20-layer random model, 20 Hz Ricker, dt from CFL 0.45, nt=2000,
-20 surface shots, 400 receivers, npml=20 (540 x 1040 cells with the
-absorbing layer), order-4 stencil, FP32.  One step = one full gradient of the
-L2 misfit, as pyfwi_misfit_gradient computes it: per pass of G shots
-(G bounded by the HBM the kept states take), batched forward modelling with
-every forward state kept (the reference's chpr = 100), residual and misfit
-against observed data on the device, and the reference-semantics adjoint
-(adjoint sweeps + imaging condition) of the pass.  `--mode reverse` instead
-keeps no states and rebuilds them by time reversal (chpr = 0).  Inputs are
-resident in HBM before the timed region.
+Workload (default): uneven_1000x2000_64shots -- nz=1000, nx=2000, dh=4 m
+(1040 x 2040 cells with the 20-cell absorbing layer), 64 surface shots, each
+with its own source position, nt (1564..3987), receiver aperture (201..1591)
+and Ricker frequency (10..25 Hz), order-4 stencil, FP32.  One step = one full
+gradient of the L2 misfit, as pyfwi_misfit_gradient computes it: the shots
+packed into passes whose kept states fill the free HBM, batched forward
+modelling with every forward state kept (the reference's chpr = 100),
+residual and misfit against observed data on the device, and the
+reference-semantics adjoint (adjoint sweeps + imaging condition) of each pass.
+`--workload plain_medium` runs the 500 x 1000-cell, 20-shot case instead;
+`--mode reverse` keeps no states and rebuilds them by time reversal
+(chpr = 0).  Inputs are resident in HBM before the timed region.
 
 A gradient cell-update is one (cell incl. absorbing layer, time step, shot)
 taken through forward plus adjoint; value = shots * cells * nt / seconds.
