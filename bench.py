@@ -677,6 +677,7 @@ def _instrument(fwi, m0, obs, budget, t_start):
             return first[1], first[2]
         if time.perf_counter() - t_start > budget:
             raise _Budget()
+        log(f"bench: evaluation {len(evals) + 1} starts")
         ts = time.perf_counter()
         out = inner(m, freq)
         evals.append((time.perf_counter() - ts, float(out[0])))
