@@ -142,11 +142,12 @@ void launch_adj_vel_grad(const DevGrid &g, int nshots, const pyfwi_shot *shots, 
                          const float *lam, const float *mu, const float *bu,
                          const float *dxp, const float *dzp, float *gacc, bool precond,
                          hipStream_t st, const float *res = nullptr);
-// Both of the above in one launch (order 4 only): the adjoint state is read
-// from `ain` and written to `aout` (distinct buffers, [nshots][5][fstride]
-// each, swapped by the caller every step); the same values, residual terms
-// and accumulators as launch_adj_str followed by launch_adj_vel_grad.
-void launch_adj_fused(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
+// Both of the above in one launch (order 4 only; false, launching nothing,
+// at order 8): the adjoint state is read from `ain` and written to `aout`
+// (distinct buffers, [nshots][5][fstride] each, swapped by the caller every
+// step); the same values, residual terms and accumulators as launch_adj_str
+// followed by launch_adj_vel_grad.
+bool launch_adj_fused(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
                       const float *fields, int64_t fields_stride, const float *ain, float *aout,
                       const float *lam, const float *mu, const float *bu, const float *dxp,
                       const float *dzp, float *gacc, bool precond, hipStream_t st,

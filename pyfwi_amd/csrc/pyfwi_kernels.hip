@@ -2538,13 +2538,13 @@ void launch_adj_vel_grad(const DevGrid &g, int nshots, const pyfwi_shot *shots, 
 #undef ARGS
 }
 
-void launch_adj_fused(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
+bool launch_adj_fused(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
                       const float *fields, int64_t fields_stride, const float *ain, float *aout,
                       const float *lam, const float *mu, const float *bu, const float *dxp,
                       const float *dzp, float *gacc, bool precond, hipStream_t st,
                       const float *res)
 {
-    if (g.sdo != 2) return;  // order 4 only; the plan takes the two launches otherwise
+    if (g.sdo != 2) return false;  // order 4 only
     DevGrid gm = g;
     const int spb = march_spb(g.sdo, nshots);
     dim3 grid = march_grid(gm, g.sdo, nshots, spb, 64 - 2 * g.sdo), block(64 * spb);
@@ -2561,6 +2561,7 @@ void launch_adj_fused(const DevGrid &g, int nshots, const pyfwi_shot *shots, int
     else LAUNCH(false, false);
 #undef LAUNCH
 #undef ARGS
+    return true;
 }
 
 void launch_exact_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,

@@ -1104,8 +1104,10 @@ int gradient_pyfwi(pyfwi_plan *p, int g0, int ng, const float *d_res, bool preco
         if (one_launch) {
             {
                 Timed tmr(p, PYFWI_K_ADJ_STEP);
-                launch_adj_fused(g, nl, dsh, t, fwd, fwd_stride, ain, aout, p->lam.p, p->mu.p,
-                                 p->bu.p, p->dxp, p->dzp, p->gacc.p, precond, p->stream, r_on);
+                if (!launch_adj_fused(g, nl, dsh, t, fwd, fwd_stride, ain, aout, p->lam.p,
+                                      p->mu.p, p->bu.p, p->dxp, p->dzp, p->gacc.p, precond,
+                                      p->stream, r_on))
+                    return fail(PYFWI_EINVAL, "the one-launch adjoint step needs order 4");
             }
             std::swap(ain, aout);
             if (int e = maybe_harvest(p)) return e;

@@ -39,6 +39,31 @@ def test_byte_model_c2_launches():
         (36 * CELLS + 8 * INTERIOR) * 10 + 16 * INTERIOR * 3
 
 
+def test_one_launch_adjoint_bytes_are_the_adjoint_cell_update():
+    """The one-launch adjoint step moves the adjoint cell-update's bytes:
+    with one shot a group, 96 B on an interior cell (adjoint state read and
+    written, forward state, model, three accumulators read and written), and
+    the two launches it replaces move 20 B more per shot on every cell (the
+    adjoint stresses read again, the adjoint velocities and b read by both) and
+    4 B more on interior cells (b read again for the rho term)."""
+    one = bench.launch_bytes("adj_step", INTERIOR, INTERIOR, 1)
+    assert one == 96 * INTERIOR
+    two = (bench.launch_bytes("adj_str", CELLS, INTERIOR, 1) +
+           bench.launch_bytes("adj_vel_grad", CELLS, INTERIOR, 1))
+    assert two - bench.launch_bytes("adj_step", CELLS, INTERIOR, 1) == 20 * CELLS + 4 * INTERIOR
+
+
+def test_tuning_candidates_are_bit_sets():
+    from pyfwi_amd import _native, tuning
+    c = tuning.default_candidates(20, 10, fused=(0, _native.TUNE_ADJ_FUSED), rows=(12,))
+    assert (10, 0, 0) in c and (10, _native.TUNE_ADJ_FUSED, 0) in c
+    assert (10, _native.TUNE_ADJ_FUSED, 12) in c
+    assert all(g <= 10 for g, _, _ in c)
+    # True stands for the fused forward / reverse steps
+    assert {f for _, f, _ in tuning.default_candidates(4, fused=(False, True))} == \
+        {0, _native.TUNE_FUSED_STEPS}
+
+
 def test_measured_traffic_lookup():
     assert bench.measured_traffic("no_such_workload", "adj_vel_grad") is None
     with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
