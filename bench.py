@@ -198,6 +198,19 @@ def exact_step(W, b, shots=None):
                   _native.MEM_DEVICE)
 
 
+def exact_kept_step(W, b, shots=None):
+    """One forward keeping every state (the reference's chpr = 100) and the
+    exact-adjoint gradient reading those states: no segment is recomputed."""
+    from pyfwi_amd import _native
+    plan = b["plan"]
+    shots = b["shots"] if shots is None else shots
+    plan.forward(shots, b["src"], b["seis"], b["chp_all"], 0, _native.MEM_DEVICE)
+    b["res"].copy_(b["seis"])
+    b["res"].sub_(b["obs"])
+    plan.gradient(shots, b["src"], b["res"], b["grad"], _native.MODE_EXACT, 0, 0,
+                  _native.MEM_DEVICE)
+
+
 def per_shot_rate(W, b, torch, nsub, step):
     """The plain per-shot path: the same kernels on one shot at a time (one
     shot per pass, the reference's schedule), on the first `nsub` shots."""
@@ -823,11 +836,13 @@ def main():
     ap.add_argument("--time-budget", type=float, default=1e9,
                     help="--mode lbfgs / timelapse: stop (and report) after this many seconds")
     ap.add_argument("--mode", default="kept",
-                    choices=["kept", "reverse", "exact", "lbfgs", "timelapse"],
+                    choices=["kept", "reverse", "exact", "exact-kept", "lbfgs", "timelapse"],
                     help="kept: every forward state kept in HBM per pass (chpr=100), misfit and "
                          "gradient in one call; reverse: chpr=0, states rebuilt by time reversal; "
                          "exact: the exact adjoint from checkpoints sized to the free HBM; "
-                         "lbfgs / timelapse: the inversion drivers on the workload")
+                         "exact-kept: the exact adjoint reading every forward state, all shots' "
+                         "states kept in HBM; lbfgs / timelapse: the inversion drivers on the "
+                         "workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -862,7 +877,8 @@ def main():
     units = int(W.ns) * cells * W.nt
     log(f"bench: {w.name}: {int(W.ns)} shots x {W.nt} steps x {cells} cells")
 
-    step = {"kept": kept_step, "reverse": gradient_step, "exact": exact_step}[args.mode]
+    step = {"kept": kept_step, "reverse": gradient_step, "exact": exact_step,
+            "exact-kept": exact_kept_step}[args.mode]
     state_bytes = 5 * 4 * (int(W.tnz) + 16) * (((int(W.tnx) + 8 + 63) // 64) * 64)
     free, _total = torch.cuda.mem_get_info()
     max_pass = int(0.8 * free // (state_bytes * W.nt)) if args.mode == "kept" else None
@@ -874,7 +890,7 @@ def main():
         g, f, r = (int(x) for x in args.tune.split(","))
         tuning = (g, bool(f), r)
         b["plan"].set_tuning(*tuning)
-    elif not args.no_tune and args.mode != "exact":
+    elif not args.no_tune and not args.mode.startswith("exact"):
         from pyfwi_amd import tuning as tn
         if args.mode == "kept":
             trial = tn.truncate(b["shots"], 120)
@@ -986,14 +1002,15 @@ def main():
                                "per pass; L2 misfit on the GPU",
                        "reverse": "reference adjoint, chpr=0: states rebuilt by time reversal",
                        "exact": "exact adjoint, two-level checkpoints sized to the free HBM, "
-                                "segments recomputed"}[args.mode],
+                                "segments recomputed",
+                       "exact-kept": "exact adjoint reading every kept forward state"}[args.mode],
                    "parallelism": "single GPU, shots batched per launch",
                    "tuning": {"shots_per_pass": tuning[0],
                               "fused_steps": bool(int(tuning[1]) & _native.TUNE_FUSED_STEPS),
                               "adjoint_one_launch": bool(int(tuning[1]) & _native.TUNE_ADJ_FUSED),
                               "strip_rows": tuning[2], "graphs": bool(graphs)}},
         "kernel_timing": ("event-bracketed launches (each holds a few microseconds of the "
-                          "event's stall)" if args.mode == "exact" else
+                          "event's stall)" if args.mode.startswith("exact") else
                           "per-step time of each time loop (HIP events around the loops) split "
                           "between its kernels by event-bracketed launches"),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),

@@ -221,7 +221,11 @@ int32_t pyfwi_forward_noalloc(pyfwi_plan *plan, const pyfwi_shot *shots, int32_t
  *         per shot, the checkpoints (ceil(nt/K) states, when recomputed),
  *         the K-state segment recomputed from each checkpoint, and the
  *         forward and adjoint states -- the two-level schedule, one extra
- *         forward sweep.  PYFWI_ENOMEM when one shot does not fit.
+ *         forward sweep.  PYFWI_ENOMEM when one shot does not fit.  When the
+ *         last forward kept every step of these shots (checkpoint list
+ *         0..nt-1) and their sources lie on active cells, the exact adjoint
+ *         reads those states instead and ckpt_every is not used: no
+ *         recomputation, the same gradient bit for bit.
  *   flags PYFWI_F_PRECOND (pyfwi mode) */
 int32_t pyfwi_gradient(pyfwi_plan *plan, const pyfwi_shot *shots,
                        int32_t nshots, const float *src, const float *res,

@@ -165,10 +165,13 @@ void launch_exact_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int
                       const float *vel, int64_t vel_stride, float *adj, float *av,
                       const float *lam, const float *mu, const float *dxp,
                       const float *dzp, float *gacc, hipStream_t st);
+// The exact adjoint's velocity kernel reads the stresses step t updates from
+// `tau` (per-shot stride tau_stride); with `src`, a kept state without step
+// t's source term, which the kernel adds to txx and tzz at the source cell.
 void launch_exact_vel(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
                       const float *tau, int64_t tau_stride, float *adj, const float *av_in,
                       float *av_out, const float *bu, const float *dxp, const float *dzp,
-                      float *gacc, hipStream_t st);
+                      float *gacc, hipStream_t st, const float *src = nullptr);
 void launch_exact_inject(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
                          float *adj, float *av, const float *res, const int32_t *rtab,
                          hipStream_t st);

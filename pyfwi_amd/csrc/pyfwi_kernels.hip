@@ -1920,7 +1920,8 @@ template <int SDO, bool ALONE, int SPB>
 __global__ __launch_bounds__(64 * SPB) void k_exact_vel(
     DevGrid g, const pyfwi_shot *shots, int t, const float *tau, int64_t tau_shot_stride,
     float *adj, const float *av_in, float *av_out, const float *__restrict__ bu,
-    const float *__restrict__ dxp, const float *__restrict__ dzp, float *gacc)
+    const float *__restrict__ dxp, const float *__restrict__ dzp, float *gacc,
+    const float *__restrict__ src)
 {
     constexpr int NV = 1;  // staged term: b
     __shared__ float part[2][ADJ_GROUP_ROWS * SPB * NV * 64];
@@ -1949,7 +1950,19 @@ __global__ __launch_bounds__(64 * SPB) void k_exact_vel(
         return xplus<SDO>([&](int k) { return B.at(o, k) * (pvx.at(o, k) * kp(r, k)); }, g.cx); };
     auto x_bz = [&](uint32_t o, int r) {
         return xminus<SDO>([&](int k) { return B.at(o, k) * (pvz.at(o, k) * kp(r, k)); }, g.cx); };
-    auto x_txx = [&](uint32_t o) { return xminus<SDO>([&](int k) { return txx.at(o, k); }, g.cx); };
+    // With `src` the stresses come from a kept state, which holds no source
+    // term: step t's is added to txx and tzz as they are loaded at the
+    // source cell, giving the values a recomputed segment holds in memory
+    // (k_inject_record adds them there before the velocity update).
+    SrcTerm st = src ? src_term(shots[s], t, src) : SrcTerm{};
+    if (st.z < i0 - SDO || st.z > i1 + SDO || st.x < j - SDO || st.x > j + SDO) st.z = -1;
+    auto x_txx = [&](uint32_t o, int r) {
+        return xminus<SDO>([&](int k) {
+            const float v = txx.at(o, k);
+            return r == st.z && j + k == st.x ? v + st.v[2] : v; }, g.cx); };
+    auto tzz_at = [&](uint32_t o, int r) {
+        const float v = tzz(o);
+        return r == st.z && j == st.x ? v + st.v[3] : v; };
     auto x_txz = [&](uint32_t o) { return xplus<SDO>([&](int k) { return txz.at(o, k); }, g.cx); };
     auto z_bz = [&](uint32_t o, int r) { return B(o) * (pvz(o) * kp(r, 0)); };
     auto z_bx = [&](uint32_t o, int r) { return B(o) * (pvx(o) * kp(r, 0)); };
@@ -1961,12 +1974,12 @@ __global__ __launch_bounds__(64 * SPB) void k_exact_vel(
         wbz.v[k + 1] = z_bz(o, r);
         wbx.v[k + 1] = z_bx(o, r);
         wtxz.v[k + 1] = txz(o);
-        wtzz.v[k + 1] = tzz(o);
+        wtzz.v[k + 1] = tzz_at(o, r);
         if (k >= SDO) {
             const int q = k - SDO;
             frxx.v[q] = x_bx(o, r);
             frxz.v[q] = x_bz(o, r);
-            ftxx.v[q] = x_txx(o);
+            ftxx.v[q] = x_txx(o, r);
             ftxz.v[q] = x_txz(o);
         }
     }
@@ -1983,9 +1996,9 @@ __global__ __launch_bounds__(64 * SPB) void k_exact_vel(
             wbz.push(z_bz(n, rn));
             wbx.push(z_bx(n, rn));
             wtxz.push(txz(n));
-            wtzz.push(tzz(n));
+            wtzz.push(tzz_at(n, rn));
             const float rxx = frxx.pop(x_bx(n, rn)), rxz_x = frxz.pop(x_bz(n, rn));
-            const float dtx = ftxx.pop(x_txx(n)), dtz = ftxz.pop(x_txz(n));
+            const float dtx = ftxx.pop(x_txx(n, rn)), dtz = ftxz.pop(x_txz(n));
             const float keep = kp(i, 0);
             const float px = pvx(cb) * keep, pz = pvz(cb) * keep;
             float *pxx = &cell_ref(A + 2 * fs, cb), *pzz = &cell_ref(A + 3 * fs, cb),
@@ -2582,13 +2595,13 @@ void launch_exact_str(const DevGrid &g, int nshots, const pyfwi_shot *shots, int
 void launch_exact_vel(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
                       const float *tau, int64_t tau_stride, float *adj, const float *av_in,
                       float *av_out, const float *bu, const float *dxp, const float *dzp,
-                      float *gacc, hipStream_t st)
+                      float *gacc, hipStream_t st, const float *src)
 {
     DevGrid gm = g;
     const int spb = march_spb(g.sdo, nshots);
     dim3 grid = march_grid(gm, g.sdo, nshots, spb, 64, true), block(64 * spb);
     const bool alone = nshots == 1;
-#define ARGS gm, shots, t, tau, tau_stride, adj, av_in, av_out, bu, dxp, dzp, gacc
+#define ARGS gm, shots, t, tau, tau_stride, adj, av_in, av_out, bu, dxp, dzp, gacc, src
     if (alone) DISPATCH_MARCH(g.sdo, spb, k_exact_vel, , true);
     else DISPATCH_MARCH(g.sdo, spb, k_exact_vel, , false);
 #undef ARGS

@@ -1191,6 +1191,51 @@ int gradient_exact(pyfwi_plan *p, int g0, int ng, const float *d_src, const floa
     return 0;
 }
 
+// Exact adjoint of one pass from the states the forward kept at every step
+// (`snaps`: the pass's first shot, per-shot stride nchp*fps, slot t = state
+// after step t): step t's velocity half step produced the velocities of slot
+// t from the stresses of slot t-1 (zero before step 0) with step t's source
+// term, which k_exact_vel adds as it loads them.  No segment is recomputed.
+int gradient_exact_kept(pyfwi_plan *p, int g0, int ng, const float *d_src, const float *d_res,
+                        const float *snaps, int nchp)
+{
+    const DevGrid &g = p->dg;
+    const int64_t fps = fields_per_shot(p);
+    const pyfwi_shot *dsh = p->d_shots + g0;
+    const int nt = max_nt(std::vector<pyfwi_shot>(p->h_shots.begin() + g0,
+                                                  p->h_shots.begin() + g0 + ng));
+    if (nt > nchp) return fail(PYFWI_EINVAL, "kept states shorter than the shots");
+    if (int e = ensure(p->adj, (size_t)ng * fps, p->stream, "adjoint fields")) return e;
+    if (int e = ensure(p->av0, (size_t)ng * 2 * g.fstride, p->stream, "adjoint velocity")) return e;
+    if (int e = ensure(p->av1, (size_t)ng * 2 * g.fstride, p->stream, "adjoint velocity")) return e;
+    if (int e = ensure(p->fields, (size_t)ng * fps, p->stream, "fields")) return e;
+    launch_zero(p->adj.p, (int64_t)ng * fps, p->stream);
+    launch_zero(p->av0.p, (int64_t)ng * 2 * g.fstride, p->stream);
+    launch_zero(p->av1.p, (int64_t)ng * 2 * g.fstride, p->stream);
+    launch_zero(p->fields.p, (int64_t)ng * fps, p->stream);  // the state before step 0
+    float *av_cur = p->av0.p, *av_nxt = p->av1.p;
+    const int64_t ss = (int64_t)nchp * fps;  // between shots
+    for (int t = nt - 1; t >= 0; --t) {
+        const float *vel = snaps + (int64_t)t * fps;
+        const float *tau = (t > 0 ? snaps + (int64_t)(t - 1) * fps : p->fields.p) + 2 * g.fstride;
+        const int64_t tau_stride = t > 0 ? ss : fps;
+        {
+            Timed tm(p, PYFWI_K_EXA_STR);
+            launch_exact_str(g, ng, dsh, t, vel, ss, p->adj.p, av_cur, p->lam.p, p->mu.p, p->dxp,
+                             p->dzp, p->gacc.p, p->stream);
+        }
+        {
+            Timed tm(p, PYFWI_K_EXA_VEL);
+            launch_exact_vel(g, ng, dsh, t, tau, tau_stride, p->adj.p, av_cur, av_nxt, p->bu.p,
+                             p->dxp, p->dzp, p->gacc.p, p->stream, d_src);
+        }
+        std::swap(av_cur, av_nxt);
+        launch_exact_inject(g, ng, dsh, t, p->adj.p, av_cur, d_res, p->d_rtab, p->stream);
+        if (int e = maybe_harvest(p)) return e;
+    }
+    return 0;
+}
+
 } // namespace
 
 int32_t pyfwi_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, const float *src,
@@ -1232,11 +1277,17 @@ int32_t pyfwi_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, c
     }
     int G = pass_size(p, nshots);
     const int nt_all = max_nt(std::vector<pyfwi_shot>(shots, shots + nshots));
-    const bool reuse = mode == PYFWI_MODE_EXACT && same && p->ckpts_all &&
+    // exact mode after a forward that kept every step of these shots (the
+    // reference's chpr = 100) with its sources on active cells: the backward
+    // reads the kept states instead of recomputing segments from checkpoints
+    const bool kept = mode == PYFWI_MODE_EXACT && same && p->snaps.p && p->src_inside &&
+                      keeps_every_step(p->chp, nt_all) &&
+                      p->snaps.n >= (size_t)nshots * p->chp.size() * fps;
+    const bool reuse = mode == PYFWI_MODE_EXACT && !kept && same && p->ckpts_all &&
                        (ckpt_every == 0 || ckpt_every == p->ckpt_every);
-    const int K = reuse ? p->ckpt_every : (ckpt_every > 0 ? ckpt_every : auto_interval(nt_all));
-    p->last_ckpt_every = mode == PYFWI_MODE_EXACT ? K : 0;
-    if (mode == PYFWI_MODE_EXACT && p->tune_group == 0) {
+    const int K = kept ? 1 : reuse ? p->ckpt_every : (ckpt_every > 0 ? ckpt_every : auto_interval(nt_all));
+    p->last_ckpt_every = mode == PYFWI_MODE_EXACT ? (kept ? 0 : K) : 0;
+    if (mode == PYFWI_MODE_EXACT && !kept && p->tune_group == 0) {
         // checkpoint schedule sized to the device memory: shots per pass from
         // what the pass's checkpoints (unless kept by the forward), segment
         // and working states take
@@ -1263,7 +1314,7 @@ int32_t pyfwi_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, c
     p->last_passes.clear();
     identity_order(p, nshots);
     // exact mode sized to memory: the passes' shots spread evenly (at most G)
-    const bool even = mode == PYFWI_MODE_EXACT && p->tune_group == 0;
+    const bool even = mode == PYFWI_MODE_EXACT && !kept && p->tune_group == 0;
     const int npass = (nshots + G - 1) / G;
     for (int g0 = 0, k = 0; g0 < nshots; g0 += G, ++k) {
         if (even) G = nshots / npass + (k < nshots % npass ? 1 : 0);
@@ -1277,6 +1328,10 @@ int32_t pyfwi_gradient(pyfwi_plan *p, const pyfwi_shot *shots, int32_t nshots, c
             const float *sn = p->snaps.p + (int64_t)g0 * p->chp.size() * fps;
             if (int e = gradient_pyfwi(p, g0, ng, d_res, (flags & PYFWI_F_PRECOND) != 0, sn,
                                        p->chp))
+                return e;
+        } else if (kept) {
+            const float *sn = p->snaps.p + (int64_t)g0 * p->chp.size() * fps;
+            if (int e = gradient_exact_kept(p, g0, ng, d_src, d_res, sn, (int)p->chp.size()))
                 return e;
         } else {
             std::vector<pyfwi_shot> shp(shots + g0, shots + g0 + ng);

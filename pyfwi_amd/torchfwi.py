@@ -9,11 +9,15 @@ Mirror of PyFWI's ``PyFWI.torchfwi.Fwi`` (src/PyFWI/torchfwi.py:7-91).
 
 The backward kernel is selected by ``W.grad_mode``:
   'pyfwi' (default)  the reference's adjoint (drop-in parity)
-  'exact'            the exact discrete adjoint, with forward states recomputed
-                     from checkpoints instead of storing every step: every
-                     ``W.ckpt_every`` steps when set, else every ceil(sqrt(nt))
-                     steps with the shots in passes sized to the free device
-                     memory (the two-level schedule of pyfwi_gradient)
+  'exact'            the exact discrete adjoint.  Without ``W.ckpt_every``, the
+                     forward keeps every state when all shots' states fit in
+                     80 % of the free device memory (or the propagator keeps
+                     every step anyway, chpr = 100) and the adjoint reads them;
+                     otherwise the forward states are recomputed from
+                     checkpoints: every ``W.ckpt_every`` steps when set, else
+                     every ceil(sqrt(nt)) steps with the shots in passes sized
+                     to the free device memory (the two-level schedule of
+                     pyfwi_gradient).  Both give the same gradient.
 """
 import numpy as np
 import torch
@@ -38,7 +42,16 @@ class Fwi(torch.autograd.Function):
         ctx.mode = mode
         if vp.is_cuda:
             k = getattr(W, "ckpt_every", 0) or _native.CKPT_AUTO
-            d = W.forward_modeling_tensor(vp, vs, rho, ckpt_every=k if mode == "exact" else 0)
+            keep_all = False
+            if mode == "exact" and not getattr(W, "ckpt_every", 0):
+                free, _ = torch.cuda.mem_get_info(vp.device)
+                keep_all = W.keeps_every_step() or \
+                    int(W.ns) * int(W.nt) * W.state_bytes() <= 0.8 * free
+            if keep_all:
+                d = W.forward_modeling_tensor(vp, vs, rho, ckpt_every=0, keep_all=True)
+            else:
+                d = W.forward_modeling_tensor(vp, vs, rho,
+                                              ckpt_every=k if mode == "exact" else 0)
             ctx.ckpt_every = max(k, 0)
             return d["taux"], d["tauz"]
         model = {"vp": vp.detach().cpu().numpy().astype(np.float32),

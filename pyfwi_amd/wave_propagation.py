@@ -277,6 +277,19 @@ class WavePreparation:
                         out[name][:, :, s, c] = buf[f]
         return out
 
+    def keeps_every_step(self):
+        """True when the checkpoint list is every time step (chpr = 100)."""
+        chp = np.asarray(self.chp)
+        return chp.size >= int(self.nt) and np.array_equal(chp[:int(self.nt)],
+                                                          np.arange(int(self.nt)))
+
+    def state_bytes(self):
+        """Device bytes of one shot's state: five fields in the plan's padded
+        layout (row pitch roundup(tnx + 8, 64), 8 zero rows above and below)."""
+        pitch = -(-(int(self.tnx) + 8) // 64) * 64
+        field = -(-((int(self.tnz) + 16) * pitch + 64) // 64) * 64
+        return 5 * field * 4
+
     def gradient_window(self):
         """Rows [z0, z1) and columns [x0, x1) of the cropped model where
         gradient_reading keeps the gradient (src/PyFWI/wave_propagation.py:
@@ -521,15 +534,18 @@ class WavePropagator(WavePreparation):
         self._src_t = torch.from_numpy(self._source_table()).to(dev)
         return dev
 
-    def forward_modeling_tensor(self, vp, vs, rho, ckpt_every=0):
+    def forward_modeling_tensor(self, vp, vs, rho, ckpt_every=0, keep_all=False):
         """Device version of ``forward_modeling`` for torch tensors on the GPU;
         returns the seismic section as device tensors.  ``ckpt_every`` > 0 also
-        keeps checkpoints for an exact-mode gradient."""
+        keeps checkpoints for an exact-mode gradient; ``keep_all`` keeps every
+        step's state (whatever chpr), which an exact-mode gradient then reads
+        instead of recomputing them."""
         import torch
         dev = self._tensor_setup(vp, vs, rho)
         ns, nt, nr = int(self.ns), int(self.nt_ext), int(self.nr)
         seis = torch.empty(ns * 5 * nt * nr, dtype=torch.float32, device=dev)
-        self._plan.forward(self._shots(list(range(ns))), self._src_t, seis, self.chp,
+        chp = checkpoint_steps(int(self.nt), 100) if keep_all else self.chp
+        self._plan.forward(self._shots(list(range(ns))), self._src_t, seis, chp,
                            ckpt_every, _native.MEM_DEVICE)
         self._forward_done = True
         self._seis_t = seis.view(ns, 5, nt, nr)

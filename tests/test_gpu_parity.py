@@ -215,6 +215,44 @@ def test_exact_gradient_vs_fp64(gpu):
         assert e < GRAD_TOL, (name, e)
 
 
+@pytest.mark.parametrize("dt_scale", [1, 2])
+def test_exact_gradient_from_kept_states(gpu, dt_scale):
+    """After a forward that kept every step (chpr = 100) the exact adjoint
+    reads the kept states instead of recomputing segments from checkpoints:
+    the same gradient as the recomputing path (same values, same shot-group
+    sums) and within the FP64 oracle's tolerance."""
+    w = _small(nt=300)
+    if dt_scale == 2:  # two internal steps a sample
+        from pyfwi_amd.acquisition import Source
+        w.inpa.update({"dh": 5.0, "dt": 0.0008, "rec_dis": 10.0})
+        w.inpa["t"] = workloads.t_for_nt(300, 0.0008)
+        w.src = Source(np.c_[w.src.i, w.src.j] * 5.0, 5.0, 0.0008)
+        w.src.Ricker(15)
+        w.rec_loc = w.rec_loc * 5.0
+    out = {}
+    for chpr in (100, 0):
+        W = WavePropagator(w.inpa, w.src, w.rec_loc, w.model_shape, components=4, chpr=chpr)
+        assert int(W.dt_scale) == dt_scale
+        d = W.forward_modeling(w.model)
+        rng = np.random.default_rng(7)
+        res = {k: rng.standard_normal(v.shape).astype(np.float32) * np.abs(v).max()
+               for k, v in d.items()}
+        out[chpr] = W.raw_gradient(res, mode="exact", ckpt_every=0 if chpr else 37)
+        if chpr:
+            # the kept-state path ran: no checkpoint interval was used
+            assert W._plan.last_passes(with_interval=True)[1] == 0
+    st = oracle_setup(w, chpr=0, components=4)
+    g3 = O.exact_gradient(st, w.model, O.residual_records(st, res, np.float64), np.float64)
+    md = st.medium(w.model, np.float64)
+    for k, ref in ((2, g3[0]), (0, g3[1]), (4, -(md["b"] ** 2) * g3[2])):
+        e_path = rel(out[100][k], out[0][k])
+        e64 = rel(out[100][k], ref)
+        same = np.array_equal(out[100][k], out[0][k])
+        print(f"exact from kept states, slot {k}: vs recomputed {e_path:.2e}"
+              f"{' (bit-equal)' if same else ''}, vs FP64 {e64:.2e}", flush=True)
+        assert e_path < 1e-6 and e64 < GRAD_TOL, (k, e_path, e64)
+
+
 @pytest.mark.parametrize("tuning", [(2, False, 0), (3, True, 0), (1, True, 0), (0, False, 16),
                                     (2, True, 12)])
 def test_execution_shapes_agree(gpu, tuning):

@@ -171,3 +171,31 @@ def test_fwi_exact_mode_central_differences_differentiation_size(gpu):
     finishes in seconds."""
     w = workloads.differentiation(ns=2)
     _fd_check(w, gpu, ckpt_every=50)
+
+
+def test_fwi_exact_mode_kept_states_equal_checkpoints(gpu):
+    """Without W.ckpt_every the exact mode keeps every forward state when they
+    fit and its adjoint reads them; with checkpoints every 37 steps it
+    recomputes them: the gradients are the same bit for bit, and the kept
+    path matches FP64 central differences (linear loss, eps = 1e-5)."""
+    w = _small(nt=400, components=0)
+    grads = []
+    for ck in (0, 37):
+        W = WavePropagator(w.inpa, w.src, w.rec_loc, w.model_shape, components=0)
+        W.grad_mode, W.ckpt_every = "exact", ck
+        m = {k: torch.tensor(v, device=gpu, requires_grad=True) for k, v in w.model.items()}
+        tx, tz = Fwi.apply(W, m["vp"], m["vs"], m["rho"])
+        if ck == 0:  # every state kept, no checkpoint interval used
+            assert W.keeps_every_step() is False
+            assert W._plan.last_passes(with_interval=True)[1] == 0
+        gen = torch.Generator().manual_seed(3)
+        wx = torch.randn(tx.shape, generator=gen).to(gpu)
+        wz = torch.randn(tz.shape, generator=gen).to(gpu)
+        ((tx * wx).sum() + (tz * wz).sum()).backward()
+        grads.append({k: m[k].grad.cpu().numpy() for k in m})
+    for k in ("vp", "vs", "rho"):
+        same = np.array_equal(grads[0][k], grads[1][k])
+        print(f"exact mode {k}: kept states vs checkpoints every 37: "
+              f"{rel(grads[0][k], grads[1][k]):.2e}{' (bit-equal)' if same else ''}", flush=True)
+        assert rel(grads[0][k], grads[1][k]) < 1e-6, k
+    _fd_check(w, gpu, ckpt_every=0, loss="linear")

@@ -110,17 +110,19 @@ def test_gradient_wells(gpu, acq_type, chpr, order):
         assert rel(a, b) < SEIS_TOL
 
 
+@pytest.mark.parametrize("chpr", [0, 100])
 @pytest.mark.parametrize("acq_type", [0, 2])
-def test_exact_gradient_wells(gpu, acq_type):
+def test_exact_gradient_wells(gpu, acq_type, chpr):
     """The exact adjoint injects each record column where it was sampled
-    (the crosswell vz column into vx)."""
-    print(f"wells exact gradient acq_type={acq_type}", flush=True)
-    W, st, model, _ = wells(acq_type, nt=300, chpr=0)
+    (the crosswell vz column into vx); with chpr = 100 it reads the kept
+    states of the forward instead of recomputing them."""
+    print(f"wells exact gradient acq_type={acq_type} chpr={chpr}", flush=True)
+    W, st, model, _ = wells(acq_type, nt=300, chpr=chpr)
     d = W.forward_modeling(model)
     rng = np.random.default_rng(5)
     res = {k: (rng.standard_normal(v.shape) * np.abs(v).max()).astype(np.float32)
            for k, v in d.items()}
-    G = W.raw_gradient(res, mode="exact", ckpt_every=37)
+    G = W.raw_gradient(res, mode="exact", ckpt_every=0 if chpr else 37)
     rr = O.residual_records(st, res, np.float64)          # record (reordered) columns
     g3 = O.exact_gradient(st, model, st.flip_wells(rr), np.float64)
     md = st.medium(model, np.float64)
