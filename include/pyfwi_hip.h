@@ -280,7 +280,10 @@ int32_t pyfwi_set_record_filter(pyfwi_plan *plan, const double *taps, int32_t ha
  * that many shots (0: all in one pass; smaller passes need less memory and
  * can keep the working set in the 256 MB on-package cache); `fused` is a
  * set of bits: PYFWI_TUNE_FUSED_STEPS does each forward and time-reversed
- * step as one launch (ping-pong buffers) instead of two half-step launches;
+ * step as one launch instead of two half-step launches (the forward step
+ * reads one state and writes the next: ping-pong buffers, or from one kept
+ * slot into the next; records and kept states bit for bit those of two
+ * launches);
  * PYFWI_TUNE_ADJ_FUSED runs the reference adjoint of each backward step as
  * one launch instead of two (stresses, then velocities; order 4 with
  * receivers on the active cells, else ignored); strip_rows > 0 fixes the

@@ -38,6 +38,9 @@ constexpr int FUSED_WAVES = 4;      // fused two-stage kernels: wavefronts per b
 #ifndef MARCH_UNROLL
 #define MARCH_UNROLL 2              // rows per unrolled iteration: two rows of loads in flight per wave
 #endif
+#ifndef FWD_STEP_UNROLL
+#define FWD_STEP_UNROLL 2           // rows per unrolled iteration of the one-launch forward step
+#endif
 #ifndef ADJ_VEL_UNROLL
 #define ADJ_VEL_UNROLL 2            // rows per unrolled iteration of the adjoint velocity + imaging kernel
 #endif
@@ -92,11 +95,17 @@ void launch_vel(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, 
                 const float *in, int64_t in_stride, float *out, int64_t out_stride,
                 const float *bu, const float *dxp, const float *dzp, hipStream_t st,
                 const float *src = nullptr);
-// Fused velocity + stress step (forward, DIR=+1) or stress + velocity step
-// (time reversed, DIR=-1): reads `in`, writes `out` (both [nshots][5][fs]).
+// Velocity + stress step in one launch (forward, dir = +1) or stress +
+// velocity step (time reversed, dir = -1): reads `in`, writes `out`
+// (distinct buffers; per-shot strides in_stride / out_stride, 5*fstride for
+// the time-reversed step).  Forward with `src` / `seis`: the source term on
+// the loaded values and the record sample of step t+1, as launch_vel and
+// launch_str.
 void launch_fused_step(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
-                       const float *in, float *out, const float *bu, const float *lam,
-                       const float *mu, const float *dxp, const float *dzp, hipStream_t st);
+                       const float *in, int64_t in_stride, float *out, int64_t out_stride,
+                       const float *bu, const float *lam, const float *mu, const float *dxp,
+                       const float *dzp, hipStream_t st, const float *src = nullptr,
+                       float *seis = nullptr);
 // Stress half step: forward reads this step's velocities from `out` and the
 // previous stresses from `in`, writes the stresses to `out`.  Forward with
 // `src`, the source term on txx, tzz at the source cell; with `seis`, the

@@ -577,8 +577,10 @@ __global__ __launch_bounds__(64 * SPB) void k_str(DevGrid g, const pyfwi_shot *s
         const float gxz = dxm_vz + wvx.dplus(g.cz);
         const float l = L(cb), m = M(cb);
         const float lm = l + 2.0f * m;
-        const float sxx = lm * exx + l * ezz;
-        const float szz = lm * ezz + l * exx;
+        // the fused multiply-adds written out, so that the one-launch step
+        // (k_fwd_step) rounds as this kernel does
+        const float sxx = fmaf(l, ezz, lm * exx);
+        const float szz = fmaf(l, exx, lm * ezz);
         const float sxz = m * gxz;
         const float damp = ddx + dzp[i];
         float a = txx(cb), bb = tzz(cb), d = txz(cb);
@@ -686,90 +688,223 @@ __device__ __forceinline__ float lane_dplus(float v, int lane, const float c[4])
     return r;
 }
 
-// Forward step: stage 1 velocities (elastic.cl:155-226), stage 2 stresses
-// (elastic.cl:235-308), damping after each update.
-template <int SDO>
-__global__ __launch_bounds__(64 * FUSED_WAVES) void k_fused_fwd(
-    DevGrid g, const pyfwi_shot *shots, int t, const float *in, float *out,
-    const float *__restrict__ bu, const float *__restrict__ lam, const float *__restrict__ mu,
-    const float *__restrict__ dxp, const float *__restrict__ dzp)
+// Tile of a two-stage launch (k_fwd_step, k_adj_fused): as tile_lane, with
+// column blocks of 64 - 2*SDO owned columns; lane l of block cb sits on
+// column cb*SPAN + l, and the lanes of the SDO columns on each side of the
+// owned ones compute stage 1 only.
+template <int SDO, int SPB>
+__device__ __forceinline__ bool tile_fused(const DevGrid &g, int &s, int &j, int &i0, int &i1,
+                                               bool &own, bool &colact)
 {
-    FusedTile<SDO> ft;
-    if (!fused_tile<SDO>(g, ft)) return;
-    if (!fwd_active(shots, ft.s, t)) return;
+    constexpr int SPAN = 64 - 2 * SDO;
+    const int lane = (int)threadIdx.x % 64;
+    int cb, st;
+    if (SPB > 1 && g.nshot == 1) {
+        const int ncq = (g.ncb + SPB - 1) / SPB;
+        const int T = ncq * g.nstrips;
+        const int per = (T + NUM_XCD - 1) / NUM_XCD;
+        const int w = (int)blockIdx.x;
+        const int logical = (w % NUM_XCD) * per + w / NUM_XCD;
+        if (logical >= T) return false;
+        cb = (logical % ncq) * SPB + wave_id();
+        st = logical / ncq;
+        s = 0;
+        if (cb >= g.ncb) return false;
+    } else {
+        const int T = g.ncb * g.nstrips * g.nbatch;
+        const int per = (T + NUM_XCD - 1) / NUM_XCD;
+        const int w = (int)blockIdx.x;
+        const int logical = (w % NUM_XCD) * per + w / NUM_XCD;
+        if (logical >= T) return false;
+        const int rest = logical / g.nbatch;
+        cb = rest % g.ncb;
+        st = rest / g.ncb;
+        s = (logical % g.nbatch) * SPB + wave_id();
+        if (s >= g.nshot) return false;
+    }
+    j = cb * SPAN + lane;
+    colact = j >= SDO && j <= g.nx - SDO;
+    own = lane >= SDO && lane < 64 - SDO && colact;
+    i0 = SDO + st * g.mrows;
+    i1 = min(i0 + g.mrows - 1, g.nz - SDO);
+    return true;
+}
+
+// A lane's value at column offsets -SDO..SDO from the neighbouring lanes
+// (wave shifts by one lane); the SDO edge lanes on each side get zeros from
+// outside the wave, so only the owned lanes of a fused tile may use them.
+template <int SDO>
+struct LaneNbr {
+    float s[2 * SDO + 1];
+    __device__ __forceinline__ explicit LaneNbr(float c)
+    {
+        s[SDO] = c;
+        float l = c, r = c;
+#pragma unroll
+        for (int k = 1; k <= SDO; ++k) {
+            l = dpp_mov<DPP_WAVE_SHR1>(0.0f, l);  // lane i <- lane i-1
+            r = dpp_mov<DPP_WAVE_SHL1>(0.0f, r);  // lane i <- lane i+1
+            s[SDO - k] = l;
+            s[SDO + k] = r;
+        }
+    }
+    __device__ __forceinline__ float operator()(int k) const { return s[SDO + k]; }
+};
+
+// Forward step in one launch (src/PyFWI/elastic.cl:155-308): stage 1
+// updates the velocities at row i, stage 2 the stresses at row q = i - SDO
+// from stage 1's new velocities, held in register windows along z and taken
+// from the neighbouring lanes (DPP) along x, so the new velocities are not
+// read back from memory.  The state of step t is read from `in` and step t+1
+// written to `out` (distinct buffers, per-shot strides in_stride and
+// out_stride): the kept-state forward steps from one slot into the next.
+// Per shot and cell: the state read and written once (40 B) and the model
+// read once (12 B), 52 B against 72 B for k_vel + k_str.
+//
+// As k_vel and k_str, with `src` the step's source term is added to the
+// values loaded at the source cell (vx, vz, txx, tzz; memory is not
+// changed), and with `seis` the record sample of step t+1 is written at the
+// receivers by the owning lane.  Every value is computed by the same
+// arithmetic as in k_vel and k_str.  Stage 1 is forced to zero off the
+// active cells, which stay zero in every state.
+template <int SDO, int SPB>
+__global__ __launch_bounds__(64 * SPB) void k_fwd_step(
+    DevGrid g, const pyfwi_shot *shots, int t, const float *__restrict__ in, int64_t in_stride,
+    float *__restrict__ out, int64_t out_stride, const float *__restrict__ bu,
+    const float *__restrict__ lam, const float *__restrict__ mu, const float *__restrict__ dxp,
+    const float *__restrict__ dzp, const float *__restrict__ src, float *seis)
+{
+    int s, j, i0, i1;
+    bool own, colact;
+    if (!tile_fused<SDO, SPB>(g, s, j, i0, i1, own, colact)) return;
+    if (!fwd_active(shots, s, t)) return;
     const int64_t fs = g.fstride;
-    const int P = g.pitch, lane = (int)threadIdx.x % 64, col = ft.col;
-    const float *I = in + (int64_t)ft.s * 5 * fs + g.origin;
-    float *O = out + (int64_t)ft.s * 5 * fs + g.origin;
-    const float *vx = I, *vz = I + fs, *txx = I + 2 * fs, *tzz = I + 3 * fs, *txz = I + 4 * fs;
-    const float *b = bu + g.origin, *L = lam + g.origin, *M = mu + g.origin;
-    const bool ld = ft.colact;  // only active columns read memory
-    ZWin<SDO> wxz{}, wzz{};     // old txz (d/dz-), old tzz (d/dz+): rows r-SDO..r+SDO
-    ZWin<SDO> nvx{}, nvz{};         // new vx (d/dz+), new vz (d/dz-): rows q-SDO..q+SDO
-    const int rs = ft.i0 - SDO;
-    const int64_t c0 = (int64_t)rs * P + col;
-    int64_t c = c0;
+    const uint32_t RB = (uint32_t)g.pitch * 4u;
+    // bases 2*SDO rows above cell (0, 0): the byte offsets of every row the
+    // kernel reads (from -2*SDO) are non-negative
+    const int64_t org = g.origin - (int64_t)2 * SDO * g.pitch;
+    const float *F = in + (int64_t)s * in_stride + org;
+    float *O = out + (int64_t)s * out_stride + org;
+    const CellRef vx{F}, vz{F + fs}, txx{F + 2 * fs}, tzz{F + 3 * fs}, txz{F + 4 * fs};
+    const CellRef B{bu + org}, L{lam + org}, M{mu + org};
+    // the step's source term on the values this lane loads (see k_vel)
+    SrcTerm st = src_term(shots[s], t, src);
+    if (st.z < i0 - 2 * SDO || st.z > i1 + 2 * SDO || st.x < j - SDO || st.x > j + SDO)
+        st.z = -1;
+    float s_txx[2 * SDO];  // txx term on the taps of d/dx- (offsets -SDO..SDO-1)
+#pragma unroll
+    for (int k = 0; k < 2 * SDO; ++k) s_txx[k] = j + k - SDO == st.x ? st.v[2] : 0.0f;
+    const bool src_col = j == st.x;
+    const float s_vx = src_col ? st.v[0] : 0.0f, s_vz = src_col ? st.v[1] : 0.0f;
+    const float s_tzz = src_col ? st.v[3] : 0.0f, s_txxc = src_col ? st.v[2] : 0.0f;
+    RecLine rl = record_line(shots[s], t + 1, seis);
+    const int rec_r = own && rl.z >= i0 && rl.z <= i1 ? rl.index(j) : -1;
+    if (rec_r < 0) rl.z = -1;
+    auto tzz_row = [&](uint32_t o, int row) {
+        float v = tzz(o);
+        if (row == st.z) v += s_tzz;
+        return v;
+    };
+    auto dx_txz = [&](uint32_t o) { return xplus<SDO>([&](int k) { return txz.at(o, k); }, g.cx); };
+
+    ZWin<SDO> wxz, wzz;    // old txz (d/dz-), old tzz (d/dz+) around stage 1's row
+    Fifo<SDO> fxz;         // d/dx+ old txz, taken on the entering row
+    Fifo<SDO> otxx;        // old txx (source included), delayed to stage 2's row
+    ZWin<SDO> nvx{}, nvz{};  // new vx (d/dz+), new vz (d/dz-) around stage 2's row
+    // first stage 1 row: i0 - SDO (the halo rows stage 2 needs above the strip)
+    uint32_t cb = ((uint32_t)(i0 - SDO + 2 * SDO) * (uint32_t)g.pitch + (uint32_t)j) * 4u;
 #pragma unroll
     for (int k = 0; k < 2 * SDO; ++k) {
-        const int64_t o = c + (int64_t)(k - SDO) * P;
-        wxz.v[k + 1] = ld ? txz[o] : 0.0f;
-        wzz.v[k + 1] = ld ? tzz[o] : 0.0f;
+        const uint32_t o = cb + (uint32_t)(k - SDO) * RB;
+        const int row = i0 - SDO + k - SDO;
+        wxz.v[k + 1] = txz(o);
+        wzz.v[k + 1] = tzz_row(o, row);
+        if (k >= SDO) fxz.v[k - SDO] = dx_txz(o);
+        if (k < SDO) {
+            // txx of stage 2's first rows, i0-2*SDO .. i0-SDO-1
+            float v = txx(o);
+            if (row == st.z) v += s_txxc;
+            otxx.v[k] = v;
+        }
     }
-    const float ddx = ld ? dxp[col] : 0.0f;
+    const float ex = colact ? dxp[j] : 0.0f;
     const float dt = g.dt;
-    // rows in groups of MARCH_UNROLL, the inner loop fully unrolled: the cross-lane
-    // shuffles stop the compiler from unrolling a loop with a run-time trip count.
-    // Rows past the strip end (at most MARCH_UNROLL - 1) stay inside the zero halo
-    // and write nothing.
-    const int nrow = ft.i1 + SDO - rs + 1;
-    for (int rr = 0; rr < nrow; rr += MARCH_UNROLL) {
+    const int nrow = i1 + SDO - (i0 - SDO) + 1;
+    // rows in groups of MARCH_UNROLL, the inner loop fully unrolled (the lane
+    // moves keep the compiler from unrolling a loop with a run-time trip
+    // count); rows past the strip's end stay in the zero halo and store nothing
+    for (int rr = 0; rr < nrow; rr += FWD_STEP_UNROLL) {
 #pragma unroll
-    for (int u = 0; u < MARCH_UNROLL; ++u) {
-        const int r = rs + rr + u;
-        c = c0 + (int64_t)(rr + u) * P;
-        const int64_t n = c + (int64_t)SDO * P;
-        wxz.push(ld ? txz[n] : 0.0f);
-        wzz.push(ld ? tzz[n] : 0.0f);
-        // stage 1: velocities at row r
-        float ux = 0.0f, uz = 0.0f;
-        const bool act1 = ld && r >= SDO && r <= g.nz - SDO;
-        if (act1) {
-            const float ax = b[c] * (dminus<SDO>(txx + c, 1, g.cx) + wxz.dminus(g.cz));
-            const float az = b[c] * (wzz.dplus(g.cz) + dplus<SDO>(txz + c, 1, g.cx));
-            const float damp = ddx + dzp[r];
-            ux = fmaf(dt, ax, vx[c]);
-            ux = fmaf(dt, -damp * ux, ux);
-            uz = fmaf(dt, az, vz[c]);
-            uz = fmaf(dt, -damp * uz, uz);
-            if (ft.own && r >= ft.i0 && r <= ft.i1) {
-                O[c] = ux;
-                O[fs + c] = uz;
+        for (int u = 0; u < FWD_STEP_UNROLL; ++u, cb += RB) {
+            const int i = i0 - SDO + rr + u, q = i - SDO;
+            const uint32_t n = cb + (uint32_t)SDO * RB, cq = cb - (uint32_t)SDO * RB;
+            // loads of the row: stage 1's state and b, stage 2's model
+            wxz.push(txz(n));
+            wzz.push(tzz_row(n, i + SDO));
+            const float dxp_txz = fxz.pop(dx_txz(n));
+            Taps<SDO, 1> ttxx;
+            ttxx.load(txx, cb);
+            float ux = vx(cb), uz = vz(cb);
+            const float bc = B(cb);
+            const float l = L(cq), m = M(cq);
+            if (i == st.z) {  // the source row
+#pragma unroll
+                for (int k = 0; k < 2 * SDO; ++k) ttxx.t[k] += s_txx[k];
+                ux += s_vx;
+                uz += s_vz;
+            }
+            const float a0 = otxx.pop(ttxx(0));  // old txx at row q
+            // stage 1: velocities at row i (k_vel's arithmetic)
+            const float dxm_txx = xminus<SDO>(ttxx, g.cx);
+            const float ax = bc * (dxm_txx + wxz.dminus(g.cz));
+            const float az = bc * (wzz.dplus(g.cz) + dxp_txz);
+            const float damp1 = ex + dzp[i];
+            ux = fmaf(dt, ax, ux);
+            ux = fmaf(dt, -damp1 * ux, ux);
+            uz = fmaf(dt, az, uz);
+            uz = fmaf(dt, -damp1 * uz, uz);
+            const bool act1 = colact && i >= SDO && i <= g.nz - SDO;
+            ux = act1 ? ux : 0.0f;
+            uz = act1 ? uz : 0.0f;
+            nvx.push(ux);
+            nvz.push(uz);
+            // stage 2: stresses at row q (k_str's arithmetic), x-derivatives
+            // of the new velocities from the neighbouring lanes
+            const float exx = xplus<SDO>(LaneNbr<SDO>(nvx.centre()), g.cx);
+            const float dxm_vz = xminus<SDO>(LaneNbr<SDO>(nvz.centre()), g.cx);
+            const float ezz = nvz.dminus(g.cz);
+            const float gxz = dxm_vz + nvx.dplus(g.cz);
+            const float lm = l + 2.0f * m;
+            const float sxx = fmaf(l, ezz, lm * exx);  // as in k_str
+            const float szz = fmaf(l, exx, lm * ezz);
+            const float sxz = m * gxz;
+            const float damp2 = ex + dzp[q];
+            float a = a0, bb = wzz.v[0], d = wxz.v[0];  // old stresses at row q
+            a = fmaf(dt, sxx, a);
+            bb = fmaf(dt, szz, bb);
+            d = fmaf(dt, sxz, d);
+            a = fmaf(dt, -damp2 * a, a);
+            bb = fmaf(dt, -damp2 * bb, bb);
+            d = fmaf(dt, -damp2 * d, d);
+            // stores: the owned columns of the strip
+            if (own && i >= i0 && i <= i1) {
+                cell_ref(O, cb) = ux;
+                cell_ref(O + fs, cb) = uz;
+            }
+            if (own && q >= i0 && q <= i1) {
+                cell_ref(O + 2 * fs, cq) = a;
+                cell_ref(O + 3 * fs, cq) = bb;
+                cell_ref(O + 4 * fs, cq) = d;
+                if (q == rl.z) {  // this lane's receiver
+                    float *w = const_cast<float *>(rl.base) + rec_r;
+                    w[0] = nvx.centre();
+                    w[rl.fs] = nvz.centre();
+                    w[2 * rl.fs] = a;
+                    w[3 * rl.fs] = bb;
+                    w[4 * rl.fs] = d;
+                }
             }
         }
-        nvx.push(ux);
-        nvz.push(uz);
-        // stage 2: stresses at row q = r - SDO (every lane shuffles)
-        const int q = r - SDO;
-        const float qvx = nvx.centre(), qvz = nvz.centre();
-        const float exx = lane_dplus<SDO>(qvx, lane, g.cx);
-        const float gzx = lane_dminus<SDO>(qvz, lane, g.cx);
-        if (q < ft.i0 || q > ft.i1 || !ft.own) continue;
-        const int64_t cq = c - (int64_t)SDO * P;
-        const float ezz = nvz.dminus(g.cz);
-        const float gxz = gzx + nvx.dplus(g.cz);
-        const float l = L[cq], m = M[cq];
-        const float lm = l + 2.0f * m;
-        const float sxx = lm * exx + l * ezz;
-        const float szz = lm * ezz + l * exx;
-        const float sxz = m * gxz;
-        const float damp = ddx + dzp[q];
-        float a = fmaf(dt, sxx, txx[cq]);
-        float bb = fmaf(dt, szz, wzz.v[0]);
-        float d = fmaf(dt, sxz, wxz.v[0]);
-        O[2 * fs + cq] = fmaf(dt, -damp * a, a);
-        O[3 * fs + cq] = fmaf(dt, -damp * bb, bb);
-        O[4 * fs + cq] = fmaf(dt, -damp * d, d);
-    }
     }
 }
 
@@ -1439,67 +1574,6 @@ __global__ __launch_bounds__(64 * SPB) void k_adj_vel_grad(
 // model once (12 B): 72 B against 92 B for the two launches, plus the
 // accumulators per block's group of shots.  Order 4 only.
 
-// Tile of a fused adjoint launch: as tile_lane, with column blocks of
-// 64 - 2*SDO owned columns; lane l of block cb sits on column cb*SPAN + l.
-template <int SDO, int SPB>
-__device__ __forceinline__ bool tile_adj_fused(const DevGrid &g, int &s, int &j, int &i0, int &i1,
-                                               bool &own, bool &colact)
-{
-    constexpr int SPAN = 64 - 2 * SDO;
-    const int lane = (int)threadIdx.x % 64;
-    int cb, st;
-    if (SPB > 1 && g.nshot == 1) {
-        const int ncq = (g.ncb + SPB - 1) / SPB;
-        const int T = ncq * g.nstrips;
-        const int per = (T + NUM_XCD - 1) / NUM_XCD;
-        const int w = (int)blockIdx.x;
-        const int logical = (w % NUM_XCD) * per + w / NUM_XCD;
-        if (logical >= T) return false;
-        cb = (logical % ncq) * SPB + wave_id();
-        st = logical / ncq;
-        s = 0;
-        if (cb >= g.ncb) return false;
-    } else {
-        const int T = g.ncb * g.nstrips * g.nbatch;
-        const int per = (T + NUM_XCD - 1) / NUM_XCD;
-        const int w = (int)blockIdx.x;
-        const int logical = (w % NUM_XCD) * per + w / NUM_XCD;
-        if (logical >= T) return false;
-        const int rest = logical / g.nbatch;
-        cb = rest % g.ncb;
-        st = rest / g.ncb;
-        s = (logical % g.nbatch) * SPB + wave_id();
-        if (s >= g.nshot) return false;
-    }
-    j = cb * SPAN + lane;
-    colact = j >= SDO && j <= g.nx - SDO;
-    own = lane >= SDO && lane < 64 - SDO && colact;
-    i0 = SDO + st * g.mrows;
-    i1 = min(i0 + g.mrows - 1, g.nz - SDO);
-    return true;
-}
-
-// A lane's value at column offsets -SDO..SDO from the neighbouring lanes
-// (wave shifts by one lane); the SDO edge lanes on each side get zeros from
-// outside the wave, so only the owned lanes of a fused tile may use them.
-template <int SDO>
-struct LaneNbr {
-    float s[2 * SDO + 1];
-    __device__ __forceinline__ explicit LaneNbr(float c)
-    {
-        s[SDO] = c;
-        float l = c, r = c;
-#pragma unroll
-        for (int k = 1; k <= SDO; ++k) {
-            l = dpp_mov<DPP_WAVE_SHR1>(0.0f, l);  // lane i <- lane i-1
-            r = dpp_mov<DPP_WAVE_SHL1>(0.0f, r);  // lane i <- lane i+1
-            s[SDO - k] = l;
-            s[SDO + k] = r;
-        }
-    }
-    __device__ __forceinline__ float operator()(int k) const { return s[SDO + k]; }
-};
-
 template <int SDO, bool PRECOND, bool ALONE, int SPB>
 __global__ __launch_bounds__(64 * SPB) void k_adj_fused(
     DevGrid g, const pyfwi_shot *shots, int t, const float *__restrict__ fields,
@@ -1515,7 +1589,7 @@ __global__ __launch_bounds__(64 * SPB) void k_adj_fused(
     __shared__ float part[2][PART];
     int s, j, i0, i1;
     bool own, colact;
-    if (!tile_adj_fused<SDO, SPB>(g, s, j, i0, i1, own, colact)) return;
+    if (!tile_fused<SDO, SPB>(g, s, j, i0, i1, own, colact)) return;
     if (!bwd_active(shots, s, t)) return;
     GroupLive gl;  // the shot group (one-shot launches have none)
     if constexpr (!ALONE) gl = group_live<SPB>(g, shots, s, t);
@@ -2477,14 +2551,22 @@ static dim3 fused_grid(DevGrid &g, int sdo, int nshots)
 }
 
 void launch_fused_step(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, int t,
-                       const float *in, float *out, const float *bu, const float *lam,
-                       const float *mu, const float *dxp, const float *dzp, hipStream_t st)
+                       const float *in, int64_t in_stride, float *out, int64_t out_stride,
+                       const float *bu, const float *lam, const float *mu, const float *dxp,
+                       const float *dzp, hipStream_t st, const float *src, float *seis)
 {
     DevGrid gm = g;
+    if (dir > 0) {
+        const int spb = march_spb(g.sdo, nshots);
+        dim3 grid = march_grid(gm, g.sdo, nshots, spb, 64 - 2 * g.sdo), block(64 * spb);
+#define ARGS gm, shots, t, in, in_stride, out, out_stride, bu, lam, mu, dxp, dzp, src, seis
+        DISPATCH_MARCH(g.sdo, spb, k_fwd_step, );
+#undef ARGS
+        return;
+    }
     dim3 grid = fused_grid(gm, g.sdo, nshots), block(64 * FUSED_WAVES);
 #define ARGS gm, shots, t, in, out, bu, lam, mu, dxp, dzp
-    if (dir > 0) DISPATCH_SDO(g.sdo, k_fused_fwd, );
-    else DISPATCH_SDO(g.sdo, k_fused_rev, );
+    DISPATCH_SDO(g.sdo, k_fused_rev, );
 #undef ARGS
 }
 

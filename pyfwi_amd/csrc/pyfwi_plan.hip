@@ -664,14 +664,19 @@ int forward_step(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t, const floa
         launch_str(g, +1, ng, dsh, t, p->cur, fps, p->cur, fps, p->lam.p, p->mu.p, p->dxp, p->dzp,
                    p->stream);
     } else if (p->tune_fused) {
-        {
+        // sources on the active cells: the kernel adds the source term to
+        // the values it loads, and writes the record sample itself unless
+        // the receivers need a gather launch
+        const bool inl = p->src_inside, rec = inl && p->rec_inline;
+        if (!rec) {
             Timed ts(p, PYFWI_K_FWD_SPARSE);
-            launch_inject_record(g, ng, dsh, t, p->cur, fps, src, seis, p->d_rtab, p->stream);
+            launch_inject_record(g, ng, dsh, t, p->cur, fps, inl ? nullptr : src, seis,
+                                 p->d_rtab, p->stream);
         }
         float *nxt = p->cur == p->fields.p ? p->fields2.p : p->fields.p;
         Timed tm(p, PYFWI_K_FWD_STEP);
-        launch_fused_step(g, +1, ng, dsh, t, p->cur, nxt, p->bu.p, p->lam.p, p->mu.p, p->dxp,
-                          p->dzp, p->stream);
+        launch_fused_step(g, +1, ng, dsh, t, p->cur, fps, nxt, fps, p->bu.p, p->lam.p, p->mu.p,
+                          p->dxp, p->dzp, p->stream, inl ? src : nullptr, rec ? seis : nullptr);
         p->cur = nxt;
     } else {
         if (!p->rec_inline) {
@@ -701,6 +706,12 @@ int forward_step_kept(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t, const
                       int64_t out_stride)
 {
     const DevGrid &g = p->dg;
+    if (p->tune_fused) {
+        Timed tm(p, PYFWI_K_FWD_STEP);
+        launch_fused_step(g, +1, ng, dsh, t, in, in_stride, out, out_stride, p->bu.p, p->lam.p,
+                          p->mu.p, p->dxp, p->dzp, p->stream, src, seis);
+        return 0;
+    }
     {
         Timed tm(p, PYFWI_K_FWD_VEL);
         launch_vel(g, +1, ng, dsh, t, in, in_stride, out, out_stride, p->bu.p, p->dxp, p->dzp,
@@ -719,8 +730,9 @@ int reverse_step(pyfwi_plan *p, const pyfwi_shot *dsh, int ng, int t)
     if (p->tune_fused) {
         float *nxt = p->cur == p->fields.p ? p->fields2.p : p->fields.p;
         Timed tm(p, PYFWI_K_REV_STEP);
-        launch_fused_step(g, -1, ng, dsh, t, p->cur, nxt, p->bu.p, p->lam.p, p->mu.p, p->dxp,
-                          p->dzp, p->stream);
+        const int64_t fps = fields_per_shot(p);
+        launch_fused_step(g, -1, ng, dsh, t, p->cur, fps, nxt, fps, p->bu.p, p->lam.p, p->mu.p,
+                          p->dxp, p->dzp, p->stream);
         p->cur = nxt;
     } else {
         const int64_t fps = fields_per_shot(p);
@@ -841,9 +853,9 @@ int forward_pass(pyfwi_plan *p, int g0, int ng, const float *d_src, float *d_sei
     KeyHash key = loop_key(p, 1, g0, ng);
     key.val(d_src).val(d_seis).val(snaps).val(ckpts).val(ncp).val(K).val(tm != nullptr);
     key.add(cl.data(), cl.size() * sizeof(int32_t));
-    // every step kept (chpr=100): the two-launch kernels step from one slot
+    // every step kept (chpr=100): the forward kernels step from one slot
     // into the next, so the kept states cost no extra traffic
-    if (snaps && keeps_every_step(cl, nt) && !p->tune_fused && p->src_inside) {
+    if (snaps && keeps_every_step(cl, nt) && p->src_inside) {
         return run_loop(p, key.h, [&]() -> int {
             const int64_t ss = tm ? fps : (int64_t)nchp * fps;  // between shots
             const float *in = p->fields.p;
