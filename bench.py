@@ -53,6 +53,7 @@ KERNEL_BYTES = {
     "exact_vel": (56, 0, 8),      # R 2 av b 3 fwd tau 3 atau, W 3 atau 2 av; group: RW g_b
     "adj_step": (52, 20, 24),     # all: R 5 adjoint lam mu b, W 5 adjoint; interior: R 5 fwd; group: RW 6 acc
     "fwd_step": (52, 0, 0),       # R 5 fields lam mu b, W 5 fields
+    "fwd_two": (72, 0, 0),        # two steps: R 5 fields lam mu b, W 2 x 5 fields
     "rev_step": (52, 0, 0),
 }
 def shots_per_block(order, shots):
@@ -359,30 +360,58 @@ def copy_rate(torch, nbytes=1 << 30, reps=5):
     return rate
 
 
-def pass_kernel_bytes(kind, pass_nts, cells, interior, order=4):
-    """Algorithmic bytes and launch count of kernel `kind` over passes of
-    uneven shots.  pass_nts: per pass, the shots' nt in the order the plan
-    runs them (decreasing).  A forward launch of step t advances the shots
-    with t < nt_s, an adjoint launch of step t (nt-1 .. 1) those with
-    1 <= t < nt_s; the accumulators move once per block's group with a live
-    shot.  Each launch takes only the live shots (the plan's time-major kept
-    states), so shots per block follow the live count, as the library picks
-    them."""
-    a, b, c = KERNEL_BYTES[kind]
+def two_step_pairs(live):
+    """Steps that start a two-step forward launch (PYFWI_TUNE_FWD_TWO), as the
+    plan pairs them: from t = 0, steps t and t+1 share a launch when the same
+    shots are live at both, else step t runs alone."""
+    starts, t = [], 0
+    while t < len(live):
+        if t + 1 < len(live) and live[t + 1] == live[t]:
+            starts.append(t)
+            t += 2
+        else:
+            t += 1
+    return np.asarray(starts, np.int64)
+
+
+def launch_live_shots(kind, pass_nts, two=False):
+    """Per launch of kernel `kind` over passes of uneven shots, the number of
+    shots it advances.  pass_nts: per pass, the shots' nt in the order the
+    plan runs them (decreasing).  A forward launch of step t advances the
+    shots with t < nt_s, an adjoint launch of step t (nt-1 .. 1) those with
+    1 <= t < nt_s: each launch takes only the live shots (the plan's
+    time-major kept states).  two: the forward runs two steps a launch where
+    it can (two_step_pairs); "fwd_two" counts those launches and the
+    forward's one-step kinds the steps left alone."""
     adjoint = kind.startswith("adj") or kind.startswith("exact")
-    total = 0.0
-    launches = 0
+    out = []
     for nts in pass_nts:
         nts = np.sort(np.asarray(nts, np.int64))[::-1]
-        tmax = int(nts[0])
-        t = np.arange(1 if adjoint else 0, tmax)
-        live = (nts[None, :] > t[:, None]).sum(1)  # shots live at each launch
-        spb = np.where((order == 4) & (live % 5 == 0) & (live % 4 != 0), 5, 4)
-        groups = -(-live // spb)
-        imaged = cells if kind.startswith("exact") else interior
-        total += float(((a * cells + b * interior) * live).sum() + (c * imaged * groups).sum())
-        launches += len(t)
-    return total, launches
+        t = np.arange(1 if adjoint else 0, int(nts[0]))
+        live = (nts[None, :] > t[:, None]).sum(1)
+        if two and not adjoint:
+            starts = two_step_pairs(live)
+            paired = np.zeros(len(t), bool)
+            paired[starts] = paired[starts + 1] = True
+            live = live[starts] if kind == "fwd_two" else live[~paired]
+        elif kind == "fwd_two":
+            live = live[:0]
+        out.append(live)
+    return np.concatenate(out) if out else np.zeros(0, np.int64)
+
+
+def pass_kernel_bytes(kind, pass_nts, cells, interior, order=4, two=False):
+    """Algorithmic bytes and launch count of kernel `kind` over passes of
+    uneven shots (launch_live_shots); the accumulators move once per block's
+    group with a live shot, shots per block following the live count as the
+    library picks them."""
+    a, b, c = KERNEL_BYTES[kind]
+    live = launch_live_shots(kind, pass_nts, two)
+    spb = np.where((order == 4) & (live % 5 == 0) & (live % 4 != 0), 5, 4)
+    groups = -(-live // spb)
+    imaged = cells if kind.startswith("exact") else interior
+    total = float(((a * cells + b * interior) * live).sum() + (c * imaged * groups).sum())
+    return total, len(live)
 
 
 def cpu_model():
@@ -451,7 +480,8 @@ def bench_uneven(args, torch):
     graphs = args.graphs if args.graphs >= 0 else int(B.tnz * B.tnx < 1_000_000)
     plan.set_graphs(bool(graphs))
     bits = (_native.TUNE_ADJ_FUSED if args.adj_fused else 0) | \
-        (_native.TUNE_FUSED_STEPS if args.fused_fwd else 0)
+        (_native.TUNE_FUSED_STEPS if args.fused_fwd else 0) | \
+        (_native.TUNE_FWD_TWO if args.fused_fwd == 2 else 0)
     plan.set_tuning(0, bits, args.rows)
 
     def run(shots):
@@ -501,7 +531,8 @@ def bench_uneven(args, torch):
     prof = kernel_profile(None, {"plan": plan}, torch, step)
     kernels = {}
     for kname, (ms, n) in prof.items():
-        nbytes, nl = pass_kernel_bytes(kname, pass_nts, cells, interior, order)
+        nbytes, nl = pass_kernel_bytes(kname, pass_nts, cells, interior, order,
+                                       two=args.fused_fwd == 2)
         if nl != n:
             log(f"bench: {kname}: {n} launches timed, {nl} in the byte model")
         kernels[kname] = {"avg_us": round(ms / n * 1e3, 2), "launches": n,
@@ -512,11 +543,8 @@ def bench_uneven(args, torch):
     achieved = kernels[dom]["GBps"]
     # launches of the dominant kernel by the number of shots they advance
     launch_mix = {}
-    t0_dom = 1 if dom.startswith("adj") else 0
-    for nts in pass_nts:
-        for t in range(t0_dom, max(nts)):
-            live = sum(1 for x in nts if x > t)
-            launch_mix[live] = launch_mix.get(live, 0) + 1
+    for live in launch_live_shots(dom, pass_nts, two=args.fused_fwd == 2):
+        launch_mix[int(live)] = launch_mix.get(int(live), 0) + 1
 
     # forward modelling alone: all shots, records only
     no_chp = np.zeros(0, np.int32)
@@ -581,7 +609,8 @@ def bench_uneven(args, torch):
                                "shots packed first-fit by decreasing nt into passes whose "
                                "kept states fill the free HBM; L2 misfit of vx, vz on the GPU",
                    "passes": passes, "cell_updates": units, "graphs": bool(graphs),
-                   "forward_step": "one launch" if args.fused_fwd else "two launches",
+                   "forward_step": {0: "two launches", 1: "one launch",
+                                    2: "two steps a launch"}[args.fused_fwd],
                    "parallelism": "single GPU, shots batched per launch"},
         "kernel_timing": "per-step time of each time loop (HIP events around the loops) split "
                          "between its kernels by event-bracketed launches",
@@ -831,9 +860,10 @@ def main():
     ap.add_argument("--adj-fused", action="store_true",
                     help="reference adjoint in one launch a step instead of two "
                          "(stresses, then velocities)")
-    ap.add_argument("--fused-fwd", type=int, default=1, choices=[0, 1],
+    ap.add_argument("--fused-fwd", type=int, default=1, choices=[0, 1, 2],
                     help="uneven_batch: each forward step in one launch (1, measured "
-                         "1.4-2.4 %% faster over the gradient) or two (0)")
+                         "1.4-2.4 %% faster over the gradient), two (0), or two steps "
+                         "a launch (2)")
     ap.add_argument("--no-tune", action="store_true", help="skip the execution-shape autotuner")
     ap.add_argument("--tune", default="", help="fixed tuning 'shots_per_pass,fused,rows'")
     ap.add_argument("--iters", type=int, default=5,
@@ -906,6 +936,7 @@ def main():
                                           chp_t, 0, 0, _native.MEM_DEVICE)
             cands = tn.default_candidates(int(W.ns), max_pass,
                                           fused=(0, _native.TUNE_FUSED_STEPS,
+                                                 _native.TUNE_FUSED_STEPS | _native.TUNE_FWD_TWO,
                                                  _native.TUNE_ADJ_FUSED))
             tuning, _ = tn.autotune(b["plan"], run, torch.cuda.synchronize, cands, log=log)
         else:
@@ -1013,6 +1044,7 @@ def main():
                    "parallelism": "single GPU, shots batched per launch",
                    "tuning": {"shots_per_pass": tuning[0],
                               "fused_steps": bool(int(tuning[1]) & _native.TUNE_FUSED_STEPS),
+                              "two_steps_a_launch": bool(int(tuning[1]) & _native.TUNE_FWD_TWO),
                               "adjoint_one_launch": bool(int(tuning[1]) & _native.TUNE_ADJ_FUSED),
                               "strip_rows": tuning[2], "graphs": bool(graphs)}},
         "kernel_timing": ("event-bracketed launches (each holds a few microseconds of the "

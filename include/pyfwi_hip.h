@@ -283,13 +283,16 @@ int32_t pyfwi_set_record_filter(pyfwi_plan *plan, const double *taps, int32_t ha
  * step as one launch instead of two half-step launches (the forward step
  * reads one state and writes the next: ping-pong buffers, or from one kept
  * slot into the next; records and kept states bit for bit those of two
- * launches);
+ * launches); PYFWI_TUNE_FWD_TWO runs the kept-state forward two steps a
+ * launch where the same shots are live at both (order 4; bit for bit those
+ * of single steps; slower than single one-launch steps at the grids
+ * measured);
  * PYFWI_TUNE_ADJ_FUSED runs the reference adjoint of each backward step as
  * one launch instead of two (stresses, then velocities; order 4 with
  * receivers on the active cells, else ignored); strip_rows > 0 fixes the
  * rows each thread walks (0: chosen per launch).  Results do not depend on
  * these choices beyond rounding. */
-enum { PYFWI_TUNE_FUSED_STEPS = 1, PYFWI_TUNE_ADJ_FUSED = 2 };
+enum { PYFWI_TUNE_FUSED_STEPS = 1, PYFWI_TUNE_ADJ_FUSED = 2, PYFWI_TUNE_FWD_TWO = 4 };
 int32_t pyfwi_plan_set_tuning(pyfwi_plan *plan, int32_t shots_per_pass, int32_t fused,
                               int32_t strip_rows);
 
@@ -362,7 +365,8 @@ enum {
     PYFWI_K_BWD_SPARSE = 13, /* residual injection launches                */
     PYFWI_K_ADJ_STEP = 14,   /* adjoint step in one launch (stresses, velocities,
                                 imaging condition)                         */
-    PYFWI_K_COUNT = 15
+    PYFWI_K_FWD_TWO = 15,    /* two forward steps in one launch            */
+    PYFWI_K_COUNT = 16
 };
 int32_t pyfwi_plan_set_timing(pyfwi_plan *plan, int32_t enable);
 int32_t pyfwi_kernel_timing(pyfwi_plan *plan, int32_t kind, double *total_ms,

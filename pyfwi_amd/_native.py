@@ -23,12 +23,12 @@ REC_VZ_FROM_VX = 1
 
 (K_FWD_VEL, K_FWD_STR, K_REV_STR, K_REV_VEL, K_ADJ_STR, K_ADJ_VEL, K_EXA_STR, K_EXA_VEL,
  K_FWD_STEP, K_REV_STEP, K_FWD_LOOP, K_BWD_LOOP, K_FWD_SPARSE, K_BWD_SPARSE,
- K_ADJ_STEP) = range(15)
+ K_ADJ_STEP, K_FWD_TWO) = range(16)
 KERNEL_NAMES = ["fwd_vel", "fwd_str", "rev_str", "rev_vel", "adj_str", "adj_vel_grad",
                 "exact_str", "exact_vel", "fwd_step", "rev_step", None, None,
-                "fwd_sparse", "bwd_sparse", "adj_step"]
+                "fwd_sparse", "bwd_sparse", "adj_step", "fwd_two"]
 # bits of set_tuning's `fused`
-TUNE_FUSED_STEPS, TUNE_ADJ_FUSED = 1, 2
+TUNE_FUSED_STEPS, TUNE_ADJ_FUSED, TUNE_FWD_TWO = 1, 2, 4
 TIMING_LAUNCHES, TIMING_LOOPS = 1, 2
 
 # exported symbols, in header order (checked by the CPU test suite)
@@ -269,8 +269,8 @@ class Plan:
         return out.value
 
     def set_tuning(self, shots_per_pass=0, fused=0, strip_rows=0):
-        """fused: bits TUNE_FUSED_STEPS, TUNE_ADJ_FUSED (True means
-        TUNE_FUSED_STEPS)."""
+        """fused: bits TUNE_FUSED_STEPS, TUNE_ADJ_FUSED, TUNE_FWD_TWO (True
+        means TUNE_FUSED_STEPS)."""
         check(self.lib.pyfwi_plan_set_tuning(self.h, int(shots_per_pass), int(fused),
                                              int(strip_rows)))
 

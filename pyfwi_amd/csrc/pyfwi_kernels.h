@@ -41,6 +41,9 @@ constexpr int FUSED_WAVES = 4;      // fused two-stage kernels: wavefronts per b
 #ifndef FWD_STEP_UNROLL
 #define FWD_STEP_UNROLL 2           // rows per unrolled iteration of the one-launch forward step
 #endif
+#ifndef FWD_TWO_ROWS
+#define FWD_TWO_ROWS 36             // strip height of the two-step forward (k_fwd_two)
+#endif
 #ifndef ADJ_VEL_UNROLL
 #define ADJ_VEL_UNROLL 2            // rows per unrolled iteration of the adjoint velocity + imaging kernel
 #endif
@@ -114,6 +117,15 @@ void launch_str(const DevGrid &g, int dir, int nshots, const pyfwi_shot *shots, 
                 const float *in, int64_t in_stride, float *out, int64_t out_stride,
                 const float *lam, const float *mu, const float *dxp, const float *dzp,
                 hipStream_t st, const float *src = nullptr, float *seis = nullptr);
+// Forward steps t and t+1 in one launch (order 4; false, launching
+// nothing, at order 8): state t-1 read from `in`, the states after steps t
+// and t+1 written to `out0` and `out1` (per-shot stride out_stride); source
+// terms and record samples as launch_fused_step.
+bool launch_fwd_two(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
+                    const float *in, int64_t in_stride, float *out0, float *out1,
+                    int64_t out_stride, const float *bu, const float *lam, const float *mu,
+                    const float *dxp, const float *dzp, hipStream_t st, const float *src,
+                    float *seis);
 // Record the state before step t's source term, then add the term in memory
 // (the fused steps, the exact mode's recomputation and sources off the
 // active cells); with src == nullptr only record (receiver tables, whose

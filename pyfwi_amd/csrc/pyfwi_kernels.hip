@@ -688,15 +688,16 @@ __device__ __forceinline__ float lane_dplus(float v, int lane, const float c[4])
     return r;
 }
 
-// Tile of a two-stage launch (k_fwd_step, k_adj_fused): as tile_lane, with
-// column blocks of 64 - 2*SDO owned columns; lane l of block cb sits on
-// column cb*SPAN + l, and the lanes of the SDO columns on each side of the
-// owned ones compute stage 1 only.
-template <int SDO, int SPB>
+// Tile of a multi-stage launch (k_fwd_step, k_fwd_two, k_adj_fused): as
+// tile_lane, with column blocks of SPAN = 64 - 2*H owned columns, H the halo
+// lanes on each side (SDO per stage after the first); lane l of block cb
+// sits on column cb*SPAN + l + SDO - H, and the halo lanes compute the early
+// stages only.
+template <int SDO, int SPB, int H = SDO>
 __device__ __forceinline__ bool tile_fused(const DevGrid &g, int &s, int &j, int &i0, int &i1,
                                                bool &own, bool &colact)
 {
-    constexpr int SPAN = 64 - 2 * SDO;
+    constexpr int SPAN = 64 - 2 * H;
     const int lane = (int)threadIdx.x % 64;
     int cb, st;
     if (SPB > 1 && g.nshot == 1) {
@@ -722,9 +723,9 @@ __device__ __forceinline__ bool tile_fused(const DevGrid &g, int &s, int &j, int
         s = (logical % g.nbatch) * SPB + wave_id();
         if (s >= g.nshot) return false;
     }
-    j = cb * SPAN + lane;
+    j = cb * SPAN + lane + SDO - H;
     colact = j >= SDO && j <= g.nx - SDO;
-    own = lane >= SDO && lane < 64 - SDO && colact;
+    own = lane >= H && lane < 64 - H && colact;
     i0 = SDO + st * g.mrows;
     i1 = min(i0 + g.mrows - 1, g.nz - SDO);
     return true;
@@ -902,6 +903,248 @@ __global__ __launch_bounds__(64 * SPB) void k_fwd_step(
                     w[2 * rl.fs] = a;
                     w[3 * rl.fs] = bb;
                     w[4 * rl.fs] = d;
+                }
+            }
+        }
+    }
+}
+
+// Two forward steps in one launch (time blocking, order 4): stages 1 and 2
+// are k_fwd_step's for step t (velocities at row r, stresses at row
+// r - SDO), stages 3 and 4 the same for step t+1 at rows r - 2*SDO and
+// r - 3*SDO, each stage fed by the one before through register windows
+// along z and lane moves along x.  State t-1 is read from `in`; the states
+// after steps t and t+1 are written to `out0` and `out1` (per-shot stride
+// out_stride), as two k_fwd_step launches would.  Per shot and cell and two
+// steps: one state read (20 B), the model read once (12 B), two states
+// written (40 B): 72 B against 104 B for two one-launch steps.  A wave owns
+// the middle 64 - 6*SDO columns; each strip runs stage 1 on 3*SDO more rows
+// above and below.  Step t+1's source term enters the values stages 3 and 4
+// use, never the stored state t; every value is computed by the arithmetic
+// of k_vel and k_str.  A shot whose last step is t stores nothing for t+1.
+template <int SPB>
+__global__ __launch_bounds__(64 * SPB) void k_fwd_two(
+    DevGrid g, const pyfwi_shot *shots, int t, const float *__restrict__ in, int64_t in_stride,
+    float *__restrict__ out0, float *__restrict__ out1, int64_t out_stride,
+    const float *__restrict__ bu, const float *__restrict__ lam, const float *__restrict__ mu,
+    const float *__restrict__ dxp, const float *__restrict__ dzp, const float *__restrict__ src,
+    float *seis)
+{
+    constexpr int SDO = 2;
+    int s, j, i0, i1;
+    bool own, colact;
+    if (!tile_fused<SDO, SPB, 3 * SDO>(g, s, j, i0, i1, own, colact)) return;
+    if (!fwd_active(shots, s, t)) return;
+    const bool second = fwd_active(shots, s, t + 1);  // wave-uniform
+    const int64_t fs = g.fstride;
+    const uint32_t RB = (uint32_t)g.pitch * 4u;
+    // bases 4*SDO rows above cell (0, 0): every row read (from -4*SDO) has a
+    // non-negative byte offset
+    const int64_t org = g.origin - (int64_t)4 * SDO * g.pitch;
+    const float *F = in + (int64_t)s * in_stride + org;
+    float *O0 = out0 + (int64_t)s * out_stride + org;
+    float *O1 = out1 + (int64_t)s * out_stride + org;
+    const CellRef vx{F}, vz{F + fs}, txx{F + 2 * fs}, tzz{F + 3 * fs}, txz{F + 4 * fs};
+    const CellRef B{bu + org}, L{lam + org}, M{mu + org};
+    // step t's source term on the loaded values (as k_fwd_step)
+    SrcTerm s0 = src_term(shots[s], t, src);
+    if (s0.z < i0 - 4 * SDO || s0.z > i1 + 4 * SDO || s0.x < j - SDO || s0.x > j + SDO)
+        s0.z = -1;
+    float s_txx[2 * SDO];  // txx term on the taps of d/dx- (offsets -SDO..SDO-1)
+#pragma unroll
+    for (int k = 0; k < 2 * SDO; ++k) s_txx[k] = j + k - SDO == s0.x ? s0.v[2] : 0.0f;
+    const bool src_col = j == s0.x;
+    const float s_vx = src_col ? s0.v[0] : 0.0f, s_vz = src_col ? s0.v[1] : 0.0f;
+    const float s_tzz = src_col ? s0.v[3] : 0.0f, s_txxc = src_col ? s0.v[2] : 0.0f;
+    // step t+1's source term, added to stage 1 and 2's results at the source
+    // cell before stages 3 and 4 use them
+    SrcTerm s1 = second ? src_term(shots[s], t + 1, src) : SrcTerm{};
+    if (s1.x != j) s1.z = -1;
+    // record samples of steps t+1 and t+2, written by the owning lane
+    RecLine r1 = record_line(shots[s], t + 1, seis);
+    const int rec1 = own && r1.z >= i0 && r1.z <= i1 ? r1.index(j) : -1;
+    if (rec1 < 0) r1.z = -1;
+    RecLine r2 = second ? record_line(shots[s], t + 2, seis) : RecLine{};
+    const int rec2 = own && r2.z >= i0 && r2.z <= i1 ? r2.index(j) : -1;
+    if (rec2 < 0) r2.z = -1;
+    auto tzz_row = [&](uint32_t o, int row) {
+        float v = tzz(o);
+        if (row == s0.z) v += s_tzz;
+        return v;
+    };
+    auto dx_txz = [&](uint32_t o) { return xplus<SDO>([&](int k) { return txz.at(o, k); }, g.cx); };
+
+    ZWin<SDO> wxz, wzz;        // state t-1: txz (d/dz-), tzz (d/dz+) around stage 1's row
+    Fifo<SDO> fxz;             // d/dx+ txz, taken on the entering row
+    Fifo<SDO> otxx;            // txx of state t-1 (source included), at stage 2's row
+    ZWin<SDO> nvx{}, nvz{};    // stage 1's velocities around stage 2's row
+    Fifo<2 * SDO> dvx{}, dvz{};  // stage 1's velocities (step t+1's source in), at stage 3's row
+    ZWin<SDO> mtxz{}, mtzz{};  // stage 2's txz, tzz (source in) around stage 3's row
+    Fifo<SDO> mtxx{};          // stage 2's txx (source in), at stage 3's row
+    Fifo<SDO> mtxx2{};         // and at stage 4's row
+    ZWin<SDO> wvx{}, wvz{};    // stage 3's velocities around stage 4's row
+    // first stage 1 row: i0 - 3*SDO
+    const int r0 = i0 - 3 * SDO;
+    // (j < 0 on the halo lanes of the first column block: the offset is taken
+    // in 64 bits, then is non-negative)
+    uint32_t cb = (uint32_t)(((int64_t)(r0 + 4 * SDO) * g.pitch + j) * 4);
+#pragma unroll
+    for (int k = 0; k < 2 * SDO; ++k) {
+        const uint32_t o = cb + (uint32_t)(k - SDO) * RB;
+        const int row = r0 + k - SDO;
+        wxz.v[k + 1] = txz(o);
+        wzz.v[k + 1] = tzz_row(o, row);
+        if (k >= SDO) fxz.v[k - SDO] = dx_txz(o);
+        if (k < SDO) {
+            float v = txx(o);
+            if (row == s0.z) v += s_txxc;
+            otxx.v[k] = v;
+        }
+    }
+    const float ex = colact ? dxp[j] : 0.0f;
+    const float dt = g.dt;
+    const int nrow = i1 + 3 * SDO - r0 + 1;
+    for (int rr = 0; rr < nrow; rr += FWD_STEP_UNROLL) {
+#pragma unroll
+        for (int u = 0; u < FWD_STEP_UNROLL; ++u, cb += RB) {
+            const int r = r0 + rr + u, q = r - SDO, p = r - 2 * SDO, o = r - 3 * SDO;
+            // (wave-uniform) no row past the strip's end: the bottom strip's
+            // next row would read below the padded fields
+            if (r > i1 + 3 * SDO) break;
+            const uint32_t n = cb + (uint32_t)SDO * RB;
+            const uint32_t cq = cb - (uint32_t)SDO * RB, cp = cq - (uint32_t)SDO * RB,
+                           co = cp - (uint32_t)SDO * RB;
+            // stages 3 and 4 above the rows they feed (p < i0 - SDO, o < i0)
+            // read the model and damping of row r instead: their results are
+            // never used, and rows that far above the strip may lie outside
+            // the padded fields
+            const bool p_ok = p >= i0 - SDO, o_ok = o >= i0;
+            const uint32_t cpl = p_ok ? cp : cb, col = o_ok ? co : cb;
+            // loads of the row
+            wxz.push(txz(n));
+            wzz.push(tzz_row(n, r + SDO));
+            const float dxp_txz = fxz.pop(dx_txz(n));
+            Taps<SDO, 1> ttxx;
+            ttxx.load(txx, cb);
+            float ux = vx(cb), uz = vz(cb);
+            const float b1 = B(cb), b3 = B(cpl);
+            const float l2 = L(cq), m2 = M(cq), l4 = L(col), m4 = M(col);
+            if (r == s0.z) {
+#pragma unroll
+                for (int k = 0; k < 2 * SDO; ++k) ttxx.t[k] += s_txx[k];
+                ux += s_vx;
+                uz += s_vz;
+            }
+            const float a0 = otxx.pop(ttxx(0));  // txx of state t-1 at row q
+            // stage 1: step t's velocities at row r
+            const float dxm_txx = xminus<SDO>(ttxx, g.cx);
+            const float ax = b1 * (dxm_txx + wxz.dminus(g.cz));
+            const float az = b1 * (wzz.dplus(g.cz) + dxp_txz);
+            const float damp1 = ex + dzp[r];
+            ux = fmaf(dt, ax, ux);
+            ux = fmaf(dt, -damp1 * ux, ux);
+            uz = fmaf(dt, az, uz);
+            uz = fmaf(dt, -damp1 * uz, uz);
+            const bool act1 = colact && r >= SDO && r <= g.nz - SDO;
+            ux = act1 ? ux : 0.0f;
+            uz = act1 ? uz : 0.0f;
+            nvx.push(ux);
+            nvz.push(uz);
+            const bool src1_r = r == s1.z;
+            const float vx3 = dvx.pop(src1_r ? ux + s1.v[0] : ux);  // at row p
+            const float vz3 = dvz.pop(src1_r ? uz + s1.v[1] : uz);
+            // stage 2: step t's stresses at row q
+            const float exx = xplus<SDO>(LaneNbr<SDO>(nvx.centre()), g.cx);
+            const float dxm_vz = xminus<SDO>(LaneNbr<SDO>(nvz.centre()), g.cx);
+            const float ezz = nvz.dminus(g.cz);
+            const float gxz = dxm_vz + nvx.dplus(g.cz);
+            const float lm2 = l2 + 2.0f * m2;
+            const float sxx = fmaf(l2, ezz, lm2 * exx);  // as in k_str
+            const float szz = fmaf(l2, exx, lm2 * ezz);
+            const float sxz = m2 * gxz;
+            const float damp2 = ex + dzp[q];
+            float a = a0, bb = wzz.v[0], d = wxz.v[0];
+            a = fmaf(dt, sxx, a);
+            bb = fmaf(dt, szz, bb);
+            d = fmaf(dt, sxz, d);
+            a = fmaf(dt, -damp2 * a, a);
+            bb = fmaf(dt, -damp2 * bb, bb);
+            d = fmaf(dt, -damp2 * d, d);
+            const bool act2 = colact && q >= SDO && q <= g.nz - SDO;
+            a = act2 ? a : 0.0f;
+            bb = act2 ? bb : 0.0f;
+            d = act2 ? d : 0.0f;
+            const bool src1_q = q == s1.z;
+            const float a_s = src1_q ? a + s1.v[2] : a, bb_s = src1_q ? bb + s1.v[3] : bb;
+            mtxz.push(d);
+            mtzz.push(bb_s);
+            const float txx3 = mtxx.pop(a_s);     // at row p
+            const float txx4 = mtxx2.pop(txx3);   // at row o
+            // stage 3: step t+1's velocities at row p
+            const float dxm_txx3 = xminus<SDO>(LaneNbr<SDO>(txx3), g.cx);
+            const float dxp_txz3 = xplus<SDO>(LaneNbr<SDO>(mtxz.centre()), g.cx);
+            const float ax3 = b3 * (dxm_txx3 + mtxz.dminus(g.cz));
+            const float az3 = b3 * (mtzz.dplus(g.cz) + dxp_txz3);
+            const float damp3 = ex + dzp[p_ok ? p : r];
+            float ux3 = fmaf(dt, ax3, vx3);
+            ux3 = fmaf(dt, -damp3 * ux3, ux3);
+            float uz3 = fmaf(dt, az3, vz3);
+            uz3 = fmaf(dt, -damp3 * uz3, uz3);
+            const bool act3 = colact && p >= SDO && p <= g.nz - SDO;
+            ux3 = act3 ? ux3 : 0.0f;
+            uz3 = act3 ? uz3 : 0.0f;
+            wvx.push(ux3);
+            wvz.push(uz3);
+            // stage 4: step t+1's stresses at row o
+            const float exx4 = xplus<SDO>(LaneNbr<SDO>(wvx.centre()), g.cx);
+            const float dxm_vz4 = xminus<SDO>(LaneNbr<SDO>(wvz.centre()), g.cx);
+            const float ezz4 = wvz.dminus(g.cz);
+            const float gxz4 = dxm_vz4 + wvx.dplus(g.cz);
+            const float lm4 = l4 + 2.0f * m4;
+            const float sxx4 = fmaf(l4, ezz4, lm4 * exx4);
+            const float szz4 = fmaf(l4, exx4, lm4 * ezz4);
+            const float sxz4 = m4 * gxz4;
+            const float damp4 = ex + dzp[o_ok ? o : r];
+            float a4 = txx4, bb4 = mtzz.v[0], d4 = mtxz.v[0];
+            a4 = fmaf(dt, sxx4, a4);
+            bb4 = fmaf(dt, szz4, bb4);
+            d4 = fmaf(dt, sxz4, d4);
+            a4 = fmaf(dt, -damp4 * a4, a4);
+            bb4 = fmaf(dt, -damp4 * bb4, bb4);
+            d4 = fmaf(dt, -damp4 * d4, d4);
+            // stores: the owned columns of the strip
+            if (own && r >= i0 && r <= i1) {
+                cell_ref(O0, cb) = ux;
+                cell_ref(O0 + fs, cb) = uz;
+            }
+            if (own && q >= i0 && q <= i1) {
+                cell_ref(O0 + 2 * fs, cq) = a;
+                cell_ref(O0 + 3 * fs, cq) = bb;
+                cell_ref(O0 + 4 * fs, cq) = d;
+                if (q == r1.z) {
+                    float *w = const_cast<float *>(r1.base) + rec1;
+                    w[0] = nvx.centre();
+                    w[r1.fs] = nvz.centre();
+                    w[2 * r1.fs] = a;
+                    w[3 * r1.fs] = bb;
+                    w[4 * r1.fs] = d;
+                }
+            }
+            if (second && own && p >= i0 && p <= i1) {
+                cell_ref(O1, cp) = ux3;
+                cell_ref(O1 + fs, cp) = uz3;
+            }
+            if (second && own && o >= i0 && o <= i1) {
+                cell_ref(O1 + 2 * fs, co) = a4;
+                cell_ref(O1 + 3 * fs, co) = bb4;
+                cell_ref(O1 + 4 * fs, co) = d4;
+                if (o == r2.z) {
+                    float *w = const_cast<float *>(r2.base) + rec2;
+                    w[0] = wvx.centre();
+                    w[r2.fs] = wvz.centre();
+                    w[2 * r2.fs] = a4;
+                    w[3 * r2.fs] = bb4;
+                    w[4 * r2.fs] = d4;
                 }
             }
         }
@@ -2568,6 +2811,24 @@ void launch_fused_step(const DevGrid &g, int dir, int nshots, const pyfwi_shot *
 #define ARGS gm, shots, t, in, out, bu, lam, mu, dxp, dzp
     DISPATCH_SDO(g.sdo, k_fused_rev, );
 #undef ARGS
+}
+
+bool launch_fwd_two(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,
+                    const float *in, int64_t in_stride, float *out0, float *out1,
+                    int64_t out_stride, const float *bu, const float *lam, const float *mu,
+                    const float *dxp, const float *dzp, hipStream_t st, const float *src,
+                    float *seis)
+{
+    if (g.sdo != 2) return false;  // order 4 only
+    DevGrid gm = g;
+    if (gm.rows_override == 0) gm.rows_override = FWD_TWO_ROWS;
+    const int spb = march_spb(g.sdo, nshots);
+    dim3 grid = march_grid(gm, g.sdo, nshots, spb, 64 - 6 * g.sdo), block(64 * spb);
+#define ARGS gm, shots, t, in, in_stride, out0, out1, out_stride, bu, lam, mu, dxp, dzp, src, seis
+    if (spb == 5) k_fwd_two<5><<<grid, block, 0, st>>>(ARGS);
+    else k_fwd_two<4><<<grid, block, 0, st>>>(ARGS);
+#undef ARGS
+    return true;
 }
 
 void launch_inject_record(const DevGrid &g, int nshots, const pyfwi_shot *shots, int t,

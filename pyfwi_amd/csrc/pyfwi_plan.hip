@@ -122,6 +122,7 @@ struct pyfwi_plan {
     // tuning: shots per pass (0: all), fused steps, strip height (0: automatic)
     int tune_group = 0, tune_fused = 0, tune_rows = 0;
     int adj_fused = 0;  // reference adjoint in one launch a step (PYFWI_TUNE_ADJ_FUSED)
+    int fwd_two = 0;    // kept-state forward two steps a launch (PYFWI_TUNE_FWD_TWO)
     bool ckpts_all = false;  // ckpts holds every shot of the last forward
     DevBuf src_stage, seis_stage, res_stage, grad_stage, shot_tab_f, resid;
     // record filter of the misfit (pyfwi_set_record_filter): taps k[0..L]
@@ -860,14 +861,33 @@ int forward_pass(pyfwi_plan *p, int g0, int ng, const float *d_src, float *d_sei
             const int64_t ss = tm ? fps : (int64_t)nchp * fps;  // between shots
             const float *in = p->fields.p;
             int64_t in_stride = fps;
-            for (int t = 0; t < nt; ++t) {
+            for (int t = 0; t < nt;) {
                 if (ckpts && t % K == 0)
                     launch_copy_shots(ckpts + (int64_t)(t / K) * fps, (int64_t)ncp * fps, in,
                                       in_stride, fps, ng, p->stream);
+                const int live = tm ? tm->live[(size_t)t] : ng;
                 float *out = snaps + (tm ? tm->off[(size_t)t] : (int64_t)t) * fps;
-                forward_step_kept(p, dsh, tm ? tm->live[(size_t)t] : ng, t, d_src,
-                                  p->rec_inline ? d_seis : nullptr, in, in_stride, out, ss);
-                in = out;
+                // two steps a launch when the same shots are live at both
+                if (p->fwd_two && p->dg.sdo == 2 && t + 1 < nt &&
+                    (tm ? tm->live[(size_t)t + 1] : ng) == live) {
+                    float *out1 = snaps + (tm ? tm->off[(size_t)t + 1] : (int64_t)t + 1) * fps;
+                    {
+                        Timed tk(p, PYFWI_K_FWD_TWO);
+                        launch_fwd_two(p->dg, live, dsh, t, in, in_stride, out, out1, ss,
+                                       p->bu.p, p->lam.p, p->mu.p, p->dxp, p->dzp, p->stream,
+                                       d_src, p->rec_inline ? d_seis : nullptr);
+                    }
+                    if (ckpts && (t + 1) % K == 0)
+                        launch_copy_shots(ckpts + (int64_t)((t + 1) / K) * fps,
+                                          (int64_t)ncp * fps, out, ss, fps, ng, p->stream);
+                    in = out1;
+                    t += 2;
+                } else {
+                    forward_step_kept(p, dsh, live, t, d_src, p->rec_inline ? d_seis : nullptr,
+                                      in, in_stride, out, ss);
+                    in = out;
+                    t += 1;
+                }
                 in_stride = ss;
                 if (int e = maybe_harvest(p)) return e;
             }
@@ -1651,6 +1671,7 @@ int32_t pyfwi_plan_set_tuning(pyfwi_plan *p, int32_t shots_per_pass, int32_t fus
     p->tune_group = shots_per_pass;
     p->tune_fused = (fused & PYFWI_TUNE_FUSED_STEPS) ? 1 : 0;
     p->adj_fused = (fused & PYFWI_TUNE_ADJ_FUSED) ? 1 : 0;
+    p->fwd_two = (fused & PYFWI_TUNE_FWD_TWO) ? 1 : 0;
     p->tune_rows = strip_rows;
     p->dg.rows_override = strip_rows;
     p->have_forward = false;

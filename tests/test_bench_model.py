@@ -118,3 +118,22 @@ def test_pass_bytes_uneven_shots():
     assert n == 9 and total == want
     total, n = bench.pass_kernel_bytes("fwd_vel", [nts], CELLS, INTERIOR)
     assert n == 10 and total == 32 * CELLS * sum(nts)
+
+
+def test_two_step_pairs_follow_the_live_shots():
+    """Two forward steps share a launch only where the same shots are live at
+    both; the byte model splits the forward's bytes between the two-step and
+    the one-step launches without losing a step."""
+    nts = [7, 5, 5]  # live 3,3,3,3,3,1,1 at t = 0..6
+    live = bench.launch_live_shots("fwd_step", [nts])
+    assert list(live) == [3, 3, 3, 3, 3, 1, 1]
+    assert list(bench.two_step_pairs(live)) == [0, 2, 5]  # step 4 alone
+    two = bench.launch_live_shots("fwd_two", [nts], two=True)
+    one = bench.launch_live_shots("fwd_step", [nts], two=True)
+    assert list(two) == [3, 3, 1] and list(one) == [3]
+    b2, n2 = bench.pass_kernel_bytes("fwd_two", [nts], CELLS, INTERIOR, two=True)
+    b1, n1 = bench.pass_kernel_bytes("fwd_step", [nts], CELLS, INTERIOR, two=True)
+    assert (n2, n1) == (3, 1)
+    # 72 B for two steps, 52 B for one: 7 shot-steps in all
+    assert b2 == 72 * CELLS * 7 and b1 == 52 * CELLS * 3
+    assert len(bench.launch_live_shots("fwd_two", [nts])) == 0
