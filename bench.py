@@ -270,7 +270,7 @@ def kernel_profile(W, b, torch, step):
     loops = {"fwd": plan.kernel_timing(_native.K_FWD_LOOP)[0],
              "bwd": plan.kernel_timing(_native.K_BWD_LOOP)[0]}
     plan.set_timing(0)
-    groups = {"fwd": ("fwd_vel", "fwd_str", "fwd_step", "fwd_sparse"),
+    groups = {"fwd": ("fwd_vel", "fwd_str", "fwd_step", "fwd_two", "fwd_sparse"),
               "bwd": ("rev_str", "rev_vel", "rev_step", "adj_str", "adj_vel_grad", "adj_step",
                       "bwd_sparse")}
     if "exact_str" in per:
