@@ -560,13 +560,13 @@ def bench_uneven(args, torch):
 
     ps = None
     if args.per_shot_sample > 0:
-        # the same kernels on one shot at a time (the reference's schedule),
-        # on shots spread over the range of nt
+        # the same kernels (the same launch shapes) on one shot at a time (the
+        # reference's schedule), on shots spread over the range of nt
         order_ids = np.argsort(B.nt)
         pick = order_ids[np.linspace(0, B.ns - 1, args.per_shot_sample).round().astype(int)]
         sub = B.shots([int(k) for k in pick])
         sub_units = cells * sum(int(B.nt[int(k)]) for k in pick)
-        plan.set_tuning(1, False, 0)
+        plan.set_tuning(1, bits, 0)
         run(sub[:1])  # warm
         torch.cuda.synchronize()
         ts = time.perf_counter()
