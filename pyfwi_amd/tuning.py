@@ -1,9 +1,10 @@
 """Execution-shape autotuner for the batched gradient.
 
 A plan can run a call's shots in passes of G shots, do each forward or
-time-reversed step as one fused launch or two in-place launches, do each
-reference-adjoint step as one launch or two, and fix the strip height of the
-column-marching kernels (``pyfwi_plan_set_tuning``).  The best choice depends
+time-reversed step as one fused launch or two in-place launches, run the
+kept-state forward two steps a launch, do each reference-adjoint step as one
+launch or two, and fix the strip height of the column-marching kernels
+(``pyfwi_plan_set_tuning``).  The best choice depends
 on the grid, the number of shots and how much of the working set the 256 MB
 on-package cache holds, so it is measured: each candidate runs a forward and
 a reference-mode gradient of the actual shots, truncated to ``nt_trial``
@@ -23,7 +24,7 @@ def default_candidates(nshots, max_pass=None, fused=(0, _native.TUNE_FUSED_STEPS
                        rows=(12, 24)):
     """(shots per pass, fused bits, strip rows) triples; 0 rows = automatic.
     `fused`: the bit sets of ``Plan.set_tuning`` to try (TUNE_FUSED_STEPS,
-    TUNE_ADJ_FUSED; True stands for TUNE_FUSED_STEPS); the fixed strip
+    TUNE_FWD_TWO, TUNE_ADJ_FUSED; True stands for TUNE_FUSED_STEPS); the fixed strip
     heights `rows` are tried with the largest pass and each bit set."""
     top = nshots if max_pass is None else max(1, min(nshots, max_pass))
     passes = sorted({g for g in (top, 10, 8, 5, 4, 2) if 1 <= g <= top}, reverse=True)
